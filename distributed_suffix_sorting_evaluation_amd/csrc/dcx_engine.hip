@@ -1,0 +1,590 @@
+// dcx_engine.hip -- MI355X DCX suffix-array engine: level driver and the C ABI of include/dcx.h.
+//
+// One recursion level (PDCX::compute_sa, lightweight_dcx/src/pdcx/pdcx.hpp:529-941) runs as:
+//   phase 1+2  sample keys (packed characters) -> refinement radix sort -> names by prefix count
+//   phase 3    names distinct: ranks directly; otherwise recursive string in (mod X, div X) order,
+//              discarding decision, recursion, ranks from the recursive SA
+//   phase 4    every suffix keyed by its packed (X-1)-character prefix -> refinement radix sort ->
+//              groups of equal prefixes ordered by the DC comparator on the sample ranks
+// All data stays in HBM; the host only reads back a handful of scalars per step.
+#include <chrono>
+#include <cstdio>
+#include <cstring>
+#include <string>
+
+#include "../../include/dcx.h"
+#include "dcx_kernels.hpp"
+
+namespace dcx {
+
+struct Options {
+    uint32_t X = 39;
+    bool pack_samples = true, pack_merging = true;
+    double discarding_threshold = 0.7;
+    uint64_t base_case_chars = 10000;
+};
+
+class Engine {
+  public:
+    int device = 0;
+    hipStream_t stream = nullptr;
+    DeviceCache cache;
+    Primitives prim;
+    uint64_t budget = 0;
+    std::string last_error;
+
+    // per call
+    Options opt;
+    const DcSet* dcs = nullptr;
+    DBuf<uint32_t> d_cmp;
+    uint32_t cmp_X = 0;
+    uint32_t sigma = 0;
+    dcx_stats* st = nullptr;
+    uint64_t alg_bytes = 0;
+    uint32_t sa_width = 8;
+
+    explicit Engine(int dev) : device(dev) {
+        DCX_HIP(hipSetDevice(device));
+        DCX_HIP(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
+        prim.init(stream, &cache);
+    }
+    ~Engine() {
+        (void)hipSetDevice(device);
+        if (stream) (void)hipStreamSynchronize(stream);
+        d_cmp.reset();
+        prim.destroy();
+        cache.release_all();
+        if (stream) (void)hipStreamDestroy(stream);
+    }
+
+    unsigned grid(uint64_t n, unsigned max_blocks = 8192) const { return Primitives::grid_for(n, 256, max_blocks); }
+
+    void set_dc(uint32_t X) {
+        dcs = find_dc(X);
+        if (!dcs) throw Error(DCX_E_INVALID_X, "X=" + std::to_string(X) + " is not a supported difference cover");
+        if (cmp_X != X) {
+            std::vector<uint32_t> t = make_cmp_table(*dcs);
+            d_cmp = DBuf<uint32_t>(cache, t.size());
+            DCX_HIP(hipMemcpyAsync(d_cmp.get(), t.data(), t.size() * 4, hipMemcpyHostToDevice, stream));
+            DCX_HIP(hipStreamSynchronize(stream));
+            cmp_X = X;
+        }
+    }
+
+    DcDev make_dcdev(uint64_t n, uint64_t* m_out) const {
+        DcDev d;
+        memset(&d, 0, sizeof(d));
+        d.X = dcs->X;
+        d.D = dcs->D;
+        std::vector<uint8_t> in(d.X, 0);
+        for (uint32_t i = 0; i < d.D; i++) {
+            d.dc[i] = dcs->dc[i];
+            in[d.dc[i]] = 1;
+        }
+        uint32_t c = 0;
+        for (uint32_t r = 0; r < d.X; r++) {
+            d.next_rank[r] = (uint8_t)c;
+            c += in[r];
+        }
+        // samples are positions p in [0, n] with p mod X in DC (compute_length_info, pdcx.hpp:473-510)
+        d.sb[0] = 0;
+        for (uint32_t t = 0; t < d.D; t++) {
+            uint64_t cnt = d.dc[t] <= n ? (n - d.dc[t]) / d.X + 1 : 0;
+            d.sb[t + 1] = (uint32_t)(d.sb[t] + cnt);
+        }
+        *m_out = d.sb[d.D];
+        return d;
+    }
+
+    // -------------------------------------------------------------------------------------------
+    // refinement sort: orders N elements by their first L characters, `bits` bits per character;
+    // on return `order` holds the element values in sorted order and H[j] = 1 where a group of
+    // equal L-character keys starts (H[N] = 1).
+    // -------------------------------------------------------------------------------------------
+    template <class TV, int MODE>
+    void refine_sort(const TV& tv, const DcDev& dc, uint32_t N, uint32_t L, uint32_t bits, DBuf<uint32_t>& order,
+                     DBuf<uint8_t>& H) {
+        DBuf<uint64_t> ka(cache, N), kb(cache, N);
+        DBuf<uint32_t> va(cache, N), vb(cache, N);
+        const uint32_t cpw = 64 / bits;
+        uint32_t c0 = L < cpw ? L : cpw;
+        k_keys_r0<TV, MODE><<<grid(N), 256, 0, stream>>>(tv, dc, N, c0, bits, ka.get(), va.get());
+        uint64_t* k = ka.get();
+        uint64_t* k2 = kb.get();
+        uint32_t* v = va.get();
+        uint32_t* v2 = vb.get();
+        prim.sort_pairs(k, k2, v, v2, N, 64 - c0 * bits, 64);
+        H = DBuf<uint8_t>(cache, (size_t)N + 1);
+        k_heads<<<grid(N + 1ull), 256, 0, stream>>>(k, N, H.get());
+        // keep the sorted values as `order`
+        if (v == va.get()) {
+            order = std::move(va);
+        } else {
+            order = std::move(vb);
+        }
+        uint32_t C = c0;
+        if (C >= L) return;
+        DBuf<uint32_t> act(cache, N), gid(cache, N);
+        while (C < L) {
+            ActiveScanOp op{H.get(), act.get(), gid.get()};
+            prim.scan(op, N, 0);
+            const uint64_t tot = prim.read_scalars(1)[0];
+            const uint32_t A = (uint32_t)tot, G = (uint32_t)(tot >> 32);
+            if (A == 0) break;
+            const uint32_t gbits = bit_width(G - 1);
+            uint32_t c = (64 - gbits) / bits;
+            if (c > L - C) c = L - C;
+            if (c == 0) throw Error(DCX_E_UNSUPPORTED, "character width too large for refinement");
+            DBuf<uint64_t> rka(cache, A), rkb(cache, A);
+            DBuf<uint32_t> rva(cache, A), rvb(cache, A);
+            k_keys_rr<TV, MODE><<<grid(A), 256, 0, stream>>>(tv, dc, A, act.get(), gid.get(), order.get(), C, c, bits,
+                                                             gbits, rka.get(), rva.get());
+            uint64_t* x = rka.get();
+            uint64_t* x2 = rkb.get();
+            uint32_t* y = rva.get();
+            uint32_t* y2 = rvb.get();
+            prim.sort_pairs(x, x2, y, y2, A, 64 - gbits - c * bits, 64);
+            k_writeback<<<grid(A), 256, 0, stream>>>(x, y, act.get(), A, order.get(), H.get(), 1);
+            C += c;
+        }
+    }
+
+    // groups of equal merge prefixes -> DC comparator on the sample ranks (tie_break_ranks,
+    // merge_samples.hpp:265-333)
+    void tie_break(uint32_t n, const DcDev& dc, DBuf<uint32_t>& order, DBuf<uint8_t>& H, const uint32_t* ranks) {
+        DBuf<uint32_t> act(cache, n), gid(cache, n);
+        ActiveScanOp op{H.get(), act.get(), gid.get()};
+        prim.scan(op, n, 0);
+        const uint64_t tot = prim.read_scalars(1)[0];
+        const uint32_t A = (uint32_t)tot, G = (uint32_t)(tot >> 32);
+        if (A == 0) return;
+        DBuf<uint32_t> gstart(cache, (size_t)G + 1);
+        k_group_starts<<<grid(A), 256, 0, stream>>>(act.get(), gid.get(), H.get(), A, gstart.get());
+        prim.fill_u32(gstart.get() + G, A, 1);
+        DBuf<uint8_t> large(cache, G);
+        k_tiebreak_small<<<grid(G), 256, 0, stream>>>(gstart.get(), G, act.get(), order.get(), dc, d_cmp.get(), ranks,
+                                                      large.get());
+        // large groups: merge sort with the composite (group, DC comparator) order
+        DBuf<uint64_t> items(cache, A), items2(cache, A);
+        DBuf<uint32_t> slots(cache, A);
+        LargeScanOp lop{large.get(), gid.get(), act.get(), order.get(), items.get(), slots.get()};
+        prim.scan(lop, A, 1);
+        const uint32_t B = (uint32_t)prim.read_scalars(2)[1];
+        if (B == 0) return;
+        k_tb_tile_sort<<<(unsigned)ceil_div(B, TB_TILE), 256, 0, stream>>>(items.get(), B, dc, d_cmp.get(), ranks);
+        uint64_t* in = items.get();
+        uint64_t* out = items2.get();
+        for (uint64_t w = TB_TILE; w < B; w *= 2) {
+            k_tb_merge<<<grid(B), 256, 0, stream>>>(in, out, B, (uint32_t)w, dc, d_cmp.get(), ranks);
+            std::swap(in, out);
+        }
+        k_tb_scatter<<<grid(B), 256, 0, stream>>>(in, slots.get(), B, order.get());
+    }
+
+    template <class TV>
+    void base_case(const TV& tv, uint32_t n, uint32_t* sa_out) {
+        if (n == 0) return;
+        DBuf<uint32_t> rank(cache, (size_t)n + 1);
+        k_base_case<TV><<<1, 1024, 0, stream>>>(tv, n, sa_out, rank.get());
+    }
+
+    dcx_level_stats* level_stat(int depth) {
+        if (!st || depth >= DCX_MAX_LEVELS) return nullptr;
+        if ((uint32_t)depth + 1 > st->levels) st->levels = depth + 1;
+        dcx_level_stats* ls = &st->level[depth];
+        memset(ls, 0, sizeof(*ls));
+        return ls;
+    }
+
+    // bits per character of the reference's packed variants (cli.cpp:488-519), extended by 9 bits
+    static uint32_t ref_pack_bits(uint32_t sig) { return sig <= 7 ? 3 : sig <= 31 ? 5 : sig <= 255 ? 8 : 9; }
+
+    // -------------------------------------------------------------------------------------------
+    // one recursion level: SA of tv[0..n) into sa_out (device, n entries)
+    // -------------------------------------------------------------------------------------------
+    template <class TV>
+    void level(const TV& tv, uint32_t n, uint32_t bits, int depth, uint32_t* sa_out, hipEvent_t* ev) {
+        const uint32_t X = dcs->X, D = dcs->D;
+        dcx_level_stats* ls = level_stat(depth);
+        if (ls) ls->n = n;
+        const uint64_t threshold = opt.base_case_chars > 2ull * X ? opt.base_case_chars : 2ull * X;
+        if (n <= threshold) {
+            if (ls) ls->base_case = 1;
+            base_case(tv, n, sa_out);
+            return;
+        }
+        uint64_t m64 = 0;
+        const DcDev dc = make_dcdev(n, &m64);
+        const uint32_t m = (uint32_t)m64;
+
+        // key lengths (materialize_sample / materialize_characters with the packing ratio)
+        const bool ps = opt.pack_samples && depth == 0;
+        const bool pm = opt.pack_merging && depth == 0;
+        double ratio = 1.0;
+        uint32_t pbits = 0;
+        if (depth == 0 && (opt.pack_samples || opt.pack_merging)) {
+            pbits = ref_pack_bits(sigma);
+            const uint32_t cpw = 64 / pbits;
+            const uint32_t w = (X + cpw - 1) / cpw;
+            ratio = (double)(w * cpw) / (double)X;
+        }
+        const uint32_t Ls = (uint32_t)((double)X * (ps ? ratio : 1.0));
+        const uint32_t Lm = (uint32_t)((double)X * (pm ? ratio : 1.0)) - 1;
+        if (ls) {
+            ls->samples = m;
+            ls->key_chars = Ls;
+            ls->merge_chars = Lm;
+            ls->bits_per_char = (ps || pm) ? pbits : 0;
+        }
+        {  // algorithmic bytes of this level
+            const uint64_t b = depth == 0 ? (pbits ? pbits : ref_pack_bits(sigma)) : bits;
+            const uint64_t c = depth == 0 ? 1 : (bits + 7) / 8;
+            const uint64_t K1 = 8 * ceil_div((uint64_t)X * b, 64), K4 = 8 * ceil_div((uint64_t)(X - 1) * b, 64);
+            const uint64_t I = 4, S = depth == 0 ? sa_width : I;
+            alg_bytes += (uint64_t)n * c + 2ull * m * (K1 + I) + 2ull * n * (K4 + I) + (uint64_t)n * S;
+        }
+
+        // ---------------- phase 1 + 2: sample sort and names ----------------
+        DBuf<uint32_t> ranks(cache, (size_t)m + D);
+        prim.fill_u32(ranks.get() + m, 0, D);
+        {
+            DBuf<uint32_t> order;
+            DBuf<uint8_t> H;
+            refine_sort<TV, 0>(tv, dc, m, Ls, bits, order, H);
+            DBuf<uint32_t> names(cache, m);
+            prim.scan(NamesOp{H.get(), names.get()}, m, 0);
+            const uint32_t M = (uint32_t)prim.read_scalars(1)[0];
+            if (ls) ls->names = M;
+            if (ev && depth == 0) DCX_HIP(hipEventRecord(ev[2], stream));
+
+            // ---------------- phase 3: recursion ----------------
+            if (M >= m) {
+                k_rank_from_order<<<grid(m), 256, 0, stream>>>(order.get(), m, ranks.get());
+            } else {
+                if (ls) ls->recursed = 1;
+                DBuf<uint32_t> R(cache, m);
+                DBuf<uint8_t> U(cache, m);
+                k_build_rec<<<grid(m), 256, 0, stream>>>(order.get(), names.get(), H.get(), m, dc, R.get(), U.get());
+                names.reset();
+                DBuf<uint32_t> Rr(cache, m), red(cache, m);
+                prim.scan(KeepOp{U.get(), R.get(), Rr.get(), red.get()}, m, 0);
+                const uint32_t after = (uint32_t)prim.read_scalars(1)[0];
+                const bool use_disc = ((double)after / (double)m) <= opt.discarding_threshold && n > 80;
+                const uint32_t rbits = bit_width(M);
+                if (!use_disc) {
+                    Rr.reset();
+                    red.reset();
+                    U.reset();
+                    DBuf<uint32_t> sar(cache, m);
+                    TextU32 rv{R.get(), m};
+                    if (n <= 80)
+                        base_case(rv, m, sar.get());  // sequential_sa_on_local_ranks (pdcx.hpp:115-121)
+                    else
+                        level(rv, m, rbits, depth + 1, sar.get(), nullptr);
+                    k_ranks_from_sa<<<grid(m), 256, 0, stream>>>(sar.get(), m, dc, ranks.get());
+                } else {
+                    if (ls) {
+                        ls->discarding = 1;
+                        ls->reduced = after;
+                    }
+                    R.reset();
+                    DBuf<uint32_t> sar(cache, after), isar(cache, after);
+                    level(TextU32{Rr.get(), after}, after, rbits, depth + 1, sar.get(), nullptr);
+                    k_isa<<<grid(after), 256, 0, stream>>>(sar.get(), after, isar.get());
+                    sar.reset();
+                    Rr.reset();
+                    // order of (name, rank2) = sorted-by-name order refined inside non-unique groups
+                    DBuf<uint32_t> act(cache, m), gid(cache, m);
+                    prim.scan(ActiveScanOp{H.get(), act.get(), gid.get()}, m, 0);
+                    const uint64_t tot = prim.read_scalars(1)[0];
+                    const uint32_t A = (uint32_t)tot, G = (uint32_t)(tot >> 32);
+                    if (A > 0) {
+                        const uint32_t gb = bit_width(G - 1), r2b = bit_width(after);
+                        DBuf<uint64_t> ka(cache, A), kb(cache, A);
+                        DBuf<uint32_t> va(cache, A), vb(cache, A);
+                        k_keys_rank2<<<grid(A), 256, 0, stream>>>(A, act.get(), gid.get(), order.get(), dc, red.get(),
+                                                                  isar.get(), r2b, ka.get(), va.get());
+                        uint64_t* x = ka.get();
+                        uint64_t* x2 = kb.get();
+                        uint32_t* y = va.get();
+                        uint32_t* y2 = vb.get();
+                        prim.sort_pairs(x, x2, y, y2, A, 0, gb + r2b);
+                        k_writeback<<<grid(A), 256, 0, stream>>>(x, y, act.get(), A, order.get(), H.get(), 0);
+                    }
+                    k_rank_from_order<<<grid(m), 256, 0, stream>>>(order.get(), m, ranks.get());
+                }
+            }
+        }
+        if (ev && depth == 0) DCX_HIP(hipEventRecord(ev[3], stream));
+
+        // ---------------- phase 4: merge ----------------
+        {
+            DBuf<uint32_t> order;
+            DBuf<uint8_t> H;
+            refine_sort<TV, 1>(tv, dc, n, Lm, bits, order, H);
+            tie_break(n, dc, order, H, ranks.get());
+            DCX_HIP(hipMemcpyAsync(sa_out, order.get(), (size_t)n * 4, hipMemcpyDeviceToDevice, stream));
+        }
+    }
+
+    // -------------------------------------------------------------------------------------------
+    int build_sa(const uint8_t* text, uint64_t n, const dcx_options* o, void* sa_out, uint32_t width, dcx_stats* stats) {
+        auto t0 = std::chrono::steady_clock::now();
+        DCX_HIP(hipSetDevice(device));
+        if (!o || (!text && n) || (!sa_out && n)) throw Error(DCX_E_ARG, "null argument");
+        if (width != 4 && width != 5 && width != 8) throw Error(DCX_E_ARG, "sa_width_bytes must be 4, 5 or 8");
+        opt.X = o->X;
+        opt.pack_samples = o->pack_samples != 0;
+        opt.pack_merging = o->pack_merging != 0;
+        opt.discarding_threshold = o->discarding_threshold;
+        opt.base_case_chars = o->base_case_chars;
+        if (!(opt.discarding_threshold >= 0.0 && opt.discarding_threshold <= 1.0))
+            throw Error(DCX_E_ARG, "discarding_threshold must lie in [0, 1]");
+        set_dc(opt.X);
+        if (opt.base_case_chars > BASE_CASE_MAX)
+            throw Error(DCX_E_ARG, "base_case_chars must be <= " + std::to_string(BASE_CASE_MAX));
+        if (n >= (1ull << 32) - (1ull << 16))
+            throw Error(DCX_E_UNSUPPORTED, "texts of 2^32 - 2^16 characters or more are not supported by this build");
+        if (width == 4 && n > (1ull << 32)) throw Error(DCX_E_WIDTH, "32-bit SA cannot index this text");
+        if (budget) {
+            uint64_t need = estimate_bytes(n);
+            if (need > budget)
+                throw Error(DCX_E_ADMISSION, "footprint " + std::to_string(need) + " exceeds budget " +
+                                                 std::to_string(budget));
+        }
+        st = stats;
+        if (st) memset(st, 0, sizeof(*st));
+        alg_bytes = 0;
+        sa_width = width;
+        cache.reset_peak();
+
+        hipEvent_t ev[6];
+        for (auto& e : ev) DCX_HIP(hipEventCreate(&e));
+        struct EvGuard {
+            hipEvent_t* e;
+            ~EvGuard() {
+                for (int i = 0; i < 6; i++) (void)hipEventDestroy(e[i]);
+            }
+        } evg{ev};
+
+        DCX_HIP(hipEventRecord(ev[0], stream));
+        if (n == 0) {
+            if (dcx_level_stats* ls = level_stat(0)) ls->base_case = 1;
+            DCX_HIP(hipStreamSynchronize(stream));
+            return DCX_OK;
+        }
+        prim.drop_profile();
+        prim.profile = o->profile_kernels != 0;
+        struct ProfGuard {
+            Primitives* p;
+            ~ProfGuard() {
+                p->profile = false;
+                p->drop_profile();
+            }
+        } profg{&prim};
+        // text into HBM (the engine never modifies the caller's text)
+        DBuf<uint8_t> dtext;
+        const uint8_t* d_text = text;
+        if (!o->text_on_device) {
+            dtext = DBuf<uint8_t>(cache, n + 16);
+            DCX_HIP(hipMemcpyAsync(dtext.get(), text, n, hipMemcpyHostToDevice, stream));
+            d_text = dtext.get();
+        }
+        DCX_HIP(hipEventRecord(ev[1], stream));  // text resident
+        // phase 0: alphabet (compress_alphabet, cli.cpp:417-463): used bytes -> 1..sigma
+        DBuf<unsigned long long> hist(cache, 256);
+        DCX_HIP(hipMemsetAsync(hist.get(), 0, 256 * 8, stream));
+        k_byte_hist<<<grid(ceil_div(n, 16), 2048), 256, 0, stream>>>(d_text, n, hist.get());
+        unsigned long long hh[256];
+        DCX_HIP(hipMemcpyAsync(hh, hist.get(), sizeof(hh), hipMemcpyDeviceToHost, stream));
+        DCX_HIP(hipStreamSynchronize(stream));
+        uint16_t lut[256];
+        uint32_t next = 1;
+        for (int c = 0; c < 256; c++) lut[c] = hh[c] ? (uint16_t)next++ : 0;
+        sigma = next - 1;
+        if (st) st->sigma = sigma;
+        DBuf<uint16_t> dlut(cache, 256);
+        DCX_HIP(hipMemcpyAsync(dlut.get(), lut, sizeof(lut), hipMemcpyHostToDevice, stream));
+
+        DBuf<uint32_t> sa(cache, n);
+        level(TextU8{d_text, (uint32_t)n, dlut.get()}, (uint32_t)n, bit_width(sigma), 0, sa.get(), ev);
+        DCX_HIP(hipEventRecord(ev[4], stream));  // SA resident in HBM
+        // output in the requested width
+        if (o->sa_on_device) {
+            k_widen<<<grid(n), 256, 0, stream>>>(sa.get(), n, width, static_cast<uint8_t*>(sa_out));
+        } else if (width == 4) {
+            DCX_HIP(hipMemcpyAsync(sa_out, sa.get(), n * 4, hipMemcpyDeviceToHost, stream));
+        } else {
+            DBuf<uint8_t> wide(cache, n * width);
+            k_widen<<<grid(n), 256, 0, stream>>>(sa.get(), n, width, wide.get());
+            DCX_HIP(hipMemcpyAsync(sa_out, wide.get(), n * width, hipMemcpyDeviceToHost, stream));
+            DCX_HIP(hipStreamSynchronize(stream));
+        }
+        DCX_HIP(hipEventRecord(ev[5], stream));
+        DCX_HIP(hipStreamSynchronize(stream));
+        if (st) {
+            float a = 0;
+            DCX_HIP(hipEventElapsedTime(&a, ev[1], ev[4]));
+            st->ms_device = a;
+            DCX_HIP(hipEventElapsedTime(&a, ev[0], ev[1]));
+            float h2d = a;
+            DCX_HIP(hipEventElapsedTime(&a, ev[1], ev[2]));
+            st->ms_phase[DCX_PHASE_SAMPLES] = a;
+            st->ms_phase[DCX_PHASE_PREPARATION] = h2d;
+            DCX_HIP(hipEventElapsedTime(&a, ev[2], ev[3]));
+            st->ms_phase[DCX_PHASE_RECURSION] = a;
+            DCX_HIP(hipEventElapsedTime(&a, ev[3], ev[4]));
+            st->ms_phase[DCX_PHASE_MERGE] = a;
+            DCX_HIP(hipEventElapsedTime(&a, ev[4], ev[5]));
+            st->ms_phase[DCX_PHASE_OUTPUT] = a;
+            st->alg_bytes = alg_bytes;
+            if (prim.profile) prim.collect_profile(&st->sort_passes, &st->sort_pass_items, &st->sort_pass_ms);
+            st->peak_device_bytes = cache.peak();
+            st->ms_total =
+                std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+        }
+        return DCX_OK;
+    }
+
+    // conservative device footprint of one build (level 0 dominates; deeper levels shrink by D/X)
+    uint64_t estimate_bytes(uint64_t n) const {
+        return n * 64 + (64ull << 20);
+    }
+
+    int check_sa(const uint8_t* text, uint64_t n, const void* sa, uint32_t width, int on_device, int* ok) {
+        DCX_HIP(hipSetDevice(device));
+        if (!ok || (n && (!text || !sa))) throw Error(DCX_E_ARG, "null argument");
+        if (width != 4 && width != 5 && width != 8) throw Error(DCX_E_ARG, "sa_width_bytes must be 4, 5 or 8");
+        if (n >= (1ull << 32) - 1) throw Error(DCX_E_UNSUPPORTED, "text too long for this build");
+        *ok = 1;
+        if (n == 0) return DCX_OK;
+        DBuf<uint8_t> dt, ds;
+        const uint8_t* d_text = text;
+        const uint8_t* d_sa = static_cast<const uint8_t*>(sa);
+        if (!on_device) {
+            dt = DBuf<uint8_t>(cache, n);
+            ds = DBuf<uint8_t>(cache, n * width);
+            DCX_HIP(hipMemcpyAsync(dt.get(), text, n, hipMemcpyHostToDevice, stream));
+            DCX_HIP(hipMemcpyAsync(ds.get(), sa, n * width, hipMemcpyHostToDevice, stream));
+            d_text = dt.get();
+            d_sa = ds.get();
+        }
+        DBuf<uint32_t> rank(cache, n + 1);
+        DBuf<uint32_t> bad(cache, 1);
+        prim.fill_u32(rank.get(), 0, n + 1);
+        prim.fill_u32(bad.get(), 0, 1);
+        k_check_perm<<<grid(n), 256, 0, stream>>>(d_sa, width, n, rank.get(), bad.get());
+        k_check_order<<<grid(n), 256, 0, stream>>>(d_sa, width, n, d_text, rank.get(), bad.get());
+        uint32_t hb = 0;
+        DCX_HIP(hipMemcpyAsync(&hb, bad.get(), 4, hipMemcpyDeviceToHost, stream));
+        DCX_HIP(hipStreamSynchronize(stream));
+        *ok = hb == 0;
+        return DCX_OK;
+    }
+};
+
+}  // namespace dcx
+
+struct dcx_handle {
+    dcx::Engine* eng;
+};
+
+using dcx::Error;
+
+extern "C" {
+
+void dcx_default_options(dcx_options* opt) {
+    if (!opt) return;
+    memset(opt, 0, sizeof(*opt));
+    opt->X = 39;
+    opt->pack_samples = 1;
+    opt->pack_merging = 1;
+    opt->discarding_threshold = 0.7;
+    opt->base_case_chars = 10000;
+}
+
+int dcx_create(dcx_handle** out, int device, uint64_t hbm_budget_bytes) {
+    if (!out) return DCX_E_ARG;
+    *out = nullptr;
+    int count = 0;
+    if (hipGetDeviceCount(&count) != hipSuccess || count == 0) {
+        (void)hipGetLastError();
+        return DCX_E_NO_GPU;
+    }
+    if (device < 0 || device >= count) return DCX_E_ARG;
+    try {
+        dcx_handle* h = new dcx_handle;
+        h->eng = new dcx::Engine(device);
+        h->eng->budget = hbm_budget_bytes;
+        *out = h;
+        return DCX_OK;
+    } catch (const Error& e) {
+        return e.code;
+    } catch (...) {
+        return DCX_E_HIP;
+    }
+}
+
+int dcx_destroy(dcx_handle* h) {
+    if (!h) return DCX_E_ARG;
+    delete h->eng;
+    delete h;
+    return DCX_OK;
+}
+
+int dcx_build_sa(dcx_handle* h, const uint8_t* text, uint64_t n, const dcx_options* opt, void* sa_out,
+                 uint32_t sa_width_bytes, dcx_stats* stats) {
+    if (!h || !h->eng) return DCX_E_ARG;
+    try {
+        h->eng->last_error.clear();
+        return h->eng->build_sa(text, n, opt, sa_out, sa_width_bytes, stats);
+    } catch (const Error& e) {
+        h->eng->last_error = e.what();
+        (void)hipStreamSynchronize(h->eng->stream);
+        (void)hipGetLastError();
+        return e.code;
+    } catch (const std::exception& e) {
+        h->eng->last_error = e.what();
+        return DCX_E_HIP;
+    }
+}
+
+int dcx_check_sa(dcx_handle* h, const uint8_t* text, uint64_t n, const void* sa, uint32_t sa_width_bytes, int on_device,
+                 int* ok) {
+    if (!h || !h->eng) return DCX_E_ARG;
+    try {
+        h->eng->last_error.clear();
+        return h->eng->check_sa(text, n, sa, sa_width_bytes, on_device, ok);
+    } catch (const Error& e) {
+        h->eng->last_error = e.what();
+        return e.code;
+    } catch (const std::exception& e) {
+        h->eng->last_error = e.what();
+        return DCX_E_HIP;
+    }
+}
+
+const char* dcx_strerror(int code) {
+    switch (code) {
+        case DCX_OK: return "ok";
+        case DCX_E_INVALID_X: return "unsupported difference cover X";
+        case DCX_E_ALPHABET: return "alphabet cannot be represented";
+        case DCX_E_NOMEM: return "device memory exhausted";
+        case DCX_E_HIP: return "HIP runtime error";
+        case DCX_E_IO: return "file I/O error";
+        case DCX_E_ARG: return "invalid argument";
+        case DCX_E_WIDTH: return "SA width too small for the text length";
+        case DCX_E_ADMISSION: return "request exceeds the HBM budget";
+        case DCX_E_UNSUPPORTED: return "unsupported size";
+        case DCX_E_CANCELLED: return "request cancelled";
+        case DCX_E_NO_GPU: return "no HIP device available";
+        default: return "unknown error";
+    }
+}
+
+const char* dcx_last_error(dcx_handle* h) {
+    if (!h || !h->eng) return "";
+    return h->eng->last_error.c_str();
+}
+
+}  // extern "C"

@@ -1,0 +1,533 @@
+// dcx_kernels.hpp -- DCX-specific device code: difference-cover parameters, packed tuple
+// construction, naming, recursion bookkeeping, the DC comparator and the single-workgroup base case.
+#pragma once
+
+#include "primitives.hpp"
+
+namespace dcx {
+
+constexpr uint32_t MAX_X = 133;
+constexpr uint32_t MAX_D = 12;
+constexpr uint32_t BASE_CASE_MAX = 16384;
+
+// Difference covers of lightweight_dcx/src/pdcx/difference_cover.hpp (DC3Param .. DC133Param, DC[D]).
+struct DcSet {
+    uint32_t X, D;
+    uint32_t dc[MAX_D];
+};
+static const DcSet kDcSets[] = {
+    {3, 2, {1, 2}},
+    {7, 3, {1, 2, 4}},
+    {13, 4, {1, 2, 4, 10}},
+    {21, 5, {1, 2, 7, 9, 19}},
+    {31, 6, {1, 2, 4, 9, 13, 19}},
+    {39, 7, {1, 2, 17, 21, 23, 28, 31}},
+    {57, 8, {1, 2, 10, 12, 15, 36, 40, 52}},
+    {73, 9, {1, 2, 4, 8, 16, 32, 37, 55, 64}},
+    {91, 10, {1, 2, 8, 17, 28, 57, 61, 69, 71, 74}},
+    {95, 11, {1, 2, 6, 9, 19, 21, 30, 32, 46, 62, 68}},
+    {133, 12, {1, 2, 33, 43, 45, 49, 52, 60, 73, 78, 98, 112}},
+};
+
+inline const DcSet* find_dc(uint32_t X) {
+    for (const auto& s : kDcSets)
+        if (s.X == X) return &s;
+    return nullptr;
+}
+
+// cmpDepthRanks[a][b] = {l, r1, r2}: l = least j >= 0 with (a+j) mod X and (b+j) mod X both in DC,
+// r1 / r2 = number of DC positions in [a, a+l) / [b, b+l) (closed form of
+// B_DC_3_7_13/tools/calc_dcparam.cc:271-301; equals difference_cover.hpp's tables).
+// Packed as l | r1 << 8 | r2 << 16.
+inline std::vector<uint32_t> make_cmp_table(const DcSet& s) {
+    std::vector<uint8_t> in(s.X, 0);
+    for (uint32_t i = 0; i < s.D; i++) in[s.dc[i]] = 1;
+    std::vector<uint32_t> t(s.X * s.X);
+    for (uint32_t a = 0; a < s.X; a++)
+        for (uint32_t b = 0; b < s.X; b++) {
+            uint32_t j = 0, r1 = 0, r2 = 0;
+            while (j < s.X && (!in[(a + j) % s.X] || !in[(b + j) % s.X])) {
+                r1 += in[(a + j) % s.X];
+                r2 += in[(b + j) % s.X];
+                j++;
+            }
+            t[a * s.X + b] = j | (r1 << 8) | (r2 << 16);
+        }
+    return t;
+}
+
+// Per-level parameters handed to kernels by value.
+struct DcDev {
+    uint32_t X, D;
+    uint32_t dc[MAX_D];
+    uint32_t sb[MAX_D + 1];     // samples_before (pdcx.hpp:499-508): class-major offsets
+    uint8_t next_rank[MAX_X + 3];  // #{d in DC : d < r}  (difference_cover.hpp NEXT_RANK)
+};
+
+// k-th sample in text order (k = rank_pos of the sample, merge_samples.hpp:181-188)
+__device__ __forceinline__ uint32_t sample_pos(const DcDev& c, uint32_t k) {
+    uint32_t q = k / c.D;
+    return q * c.X + c.dc[k - q * c.D];
+}
+// class-major index of the k-th sample (DCRankIndex::cmp_mod_div order, compute_ranks.hpp:49-56)
+__device__ __forceinline__ uint32_t class_major(const DcDev& c, uint32_t k) {
+    uint32_t q = k / c.D;
+    return c.sb[k - q * c.D] + q;
+}
+// inverse of class_major (map_back of pdcx.hpp:83-94, expressed as the text-order sample index)
+__device__ __forceinline__ uint32_t class_major_inv(const DcDev& c, uint32_t cm) {
+    uint32_t t = c.D - 1;
+    while (cm < c.sb[t]) t--;
+    return (cm - c.sb[t]) * c.D + t;
+}
+// index of the first sample at or after text position i
+__device__ __forceinline__ uint32_t rank_pos(const DcDev& c, uint32_t i) {
+    uint32_t q = i / c.X;
+    return q * c.D + c.next_rank[i - q * c.X];
+}
+
+// ------------------------------------------------------------------------------------------------
+// Text views: characters are 1..sigma (level 0, through the alphabet map) or names >= 1 (deeper
+// levels); every position at or past n reads as the padding character 0 (sample_string.hpp:112-117).
+// ------------------------------------------------------------------------------------------------
+struct TextU8 {
+    const uint8_t* t;
+    uint32_t n;
+    const uint16_t* lut;
+    __device__ __forceinline__ uint32_t at(uint64_t i) const { return i < n ? (uint32_t)lut[t[i]] : 0u; }
+};
+struct TextU32 {
+    const uint32_t* t;
+    uint32_t n;
+    __device__ __forceinline__ uint32_t at(uint64_t i) const { return i < n ? t[i] : 0u; }
+};
+
+// Left-justified packing of `count` characters of `bits` bits (PackedInteger,
+// char_container.hpp:57-71): character k occupies bits [64-(k+1)*bits, 64-k*bits).
+template <class TV>
+__device__ __forceinline__ uint64_t pack_chars(const TV& tv, uint64_t p, uint32_t count, uint32_t bits) {
+    uint64_t w = 0;
+    for (uint32_t k = 0; k < count; k++) w |= (uint64_t)tv.at(p + k) << (64 - (k + 1) * bits);
+    return w;
+}
+
+// ------------------------------------------------------------------------------------------------
+// alphabet histogram (compress_alphabet, cli.cpp:417-463)
+// ------------------------------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_byte_hist(const uint8_t* __restrict__ t, uint64_t n,
+                                                   unsigned long long* __restrict__ hist) {
+    __shared__ uint32_t s_h[4][256];
+    for (int i = threadIdx.x; i < 1024; i += 256) (&s_h[0][0])[i] = 0;
+    __syncthreads();
+    const uint32_t w = threadIdx.x / 64;
+    const uint64_t stride = (uint64_t)gridDim.x * 256 * 16;
+    for (uint64_t base = (blockIdx.x * 256ull + threadIdx.x) * 16; base < n; base += stride) {
+        if (base + 16 <= n && ((reinterpret_cast<uintptr_t>(t + base) & 15) == 0)) {
+            uint4 v = *reinterpret_cast<const uint4*>(t + base);
+            uint32_t x[4] = {v.x, v.y, v.z, v.w};
+#pragma unroll
+            for (int q = 0; q < 4; q++)
+#pragma unroll
+                for (int b = 0; b < 4; b++) atomicAdd(&s_h[w][(x[q] >> (8 * b)) & 0xFF], 1u);
+        } else {
+            for (uint64_t i = base; i < base + 16 && i < n; i++) atomicAdd(&s_h[w][t[i]], 1u);
+        }
+    }
+    __syncthreads();
+    uint32_t c = s_h[0][threadIdx.x] + s_h[1][threadIdx.x] + s_h[2][threadIdx.x] + s_h[3][threadIdx.x];
+    if (c) atomicAdd(&hist[threadIdx.x], (unsigned long long)c);
+}
+
+// ------------------------------------------------------------------------------------------------
+// refinement sort: round 0 keys, heads, active compaction, later-round keys, write back
+// MODE 0: elements are samples (value = text-order sample index k); MODE 1: all positions.
+// ------------------------------------------------------------------------------------------------
+template <class TV, int MODE>
+__global__ __launch_bounds__(256) void k_keys_r0(TV tv, DcDev dc, uint32_t N, uint32_t count, uint32_t bits,
+                                                 uint64_t* __restrict__ keys, uint32_t* __restrict__ vals) {
+    for (uint32_t j = blockIdx.x * 256 + threadIdx.x; j < N; j += gridDim.x * 256) {
+        uint32_t p = MODE == 0 ? sample_pos(dc, j) : j;
+        keys[j] = pack_chars(tv, p, count, bits);
+        vals[j] = j;
+    }
+}
+
+__global__ __launch_bounds__(256) void k_heads(const uint64_t* __restrict__ keys, uint32_t N, uint8_t* __restrict__ H) {
+    for (uint32_t j = blockIdx.x * 256 + threadIdx.x; j <= N; j += gridDim.x * 256)
+        H[j] = (j == 0 || j == N) ? 1 : (keys[j] != keys[j - 1]);
+}
+
+// active = member of a group of size >= 2; value packs (active, active && head) as two 32-bit lanes
+struct ActiveScanOp {
+    const uint8_t* H;
+    uint32_t* act_idx;
+    uint32_t* gid;
+    __device__ __forceinline__ uint64_t value(uint64_t j) const {
+        uint32_t h = H[j];
+        bool active = !(h && H[j + 1]);
+        return active ? (1ull | ((uint64_t)h << 32)) : 0ull;
+    }
+    __device__ __forceinline__ void write(uint64_t j, uint64_t excl, uint64_t v) const {
+        if (v) {
+            uint32_t a = (uint32_t)excl;
+            act_idx[a] = (uint32_t)j;
+            gid[a] = (uint32_t)(excl >> 32) + (uint32_t)(v >> 32) - 1u;
+        }
+    }
+};
+
+template <class TV, int MODE>
+__global__ __launch_bounds__(256) void k_keys_rr(TV tv, DcDev dc, uint32_t A, const uint32_t* __restrict__ act_idx,
+                                                 const uint32_t* __restrict__ gid, const uint32_t* __restrict__ order,
+                                                 uint32_t C, uint32_t count, uint32_t bits, uint32_t gbits,
+                                                 uint64_t* __restrict__ keys, uint32_t* __restrict__ vals) {
+    for (uint32_t a = blockIdx.x * 256 + threadIdx.x; a < A; a += gridDim.x * 256) {
+        uint32_t e = order[act_idx[a]];
+        uint32_t p = MODE == 0 ? sample_pos(dc, e) : e;
+        uint64_t ch = pack_chars(tv, (uint64_t)p + C, count, bits);
+        keys[a] = gbits ? (((uint64_t)gid[a] << (64 - gbits)) | (ch >> gbits)) : ch;
+        vals[a] = e;
+    }
+}
+
+__global__ __launch_bounds__(256) void k_writeback(const uint64_t* __restrict__ keys, const uint32_t* __restrict__ vals,
+                                                   const uint32_t* __restrict__ act_idx, uint32_t A,
+                                                   uint32_t* __restrict__ order, uint8_t* __restrict__ H, int mark) {
+    for (uint32_t a = blockIdx.x * 256 + threadIdx.x; a < A; a += gridDim.x * 256) {
+        uint32_t j = act_idx[a];
+        order[j] = vals[a];
+        if (mark && a > 0 && keys[a] != keys[a - 1]) H[j] = 1;
+    }
+}
+
+// names = 1 + number of distinct keys before (compute_ranks.hpp:88-111): inclusive count of heads
+struct NamesOp {
+    const uint8_t* H;
+    uint32_t* names;
+    __device__ __forceinline__ uint64_t value(uint64_t j) const { return H[j]; }
+    __device__ __forceinline__ void write(uint64_t j, uint64_t excl, uint64_t v) const {
+        names[j] = (uint32_t)(excl + v);
+    }
+};
+
+// ranks in text order from the sorted sample order: ranks[k] = 1 + sorted position
+__global__ __launch_bounds__(256) void k_rank_from_order(const uint32_t* __restrict__ order, uint32_t m,
+                                                         uint32_t* __restrict__ ranks) {
+    for (uint32_t j = blockIdx.x * 256 + threadIdx.x; j < m; j += gridDim.x * 256) ranks[order[j]] = j + 1;
+}
+
+// recursive string in (mod X, div X) order with unique flags (handle_recursive_call,
+// pdcx.hpp:145-180; flag_unique_ranks, compute_ranks.hpp:113-134)
+__global__ __launch_bounds__(256) void k_build_rec(const uint32_t* __restrict__ order, const uint32_t* __restrict__ names,
+                                                   const uint8_t* __restrict__ H, uint32_t m, DcDev dc,
+                                                   uint32_t* __restrict__ R, uint8_t* __restrict__ U) {
+    for (uint32_t j = blockIdx.x * 256 + threadIdx.x; j < m; j += gridDim.x * 256) {
+        uint32_t c = class_major(dc, order[j]);
+        R[c] = names[j];
+        U[c] = H[j] && H[j + 1];
+    }
+}
+
+// discarding (pdcx.hpp:215-246): keep c unless c and c-1 are both unique; element 0 is kept
+struct KeepOp {
+    const uint8_t* U;
+    const uint32_t* R;
+    uint32_t* Rr;
+    uint32_t* red;
+    __device__ __forceinline__ uint64_t value(uint64_t c) const { return (c == 0 || !(U[c] && U[c - 1])) ? 1u : 0u; }
+    __device__ __forceinline__ void write(uint64_t c, uint64_t excl, uint64_t v) const {
+        red[c] = (uint32_t)excl;
+        if (v) Rr[excl] = R[c];
+    }
+};
+
+// recursive_call_direct (pdcx.hpp:182-212): rank of map_back(SA_R[j]) = j + 1
+__global__ __launch_bounds__(256) void k_ranks_from_sa(const uint32_t* __restrict__ sar, uint32_t m, DcDev dc,
+                                                       uint32_t* __restrict__ ranks) {
+    for (uint32_t j = blockIdx.x * 256 + threadIdx.x; j < m; j += gridDim.x * 256)
+        ranks[class_major_inv(dc, sar[j])] = j + 1;
+}
+
+__global__ __launch_bounds__(256) void k_isa(const uint32_t* __restrict__ sa, uint32_t n, uint32_t* __restrict__ isa) {
+    for (uint32_t j = blockIdx.x * 256 + threadIdx.x; j < n; j += gridDim.x * 256) isa[sa[j]] = j + 1;
+}
+
+// RankRankIndex keys (pdcx.hpp:297-323) restricted to the non-unique groups: (group, rank2)
+__global__ __launch_bounds__(256) void k_keys_rank2(uint32_t A, const uint32_t* __restrict__ act_idx,
+                                                    const uint32_t* __restrict__ gid, const uint32_t* __restrict__ order,
+                                                    DcDev dc, const uint32_t* __restrict__ red,
+                                                    const uint32_t* __restrict__ isar, uint32_t rbits,
+                                                    uint64_t* __restrict__ keys, uint32_t* __restrict__ vals) {
+    for (uint32_t a = blockIdx.x * 256 + threadIdx.x; a < A; a += gridDim.x * 256) {
+        uint32_t k = order[act_idx[a]];
+        uint32_t r2 = isar[red[class_major(dc, k)]];
+        keys[a] = ((uint64_t)gid[a] << rbits) | r2;
+        vals[a] = k;
+    }
+}
+
+// ------------------------------------------------------------------------------------------------
+// DC comparator (DCMergeSamples::operator<, merge_samples.hpp:87-109) for two suffixes whose first
+// compared characters are equal: ranks[rank_pos(a) + r1] < ranks[rank_pos(b) + r2].
+// ------------------------------------------------------------------------------------------------
+__device__ __forceinline__ bool dc_rank_less(uint32_t a, uint32_t b, const DcDev& dc, const uint32_t* __restrict__ cmp,
+                                             const uint32_t* __restrict__ ranks) {
+    uint32_t qa = a / dc.X, qb = b / dc.X;
+    uint32_t ma = a - qa * dc.X, mb = b - qb * dc.X;
+    uint32_t e = cmp[ma * dc.X + mb];
+    uint32_t r1 = (e >> 8) & 0xFF, r2 = (e >> 16) & 0xFF;
+    uint32_t ra = ranks[qa * dc.D + dc.next_rank[ma] + r1];
+    uint32_t rb = ranks[qb * dc.D + dc.next_rank[mb] + r2];
+    return ra < rb;
+}
+
+__global__ __launch_bounds__(256) void k_group_starts(const uint32_t* __restrict__ act_idx, const uint32_t* __restrict__ gid,
+                                                      const uint8_t* __restrict__ H, uint32_t A, uint32_t* __restrict__ gstart) {
+    for (uint32_t a = blockIdx.x * 256 + threadIdx.x; a < A; a += gridDim.x * 256)
+        if (H[act_idx[a]]) gstart[gid[a]] = a;
+}
+
+constexpr uint32_t TB_SMALL = 16;
+
+// groups of size <= TB_SMALL: one thread sorts its group by insertion sort with the DC comparator;
+// larger groups are flagged for the merge-sort path
+__global__ __launch_bounds__(256) void k_tiebreak_small(const uint32_t* __restrict__ gstart, uint32_t G,
+                                                        const uint32_t* __restrict__ act_idx, uint32_t* __restrict__ order,
+                                                        DcDev dc, const uint32_t* __restrict__ cmp,
+                                                        const uint32_t* __restrict__ ranks, uint8_t* __restrict__ large) {
+    for (uint32_t g = blockIdx.x * 256 + threadIdx.x; g < G; g += gridDim.x * 256) {
+        uint32_t a0 = gstart[g], s = gstart[g + 1] - a0;
+        if (s > TB_SMALL) {
+            large[g] = 1;
+            continue;
+        }
+        large[g] = 0;
+        uint32_t j0 = act_idx[a0];
+        uint32_t e[TB_SMALL];
+        for (uint32_t t = 0; t < s; t++) e[t] = order[j0 + t];
+        for (uint32_t t = 1; t < s; t++) {
+            uint32_t x = e[t];
+            int u = (int)t - 1;
+            while (u >= 0 && dc_rank_less(x, e[u], dc, cmp, ranks)) {
+                e[u + 1] = e[u];
+                u--;
+            }
+            e[u + 1] = x;
+        }
+        for (uint32_t t = 0; t < s; t++) order[j0 + t] = e[t];
+    }
+}
+
+// elements of the large groups: (group << 32 | suffix) plus the order slot they came from
+struct LargeScanOp {
+    const uint8_t* large;
+    const uint32_t* gid;
+    const uint32_t* act_idx;
+    const uint32_t* order;
+    uint64_t* items;
+    uint32_t* slots;
+    __device__ __forceinline__ uint64_t value(uint64_t a) const { return large[gid[a]]; }
+    __device__ __forceinline__ void write(uint64_t a, uint64_t excl, uint64_t v) const {
+        if (v) {
+            uint32_t j = act_idx[a];
+            items[excl] = ((uint64_t)gid[a] << 32) | order[j];
+            slots[excl] = j;
+        }
+    }
+};
+
+__device__ __forceinline__ bool item_less(uint64_t x, uint64_t y, const DcDev& dc, const uint32_t* __restrict__ cmp,
+                                          const uint32_t* __restrict__ ranks) {
+    uint32_t gx = (uint32_t)(x >> 32), gy = (uint32_t)(y >> 32);
+    if (gx != gy) return gx < gy;
+    if (gx == 0xFFFFFFFFu) return false;  // padding
+    return dc_rank_less((uint32_t)x, (uint32_t)y, dc, cmp, ranks);
+}
+
+constexpr uint32_t TB_TILE = 1024;
+
+// sorts each tile of TB_TILE items in LDS (bitonic network, DC comparator)
+__global__ __launch_bounds__(256) void k_tb_tile_sort(uint64_t* __restrict__ items, uint32_t B, DcDev dc,
+                                                      const uint32_t* __restrict__ cmp, const uint32_t* __restrict__ ranks) {
+    __shared__ uint64_t s[TB_TILE];
+    const uint64_t base = (uint64_t)blockIdx.x * TB_TILE;
+    for (uint32_t i = threadIdx.x; i < TB_TILE; i += 256) s[i] = base + i < B ? items[base + i] : ~0ull;
+    __syncthreads();
+    for (uint32_t k = 2; k <= TB_TILE; k <<= 1)
+        for (uint32_t j = k >> 1; j > 0; j >>= 1) {
+            for (uint32_t t = threadIdx.x; t < TB_TILE / 2; t += 256) {
+                uint32_t i = (t / j) * 2 * j + (t % j);
+                uint64_t x = s[i], y = s[i + j];
+                bool asc = (i & k) == 0;
+                bool gt = item_less(y, x, dc, cmp, ranks);
+                if (gt == asc) {
+                    s[i] = y;
+                    s[i + j] = x;
+                }
+            }
+            __syncthreads();
+        }
+    for (uint32_t i = threadIdx.x; i < TB_TILE; i += 256)
+        if (base + i < B) items[base + i] = s[i];
+}
+
+// one merge pass: runs of width w are merged pairwise; every item finds its rank in the partner run
+__global__ __launch_bounds__(256) void k_tb_merge(const uint64_t* __restrict__ in, uint64_t* __restrict__ out, uint32_t B,
+                                                  uint32_t w, DcDev dc, const uint32_t* __restrict__ cmp,
+                                                  const uint32_t* __restrict__ ranks) {
+    for (uint64_t i = blockIdx.x * 256ull + threadIdx.x; i < B; i += (uint64_t)gridDim.x * 256) {
+        uint64_t s = (i / (2ull * w)) * 2ull * w;
+        uint64_t x = in[i];
+        bool left = (i - s) < w;
+        uint64_t lo, hi;
+        if (left) {
+            lo = s + w < B ? s + w : B;
+            hi = s + 2ull * w < B ? s + 2ull * w : B;
+        } else {
+            lo = s;
+            hi = s + w;
+        }
+        // count partner items < x (items are distinct)
+        uint64_t a = lo, b = hi;
+        while (a < b) {
+            uint64_t mid = (a + b) >> 1;
+            if (item_less(in[mid], x, dc, cmp, ranks))
+                a = mid + 1;
+            else
+                b = mid;
+        }
+        uint64_t before = a - lo;
+        uint64_t own = left ? (i - s) : (i - s - w);
+        out[s + own + before] = x;
+    }
+}
+
+__global__ __launch_bounds__(256) void k_tb_scatter(const uint64_t* __restrict__ items, const uint32_t* __restrict__ slots,
+                                                    uint32_t B, uint32_t* __restrict__ order) {
+    for (uint32_t b = blockIdx.x * 256 + threadIdx.x; b < B; b += gridDim.x * 256) order[slots[b]] = (uint32_t)items[b];
+}
+
+// ------------------------------------------------------------------------------------------------
+// Base case (slow_suffixarray, string_util.hpp:87-106; sequential_sa.hpp:17-28): one workgroup,
+// prefix doubling with a bitonic sort of (rank, rank+h, index) keys held in LDS.  n <= 16384.
+// ------------------------------------------------------------------------------------------------
+__device__ __forceinline__ void lds_bitonic(uint64_t* s, uint32_t P) {
+    for (uint32_t k = 2; k <= P; k <<= 1)
+        for (uint32_t j = k >> 1; j > 0; j >>= 1) {
+            for (uint32_t t = threadIdx.x; t < P / 2; t += blockDim.x) {
+                uint32_t i = (t / j) * 2 * j + (t % j);
+                uint64_t x = s[i], y = s[i + j];
+                bool asc = (i & k) == 0;
+                if ((x > y) == asc) {
+                    s[i] = y;
+                    s[i + j] = x;
+                }
+            }
+            __syncthreads();
+        }
+}
+
+// assigns dense ranks (1-based) to the sorted keys; groups compare on key >> 14
+__device__ __forceinline__ uint32_t lds_assign_ranks(const uint64_t* s, uint32_t n, uint32_t* rank, uint32_t* s_tmp) {
+    const uint32_t per = (n + blockDim.x - 1) / blockDim.x;
+    const uint32_t b = threadIdx.x * per;
+    uint32_t cnt = 0;
+    for (uint32_t j = b; j < b + per && j < n; j++) cnt += (j == 0) || ((s[j] >> 14) != (s[j - 1] >> 14));
+    uint32_t total = 0;
+    uint32_t r = block_excl_scan<1024>(cnt, s_tmp, &total);
+    for (uint32_t j = b; j < b + per && j < n; j++) {
+        r += (j == 0) || ((s[j] >> 14) != (s[j - 1] >> 14));
+        rank[s[j] & 0x3FFF] = r;
+    }
+    __syncthreads();
+    return total;
+}
+
+template <class TV>
+__global__ __launch_bounds__(1024) void k_base_case(TV tv, uint32_t n, uint32_t* __restrict__ sa,
+                                                    uint32_t* __restrict__ rank) {
+    __shared__ uint64_t s[BASE_CASE_MAX];
+    __shared__ uint32_t s_tmp[16];
+    uint32_t P = 2;
+    while (P < n) P <<= 1;
+    for (uint32_t i = threadIdx.x; i < P; i += 1024) s[i] = i < n ? (((uint64_t)tv.at(i) << 14) | i) : ~0ull;
+    __syncthreads();
+    lds_bitonic(s, P);
+    uint32_t maxr = lds_assign_ranks(s, n, rank, s_tmp);
+    for (uint32_t h = 1; maxr < n; h <<= 1) {
+        for (uint32_t i = threadIdx.x; i < P; i += 1024) {
+            if (i < n) {
+                uint64_t r1 = rank[i];
+                uint64_t r2 = i + h < n ? rank[i + h] : 0;
+                s[i] = (r1 << 29) | (r2 << 14) | i;
+            } else {
+                s[i] = ~0ull;
+            }
+        }
+        __syncthreads();
+        lds_bitonic(s, P);
+        maxr = lds_assign_ranks(s, n, rank, s_tmp);
+    }
+    for (uint32_t j = threadIdx.x; j < n; j += 1024) sa[j] = (uint32_t)(s[j] & 0x3FFF);
+}
+
+// ------------------------------------------------------------------------------------------------
+// output width conversion: uint32 SA -> 4 / 5 (uint40, uint_types.hpp:262) / 8 bytes little endian
+// ------------------------------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_widen(const uint32_t* __restrict__ sa, uint64_t n, uint32_t width,
+                                               uint8_t* __restrict__ out) {
+    for (uint64_t j = blockIdx.x * 256ull + threadIdx.x; j < n; j += (uint64_t)gridDim.x * 256) {
+        uint32_t v = sa[j];
+        if (width == 8) {
+            reinterpret_cast<uint64_t*>(out)[j] = v;
+        } else if (width == 4) {
+            reinterpret_cast<uint32_t*>(out)[j] = v;
+        } else {
+            uint8_t* o = out + 5 * j;
+            o[0] = (uint8_t)v;
+            o[1] = (uint8_t)(v >> 8);
+            o[2] = (uint8_t)(v >> 16);
+            o[3] = (uint8_t)(v >> 24);
+            o[4] = 0;
+        }
+    }
+}
+
+// ------------------------------------------------------------------------------------------------
+// SA validity check (check_suffixarray, sa_check.hpp:56-184)
+// ------------------------------------------------------------------------------------------------
+__device__ __forceinline__ uint64_t load_sa(const uint8_t* sa, uint32_t width, uint64_t k) {
+    if (width == 8) return reinterpret_cast<const uint64_t*>(sa)[k];
+    if (width == 4) return reinterpret_cast<const uint32_t*>(sa)[k];
+    const uint8_t* p = sa + 5 * k;
+    return (uint64_t)p[0] | ((uint64_t)p[1] << 8) | ((uint64_t)p[2] << 16) | ((uint64_t)p[3] << 24) |
+           ((uint64_t)p[4] << 32);
+}
+
+__global__ __launch_bounds__(256) void k_check_perm(const uint8_t* __restrict__ sa, uint32_t width, uint64_t n,
+                                                    uint32_t* __restrict__ rank, uint32_t* __restrict__ bad) {
+    for (uint64_t k = blockIdx.x * 256ull + threadIdx.x; k < n; k += (uint64_t)gridDim.x * 256) {
+        uint64_t p = load_sa(sa, width, k);
+        if (p >= n) {
+            atomicOr(bad, 1u);
+            continue;
+        }
+        uint32_t old = atomicExch(&rank[p], (uint32_t)(k + 1));
+        if (old != 0) atomicOr(bad, 1u);
+    }
+}
+
+__global__ __launch_bounds__(256) void k_check_order(const uint8_t* __restrict__ sa, uint32_t width, uint64_t n,
+                                                     const uint8_t* __restrict__ text, const uint32_t* __restrict__ rank,
+                                                     uint32_t* __restrict__ bad) {
+    for (uint64_t k = blockIdx.x * 256ull + threadIdx.x; k + 1 < n; k += (uint64_t)gridDim.x * 256) {
+        uint64_t p = load_sa(sa, width, k), q = load_sa(sa, width, k + 1);
+        if (p >= n || q >= n) continue;  // reported by the permutation check
+        uint8_t cp = text[p], cq = text[q];
+        uint32_t rp = rank[p + 1], rq = rank[q + 1];
+        bool ok = cp != cq ? (cp < cq) : (rp <= rq);
+        if (!ok) atomicOr(bad, 2u);
+    }
+}
+
+}  // namespace dcx

@@ -1,0 +1,458 @@
+// primitives.hpp -- device-wide building blocks of the DCX engine, written for gfx950 (wave64):
+//
+//  * LSD radix sort of (u64 key, u32 value) pairs, one kernel launch per 8-bit digit.  Each pass is
+//    a "onesweep" pass: a workgroup ranks a 4096-key tile with wavefront ballots (no per-key
+//    atomics), publishes its per-digit counts with a decoupled lookback over the preceding tiles,
+//    reorders the tile in LDS by digit and writes each digit run contiguously.  One upfront kernel
+//    builds the histograms of every pass from a single read of the keys.
+//  * single-pass device scan with decoupled lookback over a functor (flags -> positions, names ...).
+//
+// These replace the reference's distributed sorts and exclusive prefix sums
+// (lightweight_dcx/src/sorters/sample_sort.hpp:37-171, lightweight_dcx/src/mpi/reduce.hpp ex_prefix_sum).
+#pragma once
+
+#include "common.hpp"
+
+namespace dcx {
+
+constexpr int RS_THREADS = 256;
+constexpr int RS_ITEMS = 16;
+constexpr int RS_TILE = RS_THREADS * RS_ITEMS;  // 4096 keys per workgroup
+constexpr int RS_WAVES = RS_THREADS / 64;
+constexpr int RS_RADIX = 256;
+
+// lookback word: [63:42] tag  [41:40] flag  [39:0] count
+constexpr uint64_t LB_VALUE_MASK = (1ull << 40) - 1;
+constexpr uint64_t LB_AGG = 1ull << 40;
+constexpr uint64_t LB_INC = 2ull << 40;
+constexpr uint64_t LB_FLAG_MASK = 3ull << 40;
+constexpr uint32_t LB_TAG_LIMIT = (1u << 22) - 2;
+
+__device__ __forceinline__ uint64_t lb_pack(uint32_t tag, uint64_t flag, uint64_t v) {
+    return ((uint64_t)tag << 42) | flag | v;
+}
+
+__global__ void k_fill_u32(uint32_t* p, uint32_t v, uint64_t n) {
+    for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x)
+        p[i] = v;
+}
+__global__ void k_fill_u64(uint64_t* p, uint64_t v, uint64_t n) {
+    for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x)
+        p[i] = v;
+}
+__global__ void k_fill_u8(uint8_t* p, uint8_t v, uint64_t n) {
+    for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x)
+        p[i] = v;
+}
+
+// Histograms of all digit passes [lo_bit, lo_bit + 8*npasses) from one read of the keys.
+__global__ __launch_bounds__(256) void k_radix_hist(const uint64_t* __restrict__ keys, uint32_t N, uint32_t lo_bit,
+                                                    uint32_t npasses, uint32_t* __restrict__ hist) {
+    __shared__ uint32_t s_h[8][RS_RADIX];
+    for (int i = threadIdx.x; i < 8 * RS_RADIX; i += 256) (&s_h[0][0])[i] = 0;
+    __syncthreads();
+    const uint64_t stride = (uint64_t)gridDim.x * 256;
+    for (uint64_t i = blockIdx.x * 256ull + threadIdx.x; i < N; i += stride) {
+        uint64_t k = keys[i];
+        for (uint32_t p = 0; p < npasses; p++) atomicAdd(&s_h[p][(k >> (lo_bit + 8 * p)) & 0xFF], 1u);
+    }
+    __syncthreads();
+    for (uint32_t p = 0; p < npasses; p++) {
+        uint32_t c = s_h[p][threadIdx.x];
+        if (c) atomicAdd(&hist[p * RS_RADIX + threadIdx.x], c);
+    }
+}
+
+// In-place exclusive scan of each pass's 256 counts -> global digit offsets.
+__global__ __launch_bounds__(256) void k_radix_hist_scan(uint32_t* hist) {
+    __shared__ uint32_t s_tmp[4];
+    uint32_t* h = hist + blockIdx.x * RS_RADIX;
+    uint32_t v = h[threadIdx.x];
+    uint32_t e = block_excl_scan<256>(v, s_tmp, nullptr);
+    h[threadIdx.x] = e;
+}
+
+// One LSD pass over digit (key >> shift) & 255.  Stable.
+template <bool VALS>
+__global__ __launch_bounds__(RS_THREADS) void k_onesweep(const uint64_t* __restrict__ kin, uint64_t* __restrict__ kout,
+                                                         const uint32_t* __restrict__ vin, uint32_t* __restrict__ vout,
+                                                         uint32_t N, uint32_t shift, const uint32_t* __restrict__ dbase,
+                                                         uint64_t* __restrict__ status, uint32_t tag,
+                                                         unsigned long long* __restrict__ ticket,
+                                                         unsigned long long ticket_base) {
+    __shared__ uint64_t s_keys[RS_TILE];
+    __shared__ uint8_t s_dig[RS_TILE];
+    __shared__ uint32_t s_wcnt[RS_WAVES][RS_RADIX];
+    __shared__ uint32_t s_start[RS_RADIX];
+    __shared__ uint32_t s_gofs[RS_RADIX];
+    __shared__ uint32_t s_scan[RS_WAVES];
+    __shared__ uint32_t s_tile;
+
+    const uint32_t tid = threadIdx.x;
+    const uint32_t lane = __lane_id();
+    const uint32_t wave = tid / 64;
+    if (tid == 0) s_tile = (uint32_t)(atomicAdd(ticket, 1ull) - ticket_base);
+    for (int i = tid; i < RS_WAVES * RS_RADIX; i += RS_THREADS) (&s_wcnt[0][0])[i] = 0;
+    __syncthreads();
+    const uint32_t tile = s_tile;
+    const uint64_t tile_base = (uint64_t)tile * RS_TILE;
+    const uint64_t wbase = tile_base + (uint64_t)wave * (64 * RS_ITEMS);
+
+    uint64_t key[RS_ITEMS];
+    uint32_t val[RS_ITEMS];
+    uint32_t rk[RS_ITEMS];
+#pragma unroll
+    for (int k = 0; k < RS_ITEMS; k++) {
+        uint64_t idx = wbase + k * 64 + lane;
+        bool valid = idx < N;
+        key[k] = valid ? kin[idx] : 0ull;
+        if (VALS) val[k] = valid ? vin[idx] : 0u;
+    }
+    // rank every key among the keys of its wave with the same digit, in tile order
+#pragma unroll
+    for (int k = 0; k < RS_ITEMS; k++) {
+        uint64_t idx = wbase + k * 64 + lane;
+        bool valid = idx < N;
+        uint32_t d = (uint32_t)(key[k] >> shift) & 0xFFu;
+        uint64_t peers = __ballot(valid);
+#pragma unroll
+        for (int b = 0; b < 8; b++) {
+            uint32_t bit = (d >> b) & 1u;
+            uint64_t bb = __ballot(bit);
+            peers &= bit ? bb : ~bb;
+        }
+        uint32_t below = (uint32_t)__popcll(peers & lanemask_lt());
+        uint32_t cnt = 0;
+        if (valid) cnt = s_wcnt[wave][d];
+        __builtin_amdgcn_wave_barrier();
+        if (valid && below == 0) s_wcnt[wave][d] = cnt + (uint32_t)__popcll(peers);
+        __builtin_amdgcn_wave_barrier();
+        rk[k] = d | ((cnt + below) << 8);
+    }
+    __syncthreads();
+    // per digit: offsets of the waves and the tile total
+    const uint32_t d = tid;
+    uint32_t total = 0;
+#pragma unroll
+    for (int w = 0; w < RS_WAVES; w++) {
+        uint32_t c = s_wcnt[w][d];
+        s_wcnt[w][d] = total;
+        total += c;
+    }
+    uint32_t start = block_excl_scan<RS_THREADS>(total, s_scan, nullptr);
+    s_start[d] = start;
+    // decoupled lookback: exclusive count of this digit over all preceding tiles
+    uint64_t* st = status + (uint64_t)tile * RS_RADIX + d;
+    uint64_t excl = 0;
+    if (tile == 0) {
+        st_agent(st, lb_pack(tag, LB_INC, total));
+    } else {
+        st_agent(st, lb_pack(tag, LB_AGG, total));
+        int64_t j = (int64_t)tile - 1;
+        while (true) {
+            uint64_t w = ld_agent(status + (uint64_t)j * RS_RADIX + d);
+            if ((uint32_t)(w >> 42) != tag || (w & LB_FLAG_MASK) == 0) {
+                __builtin_amdgcn_s_sleep(1);
+                continue;
+            }
+            excl += w & LB_VALUE_MASK;
+            if (w & LB_INC) break;
+            --j;
+        }
+        st_agent(st, lb_pack(tag, LB_INC, excl + total));
+    }
+    s_gofs[d] = dbase[d] + (uint32_t)excl - start;
+    __syncthreads();
+    // reorder the tile by digit in LDS, then write each digit run contiguously
+    uint32_t lpos[RS_ITEMS];
+#pragma unroll
+    for (int k = 0; k < RS_ITEMS; k++) {
+        uint64_t idx = wbase + k * 64 + lane;
+        uint32_t dk = rk[k] & 0xFFu;
+        lpos[k] = s_start[dk] + s_wcnt[wave][dk] + (rk[k] >> 8);
+        if (idx < N) {
+            s_keys[lpos[k]] = key[k];
+            s_dig[lpos[k]] = (uint8_t)dk;
+        }
+    }
+    __syncthreads();
+    const uint32_t tile_n = (uint32_t)((N - tile_base) < (uint64_t)RS_TILE ? (N - tile_base) : RS_TILE);
+    for (uint32_t i = tid; i < tile_n; i += RS_THREADS) kout[s_gofs[s_dig[i]] + i] = s_keys[i];
+    if (VALS) {
+        __syncthreads();
+        uint32_t* s_vals = reinterpret_cast<uint32_t*>(s_keys);
+#pragma unroll
+        for (int k = 0; k < RS_ITEMS; k++) {
+            uint64_t idx = wbase + k * 64 + lane;
+            if (idx < N) s_vals[lpos[k]] = val[k];
+        }
+        __syncthreads();
+        for (uint32_t i = tid; i < tile_n; i += RS_THREADS) vout[s_gofs[s_dig[i]] + i] = s_vals[i];
+    }
+}
+
+// ------------------------------------------------------------------------------------------------
+// single-pass scan (decoupled lookback) over a functor:
+//   op.value(i) -> uint64_t     op.write(i, exclusive_prefix, value)
+// ------------------------------------------------------------------------------------------------
+constexpr int SC_THREADS = 256;
+constexpr int SC_ITEMS = 16;
+constexpr int SC_TILE = SC_THREADS * SC_ITEMS;
+
+template <class Op>
+__global__ __launch_bounds__(SC_THREADS) void k_scan(Op op, uint64_t N, uint64_t* __restrict__ st_agg,
+                                                     uint64_t* __restrict__ st_inc, uint32_t* __restrict__ st_flag,
+                                                     uint32_t tag, unsigned long long* __restrict__ ticket,
+                                                     unsigned long long ticket_base, uint64_t* __restrict__ total_out) {
+    __shared__ uint64_t s_tmp[SC_THREADS / 64];
+    __shared__ uint64_t s_excl;
+    __shared__ uint32_t s_tile;
+    const uint32_t tid = threadIdx.x;
+    if (tid == 0) s_tile = (uint32_t)(atomicAdd(ticket, 1ull) - ticket_base);
+    __syncthreads();
+    const uint32_t tile = s_tile;
+    const uint64_t base = (uint64_t)tile * SC_TILE + (uint64_t)tid * SC_ITEMS;
+    uint64_t v[SC_ITEMS];
+    uint64_t sum = 0;
+#pragma unroll
+    for (int k = 0; k < SC_ITEMS; k++) {
+        uint64_t i = base + k;
+        v[k] = i < N ? op.value(i) : 0ull;
+        sum += v[k];
+    }
+    uint64_t tile_total = 0;
+    uint64_t texcl = block_excl_scan64<SC_THREADS>(sum, s_tmp, &tile_total);
+    if (tid == 0) {
+        uint64_t excl = 0;
+        if (tile == 0) {
+            __hip_atomic_store(&st_inc[tile], tile_total, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            __hip_atomic_store(&st_flag[tile], (tag << 2) | 2u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+        } else {
+            __hip_atomic_store(&st_agg[tile], tile_total, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            __hip_atomic_store(&st_flag[tile], (tag << 2) | 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+            int64_t j = (int64_t)tile - 1;
+            while (true) {
+                uint32_t f = __hip_atomic_load(&st_flag[j], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
+                if ((f >> 2) != tag || (f & 3u) == 0) {
+                    __builtin_amdgcn_s_sleep(1);
+                    continue;
+                }
+                if (f & 2u) {
+                    excl += __hip_atomic_load(&st_inc[j], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                    break;
+                }
+                excl += __hip_atomic_load(&st_agg[j], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                --j;
+            }
+            __hip_atomic_store(&st_inc[tile], excl + tile_total, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            __hip_atomic_store(&st_flag[tile], (tag << 2) | 2u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+        }
+        s_excl = excl;
+        if (tile == gridDim.x - 1 && total_out) *total_out = excl + tile_total;
+    }
+    __syncthreads();
+    uint64_t run = s_excl + texcl;
+#pragma unroll
+    for (int k = 0; k < SC_ITEMS; k++) {
+        uint64_t i = base + k;
+        if (i < N) op.write(i, run, v[k]);
+        run += v[k];
+    }
+}
+
+// ------------------------------------------------------------------------------------------------
+// Host side: scratch for lookback state and the launch wrappers
+// ------------------------------------------------------------------------------------------------
+struct Primitives {
+    hipStream_t stream = nullptr;
+    DeviceCache* cache = nullptr;
+    // persistent scratch
+    uint64_t* sort_status = nullptr;
+    size_t sort_status_words = 0;
+    uint32_t sort_tag = 0;
+    uint64_t* scan_agg = nullptr;
+    uint64_t* scan_inc = nullptr;
+    uint32_t* scan_flag = nullptr;
+    size_t scan_tiles_cap = 0;
+    uint32_t scan_tag = 0;
+    unsigned long long* ticket = nullptr;  // device counter of tile tickets
+    unsigned long long ticket_host = 0;
+    uint64_t* scalars = nullptr;           // device scalars (scan totals)
+    uint32_t* hist = nullptr;              // 8 x 256
+    uint64_t* host_scalars = nullptr;      // pinned mirror
+    // optional per-pass timing of the radix passes
+    bool profile = false;
+    std::vector<std::pair<hipEvent_t, hipEvent_t>> pass_events;
+    std::vector<uint32_t> pass_items;
+
+    // sums the recorded pass times (synchronizes) and releases the events
+    void collect_profile(uint64_t* passes, uint64_t* items, double* ms) {
+        *passes = pass_events.size();
+        *items = 0;
+        *ms = 0;
+        for (size_t i = 0; i < pass_events.size(); i++) {
+            float a = 0;
+            DCX_HIP(hipEventSynchronize(pass_events[i].second));
+            DCX_HIP(hipEventElapsedTime(&a, pass_events[i].first, pass_events[i].second));
+            *ms += a;
+            *items += pass_items[i];
+        }
+        drop_profile();
+    }
+    void drop_profile() {
+        for (auto& e : pass_events) {
+            (void)hipEventDestroy(e.first);
+            (void)hipEventDestroy(e.second);
+        }
+        pass_events.clear();
+        pass_items.clear();
+    }
+
+    void init(hipStream_t s, DeviceCache* c) {
+        stream = s;
+        cache = c;
+        DCX_HIP(hipMalloc(&ticket, sizeof(unsigned long long)));
+        DCX_HIP(hipMemsetAsync(ticket, 0, sizeof(unsigned long long), stream));
+        DCX_HIP(hipMalloc(&scalars, 64 * sizeof(uint64_t)));
+        DCX_HIP(hipMemsetAsync(scalars, 0, 64 * sizeof(uint64_t), stream));
+        DCX_HIP(hipMalloc(&hist, 8 * RS_RADIX * sizeof(uint32_t)));
+        DCX_HIP(hipHostMalloc(&host_scalars, 64 * sizeof(uint64_t), hipHostMallocDefault));
+        ticket_host = 0;
+    }
+    void destroy() {
+        if (sort_status) (void)hipFree(sort_status);
+        if (scan_agg) (void)hipFree(scan_agg);
+        if (scan_inc) (void)hipFree(scan_inc);
+        if (scan_flag) (void)hipFree(scan_flag);
+        if (ticket) (void)hipFree(ticket);
+        if (scalars) (void)hipFree(scalars);
+        if (hist) (void)hipFree(hist);
+        if (host_scalars) (void)hipHostFree(host_scalars);
+        sort_status = nullptr;
+        scan_agg = scan_inc = nullptr;
+        scan_flag = nullptr;
+        ticket = nullptr;
+        scalars = nullptr;
+        hist = nullptr;
+        host_scalars = nullptr;
+    }
+
+    static unsigned grid_for(uint64_t n, unsigned threads, unsigned max_blocks = 8192) {
+        uint64_t b = (n + threads - 1) / threads;
+        if (b == 0) b = 1;
+        if (b > max_blocks) b = max_blocks;
+        return (unsigned)b;
+    }
+
+    void fill_u32(uint32_t* p, uint32_t v, uint64_t n) {
+        if (!n) return;
+        hipLaunchKernelGGL(k_fill_u32, dim3(grid_for(n, 256)), dim3(256), 0, stream, p, v, n);
+    }
+    void fill_u64(uint64_t* p, uint64_t v, uint64_t n) {
+        if (!n) return;
+        hipLaunchKernelGGL(k_fill_u64, dim3(grid_for(n, 256)), dim3(256), 0, stream, p, v, n);
+    }
+    void fill_u8(uint8_t* p, uint8_t v, uint64_t n) {
+        if (!n) return;
+        hipLaunchKernelGGL(k_fill_u8, dim3(grid_for(n, 256)), dim3(256), 0, stream, p, v, n);
+    }
+
+    void ensure_sort_status(uint64_t tiles) {
+        size_t words = tiles * RS_RADIX;
+        if (words <= sort_status_words) return;
+        if (sort_status) DCX_HIP(hipFree(sort_status));
+        sort_status = nullptr;
+        size_t cap = words + words / 4 + RS_RADIX;
+        DCX_HIP(hipMalloc(&sort_status, cap * sizeof(uint64_t)));
+        sort_status_words = cap;
+        fill_u64(sort_status, 0, cap);
+        sort_tag = 0;
+    }
+    void ensure_scan_status(uint64_t tiles) {
+        if (tiles <= scan_tiles_cap) return;
+        if (scan_agg) DCX_HIP(hipFree(scan_agg));
+        if (scan_inc) DCX_HIP(hipFree(scan_inc));
+        if (scan_flag) DCX_HIP(hipFree(scan_flag));
+        size_t cap = tiles + tiles / 4 + 16;
+        DCX_HIP(hipMalloc(&scan_agg, cap * sizeof(uint64_t)));
+        DCX_HIP(hipMalloc(&scan_inc, cap * sizeof(uint64_t)));
+        DCX_HIP(hipMalloc(&scan_flag, cap * sizeof(uint32_t)));
+        fill_u32(scan_flag, 0, cap);
+        scan_tiles_cap = cap;
+        scan_tag = 0;
+    }
+
+    // Sorts (keys, vals) by key bits [lo_bit, hi_bit).  Key bits inside the pass range but outside
+    // [lo_bit, hi_bit) must be equal for all keys.  On return keys/vals point at the sorted data
+    // (the buffers may have been swapped with the alternates).
+    void sort_pairs(uint64_t*& keys, uint64_t*& keys_alt, uint32_t*& vals, uint32_t*& vals_alt, uint32_t N,
+                    uint32_t lo_bit, uint32_t hi_bit) {
+        if (N <= 1 || hi_bit <= lo_bit) return;
+        uint32_t np = (hi_bit - lo_bit + 7) / 8;
+        if (np > 8) np = 8;
+        if (lo_bit + 8 * np > 64) lo_bit = 64 - 8 * np;
+        uint64_t tiles = ceil_div(N, RS_TILE);
+        ensure_sort_status(tiles);
+        fill_u32(hist, 0, 8 * RS_RADIX);
+        hipLaunchKernelGGL(k_radix_hist, dim3(grid_for(N, 256, 2048)), dim3(256), 0, stream, keys, N, lo_bit, np,
+                           hist);
+        hipLaunchKernelGGL(k_radix_hist_scan, dim3(np), dim3(256), 0, stream, hist);
+        for (uint32_t p = 0; p < np; p++) {
+            if (sort_tag >= LB_TAG_LIMIT) {
+                fill_u64(sort_status, 0, sort_status_words);
+                sort_tag = 0;
+            }
+            uint32_t tag = ++sort_tag;
+            hipEvent_t e0 = nullptr, e1 = nullptr;
+            if (profile) {
+                DCX_HIP(hipEventCreate(&e0));
+                DCX_HIP(hipEventCreate(&e1));
+                DCX_HIP(hipEventRecord(e0, stream));
+            }
+            if (vals)
+                hipLaunchKernelGGL(k_onesweep<true>, dim3((unsigned)tiles), dim3(RS_THREADS), 0, stream, keys,
+                                   keys_alt, vals, vals_alt, N, lo_bit + 8 * p, hist + p * RS_RADIX, sort_status, tag,
+                                   ticket, ticket_host);
+            else
+                hipLaunchKernelGGL(k_onesweep<false>, dim3((unsigned)tiles), dim3(RS_THREADS), 0, stream, keys,
+                                   keys_alt, (const uint32_t*)nullptr, (uint32_t*)nullptr, N, lo_bit + 8 * p,
+                                   hist + p * RS_RADIX, sort_status, tag, ticket, ticket_host);
+            if (profile) {
+                DCX_HIP(hipEventRecord(e1, stream));
+                pass_events.emplace_back(e0, e1);
+                pass_items.push_back(N);
+            }
+            ticket_host += tiles;
+            std::swap(keys, keys_alt);
+            if (vals) std::swap(vals, vals_alt);
+        }
+    }
+
+    // Launches a scan; the grand total lands in device scalar slot `slot` (read with read_scalars).
+    template <class Op>
+    void scan(const Op& op, uint64_t N, int slot) {
+        if (N == 0) {
+            fill_u64(scalars + slot, 0, 1);
+            return;
+        }
+        uint64_t tiles = ceil_div(N, SC_TILE);
+        ensure_scan_status(tiles);
+        if (scan_tag >= (1u << 29)) {
+            fill_u32(scan_flag, 0, scan_tiles_cap);
+            scan_tag = 0;
+        }
+        uint32_t tag = ++scan_tag;
+        hipLaunchKernelGGL(k_scan<Op>, dim3((unsigned)tiles), dim3(SC_THREADS), 0, stream, op, N, scan_agg, scan_inc,
+                           scan_flag, tag, ticket, ticket_host, scalars + slot);
+        ticket_host += tiles;
+    }
+
+    // Copies `count` device scalars to the host (synchronizes the stream).
+    const uint64_t* read_scalars(int count) {
+        DCX_HIP(hipMemcpyAsync(host_scalars, scalars, count * sizeof(uint64_t), hipMemcpyDeviceToHost, stream));
+        DCX_HIP(hipStreamSynchronize(stream));
+        return host_scalars;
+    }
+};
+
+}  // namespace dcx

@@ -1,0 +1,66 @@
+"""Seeded synthetic workloads (binding of csrc/textgen.c).
+
+Kinds: ``uniform_acgt`` (0), ``genome`` (1: order-3 Markov + copied segments), ``zipf256`` (2: Zipf(1.1)
+over 256 bytes + a period-17 block), ``acgtn`` (3), ``uniform_sigma`` (4).
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB = None
+
+KINDS = {"uniform_acgt": 0, "genome": 1, "zipf256": 2, "acgtn": 3, "uniform_sigma": 4}
+
+# The benchmark workloads (sizes and seeds) -- plain DCX on i.i.d. DNA, the genome-like text and the
+# byte-alphabet text.
+WORKLOADS = {
+    "dna_8mib_dc39": {"kind": "uniform_acgt", "n": 8 << 20, "seed": 1, "X": 39, "sa_width": 8},
+    "genome_1gib_dc39": {"kind": "genome", "n": 1 << 30, "seed": 2, "X": 39, "sa_width": 5},
+}
+
+# workloads whose libsais digests are committed in tests/golden/sa_digests.json
+DIGEST_WORKLOADS = {
+    "dna_8mib": {"kind": "uniform_acgt", "n": 8 << 20, "seed": 1},
+    "genome_64mib": {"kind": "genome", "n": 64 << 20, "seed": 2},
+    "genome_1gib": {"kind": "genome", "n": 1 << 30, "seed": 2},
+    "zipf_16mib": {"kind": "zipf256", "n": 16 << 20, "seed": 3},
+}
+
+
+def lib():
+    global _LIB
+    if _LIB is None:
+        path = os.path.join(HERE, "libdcx_textgen.so")
+        if not os.path.exists(path):
+            raise ImportError(f"{path} missing: run python -m distributed_suffix_sorting_evaluation_amd.build")
+        L = ctypes.CDLL(path)
+        L.dcx_gen_text.argtypes = [ctypes.c_int, ctypes.c_uint64, ctypes.c_uint64, ctypes.c_void_p]
+        L.dcx_gen_text.restype = ctypes.c_int
+        L.dcx_gen_lengths.argtypes = [ctypes.c_uint32, ctypes.c_uint64, ctypes.c_uint64, ctypes.c_uint64,
+                                      ctypes.c_void_p]
+        L.dcx_gen_lengths.restype = ctypes.c_int
+        _LIB = L
+    return _LIB
+
+
+def generate(kind: str, n: int, seed: int, sigma: int = 0, out: np.ndarray | None = None) -> np.ndarray:
+    k = KINDS[kind]
+    if k == 4:
+        seed = (seed & 0xFFFFFFFF) | ((sigma & 0xFF) << 32)
+    buf = out if out is not None else np.empty(max(n, 1), dtype=np.uint8)
+    rc = lib().dcx_gen_text(k, n, seed, buf.ctypes.data)
+    if rc != 0:
+        raise ValueError(f"dcx_gen_text rc={rc}")
+    return buf[:n]
+
+
+def lengths(count: int, lo: int, hi: int, seed: int) -> np.ndarray:
+    out = np.empty(max(count, 1), dtype=np.uint64)
+    rc = lib().dcx_gen_lengths(count, lo, hi, seed, out.ctypes.data)
+    if rc != 0:
+        raise ValueError(f"dcx_gen_lengths rc={rc}")
+    return out[:count]

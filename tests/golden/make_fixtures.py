@@ -1,0 +1,93 @@
+"""Writes the golden fixtures under tests/golden/ from data the reference holds.
+
+Run in the development container (it reads /root/reference; nothing at test time does):
+
+    python tests/golden/make_fixtures.py [--digests]
+
+Fixtures written (all data -- inputs and expected outputs):
+
+* ``abracadabra4``, ``abracadabra4.sa`` and ``abracadabra8``, ``abracadabra8.sa``: texts and 32-bit LE SAs
+  from B_DC_3_7_13/src/examples/.
+* ``mississippi``, ``aaa``: texts from B_DC_3_7_13/src/examples/ (expected SAs are closed forms / the psac
+  KAT psac/test/test_psac.cpp:105).
+* ``the_three_brothers.txt``, ``the_three_brothers_size_t_sa``: text and 64-bit LE SA from
+  dPD/tests/test_data/.
+* ``dc_tables.json``: per X, the DC set, IN_DC, NEXT_RANK and the SHA-256 of cmpDepthRanks (u32 LE,
+  row-major X*X*3) parsed from lightweight_dcx/src/pdcx/difference_cover.hpp.
+* with ``--digests``: ``sa_digests.json``: SHA-256 of the uint64 LE SA computed by libsais (oracle/_ref,
+  built from the reference's libsais sources) for the synthetic workloads made by the project generator.
+"""
+from __future__ import annotations
+
+import hashlib
+import json
+import os
+import re
+import shutil
+import sys
+
+import numpy as np
+
+REF = "/root/reference"
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(os.path.dirname(HERE))
+
+
+def copy_fixtures():
+    ex = os.path.join(REF, "B_DC_3_7_13", "src", "examples")
+    for f in ["abracadabra4", "abracadabra4.sa", "abracadabra8", "abracadabra8.sa", "mississippi", "aaa"]:
+        shutil.copyfile(os.path.join(ex, f), os.path.join(HERE, f))
+    td = os.path.join(REF, "dPD", "tests", "test_data")
+    for f in ["the_three_brothers.txt", "the_three_brothers_size_t_sa"]:
+        shutil.copyfile(os.path.join(td, f), os.path.join(HERE, f))
+
+
+def parse_dc_tables():
+    src = open(os.path.join(REF, "lightweight_dcx", "src", "pdcx", "difference_cover.hpp")).read()
+    out = {}
+    for m in re.finditer(r"struct DC(\d+)Param \{(.*?)\n\};", src, re.S):
+        body = m.group(2)
+
+        def arr(name):
+            mm = re.search(name + r"[^=]*=\s*(\{.*?\});", body, re.S)
+            return [int(x) for x in re.findall(r"\d+", mm.group(1))]
+
+        X = int(re.search(r"X = (\d+);", body).group(1))
+        D = int(re.search(r"D = (\d+);", body).group(1))
+        dc = arr(r"unsigned int DC\[D\]")
+        in_dc = arr(r"bool IN_DC\[X\]")
+        next_rank = arr(r"unsigned int NEXT_RANK\[X\]")
+        cmp = arr(r"unsigned int cmpDepthRanks\[X\]\[X\]\[3\]")
+        assert len(dc) == D and len(in_dc) == X and len(next_rank) == X and len(cmp) == 3 * X * X, X
+        digest = hashlib.sha256(np.asarray(cmp, dtype="<u4").tobytes()).hexdigest()
+        out[str(X)] = {"X": X, "D": D, "DC": dc, "IN_DC": in_dc, "NEXT_RANK": next_rank,
+                       "cmpDepthRanks_sha256": digest,
+                       "cmp_row0": cmp[: 3 * X]}
+    with open(os.path.join(HERE, "dc_tables.json"), "w") as f:
+        json.dump(out, f, indent=1, sort_keys=True)
+    return out
+
+
+def sa_digests():
+    sys.path.insert(0, ROOT)
+    import oracle  # noqa: E402
+    from distributed_suffix_sorting_evaluation_amd import textgen  # noqa: E402
+
+    res = {}
+    for name, kw in textgen.DIGEST_WORKLOADS.items():
+        text = textgen.generate(**kw)
+        sa = oracle.libsais_sa(text, threads=os.cpu_count() or 1)
+        res[name] = {"params": kw, "n": int(text.size),
+                     "text_sha256": hashlib.sha256(text.tobytes()).hexdigest(),
+                     "sa_u64_sha256": hashlib.sha256(sa.astype("<u8").tobytes()).hexdigest()}
+        print(name, res[name], flush=True)
+        del sa, text
+    with open(os.path.join(HERE, "sa_digests.json"), "w") as f:
+        json.dump(res, f, indent=1, sort_keys=True)
+
+
+if __name__ == "__main__":
+    copy_fixtures()
+    parse_dc_tables()
+    if "--digests" in sys.argv:
+        sa_digests()

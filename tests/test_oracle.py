@@ -1,0 +1,183 @@
+"""Pins the CPU oracle (oracle/dcx_oracle.c) before anything is checked against it.
+
+* golden SAs held by the reference (B_DC_3_7_13/src/examples, dPD/tests/test_data) and the psac KAT
+* libsais built from the reference's sources (oracle/_ref)
+* brute-force suffix sorting on tiny texts, every n mod X alignment (lightweight_dcx main.cpp:100-123)
+* difference-cover tables against digests of difference_cover.hpp
+* closed forms: packing layout, rank positions, uniform runs
+"""
+import hashlib
+import json
+import os
+
+import numpy as np
+import pytest
+
+import oracle
+from helpers import ALL_X, GOLDEN, brute_sa, copies_text, periodic_text, random_text, read
+
+
+def test_golden_abracadabra():
+    for name in ["abracadabra4", "abracadabra8"]:
+        text = read(name)
+        expect = np.frombuffer(read(name + ".sa"), dtype="<u4").astype(np.uint64)
+        for X in (3, 7, 39):
+            assert np.array_equal(oracle.dcx_sa(text, X=X, base_case_chars=0), expect), (name, X)
+
+
+def test_golden_three_brothers():
+    text = read("the_three_brothers.txt")
+    expect = np.frombuffer(read("the_three_brothers_size_t_sa"), dtype="<u8")
+    for X in (3, 13, 39, 133):
+        for packing in (True, False):
+            sa = oracle.dcx_sa(text, X=X, pack_samples=packing, pack_merging=packing, base_case_chars=0)
+            assert np.array_equal(sa, expect), (X, packing)
+
+
+def test_psac_mississippi_kat():
+    # psac/test/test_psac.cpp:105
+    expect = np.array([10, 7, 4, 1, 0, 9, 8, 6, 3, 5, 2], dtype=np.uint64)
+    assert np.array_equal(oracle.dcx_sa(b"mississippi", X=3, base_case_chars=0), expect)
+    assert np.array_equal(oracle.dcx_sa(b"mississippi", X=7, base_case_chars=0), expect)
+
+
+def test_aaa_closed_form():
+    text = read("aaa")
+    assert text == b"a" * len(text)
+    expect = np.arange(len(text) - 1, -1, -1, dtype=np.uint64)
+    for X in ALL_X:
+        assert np.array_equal(oracle.dcx_sa(text, X=X, base_case_chars=0), expect)
+
+
+@pytest.mark.parametrize("X", ALL_X)
+def test_brute_force_all_alignments(X):
+    # every n mod X for small n, several alphabets (main.cpp:85-123)
+    for n in range(2 * X + 1, 2 * X + 1 + X + 2):
+        for sigma, seed in ((2, n), (4, n + 1), (32, n + 2)):
+            t = random_text(n, sigma, seed)
+            assert np.array_equal(oracle.dcx_sa(t, X=X, base_case_chars=0), brute_sa(t)), (X, n, sigma)
+
+
+@pytest.mark.skipif(not oracle.sais_available(), reason="oracle/_ref/libsais.so not built")
+@pytest.mark.parametrize("X", ALL_X)
+def test_against_libsais(X):
+    for n, sigma in ((3000, 2), (20000, 4), (20000, 255), (15000, 256)):
+        t = random_text(n, sigma, X * 7 + n)
+        for ps, pm in ((True, True), (False, False), (True, False)):
+            sa = oracle.dcx_sa(t, X=X, pack_samples=ps, pack_merging=pm, base_case_chars=0)
+            assert np.array_equal(sa, oracle.libsais_sa(t)), (X, n, sigma, ps, pm)
+
+
+@pytest.mark.skipif(not oracle.sais_available(), reason="oracle/_ref/libsais.so not built")
+def test_repetitive_against_libsais():
+    cases = [periodic_text(20000, 17, 256, 1), periodic_text(12000, 1, 4, 2), periodic_text(30000, 3, 4, 3),
+             copies_text(40000, 4), copies_text(30000, 5, sigma=256)]
+    for t in cases:
+        for X in (3, 21, 39, 133):
+            sa, st = oracle.dcx_sa(t, X=X, base_case_chars=0, with_stats=True)
+            assert np.array_equal(sa, oracle.libsais_sa(t)), X
+            assert len(st) >= 2  # these inputs recurse
+
+
+def test_dc_tables_match_reference_digest():
+    ref = json.load(open(os.path.join(GOLDEN, "dc_tables.json")))
+    assert sorted(int(k) for k in ref) == list(ALL_X)
+    for X in ALL_X:
+        p = oracle.dc_params(X)
+        r = ref[str(X)]
+        assert p["D"] == r["D"] and p["DC"] == r["DC"], X
+        assert p["NEXT_RANK"] == r["NEXT_RANK"], X
+        in_dc = [1 if i in p["DC"] else 0 for i in range(X)]
+        assert in_dc == r["IN_DC"]
+        digest = hashlib.sha256(np.ascontiguousarray(p["cmp"], dtype="<u4").tobytes()).hexdigest()
+        assert digest == r["cmpDepthRanks_sha256"], X
+
+
+def test_dc_tables_are_difference_covers():
+    for X in ALL_X:
+        p = oracle.dc_params(X)
+        dc = p["DC"]
+        diffs = {(b - a) % X for a in dc for b in dc}
+        assert diffs == set(range(X)), X
+        cmp = p["cmp"]
+        for a in range(X):
+            for b in range(X):
+                l, r1, r2 = cmp[a, b]
+                assert (a + l) % X in dc and (b + l) % X in dc
+                assert all(not ((a + j) % X in dc and (b + j) % X in dc) for j in range(l))
+                assert r1 == sum((a + j) % X in dc for j in range(l))
+                assert r2 == sum((b + j) % X in dc for j in range(l))
+
+
+def test_packing_closed_form():
+    # char k of a word at bits [64-(k+1)b, 64-kb) -> word order == lexicographic order
+    rng = np.random.default_rng(5)
+    for bits in (3, 5, 8, 9):
+        cpw = 64 // bits
+        vals = rng.integers(0, 2 ** bits, (200, cpw)).astype(np.uint64)
+        words = [oracle.pack_word(v, bits) for v in vals]
+        for v, w in zip(vals, words):
+            for k in range(cpw):
+                assert (w >> (64 - (k + 1) * bits)) & ((1 << bits) - 1) == int(v[k])
+            assert w & ((1 << (64 - cpw * bits)) - 1) == 0 if 64 - cpw * bits > 0 else True
+        order_w = sorted(range(200), key=lambda i: words[i])
+        order_l = sorted(range(200), key=lambda i: tuple(vals[i]))
+        assert [tuple(vals[i]) for i in order_w] == [tuple(vals[i]) for i in order_l]
+
+
+def test_packing_ratio_truncation():
+    # cli.cpp:488-519: W = ceil(X / chars_per_word); samples use (uint64)(X * ratio) chars
+    p = oracle.packing(91, 4)
+    assert (p["bits"], p["chars_per_word"], p["words"]) == (3, 21, 5)
+    assert int(91 * p["ratio"]) == 104  # truncated, not 105
+    p = oracle.packing(39, 4)
+    assert (p["bits"], p["words"]) == (3, 2) and int(39 * p["ratio"]) == 42
+    assert oracle.packing(39, 20)["bits"] == 5 and oracle.packing(39, 200)["bits"] == 8
+    assert oracle.packing(39, 256)["bits"] == 9
+
+
+def test_stats_invariants():
+    t = copies_text(60000, 9)
+    sa, st = oracle.dcx_sa(t, X=39, base_case_chars=0, with_stats=True)
+    assert st[0]["n"] == t.size
+    # sample count = |{i in [0, n] : i mod X in DC}|
+    dc = oracle.dc_params(39)["DC"]
+    assert st[0]["samples"] == sum(1 for i in range(t.size + 1) if i % 39 in dc)
+    assert st[0]["key_chars"] == 42 and st[0]["merge_chars"] == 41 and st[0]["bits_per_char"] == 3
+    for a, b in zip(st, st[1:]):
+        assert a["recursed"] == 1
+        assert b["n"] == (a["reduced"] if a["discarding"] else a["samples"])
+        assert a["names"] < a["samples"]
+    assert st[-1]["base_case"] == 1 or st[-1]["names"] == st[-1]["samples"]
+
+
+def test_discarding_threshold_changes_path_not_output():
+    t = copies_text(50000, 11, copy_frac=0.5)
+    outs = []
+    for thr in (0.0, 0.7, 1.0):
+        sa, st = oracle.dcx_sa(t, X=21, base_case_chars=0, discarding_threshold=thr, with_stats=True)
+        outs.append(sa)
+        if thr == 0.0:
+            assert not any(l["discarding"] for l in st)
+        if thr == 1.0:
+            assert any(l["discarding"] for l in st)
+    assert all(np.array_equal(outs[0], o) for o in outs[1:])
+
+
+def test_checker_detects_errors():
+    t = random_text(5000, 4, 3)
+    sa = oracle.dcx_sa(t, X=7, base_case_chars=0)
+    assert oracle.check_sa(t, sa)
+    bad = sa.copy()
+    bad[[10, 11]] = bad[[11, 10]]
+    assert not oracle.check_sa(t, bad)
+    bad = sa.copy()
+    bad[5] = bad[6]
+    assert not oracle.check_sa(t, bad)
+
+
+def test_uint40_writer():
+    sa = np.array([0, 1, 2 ** 32 + 5, 2 ** 40 - 1], dtype=np.uint64)
+    b = oracle.uint40_bytes(sa)
+    assert len(b) == 20
+    assert b[10:15] == bytes([5, 0, 0, 0, 1])
