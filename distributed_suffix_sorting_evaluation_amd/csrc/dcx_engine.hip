@@ -407,6 +407,9 @@ class Engine {
         DCX_HIP(hipMemcpyAsync(dlut.get(), lut, sizeof(lut), hipMemcpyHostToDevice, stream));
 
         DBuf<uint32_t> sa(cache, n);
+        // phase boundaries default to "empty" (a base-case level never reaches them)
+        DCX_HIP(hipEventRecord(ev[2], stream));
+        DCX_HIP(hipEventRecord(ev[3], stream));
         level(TextU8{d_text, (uint32_t)n, dlut.get()}, (uint32_t)n, bit_width(sigma), 0, sa.get(), ev);
         DCX_HIP(hipEventRecord(ev[4], stream));  // SA resident in HBM
         // output in the requested width
