@@ -7,6 +7,7 @@
 //   phase 4    every suffix keyed by its packed (X-1)-character prefix -> refinement radix sort ->
 //              groups of equal prefixes ordered by the DC comparator on the sample ranks
 // All data stays in HBM; the host only reads back a handful of scalars per step.
+#include <algorithm>
 #include <chrono>
 #include <cstdio>
 #include <cstring>
@@ -97,77 +98,101 @@ class Engine {
     }
 
     // -------------------------------------------------------------------------------------------
-    // refinement sort: orders N elements by their first L characters, `bits` bits per character;
-    // on return `order` holds the element values in sorted order and H[j] = 1 where a group of
-    // equal L-character keys starts (H[N] = 1).
+    // active sets: members of groups of size >= 2 of the heads array H, as positions and group ids
     // -------------------------------------------------------------------------------------------
-    template <class TV, int MODE>
-    void refine_sort(const TV& tv, const DcDev& dc, uint32_t N, uint32_t L, uint32_t bits, DBuf<uint32_t>& order,
-                     DBuf<uint8_t>& H) {
-        DBuf<uint64_t> ka(cache, N), kb(cache, N);
-        DBuf<uint32_t> va(cache, N), vb(cache, N);
+    struct ActiveSet {
+        DBuf<uint32_t> act, gid;
+        uint32_t A = 0, G = 0;
+    };
+
+    // restricted to the previous active list `src` (its superset) or to all N positions (src == nullptr)
+    void active_set(const uint8_t* H, const uint32_t* src, uint32_t Nsrc, ActiveSet& out) {
+        out.act = DBuf<uint32_t>(cache, Nsrc);
+        out.gid = DBuf<uint32_t>(cache, Nsrc);
+        prim.scan(ActiveScanOp{H, src, out.act.get(), out.gid.get()}, Nsrc, 0);
+        const uint64_t tot = prim.read_scalars(1)[0];
+        out.A = (uint32_t)tot;
+        out.G = (uint32_t)(tot >> 32);
+    }
+
+    // -------------------------------------------------------------------------------------------
+    // refinement sort: orders N elements by their first L characters (`bits` bits each, taken from the
+    // text-order packed words W).  Round 0 sorts everything by the first word; every later round sorts
+    // only the members of still-tied groups by (group, next characters).  On return `order` holds the
+    // element values in sorted order, H[j] = 1 where a group of equal L-character keys starts (H[N] = 1)
+    // and `fin` lists the members of the groups that are still tied.
+    // -------------------------------------------------------------------------------------------
+    template <int MODE>
+    void refine_sort(const uint64_t* W, const DcDev& dc, uint32_t N, uint32_t L, uint32_t bits, DBuf<uint32_t>& order,
+                     DBuf<uint8_t>& H, ActiveSet& fin) {
         const uint32_t cpw = 64 / bits;
-        uint32_t c0 = L < cpw ? L : cpw;
-        k_keys_r0<TV, MODE><<<grid(N), 256, 0, stream>>>(tv, dc, N, c0, bits, ka.get(), va.get());
-        uint64_t* k = ka.get();
-        uint64_t* k2 = kb.get();
-        uint32_t* v = va.get();
-        uint32_t* v2 = vb.get();
-        prim.sort_pairs(k, k2, v, v2, N, 64 - c0 * bits, 64);
-        H = DBuf<uint8_t>(cache, (size_t)N + 1);
-        k_heads<<<grid(N + 1ull), 256, 0, stream>>>(k, N, H.get());
-        // keep the sorted values as `order`
-        if (v == va.get()) {
-            order = std::move(va);
-        } else {
-            order = std::move(vb);
+        const uint32_t c0 = L < cpw ? L : cpw;
+        {
+            DBuf<uint64_t> ka(cache, N), kb(cache, N);
+            DBuf<uint32_t> va(cache, N), vb(cache, N);
+            uint64_t* k = ka.get();
+            uint64_t* k2 = kb.get();
+            uint32_t* v = va.get();
+            uint32_t* v2 = vb.get();
+            if (MODE == 1) {
+                // every position is an element: the first pass reads the packed words directly
+                prim.sort_pairs(k, k2, v, v2, N, 64 - c0 * bits, 64, W, true);
+            } else {
+                k_keys_r0<MODE><<<grid(N), 256, 0, stream>>>(W, dc, N, c0 * bits, ka.get(), va.get());
+                prim.sort_pairs(k, k2, v, v2, N, 64 - c0 * bits, 64);
+            }
+            H = DBuf<uint8_t>(cache, (size_t)N + 1);
+            k_heads<<<grid(N + 1ull), 256, 0, stream>>>(k, N, c0 * bits, H.get());
+            if (v == va.get())
+                order = std::move(va);
+            else
+                order = std::move(vb);
         }
         uint32_t C = c0;
-        if (C >= L) return;
-        DBuf<uint32_t> act(cache, N), gid(cache, N);
-        while (C < L) {
-            ActiveScanOp op{H.get(), act.get(), gid.get()};
-            prim.scan(op, N, 0);
-            const uint64_t tot = prim.read_scalars(1)[0];
-            const uint32_t A = (uint32_t)tot, G = (uint32_t)(tot >> 32);
-            if (A == 0) break;
-            const uint32_t gbits = bit_width(G - 1);
+        ActiveSet cur;
+        active_set(H.get(), nullptr, N, cur);
+        while (C < L && cur.A > 0) {
+            const uint32_t A = cur.A;
+            const uint32_t gbits = bit_width(cur.G - 1);
             uint32_t c = (64 - gbits) / bits;
             if (c > L - C) c = L - C;
+            if (c > cpw) c = cpw;
             if (c == 0) throw Error(DCX_E_UNSUPPORTED, "character width too large for refinement");
-            DBuf<uint64_t> rka(cache, A), rkb(cache, A);
-            DBuf<uint32_t> rva(cache, A), rvb(cache, A);
-            k_keys_rr<TV, MODE><<<grid(A), 256, 0, stream>>>(tv, dc, A, act.get(), gid.get(), order.get(), C, c, bits,
-                                                             gbits, rka.get(), rva.get());
-            uint64_t* x = rka.get();
-            uint64_t* x2 = rkb.get();
-            uint32_t* y = rva.get();
-            uint32_t* y2 = rvb.get();
-            prim.sort_pairs(x, x2, y, y2, A, 64 - gbits - c * bits, 64);
-            k_writeback<<<grid(A), 256, 0, stream>>>(x, y, act.get(), A, order.get(), H.get(), 1);
+            {
+                DBuf<uint64_t> rka(cache, A), rkb(cache, A);
+                DBuf<uint32_t> rva(cache, A), rvb(cache, A);
+                k_keys_rr<MODE><<<grid(A), 256, 0, stream>>>(W, dc, A, cur.act.get(), cur.gid.get(), order.get(), C,
+                                                             c * bits, gbits, rka.get(), rva.get());
+                uint64_t* x = rka.get();
+                uint64_t* x2 = rkb.get();
+                uint32_t* y = rva.get();
+                uint32_t* y2 = rvb.get();
+                prim.sort_pairs(x, x2, y, y2, A, 64 - gbits - c * bits, 64);
+                k_writeback<<<grid(A), 256, 0, stream>>>(x, y, cur.act.get(), A, order.get(), H.get(), 1);
+            }
             C += c;
+            ActiveSet next;
+            active_set(H.get(), cur.act.get(), A, next);
+            cur = std::move(next);
         }
+        fin = std::move(cur);
     }
 
     // groups of equal merge prefixes -> DC comparator on the sample ranks (tie_break_ranks,
     // merge_samples.hpp:265-333)
-    void tie_break(uint32_t n, const DcDev& dc, DBuf<uint32_t>& order, DBuf<uint8_t>& H, const uint32_t* ranks) {
-        DBuf<uint32_t> act(cache, n), gid(cache, n);
-        ActiveScanOp op{H.get(), act.get(), gid.get()};
-        prim.scan(op, n, 0);
-        const uint64_t tot = prim.read_scalars(1)[0];
-        const uint32_t A = (uint32_t)tot, G = (uint32_t)(tot >> 32);
+    void tie_break(const DcDev& dc, DBuf<uint32_t>& order, DBuf<uint8_t>& H, ActiveSet& as, const uint32_t* ranks) {
+        const uint32_t A = as.A, G = as.G;
         if (A == 0) return;
         DBuf<uint32_t> gstart(cache, (size_t)G + 1);
-        k_group_starts<<<grid(A), 256, 0, stream>>>(act.get(), gid.get(), H.get(), A, gstart.get());
+        k_group_starts<<<grid(A), 256, 0, stream>>>(as.act.get(), as.gid.get(), H.get(), A, gstart.get());
         prim.fill_u32(gstart.get() + G, A, 1);
         DBuf<uint8_t> large(cache, G);
-        k_tiebreak_small<<<grid(G), 256, 0, stream>>>(gstart.get(), G, act.get(), order.get(), dc, d_cmp.get(), ranks,
-                                                      large.get());
+        k_tiebreak_small<<<grid(G), 256, 0, stream>>>(gstart.get(), G, as.act.get(), order.get(), dc, d_cmp.get(),
+                                                      ranks, large.get());
         // large groups: merge sort with the composite (group, DC comparator) order
         DBuf<uint64_t> items(cache, A), items2(cache, A);
         DBuf<uint32_t> slots(cache, A);
-        LargeScanOp lop{large.get(), gid.get(), act.get(), order.get(), items.get(), slots.get()};
+        LargeScanOp lop{large.get(), as.gid.get(), as.act.get(), order.get(), items.get(), slots.get()};
         prim.scan(lop, A, 1);
         const uint32_t B = (uint32_t)prim.read_scalars(2)[1];
         if (B == 0) return;
@@ -181,11 +206,15 @@ class Engine {
         k_tb_scatter<<<grid(B), 256, 0, stream>>>(in, slots.get(), B, order.get());
     }
 
-    template <class TV>
-    void base_case(const TV& tv, uint32_t n, uint32_t* sa_out) {
+    void base_case(const uint8_t* t, const uint16_t* lut, uint32_t n, uint32_t* sa_out) {
         if (n == 0) return;
         DBuf<uint32_t> rank(cache, (size_t)n + 1);
-        k_base_case<TV><<<1, 1024, 0, stream>>>(tv, n, sa_out, rank.get());
+        k_base_case<TextU8><<<1, 1024, 0, stream>>>(TextU8{t, n, lut}, n, sa_out, rank.get());
+    }
+    void base_case(const uint32_t* t, const uint16_t*, uint32_t n, uint32_t* sa_out) {
+        if (n == 0) return;
+        DBuf<uint32_t> rank(cache, (size_t)n + 1);
+        k_base_case<TextU32><<<1, 1024, 0, stream>>>(TextU32{t, n}, n, sa_out, rank.get());
     }
 
     dcx_level_stats* level_stat(int depth) {
@@ -200,17 +229,19 @@ class Engine {
     static uint32_t ref_pack_bits(uint32_t sig) { return sig <= 7 ? 3 : sig <= 31 ? 5 : sig <= 255 ? 8 : 9; }
 
     // -------------------------------------------------------------------------------------------
-    // one recursion level: SA of tv[0..n) into sa_out (device, n entries)
+    // one recursion level: SA of text[0..n) into sa_out (device, n entries).  Level 0 reads bytes
+    // through the alphabet map `lut`; deeper levels read names (uint32, >= 1).
     // -------------------------------------------------------------------------------------------
-    template <class TV>
-    void level(const TV& tv, uint32_t n, uint32_t bits, int depth, uint32_t* sa_out, hipEvent_t* ev) {
+    template <class CharT>
+    void level(const CharT* text, const uint16_t* lut, uint32_t n, uint32_t bits, int depth, uint32_t* sa_out,
+               hipEvent_t* ev) {
         const uint32_t X = dcs->X, D = dcs->D;
         dcx_level_stats* ls = level_stat(depth);
         if (ls) ls->n = n;
         const uint64_t threshold = opt.base_case_chars > 2ull * X ? opt.base_case_chars : 2ull * X;
         if (n <= threshold) {
             if (ls) ls->base_case = 1;
-            base_case(tv, n, sa_out);
+            base_case(text, lut, n, sa_out);
             return;
         }
         uint64_t m64 = 0;
@@ -244,13 +275,21 @@ class Engine {
             alg_bytes += (uint64_t)n * c + 2ull * m * (K1 + I) + 2ull * n * (K4 + I) + (uint64_t)n * S;
         }
 
+        // packed words of the whole level text (shared by the sample sort and the merge)
+        const uint32_t cpw = 64 / bits;
+        const uint64_t wtotal = (uint64_t)n + (Ls > Lm ? Ls : Lm) + 2;
+        DBuf<uint64_t> W(cache, wtotal);
+        k_words<CharT><<<(unsigned)std::min<uint64_t>(ceil_div(wtotal, WD_TILE), 8192), 256, 0, stream>>>(text, n, lut, bits, cpw, wtotal,
+                                                                                 W.get());
+
         // ---------------- phase 1 + 2: sample sort and names ----------------
         DBuf<uint32_t> ranks(cache, (size_t)m + D);
         prim.fill_u32(ranks.get() + m, 0, D);
         {
             DBuf<uint32_t> order;
             DBuf<uint8_t> H;
-            refine_sort<TV, 0>(tv, dc, m, Ls, bits, order, H);
+            ActiveSet tied;
+            refine_sort<0>(W.get(), dc, m, Ls, bits, order, H, tied);
             DBuf<uint32_t> names(cache, m);
             prim.scan(NamesOp{H.get(), names.get()}, m, 0);
             const uint32_t M = (uint32_t)prim.read_scalars(1)[0];
@@ -262,25 +301,24 @@ class Engine {
                 k_rank_from_order<<<grid(m), 256, 0, stream>>>(order.get(), m, ranks.get());
             } else {
                 if (ls) ls->recursed = 1;
+                if (M >= UNIQ_BIT) throw Error(DCX_E_UNSUPPORTED, "more than 2^31 distinct sample names");
                 DBuf<uint32_t> R(cache, m);
-                DBuf<uint8_t> U(cache, m);
-                k_build_rec<<<grid(m), 256, 0, stream>>>(order.get(), names.get(), H.get(), m, dc, R.get(), U.get());
+                k_build_rec<<<grid(m), 256, 0, stream>>>(order.get(), names.get(), H.get(), m, dc, R.get());
                 names.reset();
                 DBuf<uint32_t> Rr(cache, m), red(cache, m);
-                prim.scan(KeepOp{U.get(), R.get(), Rr.get(), red.get()}, m, 0);
+                prim.scan(KeepOp{R.get(), Rr.get(), red.get()}, m, 0);
                 const uint32_t after = (uint32_t)prim.read_scalars(1)[0];
                 const bool use_disc = ((double)after / (double)m) <= opt.discarding_threshold && n > 80;
                 const uint32_t rbits = bit_width(M);
                 if (!use_disc) {
                     Rr.reset();
                     red.reset();
-                    U.reset();
+                    k_strip_flags<<<grid(m), 256, 0, stream>>>(R.get(), m);
                     DBuf<uint32_t> sar(cache, m);
-                    TextU32 rv{R.get(), m};
                     if (n <= 80)
-                        base_case(rv, m, sar.get());  // sequential_sa_on_local_ranks (pdcx.hpp:115-121)
+                        base_case(R.get(), nullptr, m, sar.get());  // sequential_sa_on_local_ranks (pdcx.hpp:115-121)
                     else
-                        level(rv, m, rbits, depth + 1, sar.get(), nullptr);
+                        level<uint32_t>(R.get(), nullptr, m, rbits, depth + 1, sar.get(), nullptr);
                     k_ranks_from_sa<<<grid(m), 256, 0, stream>>>(sar.get(), m, dc, ranks.get());
                 } else {
                     if (ls) {
@@ -289,27 +327,24 @@ class Engine {
                     }
                     R.reset();
                     DBuf<uint32_t> sar(cache, after), isar(cache, after);
-                    level(TextU32{Rr.get(), after}, after, rbits, depth + 1, sar.get(), nullptr);
+                    level<uint32_t>(Rr.get(), nullptr, after, rbits, depth + 1, sar.get(), nullptr);
                     k_isa<<<grid(after), 256, 0, stream>>>(sar.get(), after, isar.get());
                     sar.reset();
                     Rr.reset();
                     // order of (name, rank2) = sorted-by-name order refined inside non-unique groups
-                    DBuf<uint32_t> act(cache, m), gid(cache, m);
-                    prim.scan(ActiveScanOp{H.get(), act.get(), gid.get()}, m, 0);
-                    const uint64_t tot = prim.read_scalars(1)[0];
-                    const uint32_t A = (uint32_t)tot, G = (uint32_t)(tot >> 32);
+                    const uint32_t A = tied.A, G = tied.G;
                     if (A > 0) {
                         const uint32_t gb = bit_width(G - 1), r2b = bit_width(after);
                         DBuf<uint64_t> ka(cache, A), kb(cache, A);
                         DBuf<uint32_t> va(cache, A), vb(cache, A);
-                        k_keys_rank2<<<grid(A), 256, 0, stream>>>(A, act.get(), gid.get(), order.get(), dc, red.get(),
-                                                                  isar.get(), r2b, ka.get(), va.get());
+                        k_keys_rank2<<<grid(A), 256, 0, stream>>>(A, tied.act.get(), tied.gid.get(), order.get(), dc,
+                                                                  red.get(), isar.get(), r2b, ka.get(), va.get());
                         uint64_t* x = ka.get();
                         uint64_t* x2 = kb.get();
                         uint32_t* y = va.get();
                         uint32_t* y2 = vb.get();
                         prim.sort_pairs(x, x2, y, y2, A, 0, gb + r2b);
-                        k_writeback<<<grid(A), 256, 0, stream>>>(x, y, act.get(), A, order.get(), H.get(), 0);
+                        k_writeback<<<grid(A), 256, 0, stream>>>(x, y, tied.act.get(), A, order.get(), H.get(), 0);
                     }
                     k_rank_from_order<<<grid(m), 256, 0, stream>>>(order.get(), m, ranks.get());
                 }
@@ -321,8 +356,10 @@ class Engine {
         {
             DBuf<uint32_t> order;
             DBuf<uint8_t> H;
-            refine_sort<TV, 1>(tv, dc, n, Lm, bits, order, H);
-            tie_break(n, dc, order, H, ranks.get());
+            ActiveSet tied;
+            refine_sort<1>(W.get(), dc, n, Lm, bits, order, H, tied);
+            W.reset();
+            tie_break(dc, order, H, tied, ranks.get());
             DCX_HIP(hipMemcpyAsync(sa_out, order.get(), (size_t)n * 4, hipMemcpyDeviceToDevice, stream));
         }
     }
@@ -410,7 +447,7 @@ class Engine {
         // phase boundaries default to "empty" (a base-case level never reaches them)
         DCX_HIP(hipEventRecord(ev[2], stream));
         DCX_HIP(hipEventRecord(ev[3], stream));
-        level(TextU8{d_text, (uint32_t)n, dlut.get()}, (uint32_t)n, bit_width(sigma), 0, sa.get(), ev);
+        level<uint8_t>(d_text, dlut.get(), (uint32_t)n, bit_width(sigma), 0, sa.get(), ev);
         DCX_HIP(hipEventRecord(ev[4], stream));  // SA resident in HBM
         // output in the requested width
         if (o->sa_on_device) {

@@ -102,15 +102,6 @@ struct TextU32 {
     __device__ __forceinline__ uint32_t at(uint64_t i) const { return i < n ? t[i] : 0u; }
 };
 
-// Left-justified packing of `count` characters of `bits` bits (PackedInteger,
-// char_container.hpp:57-71): character k occupies bits [64-(k+1)*bits, 64-k*bits).
-template <class TV>
-__device__ __forceinline__ uint64_t pack_chars(const TV& tv, uint64_t p, uint32_t count, uint32_t bits) {
-    uint64_t w = 0;
-    for (uint32_t k = 0; k < count; k++) w |= (uint64_t)tv.at(p + k) << (64 - (k + 1) * bits);
-    return w;
-}
-
 // ------------------------------------------------------------------------------------------------
 // alphabet histogram (compress_alphabet, cli.cpp:417-463)
 // ------------------------------------------------------------------------------------------------
@@ -142,49 +133,87 @@ __global__ __launch_bounds__(256) void k_byte_hist(const uint8_t* __restrict__ t
 // refinement sort: round 0 keys, heads, active compaction, later-round keys, write back
 // MODE 0: elements are samples (value = text-order sample index k); MODE 1: all positions.
 // ------------------------------------------------------------------------------------------------
-template <class TV, int MODE>
-__global__ __launch_bounds__(256) void k_keys_r0(TV tv, DcDev dc, uint32_t N, uint32_t count, uint32_t bits,
+// Packed words in text order: W[i] = the cpw characters T[i .. i+cpw) left-justified (PackedInteger,
+// char_container.hpp:57-71), for i in [0, total).  Characters are staged through LDS per 4096-position tile.
+constexpr int WD_TILE = 4096;
+template <class CharT>
+__global__ __launch_bounds__(256) void k_words(const CharT* __restrict__ t, uint32_t n, const uint16_t* __restrict__ lut,
+                                               uint32_t bits, uint32_t cpw, uint64_t total, uint64_t* __restrict__ W) {
+    __shared__ uint32_t s_c[WD_TILE + 64];
+    __shared__ uint16_t s_lut[256];
+    if (sizeof(CharT) == 1)
+        for (int i = threadIdx.x; i < 256; i += 256) s_lut[i] = lut[i];
+    __syncthreads();
+    for (uint64_t base = (uint64_t)blockIdx.x * WD_TILE; base < total; base += (uint64_t)gridDim.x * WD_TILE) {
+        for (uint32_t i = threadIdx.x; i < WD_TILE + cpw; i += 256) {
+            uint64_t p = base + i;
+            uint32_t c = 0;
+            if (p < n) c = sizeof(CharT) == 1 ? (uint32_t)s_lut[(uint32_t)t[p]] : (uint32_t)t[p];
+            s_c[i] = c;
+        }
+        __syncthreads();
+        for (uint32_t i = threadIdx.x; i < WD_TILE && base + i < total; i += 256) {
+            uint64_t w = 0;
+            for (uint32_t k = 0; k < cpw; k++) w |= (uint64_t)s_c[i + k] << (64 - (k + 1) * bits);
+            W[base + i] = w;
+        }
+        __syncthreads();
+    }
+}
+
+__device__ __forceinline__ uint64_t top_bits(uint64_t w, uint32_t nbits) {
+    return nbits >= 64 ? w : (nbits == 0 ? 0ull : (w & (~0ull << (64 - nbits))));
+}
+
+template <int MODE>
+__global__ __launch_bounds__(256) void k_keys_r0(const uint64_t* __restrict__ W, DcDev dc, uint32_t N, uint32_t kbits,
                                                  uint64_t* __restrict__ keys, uint32_t* __restrict__ vals) {
     for (uint32_t j = blockIdx.x * 256 + threadIdx.x; j < N; j += gridDim.x * 256) {
         uint32_t p = MODE == 0 ? sample_pos(dc, j) : j;
-        keys[j] = pack_chars(tv, p, count, bits);
+        keys[j] = top_bits(W[p], kbits);
         vals[j] = j;
     }
 }
 
-__global__ __launch_bounds__(256) void k_heads(const uint64_t* __restrict__ keys, uint32_t N, uint8_t* __restrict__ H) {
+// group heads of sorted keys, comparing only the top kbits
+__global__ __launch_bounds__(256) void k_heads(const uint64_t* __restrict__ keys, uint32_t N, uint32_t kbits,
+                                               uint8_t* __restrict__ H) {
     for (uint32_t j = blockIdx.x * 256 + threadIdx.x; j <= N; j += gridDim.x * 256)
-        H[j] = (j == 0 || j == N) ? 1 : (keys[j] != keys[j - 1]);
+        H[j] = (j == 0 || j == N) ? 1 : (top_bits(keys[j], kbits) != top_bits(keys[j - 1], kbits));
 }
 
-// active = member of a group of size >= 2; value packs (active, active && head) as two 32-bit lanes
+// active = member of a group of size >= 2; value packs (active, active && head) as two 32-bit lanes.
+// Scans either all positions (src == nullptr) or a previous active list (a subsequence of it results).
 struct ActiveScanOp {
     const uint8_t* H;
+    const uint32_t* src;
     uint32_t* act_idx;
     uint32_t* gid;
-    __device__ __forceinline__ uint64_t value(uint64_t j) const {
+    __device__ __forceinline__ uint64_t value(uint64_t a) const {
+        uint32_t j = src ? src[a] : (uint32_t)a;
         uint32_t h = H[j];
         bool active = !(h && H[j + 1]);
         return active ? (1ull | ((uint64_t)h << 32)) : 0ull;
     }
-    __device__ __forceinline__ void write(uint64_t j, uint64_t excl, uint64_t v) const {
+    __device__ __forceinline__ void write(uint64_t a, uint64_t excl, uint64_t v) const {
         if (v) {
-            uint32_t a = (uint32_t)excl;
-            act_idx[a] = (uint32_t)j;
-            gid[a] = (uint32_t)(excl >> 32) + (uint32_t)(v >> 32) - 1u;
+            uint32_t j = src ? src[a] : (uint32_t)a;
+            uint32_t o = (uint32_t)excl;
+            act_idx[o] = j;
+            gid[o] = (uint32_t)(excl >> 32) + (uint32_t)(v >> 32) - 1u;
         }
     }
 };
 
-template <class TV, int MODE>
-__global__ __launch_bounds__(256) void k_keys_rr(TV tv, DcDev dc, uint32_t A, const uint32_t* __restrict__ act_idx,
-                                                 const uint32_t* __restrict__ gid, const uint32_t* __restrict__ order,
-                                                 uint32_t C, uint32_t count, uint32_t bits, uint32_t gbits,
-                                                 uint64_t* __restrict__ keys, uint32_t* __restrict__ vals) {
+template <int MODE>
+__global__ __launch_bounds__(256) void k_keys_rr(const uint64_t* __restrict__ W, DcDev dc, uint32_t A,
+                                                 const uint32_t* __restrict__ act_idx, const uint32_t* __restrict__ gid,
+                                                 const uint32_t* __restrict__ order, uint32_t C, uint32_t cbits,
+                                                 uint32_t gbits, uint64_t* __restrict__ keys, uint32_t* __restrict__ vals) {
     for (uint32_t a = blockIdx.x * 256 + threadIdx.x; a < A; a += gridDim.x * 256) {
         uint32_t e = order[act_idx[a]];
         uint32_t p = MODE == 0 ? sample_pos(dc, e) : e;
-        uint64_t ch = pack_chars(tv, (uint64_t)p + C, count, bits);
+        uint64_t ch = top_bits(W[(uint64_t)p + C], cbits);
         keys[a] = gbits ? (((uint64_t)gid[a] << (64 - gbits)) | (ch >> gbits)) : ch;
         vals[a] = e;
     }
@@ -217,29 +246,36 @@ __global__ __launch_bounds__(256) void k_rank_from_order(const uint32_t* __restr
 }
 
 // recursive string in (mod X, div X) order with unique flags (handle_recursive_call,
-// pdcx.hpp:145-180; flag_unique_ranks, compute_ranks.hpp:113-134)
+// pdcx.hpp:145-180; flag_unique_ranks, compute_ranks.hpp:113-134).  Names stay below 2^31 (checked on
+// the host), so the unique flag rides in bit 31 of the name: one scattered word per sample.
+constexpr uint32_t UNIQ_BIT = 0x80000000u;
 __global__ __launch_bounds__(256) void k_build_rec(const uint32_t* __restrict__ order, const uint32_t* __restrict__ names,
                                                    const uint8_t* __restrict__ H, uint32_t m, DcDev dc,
-                                                   uint32_t* __restrict__ R, uint8_t* __restrict__ U) {
+                                                   uint32_t* __restrict__ R) {
     for (uint32_t j = blockIdx.x * 256 + threadIdx.x; j < m; j += gridDim.x * 256) {
         uint32_t c = class_major(dc, order[j]);
-        R[c] = names[j];
-        U[c] = H[j] && H[j + 1];
+        R[c] = names[j] | ((H[j] && H[j + 1]) ? UNIQ_BIT : 0u);
     }
 }
 
-// discarding (pdcx.hpp:215-246): keep c unless c and c-1 are both unique; element 0 is kept
+// discarding (pdcx.hpp:215-246): keep c unless c and c-1 are both unique; element 0 is kept.  Also
+// removes the flags into Rr (the reduced string) and records every element's reduced index.
 struct KeepOp {
-    const uint8_t* U;
     const uint32_t* R;
     uint32_t* Rr;
     uint32_t* red;
-    __device__ __forceinline__ uint64_t value(uint64_t c) const { return (c == 0 || !(U[c] && U[c - 1])) ? 1u : 0u; }
+    __device__ __forceinline__ uint64_t value(uint64_t c) const {
+        return (c == 0 || !((R[c] & UNIQ_BIT) && (R[c - 1] & UNIQ_BIT))) ? 1u : 0u;
+    }
     __device__ __forceinline__ void write(uint64_t c, uint64_t excl, uint64_t v) const {
         red[c] = (uint32_t)excl;
-        if (v) Rr[excl] = R[c];
+        if (v) Rr[excl] = R[c] & ~UNIQ_BIT;
     }
 };
+
+__global__ __launch_bounds__(256) void k_strip_flags(uint32_t* __restrict__ R, uint32_t m) {
+    for (uint32_t j = blockIdx.x * 256 + threadIdx.x; j < m; j += gridDim.x * 256) R[j] &= ~UNIQ_BIT;
+}
 
 // recursive_call_direct (pdcx.hpp:182-212): rank of map_back(SA_R[j]) = j + 1
 __global__ __launch_bounds__(256) void k_ranks_from_sa(const uint32_t* __restrict__ sar, uint32_t m, DcDev dc,

@@ -73,7 +73,7 @@ __global__ __launch_bounds__(256) void k_radix_hist_scan(uint32_t* hist) {
 }
 
 // One LSD pass over digit (key >> shift) & 255.  Stable.
-template <bool VALS>
+template <bool VALS, bool IDENT>
 __global__ __launch_bounds__(RS_THREADS) void k_onesweep(const uint64_t* __restrict__ kin, uint64_t* __restrict__ kout,
                                                          const uint32_t* __restrict__ vin, uint32_t* __restrict__ vout,
                                                          uint32_t N, uint32_t shift, const uint32_t* __restrict__ dbase,
@@ -106,7 +106,7 @@ __global__ __launch_bounds__(RS_THREADS) void k_onesweep(const uint64_t* __restr
         uint64_t idx = wbase + k * 64 + lane;
         bool valid = idx < N;
         key[k] = valid ? kin[idx] : 0ull;
-        if (VALS) val[k] = valid ? vin[idx] : 0u;
+        if (VALS) val[k] = IDENT ? (uint32_t)idx : (valid ? vin[idx] : 0u);
     }
     // rank every key among the keys of its wave with the same digit, in tile order
 #pragma unroll
@@ -200,63 +200,107 @@ constexpr int SC_ITEMS = 16;
 constexpr int SC_TILE = SC_THREADS * SC_ITEMS;
 
 template <class Op>
-__global__ __launch_bounds__(SC_THREADS) void k_scan(Op op, uint64_t N, uint64_t* __restrict__ st_agg,
-                                                     uint64_t* __restrict__ st_inc, uint32_t* __restrict__ st_flag,
-                                                     uint32_t tag, unsigned long long* __restrict__ ticket,
+__global__ __launch_bounds__(SC_THREADS) void k_scan(Op op, uint64_t N, uint64_t* __restrict__ st_lo,
+                                                     uint64_t* __restrict__ st_hi, uint32_t tag,
+                                                     unsigned long long* __restrict__ ticket,
                                                      unsigned long long ticket_base, uint64_t* __restrict__ total_out) {
     __shared__ uint64_t s_tmp[SC_THREADS / 64];
     __shared__ uint64_t s_excl;
     __shared__ uint32_t s_tile;
+    __shared__ uint64_t s_v[SC_TILE + SC_TILE / 16];  // padded to spread blocked reads over banks
     const uint32_t tid = threadIdx.x;
     if (tid == 0) s_tile = (uint32_t)(atomicAdd(ticket, 1ull) - ticket_base);
     __syncthreads();
     const uint32_t tile = s_tile;
-    const uint64_t base = (uint64_t)tile * SC_TILE + (uint64_t)tid * SC_ITEMS;
+    const uint64_t tbase = (uint64_t)tile * SC_TILE;
+    // striped (coalesced) loads, transposed through LDS to a blocked arrangement for the scan
     uint64_t v[SC_ITEMS];
+#pragma unroll
+    for (int k = 0; k < SC_ITEMS; k++) {
+        const uint32_t idx = k * SC_THREADS + tid;
+        const uint64_t i = tbase + idx;
+        v[k] = i < N ? op.value(i) : 0ull;
+        s_v[idx + (idx >> 4)] = v[k];
+    }
+    __syncthreads();
     uint64_t sum = 0;
 #pragma unroll
     for (int k = 0; k < SC_ITEMS; k++) {
-        uint64_t i = base + k;
-        v[k] = i < N ? op.value(i) : 0ull;
-        sum += v[k];
+        const uint32_t idx = tid * SC_ITEMS + k;
+        sum += s_v[idx + (idx >> 4)];
     }
     uint64_t tile_total = 0;
     uint64_t texcl = block_excl_scan64<SC_THREADS>(sum, s_tmp, &tile_total);
-    if (tid == 0) {
-        uint64_t excl = 0;
-        if (tile == 0) {
-            __hip_atomic_store(&st_inc[tile], tile_total, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            __hip_atomic_store(&st_flag[tile], (tag << 2) | 2u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
-        } else {
-            __hip_atomic_store(&st_agg[tile], tile_total, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            __hip_atomic_store(&st_flag[tile], (tag << 2) | 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
-            int64_t j = (int64_t)tile - 1;
-            while (true) {
-                uint32_t f = __hip_atomic_load(&st_flag[j], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
-                if ((f >> 2) != tag || (f & 3u) == 0) {
-                    __builtin_amdgcn_s_sleep(1);
-                    continue;
-                }
-                if (f & 2u) {
-                    excl += __hip_atomic_load(&st_inc[j], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-                    break;
-                }
-                excl += __hip_atomic_load(&st_agg[j], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-                --j;
-            }
-            __hip_atomic_store(&st_inc[tile], excl + tile_total, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            __hip_atomic_store(&st_flag[tile], (tag << 2) | 2u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+    if (tid < 64) {
+        // wave 0: publish the aggregates of both 32-bit lanes, then look back over 64 predecessors at a
+        // time; each lane of the value is an independent lookback (its own nearest inclusive prefix)
+        const uint32_t lane = tid;
+        const uint64_t tot_lo = tile_total & 0xFFFFFFFFull, tot_hi = tile_total >> 32;
+        if (lane == 0) {
+            const uint64_t f = tile == 0 ? LB_INC : LB_AGG;
+            st_agent(&st_lo[tile], lb_pack(tag, f, tot_lo));
+            st_agent(&st_hi[tile], lb_pack(tag, f, tot_hi));
         }
-        s_excl = excl;
-        if (tile == gridDim.x - 1 && total_out) *total_out = excl + tile_total;
+        uint64_t ex_lo = 0, ex_hi = 0;
+        if (tile > 0) {
+            bool done_lo = false, done_hi = false;
+            int64_t base = (int64_t)tile - 1;
+            while (!(done_lo && done_hi)) {
+                const int64_t j = base - (int64_t)lane;
+                uint64_t wl = LB_INC, wh = LB_INC;  // before tile 0: inclusive zeros
+                if (j >= 0) {
+                    while (true) {
+                        wl = ld_agent(&st_lo[j]);
+                        wh = ld_agent(&st_hi[j]);
+                        if ((uint32_t)(wl >> 42) == tag && (uint32_t)(wh >> 42) == tag && (wl & LB_FLAG_MASK) &&
+                            (wh & LB_FLAG_MASK))
+                            break;
+                        __builtin_amdgcn_s_sleep(1);
+                    }
+                }
+                if (!done_lo) {
+                    const uint64_t m = __ballot((wl & LB_INC) != 0);
+                    const uint32_t k = m ? (uint32_t)__ffsll((long long)m) - 1 : 63u;
+                    uint64_t part = lane <= k ? (wl & LB_VALUE_MASK) : 0ull;
+                    ex_lo += __shfl(wave_incl_scan64(part), 63, 64);
+                    done_lo = m != 0;
+                }
+                if (!done_hi) {
+                    const uint64_t m = __ballot((wh & LB_INC) != 0);
+                    const uint32_t k = m ? (uint32_t)__ffsll((long long)m) - 1 : 63u;
+                    uint64_t part = lane <= k ? (wh & LB_VALUE_MASK) : 0ull;
+                    ex_hi += __shfl(wave_incl_scan64(part), 63, 64);
+                    done_hi = m != 0;
+                }
+                base -= 64;
+            }
+        }
+        if (lane == 0) {
+            if (tile > 0) {
+                st_agent(&st_lo[tile], lb_pack(tag, LB_INC, ex_lo + tot_lo));
+                st_agent(&st_hi[tile], lb_pack(tag, LB_INC, ex_hi + tot_hi));
+            }
+            const uint64_t excl = ex_lo | (ex_hi << 32);
+            s_excl = excl;
+            if (tile == gridDim.x - 1 && total_out) *total_out = excl + tile_total;
+        }
     }
     __syncthreads();
     uint64_t run = s_excl + texcl;
 #pragma unroll
     for (int k = 0; k < SC_ITEMS; k++) {
-        uint64_t i = base + k;
-        if (i < N) op.write(i, run, v[k]);
-        run += v[k];
+        const uint32_t idx = tid * SC_ITEMS + k;
+        const uint32_t q = idx + (idx >> 4);
+        const uint64_t x = s_v[q];
+        s_v[q] = run;
+        run += x;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < SC_ITEMS; k++) {
+        const uint32_t idx = k * SC_THREADS + tid;
+        const uint64_t i = tbase + idx;
+        if (i < N) op.write(i, s_v[idx + (idx >> 4)], v[k]);
     }
 }
 
@@ -270,9 +314,8 @@ struct Primitives {
     uint64_t* sort_status = nullptr;
     size_t sort_status_words = 0;
     uint32_t sort_tag = 0;
-    uint64_t* scan_agg = nullptr;
-    uint64_t* scan_inc = nullptr;
-    uint32_t* scan_flag = nullptr;
+    uint64_t* scan_lo = nullptr;
+    uint64_t* scan_hi = nullptr;
     size_t scan_tiles_cap = 0;
     uint32_t scan_tag = 0;
     unsigned long long* ticket = nullptr;  // device counter of tile tickets
@@ -321,16 +364,14 @@ struct Primitives {
     }
     void destroy() {
         if (sort_status) (void)hipFree(sort_status);
-        if (scan_agg) (void)hipFree(scan_agg);
-        if (scan_inc) (void)hipFree(scan_inc);
-        if (scan_flag) (void)hipFree(scan_flag);
+        if (scan_lo) (void)hipFree(scan_lo);
+        if (scan_hi) (void)hipFree(scan_hi);
         if (ticket) (void)hipFree(ticket);
         if (scalars) (void)hipFree(scalars);
         if (hist) (void)hipFree(hist);
         if (host_scalars) (void)hipHostFree(host_scalars);
         sort_status = nullptr;
-        scan_agg = scan_inc = nullptr;
-        scan_flag = nullptr;
+        scan_lo = scan_hi = nullptr;
         ticket = nullptr;
         scalars = nullptr;
         hist = nullptr;
@@ -370,31 +411,38 @@ struct Primitives {
     }
     void ensure_scan_status(uint64_t tiles) {
         if (tiles <= scan_tiles_cap) return;
-        if (scan_agg) DCX_HIP(hipFree(scan_agg));
-        if (scan_inc) DCX_HIP(hipFree(scan_inc));
-        if (scan_flag) DCX_HIP(hipFree(scan_flag));
+        if (scan_lo) DCX_HIP(hipFree(scan_lo));
+        if (scan_hi) DCX_HIP(hipFree(scan_hi));
+        scan_lo = scan_hi = nullptr;
         size_t cap = tiles + tiles / 4 + 16;
-        DCX_HIP(hipMalloc(&scan_agg, cap * sizeof(uint64_t)));
-        DCX_HIP(hipMalloc(&scan_inc, cap * sizeof(uint64_t)));
-        DCX_HIP(hipMalloc(&scan_flag, cap * sizeof(uint32_t)));
-        fill_u32(scan_flag, 0, cap);
+        DCX_HIP(hipMalloc(&scan_lo, cap * sizeof(uint64_t)));
+        DCX_HIP(hipMalloc(&scan_hi, cap * sizeof(uint64_t)));
+        fill_u64(scan_lo, 0, cap);
+        fill_u64(scan_hi, 0, cap);
         scan_tiles_cap = cap;
         scan_tag = 0;
     }
-
     // Sorts (keys, vals) by key bits [lo_bit, hi_bit).  Key bits inside the pass range but outside
     // [lo_bit, hi_bit) must be equal for all keys.  On return keys/vals point at the sorted data
     // (the buffers may have been swapped with the alternates).
     void sort_pairs(uint64_t*& keys, uint64_t*& keys_alt, uint32_t*& vals, uint32_t*& vals_alt, uint32_t N,
-                    uint32_t lo_bit, uint32_t hi_bit) {
-        if (N <= 1 || hi_bit <= lo_bit) return;
+                    uint32_t lo_bit, uint32_t hi_bit, const uint64_t* first_keys = nullptr, bool ident_vals = false) {
+        if (hi_bit <= lo_bit) hi_bit = lo_bit + 1;
+        if (N <= 1) {
+            // nothing to order: materialize the inputs in place
+            if (N == 1 && first_keys)
+                DCX_HIP(hipMemcpyAsync(keys, first_keys, 8, hipMemcpyDeviceToDevice, stream));
+            if (N == 1 && ident_vals && vals) fill_u32(vals, 0, 1);
+            return;
+        }
         uint32_t np = (hi_bit - lo_bit + 7) / 8;
         if (np > 8) np = 8;
         if (lo_bit + 8 * np > 64) lo_bit = 64 - 8 * np;
         uint64_t tiles = ceil_div(N, RS_TILE);
         ensure_sort_status(tiles);
         fill_u32(hist, 0, 8 * RS_RADIX);
-        hipLaunchKernelGGL(k_radix_hist, dim3(grid_for(N, 256, 2048)), dim3(256), 0, stream, keys, N, lo_bit, np,
+        const uint64_t* hsrc = first_keys ? first_keys : keys;
+        hipLaunchKernelGGL(k_radix_hist, dim3(grid_for(N, 256, 2048)), dim3(256), 0, stream, hsrc, N, lo_bit, np,
                            hist);
         hipLaunchKernelGGL(k_radix_hist_scan, dim3(np), dim3(256), 0, stream, hist);
         for (uint32_t p = 0; p < np; p++) {
@@ -409,14 +457,20 @@ struct Primitives {
                 DCX_HIP(hipEventCreate(&e1));
                 DCX_HIP(hipEventRecord(e0, stream));
             }
-            if (vals)
-                hipLaunchKernelGGL(k_onesweep<true>, dim3((unsigned)tiles), dim3(RS_THREADS), 0, stream, keys,
-                                   keys_alt, vals, vals_alt, N, lo_bit + 8 * p, hist + p * RS_RADIX, sort_status, tag,
-                                   ticket, ticket_host);
-            else
-                hipLaunchKernelGGL(k_onesweep<false>, dim3((unsigned)tiles), dim3(RS_THREADS), 0, stream, keys,
+            const bool first = p == 0;
+            const uint64_t* kin = (first && first_keys) ? first_keys : keys;
+            if (!vals)
+                hipLaunchKernelGGL((k_onesweep<false, false>), dim3((unsigned)tiles), dim3(RS_THREADS), 0, stream, kin,
                                    keys_alt, (const uint32_t*)nullptr, (uint32_t*)nullptr, N, lo_bit + 8 * p,
                                    hist + p * RS_RADIX, sort_status, tag, ticket, ticket_host);
+            else if (first && ident_vals)
+                hipLaunchKernelGGL((k_onesweep<true, true>), dim3((unsigned)tiles), dim3(RS_THREADS), 0, stream, kin,
+                                   keys_alt, (const uint32_t*)nullptr, vals_alt, N, lo_bit + 8 * p,
+                                   hist + p * RS_RADIX, sort_status, tag, ticket, ticket_host);
+            else
+                hipLaunchKernelGGL((k_onesweep<true, false>), dim3((unsigned)tiles), dim3(RS_THREADS), 0, stream, kin,
+                                   keys_alt, vals, vals_alt, N, lo_bit + 8 * p, hist + p * RS_RADIX, sort_status, tag,
+                                   ticket, ticket_host);
             if (profile) {
                 DCX_HIP(hipEventRecord(e1, stream));
                 pass_events.emplace_back(e0, e1);
@@ -437,13 +491,14 @@ struct Primitives {
         }
         uint64_t tiles = ceil_div(N, SC_TILE);
         ensure_scan_status(tiles);
-        if (scan_tag >= (1u << 29)) {
-            fill_u32(scan_flag, 0, scan_tiles_cap);
+        if (scan_tag >= LB_TAG_LIMIT) {
+            fill_u64(scan_lo, 0, scan_tiles_cap);
+            fill_u64(scan_hi, 0, scan_tiles_cap);
             scan_tag = 0;
         }
         uint32_t tag = ++scan_tag;
-        hipLaunchKernelGGL(k_scan<Op>, dim3((unsigned)tiles), dim3(SC_THREADS), 0, stream, op, N, scan_agg, scan_inc,
-                           scan_flag, tag, ticket, ticket_host, scalars + slot);
+        hipLaunchKernelGGL(k_scan<Op>, dim3((unsigned)tiles), dim3(SC_THREADS), 0, stream, op, N, scan_lo, scan_hi,
+                           tag, ticket, ticket_host, scalars + slot);
         ticket_host += tiles;
     }
 
