@@ -126,7 +126,13 @@ class Engine {
     void refine_sort(const uint64_t* W, const DcDev& dc, uint32_t N, uint32_t L, uint32_t bits, DBuf<uint32_t>& order,
                      DBuf<uint8_t>& H, ActiveSet& fin) {
         const uint32_t cpw = 64 / bits;
-        const uint32_t c0 = L < cpw ? L : cpw;
+        uint32_t c0 = L < cpw ? L : cpw;
+        if (MODE == 1 && N >= (1u << 24)) {
+            // large merges: round 0 takes about log2(N) + 16 bits in whole 8-bit digits; the refinement
+            // rounds resolve the (few) positions still tied
+            const uint32_t want = std::min<uint32_t>(64, (bit_width(N) + 16 + 7) / 8 * 8);
+            c0 = std::max<uint32_t>(1, std::min<uint32_t>(c0, want / bits));
+        }
         {
             DBuf<uint64_t> ka(cache, N), kb(cache, N);
             DBuf<uint32_t> va(cache, N), vb(cache, N);
@@ -233,9 +239,22 @@ class Engine {
     // through the alphabet map `lut`; deeper levels read names (uint32, >= 1).
     // -------------------------------------------------------------------------------------------
     template <class CharT>
-    void level(const CharT* text, const uint16_t* lut, uint32_t n, uint32_t bits, int depth, uint32_t* sa_out,
+    void make_words(const CharT* text, const uint16_t* lut, uint32_t n, uint32_t bits, uint32_t dense, uint64_t total,
+                    uint64_t* W) {
+        const unsigned blocks = (unsigned)std::min<uint64_t>(ceil_div(total, WD_TILE), 8192);
+        k_words<CharT><<<blocks, 256, 0, stream>>>(text, n, lut, bits, 64 / bits, total, dense, W);
+    }
+
+    // characters of the level are 1..maxchar (0 is the padding)
+    template <class CharT>
+    void level(const CharT* text, const uint16_t* lut, uint32_t n, uint32_t maxchar, int depth, uint32_t* sa_out,
                hipEvent_t* ev) {
         const uint32_t X = dcs->X, D = dcs->D;
+        const uint32_t bits = bit_width(maxchar);
+        // dense merge words (c -> c - 1) when they save a bit per character
+        const uint32_t dbits = maxchar > 1 ? bit_width(maxchar - 1) : 0;
+        const bool dense = dbits >= 1 && dbits < bits;
+        const uint32_t mbits = dense ? dbits : bits;
         dcx_level_stats* ls = level_stat(depth);
         if (ls) ls->n = n;
         const uint64_t threshold = opt.base_case_chars > 2ull * X ? opt.base_case_chars : 2ull * X;
@@ -245,7 +264,7 @@ class Engine {
             return;
         }
         uint64_t m64 = 0;
-        const DcDev dc = make_dcdev(n, &m64);
+        DcDev dc = make_dcdev(n, &m64);
         const uint32_t m = (uint32_t)m64;
 
         // key lengths (materialize_sample / materialize_characters with the packing ratio)
@@ -261,6 +280,7 @@ class Engine {
         }
         const uint32_t Ls = (uint32_t)((double)X * (ps ? ratio : 1.0));
         const uint32_t Lm = (uint32_t)((double)X * (pm ? ratio : 1.0)) - 1;
+        dc.bnd = n - Lm + 1;
         if (ls) {
             ls->samples = m;
             ls->key_chars = Ls;
@@ -275,12 +295,10 @@ class Engine {
             alg_bytes += (uint64_t)n * c + 2ull * m * (K1 + I) + 2ull * n * (K4 + I) + (uint64_t)n * S;
         }
 
-        // packed words of the whole level text (shared by the sample sort and the merge)
-        const uint32_t cpw = 64 / bits;
+        // packed words of the whole level text, sentinel-coded (character 0 = padding) for the samples
         const uint64_t wtotal = (uint64_t)n + (Ls > Lm ? Ls : Lm) + 2;
         DBuf<uint64_t> W(cache, wtotal);
-        k_words<CharT><<<(unsigned)std::min<uint64_t>(ceil_div(wtotal, WD_TILE), 8192), 256, 0, stream>>>(text, n, lut, bits, cpw, wtotal,
-                                                                                 W.get());
+        make_words(text, lut, n, bits, 0, wtotal, W.get());
 
         // ---------------- phase 1 + 2: sample sort and names ----------------
         DBuf<uint32_t> ranks(cache, (size_t)m + D);
@@ -309,7 +327,6 @@ class Engine {
                 prim.scan(KeepOp{R.get(), Rr.get(), red.get()}, m, 0);
                 const uint32_t after = (uint32_t)prim.read_scalars(1)[0];
                 const bool use_disc = ((double)after / (double)m) <= opt.discarding_threshold && n > 80;
-                const uint32_t rbits = bit_width(M);
                 if (!use_disc) {
                     Rr.reset();
                     red.reset();
@@ -318,7 +335,7 @@ class Engine {
                     if (n <= 80)
                         base_case(R.get(), nullptr, m, sar.get());  // sequential_sa_on_local_ranks (pdcx.hpp:115-121)
                     else
-                        level<uint32_t>(R.get(), nullptr, m, rbits, depth + 1, sar.get(), nullptr);
+                        level<uint32_t>(R.get(), nullptr, m, M, depth + 1, sar.get(), nullptr);
                     k_ranks_from_sa<<<grid(m), 256, 0, stream>>>(sar.get(), m, dc, ranks.get());
                 } else {
                     if (ls) {
@@ -327,7 +344,7 @@ class Engine {
                     }
                     R.reset();
                     DBuf<uint32_t> sar(cache, after), isar(cache, after);
-                    level<uint32_t>(Rr.get(), nullptr, after, rbits, depth + 1, sar.get(), nullptr);
+                    level<uint32_t>(Rr.get(), nullptr, after, M, depth + 1, sar.get(), nullptr);
                     k_isa<<<grid(after), 256, 0, stream>>>(sar.get(), after, isar.get());
                     sar.reset();
                     Rr.reset();
@@ -357,7 +374,12 @@ class Engine {
             DBuf<uint32_t> order;
             DBuf<uint8_t> H;
             ActiveSet tied;
-            refine_sort<1>(W.get(), dc, n, Lm, bits, order, H, tied);
+            if (dense) {
+                W.reset();
+                W = DBuf<uint64_t>(cache, wtotal);
+                make_words(text, lut, n, mbits, 1, wtotal, W.get());
+            }
+            refine_sort<1>(W.get(), dc, n, Lm, mbits, order, H, tied);
             W.reset();
             tie_break(dc, order, H, tied, ranks.get());
             DCX_HIP(hipMemcpyAsync(sa_out, order.get(), (size_t)n * 4, hipMemcpyDeviceToDevice, stream));
@@ -447,7 +469,7 @@ class Engine {
         // phase boundaries default to "empty" (a base-case level never reaches them)
         DCX_HIP(hipEventRecord(ev[2], stream));
         DCX_HIP(hipEventRecord(ev[3], stream));
-        level<uint8_t>(d_text, dlut.get(), (uint32_t)n, bit_width(sigma), 0, sa.get(), ev);
+        level<uint8_t>(d_text, dlut.get(), (uint32_t)n, sigma, 0, sa.get(), ev);
         DCX_HIP(hipEventRecord(ev[4], stream));  // SA resident in HBM
         // output in the requested width
         if (o->sa_on_device) {

@@ -62,6 +62,7 @@ struct DcDev {
     uint32_t dc[MAX_D];
     uint32_t sb[MAX_D + 1];     // samples_before (pdcx.hpp:499-508): class-major offsets
     uint8_t next_rank[MAX_X + 3];  // #{d in DC : d < r}  (difference_cover.hpp NEXT_RANK)
+    uint32_t bnd;                  // first position whose merge window reaches past the text end
 };
 
 // k-th sample in text order (k = rank_pos of the sample, merge_samples.hpp:181-188)
@@ -136,9 +137,12 @@ __global__ __launch_bounds__(256) void k_byte_hist(const uint8_t* __restrict__ t
 // Packed words in text order: W[i] = the cpw characters T[i .. i+cpw) left-justified (PackedInteger,
 // char_container.hpp:57-71), for i in [0, total).  Characters are staged through LDS per 4096-position tile.
 constexpr int WD_TILE = 4096;
+// dense != 0 codes character c as c - 1 (the padding also reads 0): one bit less whenever the largest
+// character is a power of two.  Only the merge uses dense words; its tie-break handles the padding.
 template <class CharT>
 __global__ __launch_bounds__(256) void k_words(const CharT* __restrict__ t, uint32_t n, const uint16_t* __restrict__ lut,
-                                               uint32_t bits, uint32_t cpw, uint64_t total, uint64_t* __restrict__ W) {
+                                               uint32_t bits, uint32_t cpw, uint64_t total, uint32_t dense,
+                                               uint64_t* __restrict__ W) {
     __shared__ uint32_t s_c[WD_TILE + 64];
     __shared__ uint16_t s_lut[256];
     if (sizeof(CharT) == 1)
@@ -148,7 +152,7 @@ __global__ __launch_bounds__(256) void k_words(const CharT* __restrict__ t, uint
         for (uint32_t i = threadIdx.x; i < WD_TILE + cpw; i += 256) {
             uint64_t p = base + i;
             uint32_t c = 0;
-            if (p < n) c = sizeof(CharT) == 1 ? (uint32_t)s_lut[(uint32_t)t[p]] : (uint32_t)t[p];
+            if (p < n) c = (sizeof(CharT) == 1 ? (uint32_t)s_lut[(uint32_t)t[p]] : (uint32_t)t[p]) - dense;
             s_c[i] = c;
         }
         __syncthreads();
@@ -306,8 +310,12 @@ __global__ __launch_bounds__(256) void k_keys_rank2(uint32_t A, const uint32_t* 
 // DC comparator (DCMergeSamples::operator<, merge_samples.hpp:87-109) for two suffixes whose first
 // compared characters are equal: ranks[rank_pos(a) + r1] < ranks[rank_pos(b) + r2].
 // ------------------------------------------------------------------------------------------------
+// With dense merge words the padding codes like the smallest character, so two suffixes whose windows
+// both reach past the end can tie although they differ; the later (shorter) one is then the smaller:
+// its window is a proper prefix of the other's.
 __device__ __forceinline__ bool dc_rank_less(uint32_t a, uint32_t b, const DcDev& dc, const uint32_t* __restrict__ cmp,
                                              const uint32_t* __restrict__ ranks) {
+    if (a >= dc.bnd && b >= dc.bnd) return a > b;
     uint32_t qa = a / dc.X, qb = b / dc.X;
     uint32_t ma = a - qa * dc.X, mb = b - qb * dc.X;
     uint32_t e = cmp[ma * dc.X + mb];

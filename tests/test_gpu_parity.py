@@ -201,3 +201,20 @@ def test_genome_64mib_digest(engine):
     sa = engine.build_sa(t, sa_width=8, X=39)
     assert hashlib.sha256(sa.astype("<u8").tobytes()).hexdigest() == rec["sa_u64_sha256"]
     assert engine.check_sa(t, sa, 8)
+
+
+def test_dense_codes_text_end(engine):
+    # alphabets of 2, 4 and 256 symbols make the merge use dense character codes, where the padding
+    # reads like the smallest character; texts ending in long runs of that character exercise the
+    # tie-break rule for windows that reach past the end
+    cases = []
+    for sigma, tail in ((4, 300), (2, 150), (256, 200), (4, 7)):
+        t = random_text(50000, sigma, sigma + tail)
+        t = np.concatenate([t, np.full(tail, t.min(), dtype=np.uint8)])
+        cases.append((f"s{sigma}_tail{tail}", t))
+    p = periodic_text(30000, 5, 4, 9)
+    cases.append(("period_then_min_run", np.concatenate([p, np.full(500, p.min(), dtype=np.uint8)])))
+    for name, t in cases:
+        for X in (3, 21, 39, 133):
+            for packing in (True, False):
+                check_case(engine, name, t, X, packing, packing)
