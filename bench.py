@@ -65,7 +65,21 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-validate", action="store_true")
     ap.add_argument("--host-to-host", action="store_true", help="also time one PCIe-inclusive run")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="only run a 1 GiB device-to-device copy three times (reference kernel for the "
+                         "FETCH_SIZE / WRITE_SIZE counter scale)")
     args = ap.parse_args()
+
+    if args.copy_calibration:
+        import torch
+        a = torch.empty(1 << 30, dtype=torch.uint8, device="cuda")
+        a.fill_(1)
+        b = torch.empty_like(a)
+        for _ in range(3):
+            b.copy_(a)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_calibration_bytes": 1 << 30, "copies": 3}))
+        return
 
     if args.gpus != 1:
         print(json.dumps({"error": "this build measures one GPU: multi-GPU sharding is design-only "

@@ -15,8 +15,11 @@
 
 namespace dcx {
 
+#ifndef DCX_RS_ITEMS
+#define DCX_RS_ITEMS 16
+#endif
 constexpr int RS_THREADS = 256;
-constexpr int RS_ITEMS = 16;
+constexpr int RS_ITEMS = DCX_RS_ITEMS;  // keys per thread of one radix tile
 constexpr int RS_TILE = RS_THREADS * RS_ITEMS;  // 4096 keys per workgroup
 constexpr int RS_WAVES = RS_THREADS / 64;
 constexpr int RS_RADIX = 256;
