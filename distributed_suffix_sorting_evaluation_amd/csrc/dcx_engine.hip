@@ -387,11 +387,7 @@ class Engine {
     }
 
     // -------------------------------------------------------------------------------------------
-    int build_sa(const uint8_t* text, uint64_t n, const dcx_options* o, void* sa_out, uint32_t width, dcx_stats* stats) {
-        auto t0 = std::chrono::steady_clock::now();
-        DCX_HIP(hipSetDevice(device));
-        if (!o || (!text && n) || (!sa_out && n)) throw Error(DCX_E_ARG, "null argument");
-        if (width != 4 && width != 5 && width != 8) throw Error(DCX_E_ARG, "sa_width_bytes must be 4, 5 or 8");
+    void set_options(const dcx_options* o) {
         opt.X = o->X;
         opt.pack_samples = o->pack_samples != 0;
         opt.pack_merging = o->pack_merging != 0;
@@ -402,6 +398,171 @@ class Engine {
         set_dc(opt.X);
         if (opt.base_case_chars > BASE_CASE_MAX)
             throw Error(DCX_E_ARG, "base_case_chars must be <= " + std::to_string(BASE_CASE_MAX));
+    }
+
+    // compress_alphabet (cli.cpp:417-463): used bytes -> 1..sigma in byte order; returns sigma and the map
+    uint32_t alphabet(const uint8_t* d_text, uint64_t n, DBuf<uint16_t>& dlut) {
+        DBuf<unsigned long long> hist(cache, 256);
+        DCX_HIP(hipMemsetAsync(hist.get(), 0, 256 * 8, stream));
+        if (n) k_byte_hist<<<grid(ceil_div(n, 16), 2048), 256, 0, stream>>>(d_text, n, hist.get());
+        unsigned long long hh[256];
+        DCX_HIP(hipMemcpyAsync(hh, hist.get(), sizeof(hh), hipMemcpyDeviceToHost, stream));
+        DCX_HIP(hipStreamSynchronize(stream));
+        uint16_t lut[256];
+        uint32_t next = 1;
+        for (int c = 0; c < 256; c++) lut[c] = hh[c] ? (uint16_t)next++ : 0;
+        dlut = DBuf<uint16_t>(cache, 256);
+        DCX_HIP(hipMemcpyAsync(dlut.get(), lut, sizeof(lut), hipMemcpyHostToDevice, stream));
+        return next - 1;
+    }
+
+    // -------------------------------------------------------------------------------------------
+    // batched mode (dcx_build_sa_batch): sub-batches of texts joined by a separator below every
+    // character, one DCX pipeline per sub-batch, then a stable radix partition by text
+    // -------------------------------------------------------------------------------------------
+    int build_sa_batch(const uint8_t* texts, const uint64_t* offsets, uint32_t count, const dcx_options* o,
+                       uint32_t* sa_out, dcx_stats* stats) {
+        auto t0 = std::chrono::steady_clock::now();
+        DCX_HIP(hipSetDevice(device));
+        if (!o || !offsets) throw Error(DCX_E_ARG, "null argument");
+        set_options(o);
+        st = stats;
+        if (st) memset(st, 0, sizeof(*st));
+        alg_bytes = 0;
+        sa_width = 4;
+        cache.reset_peak();
+        if (count == 0) return DCX_OK;
+        const uint64_t base = offsets[0], total = offsets[count] - offsets[0];
+        for (uint32_t k = 0; k < count; k++) {
+            if (offsets[k + 1] < offsets[k]) throw Error(DCX_E_ARG, "offsets must be non-decreasing");
+            if (offsets[k + 1] - offsets[k] >= (1ull << 32) - (1ull << 16))
+                throw Error(DCX_E_UNSUPPORTED, "a text of the batch is too long for 32-bit positions");
+        }
+        if (total && (!texts || !sa_out)) throw Error(DCX_E_ARG, "null argument");
+        uint64_t limit = o->batch_max_chars ? o->batch_max_chars : (1ull << 31);
+        if (limit > (1ull << 32) - (1ull << 17)) limit = (1ull << 32) - (1ull << 17);
+        if (budget) {
+            const uint64_t by_budget = budget > (64ull << 20) ? (budget - (64ull << 20)) / 64 : 0;
+            if (by_budget < limit) limit = by_budget;
+        }
+        prim.drop_profile();
+        prim.profile = o->profile_kernels != 0;
+        struct ProfGuard {
+            Primitives* p;
+            ~ProfGuard() {
+                p->profile = false;
+                p->drop_profile();
+            }
+        } profg{&prim};
+        hipEvent_t e0, e1;
+        DCX_HIP(hipEventCreate(&e0));
+        DCX_HIP(hipEventCreate(&e1));
+        struct EvGuard {
+            hipEvent_t a, b;
+            ~EvGuard() {
+                (void)hipEventDestroy(a);
+                (void)hipEventDestroy(b);
+            }
+        } evg{e0, e1};
+        // texts into HBM
+        DBuf<uint8_t> dtext;
+        const uint8_t* d_texts = texts ? texts + base : nullptr;
+        if (!o->text_on_device && total) {
+            dtext = DBuf<uint8_t>(cache, total + 16);
+            DCX_HIP(hipMemcpyAsync(dtext.get(), texts + base, total, hipMemcpyHostToDevice, stream));
+            d_texts = dtext.get();
+        }
+        DCX_HIP(hipEventRecord(e0, stream));
+        DBuf<uint16_t> dlut;
+        const uint32_t sig = alphabet(d_texts, total, dlut);
+        if (st) st->sigma = sig;
+        const uint32_t maxchar = sig + 1;  // characters 2..sig+1, separator 1
+        DBuf<uint16_t> ident;
+        if (maxchar <= 255) {
+            uint16_t id[256];
+            for (int c = 0; c < 256; c++) id[c] = (uint16_t)c;
+            ident = DBuf<uint16_t>(cache, 256);
+            DCX_HIP(hipMemcpyAsync(ident.get(), id, sizeof(id), hipMemcpyHostToDevice, stream));
+        }
+        uint32_t k0 = 0;
+        while (k0 < count) {
+            // greedy sub-batch: at least one text, at most `limit` joined characters
+            uint32_t k1 = k0 + 1;
+            uint64_t joined = offsets[k1] - offsets[k0] + 1;
+            while (k1 < count && joined + (offsets[k1 + 1] - offsets[k1]) + 1 <= limit) {
+                joined += offsets[k1 + 1] - offsets[k1] + 1;
+                k1++;
+            }
+            const uint32_t cnt = k1 - k0;
+            std::vector<uint64_t> in_off(cnt + 1), co(cnt + 1);
+            for (uint32_t k = 0; k <= cnt; k++) {
+                in_off[k] = offsets[k0 + k] - base;
+                co[k] = (offsets[k0 + k] - offsets[k0]) + k;
+            }
+            DBuf<uint64_t> d_in(cache, cnt + 1), d_co(cache, cnt + 1);
+            DCX_HIP(hipMemcpyAsync(d_in.get(), in_off.data(), (cnt + 1) * 8, hipMemcpyHostToDevice, stream));
+            DCX_HIP(hipMemcpyAsync(d_co.get(), co.data(), (cnt + 1) * 8, hipMemcpyHostToDevice, stream));
+            const uint32_t N = (uint32_t)joined;
+            const uint64_t chars = offsets[k1] - offsets[k0];
+            DBuf<uint32_t> sa(cache, N);
+            sigma = maxchar;
+            if (maxchar <= 255) {
+                DBuf<uint8_t> jt(cache, (size_t)N + 16);
+                if (chars)
+                    k_batch_text<uint8_t><<<grid(chars), 256, 0, stream>>>(d_texts, in_off[0], in_off[cnt], d_in.get(),
+                                                                            d_co.get(), cnt, dlut.get(), jt.get());
+                k_batch_seps<uint8_t><<<grid(cnt), 256, 0, stream>>>(d_co.get(), cnt, jt.get());
+                level<uint8_t>(jt.get(), ident.get(), N, maxchar, 0, sa.get(), nullptr);
+            } else {
+                DBuf<uint32_t> jt(cache, (size_t)N + 16);
+                if (chars)
+                    k_batch_text<uint32_t><<<grid(chars), 256, 0, stream>>>(d_texts, in_off[0], in_off[cnt],
+                                                                             d_in.get(), d_co.get(), cnt, dlut.get(),
+                                                                             jt.get());
+                k_batch_seps<uint32_t><<<grid(cnt), 256, 0, stream>>>(d_co.get(), cnt, jt.get());
+                level<uint32_t>(jt.get(), nullptr, N, maxchar, 0, sa.get(), nullptr);
+            }
+            // stable partition of the joined SA by text; separators sort last
+            DBuf<uint64_t> ka(cache, N), kb(cache, N);
+            DBuf<uint32_t> va(cache, N), vb(cache, N);
+            k_batch_keys<<<grid(N), 256, 0, stream>>>(sa.get(), N, d_co.get(), cnt, ka.get(), va.get());
+            sa.reset();
+            uint64_t* x = ka.get();
+            uint64_t* x2 = kb.get();
+            uint32_t* y = va.get();
+            uint32_t* y2 = vb.get();
+            prim.sort_pairs(x, x2, y, y2, N, 0, bit_width(cnt));
+            uint32_t* dst = sa_out + (offsets[k0] - base);
+            if (chars) {
+                if (o->sa_on_device)
+                    DCX_HIP(hipMemcpyAsync(dst, y, chars * 4, hipMemcpyDeviceToDevice, stream));
+                else
+                    DCX_HIP(hipMemcpyAsync(dst, y, chars * 4, hipMemcpyDeviceToHost, stream));
+            }
+            DCX_HIP(hipStreamSynchronize(stream));
+            k0 = k1;
+        }
+        DCX_HIP(hipEventRecord(e1, stream));
+        DCX_HIP(hipStreamSynchronize(stream));
+        if (st) {
+            float a = 0;
+            DCX_HIP(hipEventElapsedTime(&a, e0, e1));
+            st->ms_device = a;
+            st->alg_bytes = alg_bytes;
+            if (prim.profile) prim.collect_profile(&st->sort_passes, &st->sort_pass_items, &st->sort_pass_ms);
+            st->peak_device_bytes = cache.peak();
+            st->ms_total = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+        }
+        return DCX_OK;
+    }
+
+    // -------------------------------------------------------------------------------------------
+    int build_sa(const uint8_t* text, uint64_t n, const dcx_options* o, void* sa_out, uint32_t width, dcx_stats* stats) {
+        auto t0 = std::chrono::steady_clock::now();
+        DCX_HIP(hipSetDevice(device));
+        if (!o || (!text && n) || (!sa_out && n)) throw Error(DCX_E_ARG, "null argument");
+        if (width != 4 && width != 5 && width != 8) throw Error(DCX_E_ARG, "sa_width_bytes must be 4, 5 or 8");
+        set_options(o);
         if (n >= (1ull << 32) - (1ull << 16))
             throw Error(DCX_E_UNSUPPORTED, "texts of 2^32 - 2^16 characters or more are not supported by this build");
         if (width == 4 && n > (1ull << 32)) throw Error(DCX_E_WIDTH, "32-bit SA cannot index this text");
@@ -450,20 +611,9 @@ class Engine {
             d_text = dtext.get();
         }
         DCX_HIP(hipEventRecord(ev[1], stream));  // text resident
-        // phase 0: alphabet (compress_alphabet, cli.cpp:417-463): used bytes -> 1..sigma
-        DBuf<unsigned long long> hist(cache, 256);
-        DCX_HIP(hipMemsetAsync(hist.get(), 0, 256 * 8, stream));
-        k_byte_hist<<<grid(ceil_div(n, 16), 2048), 256, 0, stream>>>(d_text, n, hist.get());
-        unsigned long long hh[256];
-        DCX_HIP(hipMemcpyAsync(hh, hist.get(), sizeof(hh), hipMemcpyDeviceToHost, stream));
-        DCX_HIP(hipStreamSynchronize(stream));
-        uint16_t lut[256];
-        uint32_t next = 1;
-        for (int c = 0; c < 256; c++) lut[c] = hh[c] ? (uint16_t)next++ : 0;
-        sigma = next - 1;
+        DBuf<uint16_t> dlut;
+        sigma = alphabet(d_text, n, dlut);
         if (st) st->sigma = sigma;
-        DBuf<uint16_t> dlut(cache, 256);
-        DCX_HIP(hipMemcpyAsync(dlut.get(), lut, sizeof(lut), hipMemcpyHostToDevice, stream));
 
         DBuf<uint32_t> sa(cache, n);
         // phase boundaries default to "empty" (a base-case level never reaches them)
@@ -600,6 +750,23 @@ int dcx_build_sa(dcx_handle* h, const uint8_t* text, uint64_t n, const dcx_optio
     try {
         h->eng->last_error.clear();
         return h->eng->build_sa(text, n, opt, sa_out, sa_width_bytes, stats);
+    } catch (const Error& e) {
+        h->eng->last_error = e.what();
+        (void)hipStreamSynchronize(h->eng->stream);
+        (void)hipGetLastError();
+        return e.code;
+    } catch (const std::exception& e) {
+        h->eng->last_error = e.what();
+        return DCX_E_HIP;
+    }
+}
+
+int dcx_build_sa_batch(dcx_handle* h, const uint8_t* texts, const uint64_t* offsets, uint32_t count,
+                       const dcx_options* opt, uint32_t* sa_out, dcx_stats* stats) {
+    if (!h || !h->eng) return DCX_E_ARG;
+    try {
+        h->eng->last_error.clear();
+        return h->eng->build_sa_batch(texts, offsets, count, opt, sa_out, stats);
     } catch (const Error& e) {
         h->eng->last_error = e.what();
         (void)hipStreamSynchronize(h->eng->stream);

@@ -574,4 +574,51 @@ __global__ __launch_bounds__(256) void k_check_order(const uint8_t* __restrict__
     }
 }
 
+// ------------------------------------------------------------------------------------------------
+// batched mode: texts joined by a separator (value 1) below every character (values >= 2)
+// ------------------------------------------------------------------------------------------------
+// largest k in [0, count) with starts[k] <= pos
+__device__ __forceinline__ uint32_t find_owner(const uint64_t* __restrict__ starts, uint32_t count, uint64_t pos) {
+    uint32_t lo = 0, hi = count;
+    while (hi - lo > 1) {
+        uint32_t mid = (lo + hi) >> 1;
+        if (starts[mid] <= pos)
+            lo = mid;
+        else
+            hi = mid;
+    }
+    return lo;
+}
+
+template <class OutT>
+__global__ __launch_bounds__(256) void k_batch_text(const uint8_t* __restrict__ texts, uint64_t begin, uint64_t end,
+                                                    const uint64_t* __restrict__ in_off, const uint64_t* __restrict__ co,
+                                                    uint32_t count, const uint16_t* __restrict__ lut,
+                                                    OutT* __restrict__ out) {
+    for (uint64_t i = begin + blockIdx.x * 256ull + threadIdx.x; i < end; i += (uint64_t)gridDim.x * 256) {
+        uint32_t k = find_owner(in_off, count, i);
+        out[co[k] + (i - in_off[k])] = (OutT)(lut[texts[i]] + 1u);
+    }
+}
+
+template <class OutT>
+__global__ __launch_bounds__(256) void k_batch_seps(const uint64_t* __restrict__ co, uint32_t count,
+                                                    OutT* __restrict__ out) {
+    for (uint32_t k = blockIdx.x * 256 + threadIdx.x; k < count; k += gridDim.x * 256) out[co[k + 1] - 1] = (OutT)1;
+}
+
+// (text, local position) of every suffix of the joined text in SA order; separators go to bucket `count`
+__global__ __launch_bounds__(256) void k_batch_keys(const uint32_t* __restrict__ sa, uint32_t N,
+                                                    const uint64_t* __restrict__ co, uint32_t count,
+                                                    uint64_t* __restrict__ keys, uint32_t* __restrict__ vals) {
+    for (uint32_t j = blockIdx.x * 256 + threadIdx.x; j < N; j += gridDim.x * 256) {
+        uint32_t p = sa[j];
+        uint32_t k = find_owner(co, count, p);
+        uint32_t local = (uint32_t)(p - co[k]);
+        bool sep = (uint64_t)p == co[k + 1] - 1;
+        keys[j] = sep ? count : k;
+        vals[j] = local;
+    }
+}
+
 }  // namespace dcx

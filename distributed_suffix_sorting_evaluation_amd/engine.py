@@ -50,7 +50,8 @@ class Options(ctypes.Structure):
         ("discarding_threshold", ctypes.c_double),
         ("base_case_chars", ctypes.c_uint64),
         ("seed", ctypes.c_uint64),
-        ("reserved", ctypes.c_uint64 * 6),
+        ("batch_max_chars", ctypes.c_uint64),
+        ("reserved", ctypes.c_uint64 * 5),
     ]
 
 
@@ -106,8 +107,8 @@ class Stats(ctypes.Structure):
 _lib = None
 _lib_lock = threading.Lock()
 
-EXPORTED = ["dcx_default_options", "dcx_create", "dcx_destroy", "dcx_build_sa", "dcx_check_sa", "dcx_strerror",
-            "dcx_last_error"]
+EXPORTED = ["dcx_default_options", "dcx_create", "dcx_destroy", "dcx_build_sa", "dcx_build_sa_batch", "dcx_check_sa",
+            "dcx_strerror", "dcx_last_error"]
 
 
 def load_library(path: str = LIB_PATH):
@@ -130,6 +131,9 @@ def load_library(path: str = LIB_PATH):
         L.dcx_build_sa.argtypes = [vp, vp, ctypes.c_uint64, ctypes.POINTER(Options), vp, ctypes.c_uint32,
                                    ctypes.POINTER(Stats)]
         L.dcx_build_sa.restype = ctypes.c_int
+        L.dcx_build_sa_batch.argtypes = [vp, vp, ctypes.POINTER(ctypes.c_uint64), ctypes.c_uint32,
+                                         ctypes.POINTER(Options), vp, ctypes.POINTER(Stats)]
+        L.dcx_build_sa_batch.restype = ctypes.c_int
         L.dcx_check_sa.argtypes = [vp, vp, ctypes.c_uint64, vp, ctypes.c_uint32, ctypes.c_int,
                                    ctypes.POINTER(ctypes.c_int)]
         L.dcx_check_sa.restype = ctypes.c_int
@@ -227,6 +231,40 @@ class Engine:
         st = Stats()
         rc = self._L.dcx_build_sa(self._h, ctypes.c_void_p(text_ptr), n, ctypes.byref(o), ctypes.c_void_p(sa_ptr),
                                   sa_width, ctypes.byref(st))
+        if rc != DCX_OK:
+            self._raise(rc)
+        return st.to_dict()
+
+    def build_sa_batch(self, texts, stats: bool = False, batch_max_chars: int = 0, **kw):
+        """Per-text 32-bit SAs of a list of texts in one batched call; returns a list of uint32 arrays."""
+        arrs = [_as_u8(t) for t in texts]
+        offsets = np.zeros(len(arrs) + 1, dtype=np.uint64)
+        if arrs:
+            offsets[1:] = np.cumsum([a.size for a in arrs])
+        joined = np.concatenate(arrs) if arrs and offsets[-1] else np.zeros(1, dtype=np.uint8)
+        out = np.zeros(max(int(offsets[-1]), 1), dtype=np.uint32)
+        o = self.make_options(**kw)
+        o.batch_max_chars = int(batch_max_chars)
+        st = Stats()
+        rc = self._L.dcx_build_sa_batch(self._h, joined.ctypes.data, offsets.ctypes.data_as(ctypes.POINTER(ctypes.c_uint64)),
+                                        len(arrs), ctypes.byref(o), out.ctypes.data, ctypes.byref(st))
+        if rc != DCX_OK:
+            self._raise(rc)
+        res = [out[int(offsets[k]):int(offsets[k + 1])] for k in range(len(arrs))]
+        return (res, st.to_dict()) if stats else res
+
+    def build_sa_batch_device(self, texts_ptr: int, offsets: np.ndarray, sa_ptr: int, batch_max_chars: int = 0,
+                              **kw) -> dict:
+        """Batched SAs of device-resident joined texts (offsets on the host) into a device uint32 buffer."""
+        off = np.ascontiguousarray(offsets, dtype=np.uint64)
+        o = self.make_options(**kw)
+        o.text_on_device = 1
+        o.sa_on_device = 1
+        o.batch_max_chars = int(batch_max_chars)
+        st = Stats()
+        rc = self._L.dcx_build_sa_batch(self._h, ctypes.c_void_p(texts_ptr),
+                                        off.ctypes.data_as(ctypes.POINTER(ctypes.c_uint64)), off.size - 1,
+                                        ctypes.byref(o), ctypes.c_void_p(sa_ptr), ctypes.byref(st))
         if rc != DCX_OK:
             self._raise(rc)
         return st.to_dict()

@@ -48,7 +48,8 @@ typedef struct dcx_options {
     double discarding_threshold;  /* --discarding-threshold (0.7, config.hpp:42)             */
     uint64_t base_case_chars;     /* sequential base case: n <= max(2X, this) (pdcx.hpp:623)  */
     uint64_t seed;                /* --seed (accepted; the SA does not depend on it)         */
-    uint64_t reserved[6];
+    uint64_t batch_max_chars;     /* dcx_build_sa_batch: characters per sub-batch (0 = auto)  */
+    uint64_t reserved[5];
 } dcx_options;
 
 typedef struct dcx_level_stats {
@@ -106,6 +107,16 @@ int dcx_destroy(dcx_handle* h);
  * from different threads. */
 int dcx_build_sa(dcx_handle* h, const uint8_t* text, uint64_t n, const dcx_options* opt, void* sa_out,
                  uint32_t sa_width_bytes, dcx_stats* stats);
+
+/* Suffix arrays of `count` independent texts in one call (batched mode).  Text k is
+ * texts[offsets[k] .. offsets[k+1]) (offsets has count + 1 entries, host memory, non-decreasing, each text
+ * shorter than 2^32); its SA is written to sa_out[offsets[k] .. offsets[k+1]) as 32-bit positions local to
+ * the text.  The texts are joined with a separator symbol smaller than every character and run through one
+ * DCX pipeline per sub-batch (at most batch_max_chars characters); a stable radix partition by text then
+ * yields the per-text suffix arrays.  No counterpart in the reference: it builds one SA per cli run
+ * (cli.cpp:747-820); this entry point serves callers with many texts at once. */
+int dcx_build_sa_batch(dcx_handle* h, const uint8_t* texts, const uint64_t* offsets, uint32_t count,
+                       const dcx_options* opt, uint32_t* sa_out, dcx_stats* stats);
 
 /* Validity check of a suffix array on the device, after check_suffixarray
  * (lightweight_dcx/src/sa_check.hpp:56-184): permutation test plus non-decreasing
