@@ -55,6 +55,115 @@ def cpu_baseline(text, sample_bytes, X):
                       f"(oracle/dcx_oracle.c), DC{X} packed, {dt:.1f} s, host nproc={os.cpu_count()}"}
 
 
+def bench_batch(args):
+    """Batched mode: many independent texts, lengths log-uniform 1 KiB - 4 MiB over {A,C,G,T,N} (N at 0.1%),
+    per-text 32-bit SAs; the batched call against the plain per-text path on a sample of the same texts."""
+    import numpy as np
+    import torch
+    from distributed_suffix_sorting_evaluation_amd import Engine, textgen
+    count = args.batch_count
+    lens = textgen.lengths(count, 1024, 4 << 20, 4)
+    off = np.zeros(count + 1, dtype=np.uint64)
+    off[1:] = np.cumsum(lens)
+    total = int(off[-1])
+    log(f"[bench] batch: {count} texts, {total / 2 ** 30:.2f} GiB")
+    text = textgen.generate("acgtn", total, 4)
+    dt = torch.from_numpy(text).cuda()
+    dsa = torch.empty(total, dtype=torch.int32, device="cuda")
+    eng = Engine(0)
+    kw = dict(X=39, pack_samples=True, pack_merging=True)
+    for _ in range(args.warmup):
+        eng.build_sa_batch_device(dt.data_ptr(), off, dsa.data_ptr(), **kw)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        eng.build_sa_batch_device(dt.data_ptr(), off, dsa.data_ptr(), **kw)
+    torch.cuda.synchronize()
+    batched_s = (time.perf_counter() - t0) / args.steps
+    # plain path on an evenly spaced sample of the texts
+    sample = np.linspace(0, count - 1, min(count, args.plain_sample)).astype(np.int64)
+    out8 = torch.empty(int(lens.max()) * 4, dtype=torch.uint8, device="cuda")
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for k in sample:
+        a, b = int(off[k]), int(off[k + 1])
+        eng.build_sa_device(dt.data_ptr() + a, b - a, out8.data_ptr(), 4, **kw)
+    torch.cuda.synchronize()
+    plain_s = time.perf_counter() - t0
+    plain_rate = len(sample) / plain_s
+    sample_chars = int(sum(int(lens[k]) for k in sample))
+    # validate a few texts of the batched output on the device
+    valid = all(eng.check_sa_device(dt.data_ptr() + int(off[k]), int(lens[k]), dsa.data_ptr() + 4 * int(off[k]), 4)
+                for k in sample[:8])
+    eng.close()
+    rate = count / batched_s
+    return {"metric": METRIC, "value": round(rate, 1), "unit": "SAs/s", "n_gpus": 1, "steps": args.steps,
+            "warmup": args.warmup, "ms_per_step": round(batched_s * 1e3, 2), "higher_is_better": True,
+            "scaling": "weak", "vs_baseline": None, "dtype": "u8", "data": "synthetic",
+            "config": {"workload": "batch_acgtn", "texts": count, "total_bytes": total, "lengths": "log-uniform 1 KiB-4 MiB",
+                       "alphabet": "ACGT + N at 0.1%", "dcx": "dc39", "sa_width_bytes": 4},
+            "batched_mb_s": round(total / 1e6 / batched_s, 1),
+            "plain": {"texts_per_s": round(plain_rate, 1), "sample_texts": int(len(sample)),
+                      "mb_s": round(sample_chars / 1e6 / plain_s, 1)},
+            "batched_over_plain": round(rate / plain_rate, 2), "sa_valid_sample": valid}
+
+
+def bench_serve(args):
+    """Serving: Poisson arrivals, sizes log-uniform 64 KiB - 256 MiB, alphabet 4 or 256 (50/50), 80 GB
+    admission budget; continuous batching (2 workers) against the plain one-at-a-time path."""
+    import numpy as np
+    import torch
+    from distributed_suffix_sorting_evaluation_amd import textgen
+    from distributed_suffix_sorting_evaluation_amd.serve import (SAServer, log_uniform_sizes, poisson_arrivals,
+                                                                 run_open_loop)
+    count, rate = args.serve_count, args.serve_rate
+    sizes = log_uniform_sizes(count, 64 << 10, 256 << 20, 5)
+    rng = np.random.default_rng(5)
+    sig256 = rng.random(count) < 0.5
+    pool_n = 1 << 30
+    pools = {False: torch.from_numpy(textgen.generate("uniform_acgt", pool_n, 5)).cuda(),
+             True: torch.from_numpy(textgen.generate("uniform_sigma", pool_n, 6, sigma=256)).cuda()}
+    starts = [int(rng.integers(0, pool_n - int(s) + 1)) for s in sizes]
+    texts = [pools[bool(b)][o:o + int(s)] for b, o, s in zip(sig256, starts, sizes)]
+    arrivals = poisson_arrivals(count, rate, 5)
+    log(f"[bench] serve: {count} requests at {rate}/s, {int(sizes.sum()) / 2 ** 30:.2f} GiB offered")
+    out = {}
+    runs = (("continuous_batching", dict(workers=2, batching=True), arrivals),
+            ("plain", dict(workers=1, batching=False), arrivals),
+            ("continuous_batching_burst", dict(workers=2, batching=True), np.zeros(count)),
+            ("plain_burst", dict(workers=1, batching=False), np.zeros(count)))
+    for name, kwargs, arr in runs:
+        srv = SAServer(hbm_budget=80 * 10 ** 9, X=39, **kwargs)
+        try:
+            res = run_open_loop(srv, texts, arr, keep_results=True)
+            reqs = res.pop("requests_list")
+            from distributed_suffix_sorting_evaluation_amd import Engine
+            with Engine(0) as chk:
+                picks = [r for r in reqs if r.error == 0][:: max(1, count // 6)]
+                res["sa_valid_sample"] = all(chk.check_sa_device(r.text.data_ptr(), r.n, r.sa.data_ptr(), 4)
+                                             for r in picks)
+            for r in reqs:
+                r.sa = None
+        finally:
+            srv.close()
+        torch.cuda.synchronize()
+        out[name] = {k: (round(v, 3) if isinstance(v, float) else v) for k, v in res.items()}
+        log(f"[bench] {name}: {out[name]}")
+    cb, pl = out["continuous_batching"], out["plain"]
+    return {"metric": METRIC, "value": cb["texts_per_s"], "unit": "texts/s", "n_gpus": 1, "steps": 1, "warmup": 0,
+            "ms_per_step": round(cb["wall_s"] * 1e3, 1), "higher_is_better": True, "scaling": "weak",
+            "vs_baseline": None, "dtype": "u8", "data": "synthetic",
+            "config": {"workload": "serve_poisson", "requests": count, "rate_per_s": rate,
+                       "sizes": "log-uniform 64 KiB-256 MiB", "alphabet": "4 or 256 (50/50)", "hbm_budget_bytes": 80 * 10 ** 9,
+                       "dcx": "dc39", "sa_width_bytes": 4},
+            "p50_ms": cb["p50_ms"], "p99_ms": cb["p99_ms"], "continuous_batching": cb, "plain": pl,
+            "burst": {"continuous_batching": out["continuous_batching_burst"], "plain": out["plain_burst"]},
+            "texts_per_s_over_plain": round(cb["texts_per_s"] / max(pl["texts_per_s"], 1e-9), 2),
+            "burst_texts_per_s_over_plain": round(out["continuous_batching_burst"]["texts_per_s"] /
+                                                  max(out["plain_burst"]["texts_per_s"], 1e-9), 2),
+            "p99_over_plain": round(cb["p99_ms"] / max(pl["p99_ms"], 1e-9), 3)}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -65,6 +174,10 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-validate", action="store_true")
     ap.add_argument("--host-to-host", action="store_true", help="also time one PCIe-inclusive run")
+    ap.add_argument("--batch-count", type=int, default=10000)
+    ap.add_argument("--plain-sample", type=int, default=500)
+    ap.add_argument("--serve-count", type=int, default=400)
+    ap.add_argument("--serve-rate", type=float, default=200.0)
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a 1 GiB device-to-device copy three times (reference kernel for the "
                          "FETCH_SIZE / WRITE_SIZE counter scale)")
@@ -93,6 +206,13 @@ def main():
     if not torch.cuda.is_available():
         print(json.dumps({"error": "no HIP device"}))
         sys.exit(1)
+
+    if args.workload == "batch_acgtn":
+        print(json.dumps(bench_batch(args)), flush=True)
+        return
+    if args.workload == "serve_poisson":
+        print(json.dumps(bench_serve(args)), flush=True)
+        return
 
     wl = textgen.WORKLOADS[args.workload]
     n, X, width = wl["n"], wl["X"], wl["sa_width"]

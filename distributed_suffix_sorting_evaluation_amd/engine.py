@@ -120,6 +120,12 @@ def load_library(path: str = LIB_PATH):
         if not os.path.exists(path):
             raise ImportError(f"{path} is missing: build the HIP engine first (python -m "
                               "distributed_suffix_sorting_evaluation_amd.build)")
+        # One HIP runtime per process: PyTorch-ROCm ships its own libamdhip64 (same soname).  Loading torch
+        # first makes libdcx bind to that copy; loading the system copy first would leave torch without GPUs.
+        try:
+            import torch  # noqa: F401
+        except ImportError:
+            pass
         L = ctypes.CDLL(path)
         vp = ctypes.c_void_p
         L.dcx_default_options.argtypes = [ctypes.POINTER(Options)]

@@ -1,0 +1,241 @@
+"""Serving loop for suffix-array build requests on one MI355X.
+
+Continuous batching: requests arrive at any time and wait in a FIFO queue.  Each worker owns an engine handle
+(its own HIP stream and device-memory cache).  A free worker admits queued requests in arrival order for as long
+as their estimated HBM footprint fits the budget that is still free, joins their texts on the device and builds
+all their suffix arrays with one ``dcx_build_sa_batch`` call.  Requests that arrive while batches run are
+admitted by the next free worker; with two workers the batches of both overlap on the device.  Results are the
+per-text 32-bit suffix arrays, bit-identical to building each text alone (the batched call is exact).
+
+The reference has no serving layer (it builds one SA per ``mpiexec`` run, lightweight_dcx/src/executables/cli.cpp);
+this module serves callers with a stream of requests.
+"""
+from __future__ import annotations
+
+import math
+import threading
+import time
+from dataclasses import dataclass, field
+
+import numpy as np
+
+from .engine import DCX_E_ADMISSION, DCX_E_CANCELLED, DCXError, Engine
+
+# device bytes per text character while a request is in flight: engine working set (see the engine's
+# estimate) plus the 4-byte SA and the joined copy of the text
+BYTES_PER_CHAR = 64 + 4 + 1
+FIXED_BYTES = 64 << 20
+
+
+def footprint(n: int) -> int:
+    return FIXED_BYTES + BYTES_PER_CHAR * int(n)
+
+
+@dataclass(eq=False)  # identity semantics: requests are compared and removed by object
+class Request:
+    text: "object"                  # torch.uint8 CUDA tensor
+    rid: int = 0
+    arrival: float = 0.0
+    admitted: float = math.nan
+    done: float = math.nan
+    sa: "object" = None             # torch.int32 CUDA tensor (uint32 positions) when finished
+    error: int = 0
+    sigma_class: int = -1           # packing width class of the alphabet (3/5/8/9-bit characters)
+    batch: int = -1
+    event: threading.Event = field(default_factory=threading.Event)
+
+    @property
+    def n(self) -> int:
+        return int(self.text.numel())
+
+    @property
+    def latency(self) -> float:
+        return self.done - self.arrival
+
+
+class SAServer:
+    """Continuous-batching server over ``workers`` engine handles."""
+
+    def __init__(self, workers: int = 2, hbm_budget: int = 80 * 10 ** 9, max_batch_texts: int = 1024,
+                 max_batch_chars: int = 64 << 20, solo_chars: int = 4 << 20, device: int = 0,
+                 batching: bool = True, **build_options):
+        import torch
+        self.torch = torch
+        self.budget = int(hbm_budget)
+        self.free = int(hbm_budget)
+        self.max_texts = max_batch_texts if batching else 1
+        self.max_chars = int(max_batch_chars)
+        self.solo_chars = int(solo_chars)  # requests this large already fill the GPU: built alone
+        self.batching = batching
+        self.opts = dict(build_options)
+        self.queue: list[Request] = []
+        self.cv = threading.Condition()
+        self.stop = False
+        self.next_id = 0
+        self.batches = 0
+        self.engines = [Engine(device) for _ in range(workers)]
+        self.threads = [threading.Thread(target=self._worker, args=(i,), daemon=True) for i in range(workers)]
+        for t in self.threads:
+            t.start()
+
+    # -- client side ---------------------------------------------------------------------------------
+    def submit(self, text) -> Request:
+        r = text if isinstance(text, Request) else Request(text=text)
+        with self.cv:
+            r.rid = self.next_id
+            self.next_id += 1
+            if not r.arrival:
+                r.arrival = time.perf_counter()
+            if self.batching and r.sigma_class < 0 and r.n < self.solo_chars:
+                r.sigma_class = self._sigma_class(r.text)
+            if footprint(r.n) > self.budget:
+                r.error = DCX_E_ADMISSION
+                r.done = time.perf_counter()
+                r.event.set()
+                return r
+            self.queue.append(r)
+            self.cv.notify_all()
+        return r
+
+    def _sigma_class(self, text) -> int:
+        # texts are only batched with texts of the same packing width: the joined alphabet decides the
+        # width of every character of the batch
+        torch = self.torch
+        if text.numel() == 0:
+            return 0
+        sigma = int((torch.bincount(text, minlength=256) > 0).sum())
+        return 0 if sigma <= 6 else 1 if sigma <= 30 else 2 if sigma <= 254 else 3
+
+    def cancel(self, r: Request) -> bool:
+        """Cancels a request that has not been admitted yet."""
+        with self.cv:
+            if r in self.queue:
+                self.queue.remove(r)
+                r.error = DCX_E_CANCELLED
+                r.done = time.perf_counter()
+                r.event.set()
+                return True
+        return False
+
+    def close(self):
+        with self.cv:
+            self.stop = True
+            self.cv.notify_all()
+        for t in self.threads:
+            t.join()
+        for e in self.engines:
+            e.close()
+
+    # -- workers -------------------------------------------------------------------------------------
+    def _take(self) -> list[Request]:
+        """Admits the queue head; a small head is joined by later small requests of the same alphabet class,
+        in arrival order, while their footprint fits the free budget and the batch limits."""
+        head = self.queue[0]
+        fp = footprint(head.n)
+        if fp > self.free:
+            return []
+        batch, chars, need = [head], head.n + 1, fp
+        if self.batching and head.n < self.solo_chars:
+            for r in self.queue[1:]:
+                if len(batch) >= self.max_texts:
+                    break
+                if r.n >= self.solo_chars or r.sigma_class != head.sigma_class:
+                    continue
+                fr = footprint(r.n)
+                if chars + r.n + 1 > self.max_chars or need + fr > self.free:
+                    continue
+                batch.append(r)
+                chars += r.n + 1
+                need += fr
+        for r in batch:
+            self.queue.remove(r)
+        self.free -= need
+        return batch
+
+    def _worker(self, wid: int):
+        torch = self.torch
+        eng = self.engines[wid]
+        stream = torch.cuda.Stream()
+        while True:
+            with self.cv:
+                while not self.stop and not (self.queue and footprint(self.queue[0].n) <= self.free):
+                    self.cv.wait(0.05)
+                if self.stop and not self.queue:
+                    return
+                batch = self._take()
+                bid = self.batches
+                self.batches += 1
+            if not batch:
+                continue
+            now = time.perf_counter()
+            for r in batch:
+                r.admitted = now
+                r.batch = bid
+            err = 0
+            try:
+                with torch.cuda.stream(stream):
+                    if self.batching and len(batch) > 1:
+                        joined = torch.cat([r.text for r in batch])
+                        total = int(joined.numel())
+                        out = torch.empty(max(total, 1), dtype=torch.int32, device=joined.device)
+                        offsets = np.zeros(len(batch) + 1, dtype=np.uint64)
+                        offsets[1:] = np.cumsum([r.n for r in batch])
+                        stream.synchronize()
+                        eng.build_sa_batch_device(joined.data_ptr(), offsets, out.data_ptr(), **self.opts)
+                        for k, r in enumerate(batch):
+                            r.sa = out[int(offsets[k]):int(offsets[k + 1])]
+                    else:
+                        r = batch[0]
+                        out = torch.empty(max(r.n, 1), dtype=torch.int32, device=r.text.device)
+                        stream.synchronize()
+                        eng.build_sa_device(r.text.data_ptr(), r.n, out.data_ptr(), 4, **self.opts)
+                        r.sa = out[:r.n]
+            except DCXError as e:
+                err = e.code
+            done = time.perf_counter()
+            with self.cv:
+                self.free += sum(footprint(r.n) for r in batch)
+                self.cv.notify_all()
+            for r in batch:
+                r.error = err
+                r.done = done
+                r.event.set()
+
+
+def poisson_arrivals(count: int, rate: float, seed: int) -> np.ndarray:
+    rng = np.random.default_rng(seed)
+    return np.cumsum(rng.exponential(1.0 / rate, count))
+
+
+def log_uniform_sizes(count: int, lo: int, hi: int, seed: int) -> np.ndarray:
+    rng = np.random.default_rng(seed + 1)
+    return np.exp(rng.uniform(math.log(lo), math.log(hi + 1), count)).astype(np.int64).clip(lo, hi)
+
+
+def run_open_loop(server: SAServer, texts: list, arrivals: np.ndarray, keep_results: bool = True) -> dict:
+    """Submits texts at the given arrival offsets (seconds) and waits for all; returns latency statistics."""
+    t0 = time.perf_counter()
+    reqs = []
+    for text, at in zip(texts, arrivals):
+        delay = t0 + at - time.perf_counter()
+        if delay > 0:
+            time.sleep(delay)
+        r = Request(text=text, arrival=t0 + at)
+        reqs.append(server.submit(r))
+    for r in reqs:
+        r.event.wait()
+    t1 = time.perf_counter()
+    ok = [r for r in reqs if r.error == 0]
+    lat = np.array([r.latency for r in ok]) if ok else np.zeros(1)
+    chars = sum(r.n for r in ok)
+    res = {
+        "requests": len(reqs), "completed": len(ok), "rejected": sum(1 for r in reqs if r.error),
+        "wall_s": t1 - t0, "texts_per_s": len(ok) / (t1 - t0), "mb_per_s": chars / 1e6 / (t1 - t0),
+        "p50_ms": float(np.percentile(lat, 50) * 1e3), "p99_ms": float(np.percentile(lat, 99) * 1e3),
+        "mean_ms": float(lat.mean() * 1e3), "batches": server.batches,
+    }
+    if not keep_results:
+        for r in reqs:
+            r.sa = None
+    res["requests_list"] = reqs
+    return res

@@ -93,3 +93,15 @@ def test_uint40_decode():
     import oracle
     sa = np.array([0, 7, 2 ** 32 + 9, 2 ** 40 - 1], dtype=np.uint64)
     assert np.array_equal(uint40_to_u64(np.frombuffer(oracle.uint40_bytes(sa), dtype=np.uint8)), sa)
+
+
+def test_serving_workload_generators():
+    from distributed_suffix_sorting_evaluation_amd.serve import footprint, log_uniform_sizes, poisson_arrivals
+    a = poisson_arrivals(20000, 200.0, 5)
+    assert np.all(np.diff(a) > 0)
+    assert abs(a[-1] / 20000 - 1 / 200.0) < 0.1 / 200.0
+    s = log_uniform_sizes(20000, 64 << 10, 256 << 20, 5)
+    assert s.min() >= 64 << 10 and s.max() <= 256 << 20
+    # log-uniform: the median is near the geometric mean of the bounds
+    assert abs(np.log(np.median(s)) - 0.5 * (np.log(64 << 10) + np.log(256 << 20))) < 0.1
+    assert footprint(10) < footprint(11)
