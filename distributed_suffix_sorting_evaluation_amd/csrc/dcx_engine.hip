@@ -85,6 +85,7 @@ class Engine {
         uint32_t c = 0;
         for (uint32_t r = 0; r < d.X; r++) {
             d.next_rank[r] = (uint8_t)c;
+            d.in_dc[r] = in[r];
             c += in[r];
         }
         // samples are positions p in [0, n] with p mod X in DC (compute_length_info, pdcx.hpp:473-510)
@@ -122,13 +123,12 @@ class Engine {
     // element values in sorted order, H[j] = 1 where a group of equal L-character keys starts (H[N] = 1)
     // and `fin` lists the members of the groups that are still tied.
     // -------------------------------------------------------------------------------------------
-    template <int MODE>
-    void refine_sort(const uint64_t* W, const DcDev& dc, uint32_t N, uint32_t L, uint32_t bits, DBuf<uint32_t>& order,
+    void refine_sort(const uint64_t* W, uint32_t N, uint32_t L, uint32_t bits, DBuf<uint32_t>& order,
                      DBuf<uint8_t>& H, ActiveSet& fin) {
         const uint32_t cpw = 64 / bits;
         uint32_t c0 = L < cpw ? L : cpw;
-        if (MODE == 1 && N >= (1u << 24)) {
-            // large merges: round 0 takes about log2(N) + 16 bits in whole 8-bit digits; the refinement
+        if (N >= (1u << 24)) {
+            // large sorts: round 0 takes about log2(N) + 16 bits in whole 8-bit digits; the refinement
             // rounds resolve the (few) positions still tied
             const uint32_t want = std::min<uint32_t>(64, (bit_width(N) + 16 + 7) / 8 * 8);
             c0 = std::max<uint32_t>(1, std::min<uint32_t>(c0, want / bits));
@@ -140,13 +140,8 @@ class Engine {
             uint64_t* k2 = kb.get();
             uint32_t* v = va.get();
             uint32_t* v2 = vb.get();
-            if (MODE == 1) {
-                // every position is an element: the first pass reads the packed words directly
-                prim.sort_pairs(k, k2, v, v2, N, 64 - c0 * bits, 64, W, true);
-            } else {
-                k_keys_r0<MODE><<<grid(N), 256, 0, stream>>>(W, dc, N, c0 * bits, ka.get(), va.get());
-                prim.sort_pairs(k, k2, v, v2, N, 64 - c0 * bits, 64);
-            }
+            // every position is an element: the first pass reads the packed words directly
+            prim.sort_pairs(k, k2, v, v2, N, 64 - c0 * bits, 64, W, true);
             H = DBuf<uint8_t>(cache, (size_t)N + 1);
             k_heads<<<grid(N + 1ull), 256, 0, stream>>>(k, N, c0 * bits, H.get());
             if (v == va.get())
@@ -167,8 +162,8 @@ class Engine {
             {
                 DBuf<uint64_t> rka(cache, A), rkb(cache, A);
                 DBuf<uint32_t> rva(cache, A), rvb(cache, A);
-                k_keys_rr<MODE><<<grid(A), 256, 0, stream>>>(W, dc, A, cur.act.get(), cur.gid.get(), order.get(), C,
-                                                             c * bits, gbits, rka.get(), rva.get());
+                k_keys_rr<<<grid(A), 256, 0, stream>>>(W, A, cur.act.get(), cur.gid.get(), order.get(), C, c * bits,
+                                                       gbits, rka.get(), rva.get());
                 uint64_t* x = rka.get();
                 uint64_t* x2 = rkb.get();
                 uint32_t* y = rva.get();
@@ -182,6 +177,40 @@ class Engine {
             cur = std::move(next);
         }
         fin = std::move(cur);
+    }
+
+    // dense words only: boundary suffixes that share a key group are moved to its front (k_boundary_fix)
+    void boundary_fixup(uint32_t bnd, DBuf<uint32_t>& order, DBuf<uint8_t>& H, ActiveSet& tied) {
+        DBuf<uint32_t> found(cache, BOUNDARY_CAP), cnt(cache, 1);
+        prim.fill_u32(cnt.get(), 0, 1);
+        k_boundary_collect<<<grid(tied.A), 256, 0, stream>>>(tied.act.get(), tied.gid.get(), order.get(), tied.A, bnd,
+                                                             found.get(), cnt.get(), BOUNDARY_CAP);
+        uint32_t hc = 0;
+        DCX_HIP(hipMemcpyAsync(&hc, cnt.get(), 4, hipMemcpyDeviceToHost, stream));
+        DCX_HIP(hipStreamSynchronize(stream));
+        if (hc == 0) return;
+        if (hc > BOUNDARY_CAP) throw Error(DCX_E_HIP, "boundary suffix count exceeds its bound");
+        std::vector<uint32_t> g(hc);
+        DCX_HIP(hipMemcpyAsync(g.data(), found.get(), hc * 4, hipMemcpyDeviceToHost, stream));
+        DBuf<uint32_t> gstart(cache, (size_t)tied.G + 1);
+        k_group_starts<<<grid(tied.A), 256, 0, stream>>>(tied.act.get(), tied.gid.get(), H.get(), tied.A, gstart.get());
+        prim.fill_u32(gstart.get() + tied.G, tied.A, 1);
+        DCX_HIP(hipStreamSynchronize(stream));
+        std::sort(g.begin(), g.end());
+        g.erase(std::unique(g.begin(), g.end()), g.end());
+        for (uint32_t gi : g) {
+            uint32_t ab[2], j0 = 0;
+            DCX_HIP(hipMemcpyAsync(ab, gstart.get() + gi, 8, hipMemcpyDeviceToHost, stream));
+            DCX_HIP(hipStreamSynchronize(stream));
+            DCX_HIP(hipMemcpyAsync(&j0, tied.act.get() + ab[0], 4, hipMemcpyDeviceToHost, stream));
+            DCX_HIP(hipStreamSynchronize(stream));
+            const uint32_t sz = ab[1] - ab[0];
+            DBuf<uint32_t> tmp(cache, sz);
+            k_boundary_fix<<<1, 1024, 0, stream>>>(order.get(), H.get(), j0, sz, bnd, tmp.get());
+        }
+        ActiveSet next;
+        active_set(H.get(), tied.act.get(), tied.A, next);
+        tied = std::move(next);
     }
 
     // groups of equal merge prefixes -> DC comparator on the sample ranks (tie_break_ranks,
@@ -280,7 +309,7 @@ class Engine {
         }
         const uint32_t Ls = (uint32_t)((double)X * (ps ? ratio : 1.0));
         const uint32_t Lm = (uint32_t)((double)X * (pm ? ratio : 1.0)) - 1;
-        dc.bnd = n - Lm + 1;
+        dc.bnd = n - Ls + 1;
         if (ls) {
             ls->samples = m;
             ls->key_chars = Ls;
@@ -295,33 +324,49 @@ class Engine {
             alg_bytes += (uint64_t)n * c + 2ull * m * (K1 + I) + 2ull * n * (K4 + I) + (uint64_t)n * S;
         }
 
-        // packed words of the whole level text, sentinel-coded (character 0 = padding) for the samples
-        const uint64_t wtotal = (uint64_t)n + (Ls > Lm ? Ls : Lm) + 2;
-        DBuf<uint64_t> W(cache, wtotal);
-        make_words(text, lut, n, bits, 0, wtotal, W.get());
+        // ---------------- phase 1 + 2: all suffixes sorted by their first L_s characters ----------------
+        // One radix refinement over every position yields both the sample names (the samples in this order,
+        // grouped by equal keys) and the key groups that the merge resolves with the DC comparator.  Words
+        // are dense-coded (character c as c - 1) when that saves a bit per character.
+        DBuf<uint32_t> order;
+        DBuf<uint8_t> H;
+        ActiveSet tied;
+        {
+            const uint64_t wtotal = (uint64_t)n + Ls + 2;
+            DBuf<uint64_t> W(cache, wtotal);
+            make_words(text, lut, n, mbits, dense ? 1 : 0, wtotal, W.get());
+            refine_sort(W.get(), n, Ls, mbits, order, H, tied);
+        }
+        if (dense && tied.A > 0) boundary_fixup(dc.bnd, order, H, tied);
 
-        // ---------------- phase 1 + 2: sample sort and names ----------------
         DBuf<uint32_t> ranks(cache, (size_t)m + D);
         prim.fill_u32(ranks.get() + m, 0, D);
         {
-            DBuf<uint32_t> order;
-            DBuf<uint8_t> H;
-            ActiveSet tied;
-            refine_sort<0>(W.get(), dc, m, Ls, bits, order, H, tied);
+            // samples in sorted order; the dummy sample (position n, key of padding only) is the smallest
+            const bool dummy = dc.in_dc[n % X] != 0;
+            DBuf<uint32_t> s_order(cache, m), s_gid(cache, m);
+            if (dummy) {
+                prim.fill_u32(s_order.get(), m - 1, 1);
+                prim.fill_u32(s_gid.get(), 0, 1);
+            }
+            prim.scan(SampleScanOp{order.get(), H.get(), dc, dummy ? 1u : 0u, s_order.get(), s_gid.get()}, n, 0);
+            DBuf<uint8_t> Hs(cache, (size_t)m + 1);
+            k_sample_heads<<<grid(m + 1ull), 256, 0, stream>>>(s_gid.get(), m, Hs.get());
+            s_gid.reset();
             DBuf<uint32_t> names(cache, m);
-            prim.scan(NamesOp{H.get(), names.get()}, m, 0);
+            prim.scan(NamesOp{Hs.get(), names.get()}, m, 0);
             const uint32_t M = (uint32_t)prim.read_scalars(1)[0];
             if (ls) ls->names = M;
             if (ev && depth == 0) DCX_HIP(hipEventRecord(ev[2], stream));
 
             // ---------------- phase 3: recursion ----------------
             if (M >= m) {
-                k_rank_from_order<<<grid(m), 256, 0, stream>>>(order.get(), m, ranks.get());
+                k_rank_from_order<<<grid(m), 256, 0, stream>>>(s_order.get(), m, ranks.get());
             } else {
                 if (ls) ls->recursed = 1;
                 if (M >= UNIQ_BIT) throw Error(DCX_E_UNSUPPORTED, "more than 2^31 distinct sample names");
                 DBuf<uint32_t> R(cache, m);
-                k_build_rec<<<grid(m), 256, 0, stream>>>(order.get(), names.get(), H.get(), m, dc, R.get());
+                k_build_rec<<<grid(m), 256, 0, stream>>>(s_order.get(), names.get(), Hs.get(), m, dc, R.get());
                 names.reset();
                 DBuf<uint32_t> Rr(cache, m), red(cache, m);
                 prim.scan(KeepOp{R.get(), Rr.get(), red.get()}, m, 0);
@@ -349,41 +394,33 @@ class Engine {
                     sar.reset();
                     Rr.reset();
                     // order of (name, rank2) = sorted-by-name order refined inside non-unique groups
-                    const uint32_t A = tied.A, G = tied.G;
+                    ActiveSet stied;
+                    active_set(Hs.get(), nullptr, m, stied);
+                    const uint32_t A = stied.A, G = stied.G;
                     if (A > 0) {
                         const uint32_t gb = bit_width(G - 1), r2b = bit_width(after);
                         DBuf<uint64_t> ka(cache, A), kb(cache, A);
                         DBuf<uint32_t> va(cache, A), vb(cache, A);
-                        k_keys_rank2<<<grid(A), 256, 0, stream>>>(A, tied.act.get(), tied.gid.get(), order.get(), dc,
-                                                                  red.get(), isar.get(), r2b, ka.get(), va.get());
+                        k_keys_rank2<<<grid(A), 256, 0, stream>>>(A, stied.act.get(), stied.gid.get(), s_order.get(),
+                                                                  dc, red.get(), isar.get(), r2b, ka.get(), va.get());
                         uint64_t* x = ka.get();
                         uint64_t* x2 = kb.get();
                         uint32_t* y = va.get();
                         uint32_t* y2 = vb.get();
                         prim.sort_pairs(x, x2, y, y2, A, 0, gb + r2b);
-                        k_writeback<<<grid(A), 256, 0, stream>>>(x, y, tied.act.get(), A, order.get(), H.get(), 0);
+                        k_writeback<<<grid(A), 256, 0, stream>>>(x, y, stied.act.get(), A, s_order.get(), Hs.get(), 0);
                     }
-                    k_rank_from_order<<<grid(m), 256, 0, stream>>>(order.get(), m, ranks.get());
+                    k_rank_from_order<<<grid(m), 256, 0, stream>>>(s_order.get(), m, ranks.get());
                 }
             }
         }
         if (ev && depth == 0) DCX_HIP(hipEventRecord(ev[3], stream));
 
         // ---------------- phase 4: merge ----------------
-        {
-            DBuf<uint32_t> order;
-            DBuf<uint8_t> H;
-            ActiveSet tied;
-            if (dense) {
-                W.reset();
-                W = DBuf<uint64_t>(cache, wtotal);
-                make_words(text, lut, n, mbits, 1, wtotal, W.get());
-            }
-            refine_sort<1>(W.get(), dc, n, Lm, mbits, order, H, tied);
-            W.reset();
-            tie_break(dc, order, H, tied, ranks.get());
-            DCX_HIP(hipMemcpyAsync(sa_out, order.get(), (size_t)n * 4, hipMemcpyDeviceToDevice, stream));
-        }
+        // suffixes with different L_s-character keys are already in order; groups of equal keys are ordered
+        // by the DC comparator on the sample ranks
+        tie_break(dc, order, H, tied, ranks.get());
+        DCX_HIP(hipMemcpyAsync(sa_out, order.get(), (size_t)n * 4, hipMemcpyDeviceToDevice, stream));
     }
 
     // -------------------------------------------------------------------------------------------

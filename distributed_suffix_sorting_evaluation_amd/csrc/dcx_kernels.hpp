@@ -62,7 +62,8 @@ struct DcDev {
     uint32_t dc[MAX_D];
     uint32_t sb[MAX_D + 1];     // samples_before (pdcx.hpp:499-508): class-major offsets
     uint8_t next_rank[MAX_X + 3];  // #{d in DC : d < r}  (difference_cover.hpp NEXT_RANK)
-    uint32_t bnd;                  // first position whose merge window reaches past the text end
+    uint8_t in_dc[MAX_X + 3];      // r in DC (difference_cover.hpp IN_DC)
+    uint32_t bnd;                  // first position whose sample-key window reaches past the text end
 };
 
 // k-th sample in text order (k = rank_pos of the sample, merge_samples.hpp:181-188)
@@ -169,16 +170,6 @@ __device__ __forceinline__ uint64_t top_bits(uint64_t w, uint32_t nbits) {
     return nbits >= 64 ? w : (nbits == 0 ? 0ull : (w & (~0ull << (64 - nbits))));
 }
 
-template <int MODE>
-__global__ __launch_bounds__(256) void k_keys_r0(const uint64_t* __restrict__ W, DcDev dc, uint32_t N, uint32_t kbits,
-                                                 uint64_t* __restrict__ keys, uint32_t* __restrict__ vals) {
-    for (uint32_t j = blockIdx.x * 256 + threadIdx.x; j < N; j += gridDim.x * 256) {
-        uint32_t p = MODE == 0 ? sample_pos(dc, j) : j;
-        keys[j] = top_bits(W[p], kbits);
-        vals[j] = j;
-    }
-}
-
 // group heads of sorted keys, comparing only the top kbits
 __global__ __launch_bounds__(256) void k_heads(const uint64_t* __restrict__ keys, uint32_t N, uint32_t kbits,
                                                uint8_t* __restrict__ H) {
@@ -209,17 +200,15 @@ struct ActiveScanOp {
     }
 };
 
-template <int MODE>
-__global__ __launch_bounds__(256) void k_keys_rr(const uint64_t* __restrict__ W, DcDev dc, uint32_t A,
+__global__ __launch_bounds__(256) void k_keys_rr(const uint64_t* __restrict__ W, uint32_t A,
                                                  const uint32_t* __restrict__ act_idx, const uint32_t* __restrict__ gid,
                                                  const uint32_t* __restrict__ order, uint32_t C, uint32_t cbits,
                                                  uint32_t gbits, uint64_t* __restrict__ keys, uint32_t* __restrict__ vals) {
     for (uint32_t a = blockIdx.x * 256 + threadIdx.x; a < A; a += gridDim.x * 256) {
-        uint32_t e = order[act_idx[a]];
-        uint32_t p = MODE == 0 ? sample_pos(dc, e) : e;
+        uint32_t p = order[act_idx[a]];
         uint64_t ch = top_bits(W[(uint64_t)p + C], cbits);
         keys[a] = gbits ? (((uint64_t)gid[a] << (64 - gbits)) | (ch >> gbits)) : ch;
-        vals[a] = e;
+        vals[a] = p;
     }
 }
 
@@ -247,6 +236,107 @@ struct NamesOp {
 __global__ __launch_bounds__(256) void k_rank_from_order(const uint32_t* __restrict__ order, uint32_t m,
                                                          uint32_t* __restrict__ ranks) {
     for (uint32_t j = blockIdx.x * 256 + threadIdx.x; j < m; j += gridDim.x * 256) ranks[order[j]] = j + 1;
+}
+
+// ------------------------------------------------------------------------------------------------
+// Phase 1+2 from the sort of all suffixes by their first L_s characters: the samples, in that order,
+// with the 1-based id of their key group (sample_string.hpp:145-164 and compute_ranks.hpp:88-111 compute
+// the same names from a separate sort of the sample keys).  `shift` leaves slot 0 to the dummy sample.
+// ------------------------------------------------------------------------------------------------
+struct SampleScanOp {
+    const uint32_t* order;
+    const uint8_t* H;
+    DcDev dc;
+    uint32_t shift;
+    uint32_t* s_order;
+    uint32_t* s_gid;
+    __device__ __forceinline__ uint64_t value(uint64_t j) const {
+        const uint32_t p = order[j];
+        const uint32_t q = p / dc.X;
+        return (uint64_t)dc.in_dc[p - q * dc.X] | ((uint64_t)H[j] << 32);
+    }
+    __device__ __forceinline__ void write(uint64_t j, uint64_t excl, uint64_t v) const {
+        if (v & 1u) {
+            const uint32_t p = order[j];
+            const uint32_t q = p / dc.X;
+            const uint32_t t = (uint32_t)excl + shift;
+            s_order[t] = q * dc.D + dc.next_rank[p - q * dc.X];
+            s_gid[t] = (uint32_t)(excl >> 32) + (uint32_t)(v >> 32);
+        }
+    }
+};
+
+__global__ __launch_bounds__(256) void k_sample_heads(const uint32_t* __restrict__ s_gid, uint32_t m,
+                                                      uint8_t* __restrict__ H) {
+    for (uint32_t t = blockIdx.x * 256 + threadIdx.x; t <= m; t += gridDim.x * 256)
+        H[t] = (t == 0 || t == m) ? 1 : (s_gid[t] != s_gid[t - 1]);
+}
+
+// ------------------------------------------------------------------------------------------------
+// Dense character codes read the padding like the smallest character, so a suffix whose key window
+// reaches past the text end (position >= bnd) can share a key group with suffixes it differs from.  In
+// such a group the boundary suffixes are the smaller ones and a later boundary suffix is smaller than an
+// earlier one (its window is a proper prefix); every boundary suffix has a key of its own.  These kernels
+// move them to the front of their group, in that order, as singleton groups.
+// ------------------------------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_boundary_collect(const uint32_t* __restrict__ act,
+                                                          const uint32_t* __restrict__ gid,
+                                                          const uint32_t* __restrict__ order, uint32_t A,
+                                                          uint32_t bnd, uint32_t* __restrict__ out,
+                                                          uint32_t* __restrict__ count, uint32_t cap) {
+    for (uint32_t a = blockIdx.x * 256 + threadIdx.x; a < A; a += gridDim.x * 256)
+        if (order[act[a]] >= bnd) {
+            uint32_t i = atomicAdd(count, 1u);
+            if (i < cap) out[i] = gid[a];
+        }
+}
+
+constexpr uint32_t BOUNDARY_CAP = 256;
+
+// one workgroup per affected group order[j0 .. j0+s)
+__global__ __launch_bounds__(1024) void k_boundary_fix(uint32_t* __restrict__ order, uint8_t* __restrict__ H,
+                                                       uint32_t j0, uint32_t s, uint32_t bnd,
+                                                       uint32_t* __restrict__ tmp) {
+    __shared__ uint32_t sB[BOUNDARY_CAP];
+    __shared__ uint32_t nB;
+    __shared__ uint32_t s_tmp[16];
+    if (threadIdx.x == 0) nB = 0;
+    __syncthreads();
+    uint32_t base = 0;
+    for (uint32_t c = 0; c < s; c += 1024) {
+        const uint32_t i = c + threadIdx.x;
+        const uint32_t p = i < s ? order[j0 + i] : 0u;
+        const bool isB = i < s && p >= bnd;
+        const bool keep = i < s && !isB;
+        if (isB) {
+            uint32_t x = atomicAdd(&nB, 1u);
+            if (x < BOUNDARY_CAP) sB[x] = p;
+        }
+        uint32_t total = 0;
+        const uint32_t e = block_excl_scan<1024>(keep ? 1u : 0u, s_tmp, &total);
+        if (keep) tmp[base + e] = p;
+        base += total;
+    }
+    __syncthreads();
+    const uint32_t b = nB < BOUNDARY_CAP ? nB : BOUNDARY_CAP;
+    if (threadIdx.x == 0) {
+        for (uint32_t t = 1; t < b; t++) {  // descending positions
+            uint32_t x = sB[t];
+            int u = (int)t - 1;
+            while (u >= 0 && sB[u] < x) {
+                sB[u + 1] = sB[u];
+                u--;
+            }
+            sB[u + 1] = x;
+        }
+    }
+    __syncthreads();
+    for (uint32_t t = threadIdx.x; t < b; t += 1024) {
+        order[j0 + t] = sB[t];
+        H[j0 + t] = 1;
+    }
+    if (threadIdx.x == 0) H[j0 + b] = 1;
+    for (uint32_t i = threadIdx.x; i < s - b; i += 1024) order[j0 + b + i] = tmp[i];
 }
 
 // recursive string in (mod X, div X) order with unique flags (handle_recursive_call,
