@@ -123,7 +123,11 @@ class Engine {
     // element values in sorted order, H[j] = 1 where a group of equal L-character keys starts (H[N] = 1)
     // and `fin` lists the members of the groups that are still tied.
     // -------------------------------------------------------------------------------------------
-    void refine_sort(const uint64_t* W, uint32_t N, uint32_t L, uint32_t bits, DBuf<uint32_t>& order,
+    // Rounds after the first either append the next packed characters (cheap when many characters fit
+    // a word) or, for wide characters, double the covered prefix with the group ids of the positions C
+    // ahead (exact: the last step takes only the L - C characters still missing).  Doubling needs the
+    // sentinel coding (dense = false): the empty suffix past the end must be the smallest key.
+    void refine_sort(const uint64_t* W, uint32_t N, uint32_t L, uint32_t bits, bool dense, DBuf<uint32_t>& order,
                      DBuf<uint8_t>& H, ActiveSet& fin) {
         const uint32_t cpw = 64 / bits;
         uint32_t c0 = L < cpw ? L : cpw;
@@ -152,12 +156,36 @@ class Engine {
         uint32_t C = c0;
         ActiveSet cur;
         active_set(H.get(), nullptr, N, cur);
+        DBuf<uint32_t> G;
         while (C < L && cur.A > 0) {
             const uint32_t A = cur.A;
             const uint32_t gbits = bit_width(cur.G - 1);
             uint32_t c = (64 - gbits) / bits;
             if (c > L - C) c = L - C;
             if (c > cpw) c = cpw;
+            if (!dense && (L - C) > 3 * std::max<uint32_t>(c, 1)) {
+                // doubling step
+                if (!G.get()) G = DBuf<uint32_t>(cache, N);
+                prim.scan(GroupScatterOp{H.get(), order.get(), G.get()}, N, 1);
+                const uint32_t groups = (uint32_t)prim.read_scalars(2)[1];
+                const uint32_t off = std::min(C, L - C);
+                const uint32_t gvb = bit_width(groups);
+                DBuf<uint64_t> rka(cache, A), rkb(cache, A);
+                DBuf<uint32_t> rva(cache, A), rvb(cache, A);
+                k_keys_double<<<grid(A), 256, 0, stream>>>(G.get(), N, A, cur.act.get(), cur.gid.get(), order.get(), off,
+                                                           gvb, rka.get(), rva.get());
+                uint64_t* x = rka.get();
+                uint64_t* x2 = rkb.get();
+                uint32_t* y = rva.get();
+                uint32_t* y2 = rvb.get();
+                prim.sort_pairs(x, x2, y, y2, A, 0, gbits + gvb);
+                k_writeback<<<grid(A), 256, 0, stream>>>(x, y, cur.act.get(), A, order.get(), H.get(), 1);
+                C += off;
+                ActiveSet next;
+                active_set(H.get(), cur.act.get(), A, next);
+                cur = std::move(next);
+                continue;
+            }
             if (c == 0) throw Error(DCX_E_UNSUPPORTED, "character width too large for refinement");
             {
                 DBuf<uint64_t> rka(cache, A), rkb(cache, A);
@@ -335,7 +363,7 @@ class Engine {
             const uint64_t wtotal = (uint64_t)n + Ls + 2;
             DBuf<uint64_t> W(cache, wtotal);
             make_words(text, lut, n, mbits, dense ? 1 : 0, wtotal, W.get());
-            refine_sort(W.get(), n, Ls, mbits, order, H, tied);
+            refine_sort(W.get(), n, Ls, mbits, dense, order, H, tied);
         }
         if (dense && tied.A > 0) boundary_fixup(dc.bnd, order, H, tied);
 

@@ -222,6 +222,32 @@ __global__ __launch_bounds__(256) void k_writeback(const uint64_t* __restrict__ 
     }
 }
 
+// prefix doubling (deeper levels): group id of every position's current key, in text order
+struct GroupScatterOp {
+    const uint8_t* H;
+    const uint32_t* order;
+    uint32_t* G;
+    __device__ __forceinline__ uint64_t value(uint64_t j) const { return H[j]; }
+    __device__ __forceinline__ void write(uint64_t j, uint64_t excl, uint64_t v) const {
+        G[order[j]] = (uint32_t)(excl + v);
+    }
+};
+
+// key of a tied position for a doubling step: (group among tied groups, group of the position `off` ahead)
+__global__ __launch_bounds__(256) void k_keys_double(const uint32_t* __restrict__ G, uint32_t n, uint32_t A,
+                                                     const uint32_t* __restrict__ act_idx,
+                                                     const uint32_t* __restrict__ gid,
+                                                     const uint32_t* __restrict__ order, uint32_t off, uint32_t gvb,
+                                                     uint64_t* __restrict__ keys, uint32_t* __restrict__ vals) {
+    for (uint32_t a = blockIdx.x * 256 + threadIdx.x; a < A; a += gridDim.x * 256) {
+        uint32_t p = order[act_idx[a]];
+        uint64_t q = (uint64_t)p + off;
+        uint32_t g2 = q < n ? G[q] : 0u;  // past the end: the empty suffix, smallest
+        keys[a] = ((uint64_t)gid[a] << gvb) | g2;
+        vals[a] = p;
+    }
+}
+
 // names = 1 + number of distinct keys before (compute_ranks.hpp:88-111): inclusive count of heads
 struct NamesOp {
     const uint8_t* H;
