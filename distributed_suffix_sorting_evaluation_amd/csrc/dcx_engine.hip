@@ -77,6 +77,8 @@ class Engine {
         memset(&d, 0, sizeof(d));
         d.X = dcs->X;
         d.D = dcs->D;
+        d.inv_X = ((1ull << 32) + d.X - 1) / d.X;
+        d.inv_D = ((1ull << 32) + d.D - 1) / d.D;
         std::vector<uint8_t> in(d.X, 0);
         for (uint32_t i = 0; i < d.D; i++) {
             d.dc[i] = dcs->dc[i];
@@ -131,13 +133,30 @@ class Engine {
                      DBuf<uint8_t>& H, ActiveSet& fin) {
         const uint32_t cpw = 64 / bits;
         uint32_t c0 = L < cpw ? L : cpw;
-        if (N >= (1u << 24)) {
-            // large sorts: round 0 takes about log2(N) + 16 bits in whole 8-bit digits; the refinement
-            // rounds resolve the (few) positions still tied
-            const uint32_t want = std::min<uint32_t>(64, (bit_width(N) + 16 + 7) / 8 * 8);
-            c0 = std::max<uint32_t>(1, std::min<uint32_t>(c0, want / bits));
-        }
-        {
+        const uint32_t pbits = bit_width(N - 1);
+        const uint32_t cpack = std::min<uint32_t>(c0, (64 - pbits) / bits);
+        if (N >= (1u << 16) && cpack * bits >= bit_width(N) + 14) {
+            // round 0 sorts one word per position holding the first characters above the position itself
+            // (16 instead of 24 bytes moved per pass) when the characters left beside the position still
+            // carry 14 bits more than log2(N): low-entropy texts would otherwise leave most positions tied
+            c0 = cpack;
+            const uint32_t kbits = c0 * bits;
+            const uint64_t hi = ~0ull << (64 - kbits);
+            DBuf<uint64_t> ka(cache, N), kb(cache, N);
+            uint64_t* k = ka.get();
+            uint64_t* k2 = kb.get();
+            uint32_t* nv = nullptr;
+            uint32_t* nv2 = nullptr;
+            prim.sort_pairs(k, k2, nv, nv2, N, 64 - kbits, 64, W, false, hi);
+            H = DBuf<uint8_t>(cache, (size_t)N + 1);
+            order = DBuf<uint32_t>(cache, N);
+            k_heads_packed<<<grid(N + 1ull), 256, 0, stream>>>(k, N, kbits, (1ull << pbits) - 1, H.get(), order.get());
+        } else {
+            if (N >= (1u << 24)) {
+                // large sorts: round 0 takes about log2(N) + 16 bits in whole 8-bit digits
+                const uint32_t want = std::min<uint32_t>(64, (bit_width(N) + 16 + 7) / 8 * 8);
+                c0 = std::max<uint32_t>(1, std::min<uint32_t>(c0, want / bits));
+            }
             DBuf<uint64_t> ka(cache, N), kb(cache, N);
             DBuf<uint32_t> va(cache, N), vb(cache, N);
             uint64_t* k = ka.get();

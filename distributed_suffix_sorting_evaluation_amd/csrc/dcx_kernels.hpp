@@ -64,16 +64,24 @@ struct DcDev {
     uint8_t next_rank[MAX_X + 3];  // #{d in DC : d < r}  (difference_cover.hpp NEXT_RANK)
     uint8_t in_dc[MAX_X + 3];      // r in DC (difference_cover.hpp IN_DC)
     uint32_t bnd;                  // first position whose sample-key window reaches past the text end
+    uint64_t inv_X, inv_D;         // ceil(2^32 / X), ceil(2^32 / D) for division without a divide
 };
+
+// floor(p / d) for 32-bit p with inv = ceil(2^32 / d): the product estimate is q or q + 1
+__device__ __forceinline__ uint32_t fast_div(uint32_t p, uint32_t d, uint64_t inv) {
+    uint32_t q = (uint32_t)(((uint64_t)p * inv) >> 32);
+    if ((uint64_t)q * d > p) q--;
+    return q;
+}
 
 // k-th sample in text order (k = rank_pos of the sample, merge_samples.hpp:181-188)
 __device__ __forceinline__ uint32_t sample_pos(const DcDev& c, uint32_t k) {
-    uint32_t q = k / c.D;
+    uint32_t q = fast_div(k, c.D, c.inv_D);
     return q * c.X + c.dc[k - q * c.D];
 }
 // class-major index of the k-th sample (DCRankIndex::cmp_mod_div order, compute_ranks.hpp:49-56)
 __device__ __forceinline__ uint32_t class_major(const DcDev& c, uint32_t k) {
-    uint32_t q = k / c.D;
+    uint32_t q = fast_div(k, c.D, c.inv_D);
     return c.sb[k - q * c.D] + q;
 }
 // inverse of class_major (map_back of pdcx.hpp:83-94, expressed as the text-order sample index)
@@ -84,7 +92,7 @@ __device__ __forceinline__ uint32_t class_major_inv(const DcDev& c, uint32_t cm)
 }
 // index of the first sample at or after text position i
 __device__ __forceinline__ uint32_t rank_pos(const DcDev& c, uint32_t i) {
-    uint32_t q = i / c.X;
+    uint32_t q = fast_div(i, c.X, c.inv_X);
     return q * c.D + c.next_rank[i - q * c.X];
 }
 
@@ -175,6 +183,17 @@ __global__ __launch_bounds__(256) void k_heads(const uint64_t* __restrict__ keys
                                                uint8_t* __restrict__ H) {
     for (uint32_t j = blockIdx.x * 256 + threadIdx.x; j <= N; j += gridDim.x * 256)
         H[j] = (j == 0 || j == N) ? 1 : (top_bits(keys[j], kbits) != top_bits(keys[j - 1], kbits));
+}
+
+// packed round 0: keys are (characters << pbits) | position; heads compare the characters, order gets the
+// positions
+__global__ __launch_bounds__(256) void k_heads_packed(const uint64_t* __restrict__ keys, uint32_t N, uint32_t kbits,
+                                                      uint64_t pmask, uint8_t* __restrict__ H,
+                                                      uint32_t* __restrict__ order) {
+    for (uint32_t j = blockIdx.x * 256 + threadIdx.x; j <= N; j += gridDim.x * 256) {
+        if (j < N) order[j] = (uint32_t)(keys[j] & pmask);
+        H[j] = (j == 0 || j == N) ? 1 : (top_bits(keys[j], kbits) != top_bits(keys[j - 1], kbits));
+    }
 }
 
 // active = member of a group of size >= 2; value packs (active, active && head) as two 32-bit lanes.
@@ -278,13 +297,13 @@ struct SampleScanOp {
     uint32_t* s_gid;
     __device__ __forceinline__ uint64_t value(uint64_t j) const {
         const uint32_t p = order[j];
-        const uint32_t q = p / dc.X;
+        const uint32_t q = fast_div(p, dc.X, dc.inv_X);
         return (uint64_t)dc.in_dc[p - q * dc.X] | ((uint64_t)H[j] << 32);
     }
     __device__ __forceinline__ void write(uint64_t j, uint64_t excl, uint64_t v) const {
         if (v & 1u) {
             const uint32_t p = order[j];
-            const uint32_t q = p / dc.X;
+            const uint32_t q = fast_div(p, dc.X, dc.inv_X);
             const uint32_t t = (uint32_t)excl + shift;
             s_order[t] = q * dc.D + dc.next_rank[p - q * dc.X];
             s_gid[t] = (uint32_t)(excl >> 32) + (uint32_t)(v >> 32);
@@ -432,7 +451,7 @@ __global__ __launch_bounds__(256) void k_keys_rank2(uint32_t A, const uint32_t* 
 __device__ __forceinline__ bool dc_rank_less(uint32_t a, uint32_t b, const DcDev& dc, const uint32_t* __restrict__ cmp,
                                              const uint32_t* __restrict__ ranks) {
     if (a >= dc.bnd && b >= dc.bnd) return a > b;
-    uint32_t qa = a / dc.X, qb = b / dc.X;
+    uint32_t qa = fast_div(a, dc.X, dc.inv_X), qb = fast_div(b, dc.X, dc.inv_X);
     uint32_t ma = a - qa * dc.X, mb = b - qb * dc.X;
     uint32_t e = cmp[ma * dc.X + mb];
     uint32_t r1 = (e >> 8) & 0xFF, r2 = (e >> 16) & 0xFF;

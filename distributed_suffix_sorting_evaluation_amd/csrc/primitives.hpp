@@ -54,14 +54,16 @@ __global__ void k_fill_u8(uint8_t* p, uint8_t v, uint64_t n) {
 }
 
 // Histograms of all digit passes [lo_bit, lo_bit + 8*npasses) from one read of the keys.
+// PACKPOS: the keys of the first pass are (kin & hi_mask) | index (position packed into the low bits)
+template <bool PACKPOS>
 __global__ __launch_bounds__(256) void k_radix_hist(const uint64_t* __restrict__ keys, uint32_t N, uint32_t lo_bit,
-                                                    uint32_t npasses, uint32_t* __restrict__ hist) {
+                                                    uint32_t npasses, uint64_t hi_mask, uint32_t* __restrict__ hist) {
     __shared__ uint32_t s_h[8][RS_RADIX];
     for (int i = threadIdx.x; i < 8 * RS_RADIX; i += 256) (&s_h[0][0])[i] = 0;
     __syncthreads();
     const uint64_t stride = (uint64_t)gridDim.x * 256;
     for (uint64_t i = blockIdx.x * 256ull + threadIdx.x; i < N; i += stride) {
-        uint64_t k = keys[i];
+        uint64_t k = PACKPOS ? ((keys[i] & hi_mask) | i) : keys[i];
         for (uint32_t p = 0; p < npasses; p++) atomicAdd(&s_h[p][(k >> (lo_bit + 8 * p)) & 0xFF], 1u);
     }
     __syncthreads();
@@ -81,13 +83,14 @@ __global__ __launch_bounds__(256) void k_radix_hist_scan(uint32_t* hist) {
 }
 
 // One LSD pass over digit (key >> shift) & 255.  Stable.
-template <bool VALS, bool IDENT>
+// IDENT: values are the input indices.  PACKPOS: keys are (kin & hi_mask) | input index (keys only).
+template <bool VALS, bool IDENT, bool PACKPOS = false>
 __global__ __launch_bounds__(RS_THREADS) void k_onesweep(const uint64_t* __restrict__ kin, uint64_t* __restrict__ kout,
                                                          const uint32_t* __restrict__ vin, uint32_t* __restrict__ vout,
                                                          uint32_t N, uint32_t shift, const uint32_t* __restrict__ dbase,
                                                          uint64_t* __restrict__ status, uint32_t tag,
                                                          unsigned long long* __restrict__ ticket,
-                                                         unsigned long long ticket_base) {
+                                                         unsigned long long ticket_base, uint64_t hi_mask) {
     __shared__ uint64_t s_keys[RS_TILE];
     __shared__ uint8_t s_dig[RS_TILE];
     __shared__ uint32_t s_wcnt[RS_WAVES][RS_RADIX];  // per-wave digit counts, then wave offsets
@@ -113,7 +116,7 @@ __global__ __launch_bounds__(RS_THREADS) void k_onesweep(const uint64_t* __restr
     for (int k = 0; k < RS_ITEMS; k++) {
         uint64_t idx = wbase + k * 64 + lane;
         bool valid = idx < N;
-        key[k] = valid ? kin[idx] : 0ull;
+        key[k] = valid ? (PACKPOS ? ((kin[idx] & hi_mask) | idx) : kin[idx]) : 0ull;
         if (VALS) val[k] = IDENT ? (uint32_t)idx : (valid ? vin[idx] : 0u);
     }
     // rank every key among the keys of its wave with the same digit, in tile order
@@ -438,8 +441,10 @@ struct Primitives {
     // Sorts (keys, vals) by key bits [lo_bit, hi_bit).  Key bits inside the pass range but outside
     // [lo_bit, hi_bit) must be equal for all keys.  On return keys/vals point at the sorted data
     // (the buffers may have been swapped with the alternates).
+    // pack_mask != 0 (keys only, with first_keys): the first pass sorts (first_keys & pack_mask) | index.
     void sort_pairs(uint64_t*& keys, uint64_t*& keys_alt, uint32_t*& vals, uint32_t*& vals_alt, uint32_t N,
-                    uint32_t lo_bit, uint32_t hi_bit, const uint64_t* first_keys = nullptr, bool ident_vals = false) {
+                    uint32_t lo_bit, uint32_t hi_bit, const uint64_t* first_keys = nullptr, bool ident_vals = false,
+                    uint64_t pack_mask = 0) {
         if (hi_bit <= lo_bit) hi_bit = lo_bit + 1;
         if (N <= 1) {
             // nothing to order: materialize the inputs in place
@@ -455,8 +460,13 @@ struct Primitives {
         ensure_sort_status(tiles);
         fill_u32(hist, 0, 8 * RS_RADIX);
         const uint64_t* hsrc = first_keys ? first_keys : keys;
-        hipLaunchKernelGGL(k_radix_hist, dim3(grid_for(N, 256, 2048)), dim3(256), 0, stream, hsrc, N, lo_bit, np,
-                           hist);
+        const bool packpos = pack_mask != 0 && first_keys && !vals;
+        if (packpos)
+            hipLaunchKernelGGL(k_radix_hist<true>, dim3(grid_for(N, 256, 2048)), dim3(256), 0, stream, hsrc, N, lo_bit,
+                               np, pack_mask, hist);
+        else
+            hipLaunchKernelGGL(k_radix_hist<false>, dim3(grid_for(N, 256, 2048)), dim3(256), 0, stream, hsrc, N,
+                               lo_bit, np, (uint64_t)0, hist);
         hipLaunchKernelGGL(k_radix_hist_scan, dim3(np), dim3(256), 0, stream, hist);
         for (uint32_t p = 0; p < np; p++) {
             if (sort_tag >= LB_TAG_LIMIT) {
@@ -472,18 +482,22 @@ struct Primitives {
             }
             const bool first = p == 0;
             const uint64_t* kin = (first && first_keys) ? first_keys : keys;
-            if (!vals)
+            if (!vals && first && packpos)
+                hipLaunchKernelGGL((k_onesweep<false, false, true>), dim3((unsigned)tiles), dim3(RS_THREADS), 0, stream,
+                                   kin, keys_alt, (const uint32_t*)nullptr, (uint32_t*)nullptr, N, lo_bit + 8 * p,
+                                   hist + p * RS_RADIX, sort_status, tag, ticket, ticket_host, pack_mask);
+            else if (!vals)
                 hipLaunchKernelGGL((k_onesweep<false, false>), dim3((unsigned)tiles), dim3(RS_THREADS), 0, stream, kin,
                                    keys_alt, (const uint32_t*)nullptr, (uint32_t*)nullptr, N, lo_bit + 8 * p,
-                                   hist + p * RS_RADIX, sort_status, tag, ticket, ticket_host);
+                                   hist + p * RS_RADIX, sort_status, tag, ticket, ticket_host, (uint64_t)0);
             else if (first && ident_vals)
                 hipLaunchKernelGGL((k_onesweep<true, true>), dim3((unsigned)tiles), dim3(RS_THREADS), 0, stream, kin,
                                    keys_alt, (const uint32_t*)nullptr, vals_alt, N, lo_bit + 8 * p,
-                                   hist + p * RS_RADIX, sort_status, tag, ticket, ticket_host);
+                                   hist + p * RS_RADIX, sort_status, tag, ticket, ticket_host, (uint64_t)0);
             else
                 hipLaunchKernelGGL((k_onesweep<true, false>), dim3((unsigned)tiles), dim3(RS_THREADS), 0, stream, kin,
                                    keys_alt, vals, vals_alt, N, lo_bit + 8 * p, hist + p * RS_RADIX, sort_status, tag,
-                                   ticket, ticket_host);
+                                   ticket, ticket_host, (uint64_t)0);
             if (profile) {
                 DCX_HIP(hipEventRecord(e1, stream));
                 pass_events.emplace_back(e0, e1);
