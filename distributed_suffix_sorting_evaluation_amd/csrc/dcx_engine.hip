@@ -108,11 +108,15 @@ class Engine {
         uint32_t A = 0, G = 0;
     };
 
-    // restricted to the previous active list `src` (its superset) or to all N positions (src == nullptr)
-    void active_set(const uint8_t* H, const uint32_t* src, uint32_t Nsrc, ActiveSet& out) {
+    // restricted to the previous active list `src` (its superset) or to all N positions (src == nullptr).
+    // H: the heads of all positions, or with `compact` the heads of the list entries (k_writeback's Ha).
+    void active_set(const uint8_t* H, const uint32_t* src, uint32_t Nsrc, ActiveSet& out, bool compact = false) {
         out.act = DBuf<uint32_t>(cache, Nsrc);
         out.gid = DBuf<uint32_t>(cache, Nsrc);
-        prim.scan(ActiveScanOp{H, src, out.act.get(), out.gid.get()}, Nsrc, 0);
+        if (src && !compact)
+            prim.scan(ActiveScanOp<true>{H, src, out.act.get(), out.gid.get()}, Nsrc, 0);
+        else
+            prim.scan(ActiveScanOp<false>{H, src, out.act.get(), out.gid.get()}, Nsrc, 0);
         const uint64_t tot = prim.read_scalars(1)[0];
         out.A = (uint32_t)tot;
         out.G = (uint32_t)(tot >> 32);
@@ -198,14 +202,16 @@ class Engine {
                 uint32_t* y = rva.get();
                 uint32_t* y2 = rvb.get();
                 prim.sort_pairs(x, x2, y, y2, A, 0, gbits + gvb);
-                k_writeback<<<grid(A), 256, 0, stream>>>(x, y, cur.act.get(), A, order.get(), H.get(), 1);
+                DBuf<uint8_t> Ha(cache, (size_t)A + 1);
+                k_writeback<<<grid(A), 256, 0, stream>>>(x, y, cur.act.get(), A, order.get(), H.get(), 1, Ha.get());
                 C += off;
                 ActiveSet next;
-                active_set(H.get(), cur.act.get(), A, next);
+                active_set(Ha.get(), cur.act.get(), A, next, true);
                 cur = std::move(next);
                 continue;
             }
             if (c == 0) throw Error(DCX_E_UNSUPPORTED, "character width too large for refinement");
+            DBuf<uint8_t> Ha;
             {
                 DBuf<uint64_t> rka(cache, A), rkb(cache, A);
                 DBuf<uint32_t> rva(cache, A), rvb(cache, A);
@@ -216,11 +222,12 @@ class Engine {
                 uint32_t* y = rva.get();
                 uint32_t* y2 = rvb.get();
                 prim.sort_pairs(x, x2, y, y2, A, 64 - gbits - c * bits, 64);
-                k_writeback<<<grid(A), 256, 0, stream>>>(x, y, cur.act.get(), A, order.get(), H.get(), 1);
+                Ha = DBuf<uint8_t>(cache, (size_t)A + 1);
+                k_writeback<<<grid(A), 256, 0, stream>>>(x, y, cur.act.get(), A, order.get(), H.get(), 1, Ha.get());
             }
             C += c;
             ActiveSet next;
-            active_set(H.get(), cur.act.get(), A, next);
+            active_set(Ha.get(), cur.act.get(), A, next, true);
             cur = std::move(next);
         }
         fin = std::move(cur);
@@ -455,7 +462,8 @@ class Engine {
                         uint32_t* y = va.get();
                         uint32_t* y2 = vb.get();
                         prim.sort_pairs(x, x2, y, y2, A, 0, gb + r2b);
-                        k_writeback<<<grid(A), 256, 0, stream>>>(x, y, stied.act.get(), A, s_order.get(), Hs.get(), 0);
+                        k_writeback<<<grid(A), 256, 0, stream>>>(x, y, stied.act.get(), A, s_order.get(), Hs.get(), 0,
+                                                                 (uint8_t*)nullptr);
                     }
                     k_rank_from_order<<<grid(m), 256, 0, stream>>>(s_order.get(), m, ranks.get());
                 }

@@ -198,22 +198,26 @@ __global__ __launch_bounds__(256) void k_heads_packed(const uint64_t* __restrict
 
 // active = member of a group of size >= 2; value packs (active, active && head) as two 32-bit lanes.
 // Scans either all positions (src == nullptr) or a previous active list (a subsequence of it results).
+// members of groups of size >= 2 as (element index, group id among those groups).  GATHER: the heads of
+// list entry a are H[src[a]] and H[src[a] + 1] (the heads array of all elements); otherwise H is indexed
+// by a itself (the full heads array when src == nullptr, or the compact heads of the list src written by
+// k_writeback, whose entry a + 1 stands for element src[a] + 1: both are 1 unless they share a group).
+template <bool GATHER>
 struct ActiveScanOp {
+    static constexpr bool kBlocked = true;
     const uint8_t* H;
     const uint32_t* src;
     uint32_t* act_idx;
     uint32_t* gid;
     __device__ __forceinline__ uint64_t value(uint64_t a) const {
-        uint32_t j = src ? src[a] : (uint32_t)a;
-        uint32_t h = H[j];
-        bool active = !(h && H[j + 1]);
-        return active ? (1ull | ((uint64_t)h << 32)) : 0ull;
+        const uint64_t j = GATHER ? src[a] : a;
+        const uint32_t h = H[j], h1 = H[j + 1];  // both loads unconditional: mergeable
+        return (h & h1) ? 0ull : (1ull | ((uint64_t)h << 32));
     }
     __device__ __forceinline__ void write(uint64_t a, uint64_t excl, uint64_t v) const {
         if (v) {
-            uint32_t j = src ? src[a] : (uint32_t)a;
-            uint32_t o = (uint32_t)excl;
-            act_idx[o] = j;
+            const uint32_t o = (uint32_t)excl;
+            act_idx[o] = src ? src[a] : (uint32_t)a;
             gid[o] = (uint32_t)(excl >> 32) + (uint32_t)(v >> 32) - 1u;
         }
     }
@@ -231,18 +235,28 @@ __global__ __launch_bounds__(256) void k_keys_rr(const uint64_t* __restrict__ W,
     }
 }
 
+// sorted active list back into `order`; with `mark`, new group starts into H and, if Ha is given, the
+// compact heads of the list (Ha[a] for entry a, Ha[A] = 1) for the next active set
 __global__ __launch_bounds__(256) void k_writeback(const uint64_t* __restrict__ keys, const uint32_t* __restrict__ vals,
                                                    const uint32_t* __restrict__ act_idx, uint32_t A,
-                                                   uint32_t* __restrict__ order, uint8_t* __restrict__ H, int mark) {
+                                                   uint32_t* __restrict__ order, uint8_t* __restrict__ H, int mark,
+                                                   uint8_t* __restrict__ Ha) {
     for (uint32_t a = blockIdx.x * 256 + threadIdx.x; a < A; a += gridDim.x * 256) {
-        uint32_t j = act_idx[a];
+        const uint32_t j = act_idx[a];
         order[j] = vals[a];
-        if (mark && a > 0 && keys[a] != keys[a - 1]) H[j] = 1;
+        if (mark) {
+            // entry 0 starts its group; a differing key starts a new one (the old group id is in the key)
+            const bool head = a == 0 || keys[a] != keys[a - 1];
+            if (head && a > 0) H[j] = 1;
+            if (Ha) Ha[a] = head ? 1 : 0;
+        }
     }
+    if (Ha && blockIdx.x == 0 && threadIdx.x == 0) Ha[A] = 1;
 }
 
 // prefix doubling (deeper levels): group id of every position's current key, in text order
 struct GroupScatterOp {
+    static constexpr bool kBlocked = true;
     const uint8_t* H;
     const uint32_t* order;
     uint32_t* G;
@@ -269,6 +283,7 @@ __global__ __launch_bounds__(256) void k_keys_double(const uint32_t* __restrict_
 
 // names = 1 + number of distinct keys before (compute_ranks.hpp:88-111): inclusive count of heads
 struct NamesOp {
+    static constexpr bool kBlocked = true;
     const uint8_t* H;
     uint32_t* names;
     __device__ __forceinline__ uint64_t value(uint64_t j) const { return H[j]; }
@@ -289,6 +304,7 @@ __global__ __launch_bounds__(256) void k_rank_from_order(const uint32_t* __restr
 // the same names from a separate sort of the sample keys).  `shift` leaves slot 0 to the dummy sample.
 // ------------------------------------------------------------------------------------------------
 struct SampleScanOp {
+    static constexpr bool kBlocked = true;
     const uint32_t* order;
     const uint8_t* H;
     DcDev dc;
@@ -400,6 +416,7 @@ __global__ __launch_bounds__(256) void k_build_rec(const uint32_t* __restrict__ 
 // discarding (pdcx.hpp:215-246): keep c unless c and c-1 are both unique; element 0 is kept.  Also
 // removes the flags into Rr (the reduced string) and records every element's reduced index.
 struct KeepOp {
+    static constexpr bool kBlocked = false;  // value(0) must not read R[-1]
     const uint32_t* R;
     uint32_t* Rr;
     uint32_t* red;
@@ -499,6 +516,7 @@ __global__ __launch_bounds__(256) void k_tiebreak_small(const uint32_t* __restri
 
 // elements of the large groups: (group << 32 | suffix) plus the order slot they came from
 struct LargeScanOp {
+    static constexpr bool kBlocked = true;
     const uint8_t* large;
     const uint32_t* gid;
     const uint32_t* act_idx;

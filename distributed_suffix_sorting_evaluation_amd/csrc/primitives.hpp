@@ -321,6 +321,168 @@ __global__ __launch_bounds__(SC_THREADS) void k_scan(Op op, uint64_t N, uint64_t
 }
 
 // ------------------------------------------------------------------------------------------------
+// reduce-then-scan for long inputs: block b owns the contiguous tiles [b * tpb, (b + 1) * tpb).
+// k_scan_reduce sums each block's values (both 32-bit lanes apart), k_scan_spine turns the block sums
+// into exclusive block prefixes, k_scan_down rescans every tile of the block from its prefix and calls
+// op.write.  value() is evaluated twice per element, but no tile waits on another one.
+//
+// Ops with Op::kBlocked read element i (and i + 1) of plain arrays without branches in value(): a thread
+// then takes 16 consecutive elements, whose loads the compiler merges into vector loads (one 16-byte
+// load instead of 16 byte loads per thread for a flag array).  The exclusive prefixes go through LDS
+// so that op.write still runs in striped order (coalesced stores); the value of an element is the
+// difference of its neighbouring prefixes.  Other ops load striped and scan through LDS.
+// ------------------------------------------------------------------------------------------------
+#ifndef DCX_SC_BLOCKS
+#define DCX_SC_BLOCKS 2048
+#endif
+constexpr uint32_t SC_BLOCKS = DCX_SC_BLOCKS;  // blocks of the two-pass scan
+#ifndef DCX_SC_SINGLE_PASS_TILES
+#define DCX_SC_SINGLE_PASS_TILES 256
+#endif
+constexpr uint64_t SC_SINGLE_PASS_TILES = DCX_SC_SINGLE_PASS_TILES;  // up to this many tiles: one pass
+static_assert(SC_BLOCKS >= 1 && SC_BLOCKS <= 4 * 1024, "k_scan_spine scans at most 4096 block sums");
+
+template <class Op>
+__device__ __forceinline__ void scan_load(const Op& op, uint64_t N, uint64_t tbase, uint32_t tid,
+                                          uint64_t (&v)[SC_ITEMS]) {
+    if (Op::kBlocked) {
+        const uint64_t base = tbase + (uint64_t)tid * SC_ITEMS;
+        if (tbase + SC_TILE <= N) {
+#pragma unroll
+            for (int k = 0; k < SC_ITEMS; k++) v[k] = op.value(base + k);
+        } else {
+#pragma unroll
+            for (int k = 0; k < SC_ITEMS; k++) v[k] = base + k < N ? op.value(base + k) : 0ull;
+        }
+    } else {
+#pragma unroll
+        for (int k = 0; k < SC_ITEMS; k++) {
+            const uint64_t i = tbase + (uint64_t)k * SC_THREADS + tid;
+            v[k] = i < N ? op.value(i) : 0ull;
+        }
+    }
+}
+
+template <class Op>
+__global__ __launch_bounds__(SC_THREADS) void k_scan_reduce(Op op, uint64_t N, uint32_t tpb,
+                                                            uint64_t* __restrict__ part) {
+    __shared__ uint64_t s_red[2][SC_THREADS / 64];
+    const uint32_t tid = threadIdx.x;
+    const uint64_t t0 = (uint64_t)blockIdx.x * tpb;
+    uint64_t lo = 0, hi = 0;
+    for (uint64_t t = t0; t < t0 + tpb; t++) {
+        const uint64_t tbase = t * SC_TILE;
+        if (tbase >= N) break;
+        uint64_t v[SC_ITEMS];
+        scan_load(op, N, tbase, tid, v);
+#pragma unroll
+        for (int k = 0; k < SC_ITEMS; k++) {
+            lo += v[k] & 0xFFFFFFFFull;
+            hi += v[k] >> 32;
+        }
+    }
+    lo = __shfl(wave_incl_scan64(lo), 63, 64);
+    hi = __shfl(wave_incl_scan64(hi), 63, 64);
+    const uint32_t wave = tid / 64;
+    if (__lane_id() == 0) {
+        s_red[0][wave] = lo;
+        s_red[1][wave] = hi;
+    }
+    __syncthreads();
+    if (tid == 0) {
+        uint64_t a = 0, b = 0;
+#pragma unroll
+        for (int w = 0; w < SC_THREADS / 64; w++) {
+            a += s_red[0][w];
+            b += s_red[1][w];
+        }
+        part[2 * blockIdx.x] = a;
+        part[2 * blockIdx.x + 1] = b;
+    }
+}
+
+// exclusive prefixes of nb <= 4 * 1024 block sums, packed as lo | hi << 32; grand total to *total_out
+__global__ __launch_bounds__(1024) void k_scan_spine(const uint64_t* __restrict__ part, uint32_t nb,
+                                                     uint64_t* __restrict__ prefix, uint64_t* __restrict__ total_out) {
+    __shared__ uint64_t s_tmp[1024 / 64];
+    const uint32_t tid = threadIdx.x;
+    uint64_t lo[4], hi[4], slo = 0, shi = 0;
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+        const uint32_t b = tid * 4 + k;
+        lo[k] = b < nb ? part[2 * b] : 0ull;
+        hi[k] = b < nb ? part[2 * b + 1] : 0ull;
+        slo += lo[k];
+        shi += hi[k];
+    }
+    uint64_t tlo = 0, thi = 0;
+    uint64_t elo = block_excl_scan64<1024>(slo, s_tmp, &tlo);
+    uint64_t ehi = block_excl_scan64<1024>(shi, s_tmp, &thi);
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+        const uint32_t b = tid * 4 + k;
+        if (b < nb) prefix[b] = elo | (ehi << 32);
+        elo += lo[k];
+        ehi += hi[k];
+    }
+    if (tid == 0 && total_out) *total_out = tlo | (thi << 32);
+}
+
+template <class Op>
+__global__ __launch_bounds__(SC_THREADS) void k_scan_down(Op op, uint64_t N, uint32_t tpb,
+                                                          const uint64_t* __restrict__ prefix) {
+    constexpr uint32_t PAD = SC_TILE + SC_TILE / 16;  // slot of the tile's end prefix
+    __shared__ uint64_t s_tmp[SC_THREADS / 64];
+    __shared__ uint64_t s_e[PAD + 1];  // padded to spread blocked accesses over banks
+    const uint32_t tid = threadIdx.x;
+    const uint64_t t0 = (uint64_t)blockIdx.x * tpb;
+    uint64_t carry = prefix[blockIdx.x];
+    for (uint64_t t = t0; t < t0 + tpb; t++) {
+        const uint64_t tbase = t * SC_TILE;
+        if (tbase >= N) break;
+        uint64_t v[SC_ITEMS];
+        scan_load(op, N, tbase, tid, v);
+        if (!Op::kBlocked) {
+            // striped values -> blocked through LDS
+#pragma unroll
+            for (int k = 0; k < SC_ITEMS; k++) {
+                const uint32_t idx = k * SC_THREADS + tid;
+                s_e[idx + (idx >> 4)] = v[k];
+            }
+            __syncthreads();
+#pragma unroll
+            for (int k = 0; k < SC_ITEMS; k++) {
+                const uint32_t idx = tid * SC_ITEMS + k;
+                v[k] = s_e[idx + (idx >> 4)];
+            }
+        }
+        uint64_t sum = 0;
+#pragma unroll
+        for (int k = 0; k < SC_ITEMS; k++) sum += v[k];
+        uint64_t tile_total = 0;
+        uint64_t run = carry + block_excl_scan64<SC_THREADS>(sum, s_tmp, &tile_total);  // synchronizes
+#pragma unroll
+        for (int k = 0; k < SC_ITEMS; k++) {
+            const uint32_t idx = tid * SC_ITEMS + k;
+            s_e[idx + (idx >> 4)] = run;
+            run += v[k];
+        }
+        if (tid == SC_THREADS - 1) s_e[PAD] = run;
+        __syncthreads();
+#pragma unroll
+        for (int k = 0; k < SC_ITEMS; k++) {
+            const uint32_t idx = k * SC_THREADS + tid;
+            const uint32_t nx = idx + 1;
+            const uint64_t i = tbase + idx;
+            const uint64_t e = s_e[idx + (idx >> 4)];
+            if (i < N) op.write(i, e, s_e[nx + (nx >> 4)] - e);
+        }
+        carry += tile_total;
+        __syncthreads();
+    }
+}
+
+// ------------------------------------------------------------------------------------------------
 // Host side: scratch for lookback state and the launch wrappers
 // ------------------------------------------------------------------------------------------------
 struct Primitives {
@@ -337,6 +499,8 @@ struct Primitives {
     unsigned long long* ticket = nullptr;  // device counter of tile tickets
     unsigned long long ticket_host = 0;
     uint64_t* scalars = nullptr;           // device scalars (scan totals)
+    uint64_t* scan_part = nullptr;         // two-pass scan: block sums (2 per block) and block prefixes
+    uint64_t* scan_prefix = nullptr;
     uint32_t* hist = nullptr;              // 8 x 256
     uint64_t* host_scalars = nullptr;      // pinned mirror
     // optional per-pass timing of the radix passes
@@ -375,6 +539,8 @@ struct Primitives {
         DCX_HIP(hipMalloc(&scalars, 64 * sizeof(uint64_t)));
         DCX_HIP(hipMemsetAsync(scalars, 0, 64 * sizeof(uint64_t), stream));
         DCX_HIP(hipMalloc(&hist, 8 * RS_RADIX * sizeof(uint32_t)));
+        DCX_HIP(hipMalloc(&scan_part, 2 * SC_BLOCKS * sizeof(uint64_t)));
+        DCX_HIP(hipMalloc(&scan_prefix, SC_BLOCKS * sizeof(uint64_t)));
         DCX_HIP(hipHostMalloc(&host_scalars, 64 * sizeof(uint64_t), hipHostMallocDefault));
         ticket_host = 0;
     }
@@ -385,6 +551,9 @@ struct Primitives {
         if (ticket) (void)hipFree(ticket);
         if (scalars) (void)hipFree(scalars);
         if (hist) (void)hipFree(hist);
+        if (scan_part) (void)hipFree(scan_part);
+        if (scan_prefix) (void)hipFree(scan_prefix);
+        scan_part = scan_prefix = nullptr;
         if (host_scalars) (void)hipHostFree(host_scalars);
         sort_status = nullptr;
         scan_lo = scan_hi = nullptr;
@@ -517,6 +686,16 @@ struct Primitives {
             return;
         }
         uint64_t tiles = ceil_div(N, SC_TILE);
+        if (tiles > SC_SINGLE_PASS_TILES) {
+            const uint32_t tpb = (uint32_t)ceil_div(tiles, (uint64_t)SC_BLOCKS);
+            const uint32_t nb = (uint32_t)ceil_div(tiles, (uint64_t)tpb);
+            hipLaunchKernelGGL(k_scan_reduce<Op>, dim3(nb), dim3(SC_THREADS), 0, stream, op, N, tpb, scan_part);
+            hipLaunchKernelGGL(k_scan_spine, dim3(1), dim3(1024), 0, stream, (const uint64_t*)scan_part, nb,
+                               scan_prefix, scalars + slot);
+            hipLaunchKernelGGL(k_scan_down<Op>, dim3(nb), dim3(SC_THREADS), 0, stream, op, N, tpb,
+                               (const uint64_t*)scan_prefix);
+            return;
+        }
         ensure_scan_status(tiles);
         if (scan_tag >= LB_TAG_LIMIT) {
             fill_u64(scan_lo, 0, scan_tiles_cap);
