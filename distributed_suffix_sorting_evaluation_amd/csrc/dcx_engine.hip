@@ -233,6 +233,28 @@ class Engine {
         fin = std::move(cur);
     }
 
+    // out[dest[i]] = val[i] + add for i < N, where dest is injective into [0, limit) (val == nullptr: the
+    // values are i).  Word-sized writes to random places cost a cache line each; one radix pass first
+    // groups the pairs by the top 8 bits of dest, so that the writes of concurrent workgroups fall into
+    // the same few MB and their lines fill up in cache before they reach HBM.
+    void scatter_u32(const uint32_t* dest, const uint32_t* val, uint32_t N, uint32_t limit, uint32_t add, uint32_t* out) {
+        if (N == 0) return;
+        const uint32_t bw = bit_width(limit > 1 ? limit - 1 : 1);
+        if (N < SCATTER_PARTITION_MIN || bw <= 8) {
+            if (val) {
+                k_scatter_pairs<<<grid(N), 256, 0, stream>>>(dest, val, N, add, out);
+            } else {
+                DBuf<uint32_t> idx(cache, N);
+                k_iota<<<grid(N), 256, 0, stream>>>(idx.get(), N);
+                k_scatter_pairs<<<grid(N), 256, 0, stream>>>(dest, idx.get(), N, add, out);
+            }
+            return;
+        }
+        DBuf<uint32_t> pd(cache, N), pv(cache, N);
+        prim.partition_u32(dest, pd.get(), val, pv.get(), N, bw - 8);
+        k_scatter_pairs<<<grid(N), 256, 0, stream>>>(pd.get(), pv.get(), N, add, out);
+    }
+
     // dense words only: boundary suffixes that share a key group are moved to its front (k_boundary_fix)
     void boundary_fixup(uint32_t bnd, DBuf<uint32_t>& order, DBuf<uint8_t>& H, ActiveSet& tied) {
         DBuf<uint32_t> found(cache, BOUNDARY_CAP), cnt(cache, 1);
@@ -415,13 +437,18 @@ class Engine {
 
             // ---------------- phase 3: recursion ----------------
             if (M >= m) {
-                k_rank_from_order<<<grid(m), 256, 0, stream>>>(s_order.get(), m, ranks.get());
+                scatter_u32(s_order.get(), nullptr, m, m, 1, ranks.get());
             } else {
                 if (ls) ls->recursed = 1;
                 if (M >= UNIQ_BIT) throw Error(DCX_E_UNSUPPORTED, "more than 2^31 distinct sample names");
                 DBuf<uint32_t> R(cache, m);
-                k_build_rec<<<grid(m), 256, 0, stream>>>(s_order.get(), names.get(), Hs.get(), m, dc, R.get());
-                names.reset();
+                {
+                    DBuf<uint32_t> dest(cache, m), word(cache, m);
+                    k_rec_pairs<<<grid(m), 256, 0, stream>>>(s_order.get(), names.get(), Hs.get(), m, dc, dest.get(),
+                                                             word.get());
+                    names.reset();
+                    scatter_u32(dest.get(), word.get(), m, m, 0, R.get());
+                }
                 DBuf<uint32_t> Rr(cache, m), red(cache, m);
                 prim.scan(KeepOp{R.get(), Rr.get(), red.get()}, m, 0);
                 const uint32_t after = (uint32_t)prim.read_scalars(1)[0];
@@ -435,7 +462,10 @@ class Engine {
                         base_case(R.get(), nullptr, m, sar.get());  // sequential_sa_on_local_ranks (pdcx.hpp:115-121)
                     else
                         level<uint32_t>(R.get(), nullptr, m, M, depth + 1, sar.get(), nullptr);
-                    k_ranks_from_sa<<<grid(m), 256, 0, stream>>>(sar.get(), m, dc, ranks.get());
+                    DBuf<uint32_t> dest(cache, m);
+                    k_sa_dest<<<grid(m), 256, 0, stream>>>(sar.get(), m, dc, dest.get());
+                    sar.reset();
+                    scatter_u32(dest.get(), nullptr, m, m, 1, ranks.get());
                 } else {
                     if (ls) {
                         ls->discarding = 1;
@@ -444,7 +474,7 @@ class Engine {
                     R.reset();
                     DBuf<uint32_t> sar(cache, after), isar(cache, after);
                     level<uint32_t>(Rr.get(), nullptr, after, M, depth + 1, sar.get(), nullptr);
-                    k_isa<<<grid(after), 256, 0, stream>>>(sar.get(), after, isar.get());
+                    scatter_u32(sar.get(), nullptr, after, after, 1, isar.get());
                     sar.reset();
                     Rr.reset();
                     // order of (name, rank2) = sorted-by-name order refined inside non-unique groups
@@ -465,7 +495,7 @@ class Engine {
                         k_writeback<<<grid(A), 256, 0, stream>>>(x, y, stied.act.get(), A, s_order.get(), Hs.get(), 0,
                                                                  (uint8_t*)nullptr);
                     }
-                    k_rank_from_order<<<grid(m), 256, 0, stream>>>(s_order.get(), m, ranks.get());
+                    scatter_u32(s_order.get(), nullptr, m, m, 1, ranks.get());
                 }
             }
         }

@@ -292,12 +292,6 @@ struct NamesOp {
     }
 };
 
-// ranks in text order from the sorted sample order: ranks[k] = 1 + sorted position
-__global__ __launch_bounds__(256) void k_rank_from_order(const uint32_t* __restrict__ order, uint32_t m,
-                                                         uint32_t* __restrict__ ranks) {
-    for (uint32_t j = blockIdx.x * 256 + threadIdx.x; j < m; j += gridDim.x * 256) ranks[order[j]] = j + 1;
-}
-
 // ------------------------------------------------------------------------------------------------
 // Phase 1+2 from the sort of all suffixes by their first L_s characters: the samples, in that order,
 // with the 1-based id of their key group (sample_string.hpp:145-164 and compute_ranks.hpp:88-111 compute
@@ -404,13 +398,27 @@ __global__ __launch_bounds__(1024) void k_boundary_fix(uint32_t* __restrict__ or
 // pdcx.hpp:145-180; flag_unique_ranks, compute_ranks.hpp:113-134).  Names stay below 2^31 (checked on
 // the host), so the unique flag rides in bit 31 of the name: one scattered word per sample.
 constexpr uint32_t UNIQ_BIT = 0x80000000u;
-__global__ __launch_bounds__(256) void k_build_rec(const uint32_t* __restrict__ order, const uint32_t* __restrict__ names,
+// its (destination, word) pairs, written to R by a partitioned scatter (scatter_u32)
+__global__ __launch_bounds__(256) void k_rec_pairs(const uint32_t* __restrict__ order, const uint32_t* __restrict__ names,
                                                    const uint8_t* __restrict__ H, uint32_t m, DcDev dc,
-                                                   uint32_t* __restrict__ R) {
+                                                   uint32_t* __restrict__ dest, uint32_t* __restrict__ word) {
     for (uint32_t j = blockIdx.x * 256 + threadIdx.x; j < m; j += gridDim.x * 256) {
-        uint32_t c = class_major(dc, order[j]);
-        R[c] = names[j] | ((H[j] && H[j + 1]) ? UNIQ_BIT : 0u);
+        dest[j] = class_major(dc, order[j]);
+        word[j] = names[j] | ((H[j] && H[j + 1]) ? UNIQ_BIT : 0u);
     }
+}
+
+// scatters of fewer words than this go straight to their destinations
+constexpr uint32_t SCATTER_PARTITION_MIN = 1u << 20;
+
+__global__ __launch_bounds__(256) void k_iota(uint32_t* __restrict__ out, uint32_t N) {
+    for (uint32_t i = blockIdx.x * 256 + threadIdx.x; i < N; i += gridDim.x * 256) out[i] = i;
+}
+
+// out[dest[i]] = val[i] + add
+__global__ __launch_bounds__(256) void k_scatter_pairs(const uint32_t* __restrict__ dest, const uint32_t* __restrict__ val,
+                                                       uint32_t N, uint32_t add, uint32_t* __restrict__ out) {
+    for (uint32_t i = blockIdx.x * 256 + threadIdx.x; i < N; i += gridDim.x * 256) out[dest[i]] = val[i] + add;
 }
 
 // discarding (pdcx.hpp:215-246): keep c unless c and c-1 are both unique; element 0 is kept.  Also
@@ -433,15 +441,10 @@ __global__ __launch_bounds__(256) void k_strip_flags(uint32_t* __restrict__ R, u
     for (uint32_t j = blockIdx.x * 256 + threadIdx.x; j < m; j += gridDim.x * 256) R[j] &= ~UNIQ_BIT;
 }
 
-// recursive_call_direct (pdcx.hpp:182-212): rank of map_back(SA_R[j]) = j + 1
-__global__ __launch_bounds__(256) void k_ranks_from_sa(const uint32_t* __restrict__ sar, uint32_t m, DcDev dc,
-                                                       uint32_t* __restrict__ ranks) {
-    for (uint32_t j = blockIdx.x * 256 + threadIdx.x; j < m; j += gridDim.x * 256)
-        ranks[class_major_inv(dc, sar[j])] = j + 1;
-}
-
-__global__ __launch_bounds__(256) void k_isa(const uint32_t* __restrict__ sa, uint32_t n, uint32_t* __restrict__ isa) {
-    for (uint32_t j = blockIdx.x * 256 + threadIdx.x; j < n; j += gridDim.x * 256) isa[sa[j]] = j + 1;
+// recursive_call_direct (pdcx.hpp:182-212): the rank of map_back(SA_R[j]) is j + 1; dest[j] = map_back(SA_R[j])
+__global__ __launch_bounds__(256) void k_sa_dest(const uint32_t* __restrict__ sar, uint32_t m, DcDev dc,
+                                                 uint32_t* __restrict__ dest) {
+    for (uint32_t j = blockIdx.x * 256 + threadIdx.x; j < m; j += gridDim.x * 256) dest[j] = class_major_inv(dc, sar[j]);
 }
 
 // RankRankIndex keys (pdcx.hpp:297-323) restricted to the non-unique groups: (group, rank2)

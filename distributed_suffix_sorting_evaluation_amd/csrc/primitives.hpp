@@ -55,15 +55,15 @@ __global__ void k_fill_u8(uint8_t* p, uint8_t v, uint64_t n) {
 
 // Histograms of all digit passes [lo_bit, lo_bit + 8*npasses) from one read of the keys.
 // PACKPOS: the keys of the first pass are (kin & hi_mask) | index (position packed into the low bits)
-template <bool PACKPOS>
-__global__ __launch_bounds__(256) void k_radix_hist(const uint64_t* __restrict__ keys, uint32_t N, uint32_t lo_bit,
+template <bool PACKPOS, class KeyT = uint64_t>
+__global__ __launch_bounds__(256) void k_radix_hist(const KeyT* __restrict__ keys, uint32_t N, uint32_t lo_bit,
                                                     uint32_t npasses, uint64_t hi_mask, uint32_t* __restrict__ hist) {
     __shared__ uint32_t s_h[8][RS_RADIX];
     for (int i = threadIdx.x; i < 8 * RS_RADIX; i += 256) (&s_h[0][0])[i] = 0;
     __syncthreads();
     const uint64_t stride = (uint64_t)gridDim.x * 256;
     for (uint64_t i = blockIdx.x * 256ull + threadIdx.x; i < N; i += stride) {
-        uint64_t k = PACKPOS ? ((keys[i] & hi_mask) | i) : keys[i];
+        uint64_t k = PACKPOS ? (((uint64_t)keys[i] & hi_mask) | i) : (uint64_t)keys[i];
         for (uint32_t p = 0; p < npasses; p++) atomicAdd(&s_h[p][(k >> (lo_bit + 8 * p)) & 0xFF], 1u);
     }
     __syncthreads();
@@ -84,14 +84,15 @@ __global__ __launch_bounds__(256) void k_radix_hist_scan(uint32_t* hist) {
 
 // One LSD pass over digit (key >> shift) & 255.  Stable.
 // IDENT: values are the input indices.  PACKPOS: keys are (kin & hi_mask) | input index (keys only).
-template <bool VALS, bool IDENT, bool PACKPOS = false>
-__global__ __launch_bounds__(RS_THREADS) void k_onesweep(const uint64_t* __restrict__ kin, uint64_t* __restrict__ kout,
+// KeyT: 64-bit sort keys or 32-bit keys (radix partitions by destination index).
+template <bool VALS, bool IDENT, bool PACKPOS = false, class KeyT = uint64_t>
+__global__ __launch_bounds__(RS_THREADS) void k_onesweep(const KeyT* __restrict__ kin, KeyT* __restrict__ kout,
                                                          const uint32_t* __restrict__ vin, uint32_t* __restrict__ vout,
                                                          uint32_t N, uint32_t shift, const uint32_t* __restrict__ dbase,
                                                          uint64_t* __restrict__ status, uint32_t tag,
                                                          unsigned long long* __restrict__ ticket,
                                                          unsigned long long ticket_base, uint64_t hi_mask) {
-    __shared__ uint64_t s_keys[RS_TILE];
+    __shared__ KeyT s_keys[RS_TILE];
     __shared__ uint8_t s_dig[RS_TILE];
     __shared__ uint32_t s_wcnt[RS_WAVES][RS_RADIX];  // per-wave digit counts, then wave offsets
     __shared__ uint32_t s_start[RS_RADIX];
@@ -116,7 +117,7 @@ __global__ __launch_bounds__(RS_THREADS) void k_onesweep(const uint64_t* __restr
     for (int k = 0; k < RS_ITEMS; k++) {
         uint64_t idx = wbase + k * 64 + lane;
         bool valid = idx < N;
-        key[k] = valid ? (PACKPOS ? ((kin[idx] & hi_mask) | idx) : kin[idx]) : 0ull;
+        key[k] = valid ? (PACKPOS ? (((uint64_t)kin[idx] & hi_mask) | idx) : (uint64_t)kin[idx]) : 0ull;
         if (VALS) val[k] = IDENT ? (uint32_t)idx : (valid ? vin[idx] : 0u);
     }
     // rank every key among the keys of its wave with the same digit, in tile order
@@ -186,7 +187,7 @@ __global__ __launch_bounds__(RS_THREADS) void k_onesweep(const uint64_t* __restr
         uint32_t dk = rk[k] & 0xFFu;
         if (idx < N) {
             uint32_t lp = s_start[dk] + s_wcnt[wave][dk] + (rk[k] >> 8);
-            s_keys[lp] = key[k];
+            s_keys[lp] = (KeyT)key[k];
             s_dig[lp] = (uint8_t)dk;
         }
     }
@@ -676,6 +677,33 @@ struct Primitives {
             std::swap(keys, keys_alt);
             if (vals) std::swap(vals, vals_alt);
         }
+    }
+
+    // One stable radix pass of (u32 key, u32 value) pairs over key bits [lo_bit, lo_bit + 8), into
+    // keys_out / vals_out.  vals == nullptr: the values are the input indices.
+    void partition_u32(const uint32_t* keys, uint32_t* keys_out, const uint32_t* vals, uint32_t* vals_out, uint32_t N,
+                       uint32_t lo_bit) {
+        if (N == 0) return;
+        const uint64_t tiles = ceil_div(N, RS_TILE);
+        ensure_sort_status(tiles);
+        fill_u32(hist, 0, RS_RADIX);
+        hipLaunchKernelGGL((k_radix_hist<false, uint32_t>), dim3(grid_for(N, 256, 2048)), dim3(256), 0, stream, keys, N,
+                           lo_bit, 1u, (uint64_t)0, hist);
+        hipLaunchKernelGGL(k_radix_hist_scan, dim3(1), dim3(256), 0, stream, hist);
+        if (sort_tag >= LB_TAG_LIMIT) {
+            fill_u64(sort_status, 0, sort_status_words);
+            sort_tag = 0;
+        }
+        const uint32_t tag = ++sort_tag;
+        if (vals)
+            hipLaunchKernelGGL((k_onesweep<true, false, false, uint32_t>), dim3((unsigned)tiles), dim3(RS_THREADS), 0,
+                               stream, keys, keys_out, vals, vals_out, N, lo_bit, (const uint32_t*)hist, sort_status, tag,
+                               ticket, ticket_host, (uint64_t)0);
+        else
+            hipLaunchKernelGGL((k_onesweep<true, true, false, uint32_t>), dim3((unsigned)tiles), dim3(RS_THREADS), 0,
+                               stream, keys, keys_out, (const uint32_t*)nullptr, vals_out, N, lo_bit,
+                               (const uint32_t*)hist, sort_status, tag, ticket, ticket_host, (uint64_t)0);
+        ticket_host += tiles;
     }
 
     // Launches a scan; the grand total lands in device scalar slot `slot` (read with read_scalars).
