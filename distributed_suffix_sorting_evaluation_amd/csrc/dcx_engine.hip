@@ -233,14 +233,14 @@ class Engine {
         fin = std::move(cur);
     }
 
-    // out[dest[i]] = val[i] + add for i < N, where dest is injective into [0, limit) (val == nullptr: the
-    // values are i).  Word-sized writes to random places cost a cache line each; one radix pass first
-    // groups the pairs by the top 8 bits of dest, so that the writes of concurrent workgroups fall into
-    // the same few MB and their lines fill up in cache before they reach HBM.
-    void scatter_u32(const uint32_t* dest, const uint32_t* val, uint32_t N, uint32_t limit, uint32_t add, uint32_t* out) {
+    // out[dest[i]] = val[i] + add for i < N, where dest is a permutation of [0, N) (val == nullptr: the values
+    // are i).  Word-sized writes to random places cost a cache line each; instead an LSD radix sort by the
+    // high bits of dest (all but the low SCATTER_WINDOW_BITS or fewer) orders the pairs by destination
+    // window, and each window is assembled in LDS and written out whole.
+    void scatter_u32(const uint32_t* dest, const uint32_t* val, uint32_t N, uint32_t add, uint32_t* out) {
         if (N == 0) return;
-        const uint32_t bw = bit_width(limit > 1 ? limit - 1 : 1);
-        if (N < SCATTER_PARTITION_MIN || bw <= 8) {
+        const uint32_t bw = bit_width(N - 1);
+        if (N < SCATTER_PARTITION_MIN || bw <= SCATTER_WINDOW_BITS) {
             if (val) {
                 k_scatter_pairs<<<grid(N), 256, 0, stream>>>(dest, val, N, add, out);
             } else {
@@ -250,9 +250,12 @@ class Engine {
             }
             return;
         }
-        DBuf<uint32_t> pd(cache, N), pv(cache, N);
-        prim.partition_u32(dest, pd.get(), val, pv.get(), N, bw - 8);
-        k_scatter_pairs<<<grid(N), 256, 0, stream>>>(pd.get(), pv.get(), N, add, out);
+        const uint32_t np = (bw - SCATTER_WINDOW_BITS + 7) / 8;
+        const uint32_t wbits = bw - 8 * np;
+        DBuf<uint32_t> k1(cache, N), v1(cache, N), k2(cache, N), v2(cache, N);
+        uint32_t *ks = nullptr, *vs = nullptr;
+        prim.sort_u32(dest, val, k1.get(), v1.get(), k2.get(), v2.get(), N, wbits, np, &ks, &vs);
+        k_scatter_window<<<(unsigned)ceil_div((uint64_t)N, 1ull << wbits), 256, 0, stream>>>(ks, vs, N, add, wbits, out);
     }
 
     // dense words only: boundary suffixes that share a key group are moved to its front (k_boundary_fix)
@@ -437,7 +440,7 @@ class Engine {
 
             // ---------------- phase 3: recursion ----------------
             if (M >= m) {
-                scatter_u32(s_order.get(), nullptr, m, m, 1, ranks.get());
+                scatter_u32(s_order.get(), nullptr, m, 1, ranks.get());
             } else {
                 if (ls) ls->recursed = 1;
                 if (M >= UNIQ_BIT) throw Error(DCX_E_UNSUPPORTED, "more than 2^31 distinct sample names");
@@ -447,7 +450,7 @@ class Engine {
                     k_rec_pairs<<<grid(m), 256, 0, stream>>>(s_order.get(), names.get(), Hs.get(), m, dc, dest.get(),
                                                              word.get());
                     names.reset();
-                    scatter_u32(dest.get(), word.get(), m, m, 0, R.get());
+                    scatter_u32(dest.get(), word.get(), m, 0, R.get());
                 }
                 DBuf<uint32_t> Rr(cache, m), red(cache, m);
                 prim.scan(KeepOp{R.get(), Rr.get(), red.get()}, m, 0);
@@ -465,7 +468,7 @@ class Engine {
                     DBuf<uint32_t> dest(cache, m);
                     k_sa_dest<<<grid(m), 256, 0, stream>>>(sar.get(), m, dc, dest.get());
                     sar.reset();
-                    scatter_u32(dest.get(), nullptr, m, m, 1, ranks.get());
+                    scatter_u32(dest.get(), nullptr, m, 1, ranks.get());
                 } else {
                     if (ls) {
                         ls->discarding = 1;
@@ -474,7 +477,7 @@ class Engine {
                     R.reset();
                     DBuf<uint32_t> sar(cache, after), isar(cache, after);
                     level<uint32_t>(Rr.get(), nullptr, after, M, depth + 1, sar.get(), nullptr);
-                    scatter_u32(sar.get(), nullptr, after, after, 1, isar.get());
+                    scatter_u32(sar.get(), nullptr, after, 1, isar.get());
                     sar.reset();
                     Rr.reset();
                     // order of (name, rank2) = sorted-by-name order refined inside non-unique groups
@@ -495,7 +498,7 @@ class Engine {
                         k_writeback<<<grid(A), 256, 0, stream>>>(x, y, stied.act.get(), A, s_order.get(), Hs.get(), 0,
                                                                  (uint8_t*)nullptr);
                     }
-                    scatter_u32(s_order.get(), nullptr, m, m, 1, ranks.get());
+                    scatter_u32(s_order.get(), nullptr, m, 1, ranks.get());
                 }
             }
         }

@@ -679,31 +679,47 @@ struct Primitives {
         }
     }
 
-    // One stable radix pass of (u32 key, u32 value) pairs over key bits [lo_bit, lo_bit + 8), into
-    // keys_out / vals_out.  vals == nullptr: the values are the input indices.
-    void partition_u32(const uint32_t* keys, uint32_t* keys_out, const uint32_t* vals, uint32_t* vals_out, uint32_t N,
-                       uint32_t lo_bit) {
-        if (N == 0) return;
+    // Stable LSD radix sort of (u32 key, u32 value) pairs by key bits [lo_bit, lo_bit + 8 * np) (np <= 4).
+    // The input arrays are left alone; the result ends in *keys_out / *vals_out, which point to either
+    // (k1, v1) or (k2, v2).  vals == nullptr: the values are the input indices.
+    void sort_u32(const uint32_t* keys, const uint32_t* vals, uint32_t* k1, uint32_t* v1, uint32_t* k2, uint32_t* v2,
+                  uint32_t N, uint32_t lo_bit, uint32_t np, uint32_t** keys_out, uint32_t** vals_out) {
+        const uint32_t* kin = keys;
+        const uint32_t* vin = vals;
+        uint32_t* ko = k1;
+        uint32_t* vo = v1;
+        *keys_out = k1;
+        *vals_out = v1;
+        if (N == 0 || np == 0) return;
         const uint64_t tiles = ceil_div(N, RS_TILE);
         ensure_sort_status(tiles);
-        fill_u32(hist, 0, RS_RADIX);
+        fill_u32(hist, 0, np * RS_RADIX);
         hipLaunchKernelGGL((k_radix_hist<false, uint32_t>), dim3(grid_for(N, 256, 2048)), dim3(256), 0, stream, keys, N,
-                           lo_bit, 1u, (uint64_t)0, hist);
-        hipLaunchKernelGGL(k_radix_hist_scan, dim3(1), dim3(256), 0, stream, hist);
-        if (sort_tag >= LB_TAG_LIMIT) {
-            fill_u64(sort_status, 0, sort_status_words);
-            sort_tag = 0;
+                           lo_bit, np, (uint64_t)0, hist);
+        hipLaunchKernelGGL(k_radix_hist_scan, dim3(np), dim3(256), 0, stream, hist);
+        for (uint32_t p = 0; p < np; p++) {
+            if (sort_tag >= LB_TAG_LIMIT) {
+                fill_u64(sort_status, 0, sort_status_words);
+                sort_tag = 0;
+            }
+            const uint32_t tag = ++sort_tag;
+            if (vin)
+                hipLaunchKernelGGL((k_onesweep<true, false, false, uint32_t>), dim3((unsigned)tiles), dim3(RS_THREADS), 0,
+                                   stream, kin, ko, vin, vo, N, lo_bit + 8 * p, (const uint32_t*)(hist + p * RS_RADIX),
+                                   sort_status, tag, ticket, ticket_host, (uint64_t)0);
+            else
+                hipLaunchKernelGGL((k_onesweep<true, true, false, uint32_t>), dim3((unsigned)tiles), dim3(RS_THREADS), 0,
+                                   stream, kin, ko, (const uint32_t*)nullptr, vo, N, lo_bit + 8 * p,
+                                   (const uint32_t*)(hist + p * RS_RADIX), sort_status, tag, ticket, ticket_host,
+                                   (uint64_t)0);
+            ticket_host += tiles;
+            *keys_out = ko;
+            *vals_out = vo;
+            kin = ko;
+            vin = vo;
+            ko = (ko == k1) ? k2 : k1;
+            vo = (vo == v1) ? v2 : v1;
         }
-        const uint32_t tag = ++sort_tag;
-        if (vals)
-            hipLaunchKernelGGL((k_onesweep<true, false, false, uint32_t>), dim3((unsigned)tiles), dim3(RS_THREADS), 0,
-                               stream, keys, keys_out, vals, vals_out, N, lo_bit, (const uint32_t*)hist, sort_status, tag,
-                               ticket, ticket_host, (uint64_t)0);
-        else
-            hipLaunchKernelGGL((k_onesweep<true, true, false, uint32_t>), dim3((unsigned)tiles), dim3(RS_THREADS), 0,
-                               stream, keys, keys_out, (const uint32_t*)nullptr, vals_out, N, lo_bit,
-                               (const uint32_t*)hist, sort_status, tag, ticket, ticket_host, (uint64_t)0);
-        ticket_host += tiles;
     }
 
     // Launches a scan; the grand total lands in device scalar slot `slot` (read with read_scalars).
