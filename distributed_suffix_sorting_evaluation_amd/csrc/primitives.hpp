@@ -1,7 +1,7 @@
 // primitives.hpp -- device-wide building blocks of the DCX engine, written for gfx950 (wave64):
 //
 //  * LSD radix sort of (u64 key, u32 value) pairs, one kernel launch per 8-bit digit.  Each pass is
-//    a "onesweep" pass: a workgroup ranks a 4096-key tile with wavefront ballots (no per-key
+//    a "onesweep" pass: a workgroup ranks a 7168-key tile with wavefront ballots (no per-key
 //    atomics), publishes its per-digit counts with a decoupled lookback over the preceding tiles,
 //    reorders the tile in LDS by digit and writes each digit run contiguously.  One upfront kernel
 //    builds the histograms of every pass from a single read of the keys.
@@ -15,10 +15,11 @@
 
 namespace dcx {
 
-// 512 threads x 12 keys: 6144-key tiles (longer digit runs per tile) at two workgroups per CU; measured
-// faster on gfx950 than 256 x 16, 256 x 24, 512 x 16, 768 x 8 and 1024 x 6/8 for 64-bit keys
+// 512 threads x 14 keys: 7168-key tiles at two workgroups per CU.  Fewer, larger tiles mean fewer
+// lookbacks; measured on gfx950 for 64-bit keys against 512 x 8/10/12/13/15/16, 384 x 10/12, 256 x 12/16/24,
+// 768 x 8 and 1024 x 6/8 (15 and 16 keys per thread fall off sharply)
 #ifndef DCX_RS_ITEMS
-#define DCX_RS_ITEMS 12
+#define DCX_RS_ITEMS 14
 #endif
 #ifndef DCX_RS_THREADS
 #define DCX_RS_THREADS 512
@@ -93,7 +94,6 @@ __global__ __launch_bounds__(RS_THREADS) void k_onesweep(const KeyT* __restrict_
                                                          unsigned long long* __restrict__ ticket,
                                                          unsigned long long ticket_base, uint64_t hi_mask) {
     __shared__ KeyT s_keys[RS_TILE];
-    __shared__ uint8_t s_dig[RS_TILE];
     __shared__ uint32_t s_wcnt[RS_WAVES][RS_RADIX];  // per-wave digit counts, then wave offsets
     __shared__ uint32_t s_start[RS_RADIX];
     __shared__ uint32_t s_gofs[RS_RADIX];
@@ -188,12 +188,23 @@ __global__ __launch_bounds__(RS_THREADS) void k_onesweep(const KeyT* __restrict_
         if (idx < N) {
             uint32_t lp = s_start[dk] + s_wcnt[wave][dk] + (rk[k] >> 8);
             s_keys[lp] = (KeyT)key[k];
-            s_dig[lp] = (uint8_t)dk;
         }
     }
     __syncthreads();
+    // write out: position i of the reordered tile goes to gofs[digit] + i; the digits of a thread's
+    // positions stay in registers for the values
     const uint32_t tile_n = (uint32_t)((N - tile_base) < (uint64_t)RS_TILE ? (N - tile_base) : RS_TILE);
-    for (uint32_t i = tid; i < tile_n; i += RS_THREADS) kout[s_gofs[s_dig[i]] + i] = s_keys[i];
+    uint32_t dg[RS_ITEMS];
+#pragma unroll
+    for (int r = 0; r < RS_ITEMS; r++) {
+        const uint32_t i = tid + r * RS_THREADS;
+        dg[r] = 0;
+        if (i < tile_n) {
+            const KeyT kk = s_keys[i];
+            dg[r] = (uint32_t)((uint64_t)kk >> shift) & 0xFFu;
+            kout[s_gofs[dg[r]] + i] = kk;
+        }
+    }
     if (VALS) {
         __syncthreads();
         uint32_t* s_vals = reinterpret_cast<uint32_t*>(s_keys);
@@ -204,7 +215,11 @@ __global__ __launch_bounds__(RS_THREADS) void k_onesweep(const KeyT* __restrict_
             if (idx < N) s_vals[s_start[dk] + s_wcnt[wave][dk] + (rk[k] >> 8)] = val[k];
         }
         __syncthreads();
-        for (uint32_t i = tid; i < tile_n; i += RS_THREADS) vout[s_gofs[s_dig[i]] + i] = s_vals[i];
+#pragma unroll
+        for (int r = 0; r < RS_ITEMS; r++) {
+            const uint32_t i = tid + r * RS_THREADS;
+            if (i < tile_n) vout[s_gofs[dg[r]] + i] = s_vals[i];
+        }
     }
 }
 
