@@ -88,6 +88,7 @@ class Engine {
         for (uint32_t r = 0; r < d.X; r++) {
             d.next_rank[r] = (uint8_t)c;
             d.in_dc[r] = in[r];
+            if (in[r]) d.in_mask[r >> 6] |= 1ull << (r & 63);
             c += in[r];
         }
         // samples are positions p in [0, n] with p mod X in DC (compute_length_info, pdcx.hpp:473-510)
@@ -133,7 +134,7 @@ class Engine {
     // a word) or, for wide characters, double the covered prefix with the group ids of the positions C
     // ahead (exact: the last step takes only the L - C characters still missing).  Doubling needs the
     // sentinel coding (dense = false): the empty suffix past the end must be the smallest key.
-    void refine_sort(const uint64_t* W, uint32_t N, uint32_t L, uint32_t bits, bool dense, DBuf<uint32_t>& order,
+    void refine_sort(const uint64_t* PT, uint32_t N, uint32_t L, uint32_t bits, bool dense, DBuf<uint32_t>& order,
                      DBuf<uint8_t>& H, ActiveSet& fin) {
         const uint32_t cpw = 64 / bits;
         uint32_t c0 = L < cpw ? L : cpw;
@@ -151,7 +152,7 @@ class Engine {
             uint64_t* k2 = kb.get();
             uint32_t* nv = nullptr;
             uint32_t* nv2 = nullptr;
-            prim.sort_pairs(k, k2, nv, nv2, N, 64 - kbits, 64, W, false, hi);
+            prim.sort_pairs_from(StreamKeys<true>{PT, bits, hi}, k, k2, nv, nv2, N, 64 - kbits, 64, false);
             H = DBuf<uint8_t>(cache, (size_t)N + 1);
             order = DBuf<uint32_t>(cache, N);
             k_heads_packed<<<grid(N + 1ull), 256, 0, stream>>>(k, N, kbits, (1ull << pbits) - 1, H.get(), order.get());
@@ -168,7 +169,8 @@ class Engine {
             uint32_t* v = va.get();
             uint32_t* v2 = vb.get();
             // every position is an element: the first pass reads the packed words directly
-            prim.sort_pairs(k, k2, v, v2, N, 64 - c0 * bits, 64, W, true);
+            prim.sort_pairs_from(StreamKeys<false>{PT, bits, top_mask(c0 * bits)}, k, k2, v, v2, N, 64 - c0 * bits, 64,
+                                 true);
             H = DBuf<uint8_t>(cache, (size_t)N + 1);
             k_heads<<<grid(N + 1ull), 256, 0, stream>>>(k, N, c0 * bits, H.get());
             if (v == va.get())
@@ -215,7 +217,7 @@ class Engine {
             {
                 DBuf<uint64_t> rka(cache, A), rkb(cache, A);
                 DBuf<uint32_t> rva(cache, A), rvb(cache, A);
-                k_keys_rr<<<grid(A), 256, 0, stream>>>(W, A, cur.act.get(), cur.gid.get(), order.get(), C, c * bits,
+                k_keys_rr<<<grid(A), 256, 0, stream>>>(PT, bits, A, cur.act.get(), cur.gid.get(), order.get(), C, c * bits,
                                                        gbits, rka.get(), rva.get());
                 uint64_t* x = rka.get();
                 uint64_t* x2 = rkb.get();
@@ -346,17 +348,30 @@ class Engine {
     // one recursion level: SA of text[0..n) into sa_out (device, n entries).  Level 0 reads bytes
     // through the alphabet map `lut`; deeper levels read names (uint32, >= 1).
     // -------------------------------------------------------------------------------------------
+    // the level's characters as a packed code stream (k_pack_text) of `chars` characters
     template <class CharT>
-    void make_words(const CharT* text, const uint16_t* lut, uint32_t n, uint32_t bits, uint32_t dense, uint64_t total,
-                    uint64_t* W) {
-        const unsigned blocks = (unsigned)std::min<uint64_t>(ceil_div(total, WD_TILE), 8192);
-        k_words<CharT><<<blocks, 256, 0, stream>>>(text, n, lut, bits, 64 / bits, total, dense, W);
+    void make_stream(const CharT* text, const uint16_t* lut, uint32_t n, uint32_t bits, uint32_t dense, uint64_t chars,
+                     DBuf<uint64_t>& PT) {
+        const uint64_t nwords = stream_words(chars, bits);
+        PT = DBuf<uint64_t>(cache, nwords);
+        const unsigned wgrid = (unsigned)std::min<uint64_t>(ceil_div(nwords, 256), 16384);
+        switch (bits) {
+            case 1: k_pack_aligned<CharT, 1><<<wgrid, 256, 0, stream>>>(text, n, lut, dense, nwords, PT.get()); return;
+            case 2: k_pack_aligned<CharT, 2><<<wgrid, 256, 0, stream>>>(text, n, lut, dense, nwords, PT.get()); return;
+            case 4: k_pack_aligned<CharT, 4><<<wgrid, 256, 0, stream>>>(text, n, lut, dense, nwords, PT.get()); return;
+            case 8: k_pack_aligned<CharT, 8><<<wgrid, 256, 0, stream>>>(text, n, lut, dense, nwords, PT.get()); return;
+            default: break;
+        }
+        const uint64_t chunks = ceil_div(nwords, (uint64_t)PK_CHARS * bits / 64);
+        k_pack_text<CharT><<<(unsigned)std::min<uint64_t>(chunks, 16384), 256, 0, stream>>>(text, n, lut, bits, dense,
+                                                                                             nwords, PT.get());
     }
 
     // characters of the level are 1..maxchar (0 is the padding)
     template <class CharT>
+    // The SA goes to sa_out, or with sa_take != nullptr into *sa_take (the level's own buffer, no copy).
     void level(const CharT* text, const uint16_t* lut, uint32_t n, uint32_t maxchar, int depth, uint32_t* sa_out,
-               hipEvent_t* ev) {
+               hipEvent_t* ev, DBuf<uint32_t>* sa_take = nullptr) {
         const uint32_t X = dcs->X, D = dcs->D;
         const uint32_t bits = bit_width(maxchar);
         // dense merge words (c -> c - 1) when they save a bit per character
@@ -368,6 +383,10 @@ class Engine {
         const uint64_t threshold = opt.base_case_chars > 2ull * X ? opt.base_case_chars : 2ull * X;
         if (n <= threshold) {
             if (ls) ls->base_case = 1;
+            if (sa_take) {
+                *sa_take = DBuf<uint32_t>(cache, n > 0 ? n : 1);
+                sa_out = sa_take->get();
+            }
             base_case(text, lut, n, sa_out);
             return;
         }
@@ -411,10 +430,9 @@ class Engine {
         DBuf<uint8_t> H;
         ActiveSet tied;
         {
-            const uint64_t wtotal = (uint64_t)n + Ls + 2;
-            DBuf<uint64_t> W(cache, wtotal);
-            make_words(text, lut, n, mbits, dense ? 1 : 0, wtotal, W.get());
-            refine_sort(W.get(), n, Ls, mbits, dense, order, H, tied);
+            DBuf<uint64_t> PT;
+            make_stream(text, lut, n, mbits, dense ? 1 : 0, (uint64_t)n + Ls + 2, PT);
+            refine_sort(PT.get(), n, Ls, mbits, dense, order, H, tied);
         }
         if (dense && tied.A > 0) boundary_fixup(dc.bnd, order, H, tied);
 
@@ -460,11 +478,13 @@ class Engine {
                     Rr.reset();
                     red.reset();
                     k_strip_flags<<<grid(m), 256, 0, stream>>>(R.get(), m);
-                    DBuf<uint32_t> sar(cache, m);
-                    if (n <= 80)
+                    DBuf<uint32_t> sar;
+                    if (n <= 80) {
+                        sar = DBuf<uint32_t>(cache, m);
                         base_case(R.get(), nullptr, m, sar.get());  // sequential_sa_on_local_ranks (pdcx.hpp:115-121)
-                    else
-                        level<uint32_t>(R.get(), nullptr, m, M, depth + 1, sar.get(), nullptr);
+                    } else {
+                        level<uint32_t>(R.get(), nullptr, m, M, depth + 1, nullptr, nullptr, &sar);
+                    }
                     DBuf<uint32_t> dest(cache, m);
                     k_sa_dest<<<grid(m), 256, 0, stream>>>(sar.get(), m, dc, dest.get());
                     sar.reset();
@@ -475,8 +495,8 @@ class Engine {
                         ls->reduced = after;
                     }
                     R.reset();
-                    DBuf<uint32_t> sar(cache, after), isar(cache, after);
-                    level<uint32_t>(Rr.get(), nullptr, after, M, depth + 1, sar.get(), nullptr);
+                    DBuf<uint32_t> sar, isar(cache, after);
+                    level<uint32_t>(Rr.get(), nullptr, after, M, depth + 1, nullptr, nullptr, &sar);
                     scatter_u32(sar.get(), nullptr, after, 1, isar.get());
                     sar.reset();
                     Rr.reset();
@@ -508,7 +528,10 @@ class Engine {
         // suffixes with different L_s-character keys are already in order; groups of equal keys are ordered
         // by the DC comparator on the sample ranks
         tie_break(dc, order, H, tied, ranks.get());
-        DCX_HIP(hipMemcpyAsync(sa_out, order.get(), (size_t)n * 4, hipMemcpyDeviceToDevice, stream));
+        if (sa_take)
+            *sa_take = std::move(order);
+        else
+            DCX_HIP(hipMemcpyAsync(sa_out, order.get(), (size_t)n * 4, hipMemcpyDeviceToDevice, stream));
     }
 
     // -------------------------------------------------------------------------------------------
@@ -527,15 +550,15 @@ class Engine {
 
     // compress_alphabet (cli.cpp:417-463): used bytes -> 1..sigma in byte order; returns sigma and the map
     uint32_t alphabet(const uint8_t* d_text, uint64_t n, DBuf<uint16_t>& dlut) {
-        DBuf<unsigned long long> hist(cache, 256);
-        DCX_HIP(hipMemsetAsync(hist.get(), 0, 256 * 8, stream));
-        if (n) k_byte_hist<<<grid(ceil_div(n, 16), 2048), 256, 0, stream>>>(d_text, n, hist.get());
-        unsigned long long hh[256];
-        DCX_HIP(hipMemcpyAsync(hh, hist.get(), sizeof(hh), hipMemcpyDeviceToHost, stream));
+        DBuf<unsigned long long> mask(cache, 4);
+        DCX_HIP(hipMemsetAsync(mask.get(), 0, 4 * 8, stream));
+        if (n) k_byte_mask<<<grid(ceil_div(n, 16), 2048), 256, 0, stream>>>(d_text, n, mask.get());
+        unsigned long long hm[4];
+        DCX_HIP(hipMemcpyAsync(hm, mask.get(), sizeof(hm), hipMemcpyDeviceToHost, stream));
         DCX_HIP(hipStreamSynchronize(stream));
         uint16_t lut[256];
         uint32_t next = 1;
-        for (int c = 0; c < 256; c++) lut[c] = hh[c] ? (uint16_t)next++ : 0;
+        for (int c = 0; c < 256; c++) lut[c] = ((hm[c >> 6] >> (c & 63)) & 1) ? (uint16_t)next++ : 0;
         dlut = DBuf<uint16_t>(cache, 256);
         DCX_HIP(hipMemcpyAsync(dlut.get(), lut, sizeof(lut), hipMemcpyHostToDevice, stream));
         return next - 1;
@@ -629,7 +652,7 @@ class Engine {
             DCX_HIP(hipMemcpyAsync(d_co.get(), co.data(), (cnt + 1) * 8, hipMemcpyHostToDevice, stream));
             const uint32_t N = (uint32_t)joined;
             const uint64_t chars = offsets[k1] - offsets[k0];
-            DBuf<uint32_t> sa(cache, N);
+            DBuf<uint32_t> sa;
             sigma = maxchar;
             if (maxchar <= 255) {
                 DBuf<uint8_t> jt(cache, (size_t)N + 16);
@@ -637,7 +660,7 @@ class Engine {
                     k_batch_text<uint8_t><<<grid(chars), 256, 0, stream>>>(d_texts, in_off[0], in_off[cnt], d_in.get(),
                                                                             d_co.get(), cnt, dlut.get(), jt.get());
                 k_batch_seps<uint8_t><<<grid(cnt), 256, 0, stream>>>(d_co.get(), cnt, jt.get());
-                level<uint8_t>(jt.get(), ident.get(), N, maxchar, 0, sa.get(), nullptr);
+                level<uint8_t>(jt.get(), ident.get(), N, maxchar, 0, nullptr, nullptr, &sa);
             } else {
                 DBuf<uint32_t> jt(cache, (size_t)N + 16);
                 if (chars)
@@ -645,7 +668,7 @@ class Engine {
                                                                              d_in.get(), d_co.get(), cnt, dlut.get(),
                                                                              jt.get());
                 k_batch_seps<uint32_t><<<grid(cnt), 256, 0, stream>>>(d_co.get(), cnt, jt.get());
-                level<uint32_t>(jt.get(), nullptr, N, maxchar, 0, sa.get(), nullptr);
+                level<uint32_t>(jt.get(), nullptr, N, maxchar, 0, nullptr, nullptr, &sa);
             }
             // stable partition of the joined SA by text; separators sort last
             DBuf<uint64_t> ka(cache, N), kb(cache, N);
@@ -740,11 +763,11 @@ class Engine {
         sigma = alphabet(d_text, n, dlut);
         if (st) st->sigma = sigma;
 
-        DBuf<uint32_t> sa(cache, n);
+        DBuf<uint32_t> sa;
         // phase boundaries default to "empty" (a base-case level never reaches them)
         DCX_HIP(hipEventRecord(ev[2], stream));
         DCX_HIP(hipEventRecord(ev[3], stream));
-        level<uint8_t>(d_text, dlut.get(), (uint32_t)n, sigma, 0, sa.get(), ev);
+        level<uint8_t>(d_text, dlut.get(), (uint32_t)n, sigma, 0, nullptr, ev, &sa);
         DCX_HIP(hipEventRecord(ev[4], stream));  // SA resident in HBM
         // output in the requested width
         if (o->sa_on_device) {

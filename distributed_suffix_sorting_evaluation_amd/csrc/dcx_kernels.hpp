@@ -65,7 +65,24 @@ struct DcDev {
     uint8_t in_dc[MAX_X + 3];      // r in DC (difference_cover.hpp IN_DC)
     uint32_t bnd;                  // first position whose sample-key window reaches past the text end
     uint64_t inv_X, inv_D;         // ceil(2^32 / X), ceil(2^32 / D) for division without a divide
+    uint64_t in_mask[3];           // bit r set iff r in DC (X <= 133 < 192)
 };
+
+// r in DC, from the mask (no table load)
+__device__ __forceinline__ uint32_t dc_in(const DcDev& c, uint32_t r) {
+    const uint32_t g = r >> 6;
+    const uint64_t w = g == 0 ? c.in_mask[0] : (g == 1 ? c.in_mask[1] : c.in_mask[2]);
+    return (uint32_t)(w >> (r & 63)) & 1u;
+}
+// #{d in DC : d < r} (NEXT_RANK), as population counts of the mask below r
+__device__ __forceinline__ uint32_t dc_next_rank(const DcDev& c, uint32_t r) {
+    const uint32_t g = r >> 6;
+    const uint64_t below = (1ull << (r & 63)) - 1;
+    const uint64_t a0 = g == 0 ? (c.in_mask[0] & below) : c.in_mask[0];
+    const uint64_t a1 = g == 0 ? 0ull : (g == 1 ? (c.in_mask[1] & below) : c.in_mask[1]);
+    const uint64_t a2 = g == 2 ? (c.in_mask[2] & below) : 0ull;
+    return (uint32_t)(__popcll(a0) + __popcll(a1) + __popcll(a2));
+}
 
 // floor(p / d) for 32-bit p with inv = ceil(2^32 / d): the product estimate is q or q + 1
 __device__ __forceinline__ uint32_t fast_div(uint32_t p, uint32_t d, uint64_t inv) {
@@ -93,7 +110,7 @@ __device__ __forceinline__ uint32_t class_major_inv(const DcDev& c, uint32_t cm)
 // index of the first sample at or after text position i
 __device__ __forceinline__ uint32_t rank_pos(const DcDev& c, uint32_t i) {
     uint32_t q = fast_div(i, c.X, c.inv_X);
-    return q * c.D + c.next_rank[i - q * c.X];
+    return q * c.D + dc_next_rank(c, i - q * c.X);
 }
 
 // ------------------------------------------------------------------------------------------------
@@ -113,66 +130,133 @@ struct TextU32 {
 };
 
 // ------------------------------------------------------------------------------------------------
-// alphabet histogram (compress_alphabet, cli.cpp:417-463)
+// byte alphabet (compress_alphabet, cli.cpp:417-463): which byte values occur, as a 256-bit mask.  Only
+// presence matters for the alphabet map, and a register mask per thread avoids the same-address LDS
+// atomics that per-byte counting of a small alphabet serializes on.
 // ------------------------------------------------------------------------------------------------
-__global__ __launch_bounds__(256) void k_byte_hist(const uint8_t* __restrict__ t, uint64_t n,
-                                                   unsigned long long* __restrict__ hist) {
-    __shared__ uint32_t s_h[4][256];
-    for (int i = threadIdx.x; i < 1024; i += 256) (&s_h[0][0])[i] = 0;
+__device__ __forceinline__ void mark_byte(uint32_t c, uint64_t (&m)[4]) {
+    const uint64_t bit = 1ull << (c & 63);
+    const uint32_t g = c >> 6;
+    m[0] |= g == 0 ? bit : 0ull;
+    m[1] |= g == 1 ? bit : 0ull;
+    m[2] |= g == 2 ? bit : 0ull;
+    m[3] |= g == 3 ? bit : 0ull;
+}
+
+__global__ __launch_bounds__(256) void k_byte_mask(const uint8_t* __restrict__ t, uint64_t n,
+                                                   unsigned long long* __restrict__ mask) {
+    __shared__ unsigned long long s_m[4];
+    if (threadIdx.x < 4) s_m[threadIdx.x] = 0;
     __syncthreads();
-    const uint32_t w = threadIdx.x / 64;
+    uint64_t m[4] = {0, 0, 0, 0};
     const uint64_t stride = (uint64_t)gridDim.x * 256 * 16;
     for (uint64_t base = (blockIdx.x * 256ull + threadIdx.x) * 16; base < n; base += stride) {
         if (base + 16 <= n && ((reinterpret_cast<uintptr_t>(t + base) & 15) == 0)) {
-            uint4 v = *reinterpret_cast<const uint4*>(t + base);
-            uint32_t x[4] = {v.x, v.y, v.z, v.w};
+            const uint4 v = *reinterpret_cast<const uint4*>(t + base);
+            const uint32_t x[4] = {v.x, v.y, v.z, v.w};
 #pragma unroll
             for (int q = 0; q < 4; q++)
 #pragma unroll
-                for (int b = 0; b < 4; b++) atomicAdd(&s_h[w][(x[q] >> (8 * b)) & 0xFF], 1u);
+                for (int b = 0; b < 4; b++) mark_byte((x[q] >> (8 * b)) & 0xFF, m);
         } else {
-            for (uint64_t i = base; i < base + 16 && i < n; i++) atomicAdd(&s_h[w][t[i]], 1u);
+            for (uint64_t i = base; i < base + 16 && i < n; i++) mark_byte(t[i], m);
         }
     }
+#pragma unroll
+    for (int g = 0; g < 4; g++) {
+#pragma unroll
+        for (int d = 1; d < 64; d <<= 1) m[g] |= __shfl_xor(m[g], d, 64);
+    }
+    if (__lane_id() == 0)
+        for (int g = 0; g < 4; g++)
+            if (m[g]) atomicOr(&s_m[g], (unsigned long long)m[g]);
     __syncthreads();
-    uint32_t c = s_h[0][threadIdx.x] + s_h[1][threadIdx.x] + s_h[2][threadIdx.x] + s_h[3][threadIdx.x];
-    if (c) atomicAdd(&hist[threadIdx.x], (unsigned long long)c);
+    if (threadIdx.x < 4 && s_m[threadIdx.x]) atomicOr(&mask[threadIdx.x], s_m[threadIdx.x]);
 }
 
 // ------------------------------------------------------------------------------------------------
-// refinement sort: round 0 keys, heads, active compaction, later-round keys, write back
-// MODE 0: elements are samples (value = text-order sample index k); MODE 1: all positions.
+// refinement sort: packed characters, heads, active compaction, later-round keys, write back
 // ------------------------------------------------------------------------------------------------
-// Packed words in text order: W[i] = the cpw characters T[i .. i+cpw) left-justified (PackedInteger,
-// char_container.hpp:57-71), for i in [0, total).  Characters are staged through LDS per 4096-position tile.
-constexpr int WD_TILE = 4096;
-// dense != 0 codes character c as c - 1 (the padding also reads 0): one bit less whenever the largest
-// character is a power of two.  Only the merge uses dense words; its tie-break handles the padding.
+// The reference packs a fixed number of characters per word (PackedInteger, char_container.hpp:57-71);
+// here the whole text becomes one bit stream and any window of it is a key.  dense != 0 codes character
+// c as c - 1 (the padding also reads 0): one bit less whenever the largest character is a power of two.
+// Character codes packed into an MSB-first bit stream, `bits` bits per character: code of character q
+// at stream bits [q * bits, (q + 1) * bits), codes lut[c] - dense for q < n and 0 (padding) beyond.
+// The first L characters of suffix i are the top L * bits bits of the 64-bit window at bit i * bits
+// (StreamKeys): 1/8 to 1/32 of the bytes of one word per position.  A workgroup packs PK_CHARS
+// characters (a whole number of words for every width) staged in LDS.
+constexpr uint32_t PK_CHARS = 8192;
 template <class CharT>
-__global__ __launch_bounds__(256) void k_words(const CharT* __restrict__ t, uint32_t n, const uint16_t* __restrict__ lut,
-                                               uint32_t bits, uint32_t cpw, uint64_t total, uint32_t dense,
-                                               uint64_t* __restrict__ W) {
-    __shared__ uint32_t s_c[WD_TILE + 64];
+__global__ __launch_bounds__(256) void k_pack_text(const CharT* __restrict__ t, uint32_t n,
+                                                   const uint16_t* __restrict__ lut, uint32_t bits, uint32_t dense,
+                                                   uint64_t nwords, uint64_t* __restrict__ PT) {
+    __shared__ uint32_t s_c[PK_CHARS];
     __shared__ uint16_t s_lut[256];
-    if (sizeof(CharT) == 1)
-        for (int i = threadIdx.x; i < 256; i += 256) s_lut[i] = lut[i];
+    const uint32_t tid = threadIdx.x;
+    if (sizeof(CharT) == 1) s_lut[tid] = lut[tid];
     __syncthreads();
-    for (uint64_t base = (uint64_t)blockIdx.x * WD_TILE; base < total; base += (uint64_t)gridDim.x * WD_TILE) {
-        for (uint32_t i = threadIdx.x; i < WD_TILE + cpw; i += 256) {
-            uint64_t p = base + i;
+    const uint32_t wpb = PK_CHARS * bits / 64;  // words per chunk of PK_CHARS characters
+    for (uint64_t chunk = blockIdx.x; chunk * wpb < nwords; chunk += gridDim.x) {
+        const uint64_t q0 = chunk * PK_CHARS;
+        for (uint32_t i = tid; i < PK_CHARS; i += 256) {
+            const uint64_t q = q0 + i;
             uint32_t c = 0;
-            if (p < n) c = (sizeof(CharT) == 1 ? (uint32_t)s_lut[(uint32_t)t[p]] : (uint32_t)t[p]) - dense;
+            if (q < n) c = (sizeof(CharT) == 1 ? (uint32_t)s_lut[(uint32_t)t[q]] : (uint32_t)t[q]) - dense;
             s_c[i] = c;
         }
         __syncthreads();
-        for (uint32_t i = threadIdx.x; i < WD_TILE && base + i < total; i += 256) {
-            uint64_t w = 0;
-            for (uint32_t k = 0; k < cpw; k++) w |= (uint64_t)s_c[i + k] << (64 - (k + 1) * bits);
-            W[base + i] = w;
+        for (uint32_t wl = tid; wl < wpb; wl += 256) {
+            const uint64_t w = chunk * wpb + wl;
+            if (w >= nwords) break;
+            uint64_t word = 0;
+            const uint32_t qa = (64 * wl) / bits, qb = (64 * wl + 63) / bits;  // characters overlapping the word
+            for (uint32_t q = qa; q <= qb; q++) {
+                // bit position of the character's lowest bit within the word (negative: it continues in the
+                // next word)
+                const int32_t pos = (int32_t)(64 * (wl + 1)) - (int32_t)((q + 1) * bits);
+                const uint64_t c = s_c[q];
+                word |= pos >= 0 ? (c << pos) : (c >> (-pos));
+            }
+            PT[w] = word;
         }
         __syncthreads();
     }
 }
+
+// the same for widths that divide 64 (no character straddles two words): a thread packs the 64 / bits
+// characters of one word straight from global memory (consecutive loads the compiler merges)
+template <class CharT, uint32_t BITS>
+__global__ __launch_bounds__(256) void k_pack_aligned(const CharT* __restrict__ t, uint32_t n,
+                                                      const uint16_t* __restrict__ lut, uint32_t dense, uint64_t nwords,
+                                                      uint64_t* __restrict__ PT) {
+    constexpr uint32_t CPW = 64 / BITS;
+    __shared__ uint16_t s_lut[256];
+    if (sizeof(CharT) == 1) s_lut[threadIdx.x] = lut[threadIdx.x];
+    __syncthreads();
+    for (uint64_t w = blockIdx.x * 256ull + threadIdx.x; w < nwords; w += (uint64_t)gridDim.x * 256) {
+        const uint64_t q0 = w * CPW;
+        uint32_t c[CPW];
+        if (q0 + CPW <= n) {
+#pragma unroll
+            for (uint32_t k = 0; k < CPW; k++) c[k] = (uint32_t)t[q0 + k];
+        } else {
+#pragma unroll
+            for (uint32_t k = 0; k < CPW; k++) c[k] = q0 + k < n ? (uint32_t)t[q0 + k] : 0xFFFFFFFFu;
+        }
+        uint64_t word = 0;
+#pragma unroll
+        for (uint32_t k = 0; k < CPW; k++) {
+            const uint32_t code = c[k] == 0xFFFFFFFFu ? 0u : (sizeof(CharT) == 1 ? (uint32_t)s_lut[c[k]] : c[k]) - dense;
+            word |= (uint64_t)code << (64 - (k + 1) * BITS);
+        }
+        PT[w] = word;
+    }
+}
+
+// words of the stream of `chars` characters, with the slack word StreamKeys reads past a window
+__host__ __device__ inline uint64_t stream_words(uint64_t chars, uint32_t bits) { return (chars * bits + 63) / 64 + 2; }
+
+__host__ __device__ inline uint64_t top_mask(uint32_t nbits) { return nbits >= 64 ? ~0ull : (nbits == 0 ? 0ull : ~0ull << (64 - nbits)); }
 
 __device__ __forceinline__ uint64_t top_bits(uint64_t w, uint32_t nbits) {
     return nbits >= 64 ? w : (nbits == 0 ? 0ull : (w & (~0ull << (64 - nbits))));
@@ -223,13 +307,14 @@ struct ActiveScanOp {
     }
 };
 
-__global__ __launch_bounds__(256) void k_keys_rr(const uint64_t* __restrict__ W, uint32_t A,
+__global__ __launch_bounds__(256) void k_keys_rr(const uint64_t* __restrict__ PT, uint32_t bits, uint32_t A,
                                                  const uint32_t* __restrict__ act_idx, const uint32_t* __restrict__ gid,
                                                  const uint32_t* __restrict__ order, uint32_t C, uint32_t cbits,
                                                  uint32_t gbits, uint64_t* __restrict__ keys, uint32_t* __restrict__ vals) {
+    const StreamKeys<false> chars{PT, bits, top_mask(cbits)};
     for (uint32_t a = blockIdx.x * 256 + threadIdx.x; a < A; a += gridDim.x * 256) {
         uint32_t p = order[act_idx[a]];
-        uint64_t ch = top_bits(W[(uint64_t)p + C], cbits);
+        uint64_t ch = chars((uint64_t)p + C);
         keys[a] = gbits ? (((uint64_t)gid[a] << (64 - gbits)) | (ch >> gbits)) : ch;
         vals[a] = p;
     }
@@ -308,14 +393,14 @@ struct SampleScanOp {
     __device__ __forceinline__ uint64_t value(uint64_t j) const {
         const uint32_t p = order[j];
         const uint32_t q = fast_div(p, dc.X, dc.inv_X);
-        return (uint64_t)dc.in_dc[p - q * dc.X] | ((uint64_t)H[j] << 32);
+        return (uint64_t)dc_in(dc, p - q * dc.X) | ((uint64_t)H[j] << 32);
     }
     __device__ __forceinline__ void write(uint64_t j, uint64_t excl, uint64_t v) const {
         if (v & 1u) {
             const uint32_t p = order[j];
             const uint32_t q = fast_div(p, dc.X, dc.inv_X);
             const uint32_t t = (uint32_t)excl + shift;
-            s_order[t] = q * dc.D + dc.next_rank[p - q * dc.X];
+            s_order[t] = q * dc.D + dc_next_rank(dc, p - q * dc.X);
             s_gid[t] = (uint32_t)(excl >> 32) + (uint32_t)(v >> 32);
         }
     }
@@ -494,8 +579,8 @@ __device__ __forceinline__ bool dc_rank_less(uint32_t a, uint32_t b, const DcDev
     uint32_t ma = a - qa * dc.X, mb = b - qb * dc.X;
     uint32_t e = cmp[ma * dc.X + mb];
     uint32_t r1 = (e >> 8) & 0xFF, r2 = (e >> 16) & 0xFF;
-    uint32_t ra = ranks[qa * dc.D + dc.next_rank[ma] + r1];
-    uint32_t rb = ranks[qb * dc.D + dc.next_rank[mb] + r2];
+    uint32_t ra = ranks[qa * dc.D + dc_next_rank(dc, ma) + r1];
+    uint32_t rb = ranks[qb * dc.D + dc_next_rank(dc, mb) + r2];
     return ra < rb;
 }
 

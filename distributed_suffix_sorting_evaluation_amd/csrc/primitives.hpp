@@ -54,17 +54,43 @@ __global__ void k_fill_u8(uint8_t* p, uint8_t v, uint64_t n) {
         p[i] = v;
 }
 
+// ------------------------------------------------------------------------------------------------
+// Key sources of a radix sort's first pass: key(i) for input index i.
+// ------------------------------------------------------------------------------------------------
+template <class KeyT>
+struct ArrayKeys {  // keys[i]
+    const KeyT* k;
+    __device__ __forceinline__ uint64_t operator()(uint64_t i) const { return (uint64_t)k[i]; }
+};
+
+// The 64-bit window at bit offset i * bits of an MSB-first bit stream (one element of `bits` bits per
+// index), masked to its top bits; PACKPOS: the index in the low bits (below the mask).  The stream
+// needs one readable word past the last window's first word.
+template <bool PACKPOS>
+struct StreamKeys {
+    const uint64_t* s;
+    uint32_t bits;
+    uint64_t mask;
+    __device__ __forceinline__ uint64_t operator()(uint64_t i) const {
+        const uint64_t o = i * bits;
+        const uint64_t w = o >> 6;
+        const uint32_t sh = (uint32_t)(o & 63);
+        uint64_t x = s[w] << sh;
+        if (sh) x |= s[w + 1] >> (64 - sh);
+        return PACKPOS ? ((x & mask) | i) : (x & mask);
+    }
+};
+
 // Histograms of all digit passes [lo_bit, lo_bit + 8*npasses) from one read of the keys.
-// PACKPOS: the keys of the first pass are (kin & hi_mask) | index (position packed into the low bits)
-template <bool PACKPOS, class KeyT = uint64_t>
-__global__ __launch_bounds__(256) void k_radix_hist(const KeyT* __restrict__ keys, uint32_t N, uint32_t lo_bit,
-                                                    uint32_t npasses, uint64_t hi_mask, uint32_t* __restrict__ hist) {
+template <class Src>
+__global__ __launch_bounds__(256) void k_radix_hist(Src src, uint32_t N, uint32_t lo_bit, uint32_t npasses,
+                                                    uint32_t* __restrict__ hist) {
     __shared__ uint32_t s_h[8][RS_RADIX];
     for (int i = threadIdx.x; i < 8 * RS_RADIX; i += 256) (&s_h[0][0])[i] = 0;
     __syncthreads();
     const uint64_t stride = (uint64_t)gridDim.x * 256;
     for (uint64_t i = blockIdx.x * 256ull + threadIdx.x; i < N; i += stride) {
-        uint64_t k = PACKPOS ? (((uint64_t)keys[i] & hi_mask) | i) : (uint64_t)keys[i];
+        const uint64_t k = src(i);
         for (uint32_t p = 0; p < npasses; p++) atomicAdd(&s_h[p][(k >> (lo_bit + 8 * p)) & 0xFF], 1u);
     }
     __syncthreads();
@@ -72,6 +98,12 @@ __global__ __launch_bounds__(256) void k_radix_hist(const KeyT* __restrict__ key
         uint32_t c = s_h[p][threadIdx.x];
         if (c) atomicAdd(&hist[p * RS_RADIX + threadIdx.x], c);
     }
+}
+
+// keys[i] = src(i) for i < N (sorts of a single element)
+template <class Src>
+__global__ void k_materialize(Src src, uint32_t N, uint64_t* __restrict__ keys) {
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < N; i += gridDim.x * blockDim.x) keys[i] = src(i);
 }
 
 // In-place exclusive scan of each pass's 256 counts -> global digit offsets.
@@ -83,16 +115,16 @@ __global__ __launch_bounds__(256) void k_radix_hist_scan(uint32_t* hist) {
     h[threadIdx.x] = e;
 }
 
-// One LSD pass over digit (key >> shift) & 255.  Stable.
-// IDENT: values are the input indices.  PACKPOS: keys are (kin & hi_mask) | input index (keys only).
-// KeyT: 64-bit sort keys or 32-bit keys (radix partitions by destination index).
-template <bool VALS, bool IDENT, bool PACKPOS = false, class KeyT = uint64_t>
-__global__ __launch_bounds__(RS_THREADS) void k_onesweep(const KeyT* __restrict__ kin, KeyT* __restrict__ kout,
+// One LSD pass over digit (key >> shift) & 255.  Stable.  Keys come from `src` (the previous pass's
+// output, or a first-pass key source) and go to kout as KeyT: 64-bit sort keys or 32-bit keys (radix
+// partitions by destination index).  IDENT: values are the input indices.
+template <bool VALS, bool IDENT, class Src, class KeyT = uint64_t>
+__global__ __launch_bounds__(RS_THREADS) void k_onesweep(Src src, KeyT* __restrict__ kout,
                                                          const uint32_t* __restrict__ vin, uint32_t* __restrict__ vout,
                                                          uint32_t N, uint32_t shift, const uint32_t* __restrict__ dbase,
                                                          uint64_t* __restrict__ status, uint32_t tag,
                                                          unsigned long long* __restrict__ ticket,
-                                                         unsigned long long ticket_base, uint64_t hi_mask) {
+                                                         unsigned long long ticket_base) {
     __shared__ KeyT s_keys[RS_TILE];
     __shared__ uint32_t s_wcnt[RS_WAVES][RS_RADIX];  // per-wave digit counts, then wave offsets
     __shared__ uint32_t s_start[RS_RADIX];
@@ -117,7 +149,7 @@ __global__ __launch_bounds__(RS_THREADS) void k_onesweep(const KeyT* __restrict_
     for (int k = 0; k < RS_ITEMS; k++) {
         uint64_t idx = wbase + k * 64 + lane;
         bool valid = idx < N;
-        key[k] = valid ? (PACKPOS ? (((uint64_t)kin[idx] & hi_mask) | idx) : (uint64_t)kin[idx]) : 0ull;
+        key[k] = valid ? src(idx) : 0ull;
         if (VALS) val[k] = IDENT ? (uint32_t)idx : (valid ? vin[idx] : 0u);
     }
     // rank every key among the keys of its wave with the same digit, in tile order
@@ -623,18 +655,19 @@ struct Primitives {
         scan_tiles_cap = cap;
         scan_tag = 0;
     }
-    // Sorts (keys, vals) by key bits [lo_bit, hi_bit).  Key bits inside the pass range but outside
-    // [lo_bit, hi_bit) must be equal for all keys.  On return keys/vals point at the sorted data
-    // (the buffers may have been swapped with the alternates).
-    // pack_mask != 0 (keys only, with first_keys): the first pass sorts (first_keys & pack_mask) | index.
-    void sort_pairs(uint64_t*& keys, uint64_t*& keys_alt, uint32_t*& vals, uint32_t*& vals_alt, uint32_t N,
-                    uint32_t lo_bit, uint32_t hi_bit, const uint64_t* first_keys = nullptr, bool ident_vals = false,
-                    uint64_t pack_mask = 0) {
+
+    // LSD radix sort of (u64 key, u32 value) pairs by key bits [lo_bit, hi_bit) (rounded up to whole
+    // bytes); bits above hi_bit must be equal for all keys.  The first pass reads its keys from `first`
+    // (ArrayKeys{keys}: the keys array itself); with ident_vals its values are the input indices (vals
+    // is then only an output).  vals == nullptr sorts keys only.  On return keys/vals point at the
+    // sorted data (the buffers may have been swapped with the alternates).
+    template <class Src>
+    void sort_pairs_from(const Src& first, uint64_t*& keys, uint64_t*& keys_alt, uint32_t*& vals,
+                         uint32_t*& vals_alt, uint32_t N, uint32_t lo_bit, uint32_t hi_bit, bool ident_vals) {
         if (hi_bit <= lo_bit) hi_bit = lo_bit + 1;
         if (N <= 1) {
             // nothing to order: materialize the inputs in place
-            if (N == 1 && first_keys)
-                DCX_HIP(hipMemcpyAsync(keys, first_keys, 8, hipMemcpyDeviceToDevice, stream));
+            if (N == 1) hipLaunchKernelGGL(k_materialize<Src>, dim3(1), dim3(64), 0, stream, first, N, keys);
             if (N == 1 && ident_vals && vals) fill_u32(vals, 0, 1);
             return;
         }
@@ -644,14 +677,8 @@ struct Primitives {
         uint64_t tiles = ceil_div(N, RS_TILE);
         ensure_sort_status(tiles);
         fill_u32(hist, 0, 8 * RS_RADIX);
-        const uint64_t* hsrc = first_keys ? first_keys : keys;
-        const bool packpos = pack_mask != 0 && first_keys && !vals;
-        if (packpos)
-            hipLaunchKernelGGL(k_radix_hist<true>, dim3(grid_for(N, 256, 2048)), dim3(256), 0, stream, hsrc, N, lo_bit,
-                               np, pack_mask, hist);
-        else
-            hipLaunchKernelGGL(k_radix_hist<false>, dim3(grid_for(N, 256, 2048)), dim3(256), 0, stream, hsrc, N,
-                               lo_bit, np, (uint64_t)0, hist);
+        hipLaunchKernelGGL(k_radix_hist<Src>, dim3(grid_for(N, 256, 2048)), dim3(256), 0, stream, first, N, lo_bit, np,
+                           hist);
         hipLaunchKernelGGL(k_radix_hist_scan, dim3(np), dim3(256), 0, stream, hist);
         for (uint32_t p = 0; p < np; p++) {
             if (sort_tag >= LB_TAG_LIMIT) {
@@ -665,24 +692,33 @@ struct Primitives {
                 DCX_HIP(hipEventCreate(&e1));
                 DCX_HIP(hipEventRecord(e0, stream));
             }
-            const bool first = p == 0;
-            const uint64_t* kin = (first && first_keys) ? first_keys : keys;
-            if (!vals && first && packpos)
-                hipLaunchKernelGGL((k_onesweep<false, false, true>), dim3((unsigned)tiles), dim3(RS_THREADS), 0, stream,
-                                   kin, keys_alt, (const uint32_t*)nullptr, (uint32_t*)nullptr, N, lo_bit + 8 * p,
-                                   hist + p * RS_RADIX, sort_status, tag, ticket, ticket_host, pack_mask);
-            else if (!vals)
-                hipLaunchKernelGGL((k_onesweep<false, false>), dim3((unsigned)tiles), dim3(RS_THREADS), 0, stream, kin,
-                                   keys_alt, (const uint32_t*)nullptr, (uint32_t*)nullptr, N, lo_bit + 8 * p,
-                                   hist + p * RS_RADIX, sort_status, tag, ticket, ticket_host, (uint64_t)0);
-            else if (first && ident_vals)
-                hipLaunchKernelGGL((k_onesweep<true, true>), dim3((unsigned)tiles), dim3(RS_THREADS), 0, stream, kin,
-                                   keys_alt, (const uint32_t*)nullptr, vals_alt, N, lo_bit + 8 * p,
-                                   hist + p * RS_RADIX, sort_status, tag, ticket, ticket_host, (uint64_t)0);
-            else
-                hipLaunchKernelGGL((k_onesweep<true, false>), dim3((unsigned)tiles), dim3(RS_THREADS), 0, stream, kin,
-                                   keys_alt, vals, vals_alt, N, lo_bit + 8 * p, hist + p * RS_RADIX, sort_status, tag,
-                                   ticket, ticket_host, (uint64_t)0);
+            const uint32_t shift = lo_bit + 8 * p;
+            const uint32_t* dbase = hist + p * RS_RADIX;
+            const dim3 g((unsigned)tiles), b(RS_THREADS);
+            if (p == 0) {
+                if (!vals)
+                    hipLaunchKernelGGL((k_onesweep<false, false, Src>), g, b, 0, stream, first, keys_alt,
+                                       (const uint32_t*)nullptr, (uint32_t*)nullptr, N, shift, dbase, sort_status, tag,
+                                       ticket, ticket_host);
+                else if (ident_vals)
+                    hipLaunchKernelGGL((k_onesweep<true, true, Src>), g, b, 0, stream, first, keys_alt,
+                                       (const uint32_t*)nullptr, vals_alt, N, shift, dbase, sort_status, tag, ticket,
+                                       ticket_host);
+                else
+                    hipLaunchKernelGGL((k_onesweep<true, false, Src>), g, b, 0, stream, first, keys_alt,
+                                       (const uint32_t*)vals, vals_alt, N, shift, dbase, sort_status, tag, ticket,
+                                       ticket_host);
+            } else {
+                const ArrayKeys<uint64_t> in{keys};
+                if (!vals)
+                    hipLaunchKernelGGL((k_onesweep<false, false, ArrayKeys<uint64_t>>), g, b, 0, stream, in, keys_alt,
+                                       (const uint32_t*)nullptr, (uint32_t*)nullptr, N, shift, dbase, sort_status, tag,
+                                       ticket, ticket_host);
+                else
+                    hipLaunchKernelGGL((k_onesweep<true, false, ArrayKeys<uint64_t>>), g, b, 0, stream, in, keys_alt,
+                                       (const uint32_t*)vals, vals_alt, N, shift, dbase, sort_status, tag, ticket,
+                                       ticket_host);
+            }
             if (profile) {
                 DCX_HIP(hipEventRecord(e1, stream));
                 pass_events.emplace_back(e0, e1);
@@ -692,6 +728,12 @@ struct Primitives {
             std::swap(keys, keys_alt);
             if (vals) std::swap(vals, vals_alt);
         }
+    }
+
+    // the same with the keys array as the first pass's key source
+    void sort_pairs(uint64_t*& keys, uint64_t*& keys_alt, uint32_t*& vals, uint32_t*& vals_alt, uint32_t N,
+                    uint32_t lo_bit, uint32_t hi_bit) {
+        sort_pairs_from(ArrayKeys<uint64_t>{keys}, keys, keys_alt, vals, vals_alt, N, lo_bit, hi_bit, false);
     }
 
     // Stable LSD radix sort of (u32 key, u32 value) pairs by key bits [lo_bit, lo_bit + 8 * np) (np <= 4).
@@ -709,8 +751,8 @@ struct Primitives {
         const uint64_t tiles = ceil_div(N, RS_TILE);
         ensure_sort_status(tiles);
         fill_u32(hist, 0, np * RS_RADIX);
-        hipLaunchKernelGGL((k_radix_hist<false, uint32_t>), dim3(grid_for(N, 256, 2048)), dim3(256), 0, stream, keys, N,
-                           lo_bit, np, (uint64_t)0, hist);
+        hipLaunchKernelGGL(k_radix_hist<ArrayKeys<uint32_t>>, dim3(grid_for(N, 256, 2048)), dim3(256), 0, stream,
+                           ArrayKeys<uint32_t>{keys}, N, lo_bit, np, hist);
         hipLaunchKernelGGL(k_radix_hist_scan, dim3(np), dim3(256), 0, stream, hist);
         for (uint32_t p = 0; p < np; p++) {
             if (sort_tag >= LB_TAG_LIMIT) {
@@ -718,15 +760,15 @@ struct Primitives {
                 sort_tag = 0;
             }
             const uint32_t tag = ++sort_tag;
+            const ArrayKeys<uint32_t> in{kin};
             if (vin)
-                hipLaunchKernelGGL((k_onesweep<true, false, false, uint32_t>), dim3((unsigned)tiles), dim3(RS_THREADS), 0,
-                                   stream, kin, ko, vin, vo, N, lo_bit + 8 * p, (const uint32_t*)(hist + p * RS_RADIX),
-                                   sort_status, tag, ticket, ticket_host, (uint64_t)0);
+                hipLaunchKernelGGL((k_onesweep<true, false, ArrayKeys<uint32_t>, uint32_t>), dim3((unsigned)tiles),
+                                   dim3(RS_THREADS), 0, stream, in, ko, vin, vo, N, lo_bit + 8 * p,
+                                   (const uint32_t*)(hist + p * RS_RADIX), sort_status, tag, ticket, ticket_host);
             else
-                hipLaunchKernelGGL((k_onesweep<true, true, false, uint32_t>), dim3((unsigned)tiles), dim3(RS_THREADS), 0,
-                                   stream, kin, ko, (const uint32_t*)nullptr, vo, N, lo_bit + 8 * p,
-                                   (const uint32_t*)(hist + p * RS_RADIX), sort_status, tag, ticket, ticket_host,
-                                   (uint64_t)0);
+                hipLaunchKernelGGL((k_onesweep<true, true, ArrayKeys<uint32_t>, uint32_t>), dim3((unsigned)tiles),
+                                   dim3(RS_THREADS), 0, stream, in, ko, (const uint32_t*)nullptr, vo, N, lo_bit + 8 * p,
+                                   (const uint32_t*)(hist + p * RS_RADIX), sort_status, tag, ticket, ticket_host);
             ticket_host += tiles;
             *keys_out = ko;
             *vals_out = vo;
