@@ -29,6 +29,10 @@ constexpr int RS_ITEMS = DCX_RS_ITEMS;  // keys per thread of one radix tile
 constexpr int RS_TILE = RS_THREADS * RS_ITEMS;  // keys per workgroup
 constexpr int RS_WAVES = RS_THREADS / 64;
 constexpr int RS_RADIX = 256;
+#ifndef DCX_RS_LOOKBACK
+#define DCX_RS_LOOKBACK 16
+#endif
+constexpr int RS_LOOKBACK = DCX_RS_LOOKBACK;  // predecessors read per lookback round trip after the first
 
 // lookback word: [63:42] tag  [41:40] flag  [39:0] count
 constexpr uint64_t LB_VALUE_MASK = (1ull << 40) - 1;
@@ -187,16 +191,31 @@ __global__ __launch_bounds__(RS_THREADS) void k_onesweep(Src src, KeyT* __restri
         }
     }
     uint32_t start = block_excl_scan<RS_THREADS>(dthr ? total : 0u, s_scan, nullptr);
+    uint64_t* st = status + (uint64_t)tile * RS_RADIX + d;
     if (dthr) {
         s_start[d] = start;
-        // decoupled lookback: exclusive count of this digit over all preceding tiles
-        uint64_t* st = status + (uint64_t)tile * RS_RADIX + d;
+        // publish this tile's count of the digit: inclusive right away for tile 0
+        st_agent(st, lb_pack(tag, tile == 0 ? LB_INC : LB_AGG, total));
+    }
+    __syncthreads();
+    // reorder the tile by digit in LDS (the keys' registers are free for the lookback afterwards)
+#pragma unroll
+    for (int k = 0; k < RS_ITEMS; k++) {
+        uint64_t idx = wbase + k * 64 + lane;
+        uint32_t dk = rk[k] & 0xFFu;
+        if (idx < N) {
+            uint32_t lp = s_start[dk] + s_wcnt[wave][dk] + (rk[k] >> 8);
+            s_keys[lp] = (KeyT)key[k];
+        }
+    }
+    if (dthr) {
+        // decoupled lookback: exclusive count of this digit over all preceding tiles.  Walks run to a
+        // dozen tiles and more while tiles stream through the machine, so after the nearest predecessor
+        // (polled alone: it is the one still likely to be unpublished) RS_LOOKBACK predecessors are read
+        // per memory round trip.
         uint64_t excl = 0;
-        if (tile == 0) {
-            st_agent(st, lb_pack(tag, LB_INC, total));
-        } else {
-            st_agent(st, lb_pack(tag, LB_AGG, total));
-            int64_t j = (int64_t)tile - 1;
+        if (tile > 0) {
+            int64_t j = (int64_t)tile - 1;  // next predecessor to add; tile 0 is always inclusive
             while (true) {
                 uint64_t w = ld_agent(status + (uint64_t)j * RS_RADIX + d);
                 if ((uint32_t)(w >> 42) != tag || (w & LB_FLAG_MASK) == 0) {
@@ -206,21 +225,28 @@ __global__ __launch_bounds__(RS_THREADS) void k_onesweep(Src src, KeyT* __restri
                 excl += w & LB_VALUE_MASK;
                 if (w & LB_INC) break;
                 --j;
+                const int cnt = (int)(j + 1 < RS_LOOKBACK ? j + 1 : RS_LOOKBACK);
+                uint64_t wb[RS_LOOKBACK];
+#pragma unroll
+                for (int b = 0; b < RS_LOOKBACK; b++)
+                    wb[b] = b < cnt ? ld_agent(status + (uint64_t)(j - b) * RS_RADIX + d) : 0ull;
+                bool stop = false, inc = false;
+#pragma unroll
+                for (int b = 0; b < RS_LOOKBACK; b++) {
+                    if (stop || b >= cnt) continue;
+                    if ((uint32_t)(wb[b] >> 42) != tag || (wb[b] & LB_FLAG_MASK) == 0) {
+                        stop = true;  // unpublished: polled alone at the top of the loop
+                        continue;
+                    }
+                    excl += wb[b] & LB_VALUE_MASK;
+                    --j;
+                    if (wb[b] & LB_INC) inc = stop = true;
+                }
+                if (inc) break;
             }
             st_agent(st, lb_pack(tag, LB_INC, excl + total));
         }
         s_gofs[d] = dbase[d] + (uint32_t)excl - start;
-    }
-    __syncthreads();
-    // reorder the tile by digit in LDS, then write each digit run contiguously
-#pragma unroll
-    for (int k = 0; k < RS_ITEMS; k++) {
-        uint64_t idx = wbase + k * 64 + lane;
-        uint32_t dk = rk[k] & 0xFFu;
-        if (idx < N) {
-            uint32_t lp = s_start[dk] + s_wcnt[wave][dk] + (rk[k] >> 8);
-            s_keys[lp] = (KeyT)key[k];
-        }
     }
     __syncthreads();
     // write out: position i of the reordered tile goes to gofs[digit] + i; the digits of a thread's
