@@ -1,11 +1,13 @@
 // dcx_engine.hip -- MI355X DCX suffix-array engine: level driver and the C ABI of include/dcx.h.
 //
 // One recursion level (PDCX::compute_sa, lightweight_dcx/src/pdcx/pdcx.hpp:529-941) runs as:
-//   phase 1+2  sample keys (packed characters) -> refinement radix sort -> names by prefix count
+//   phase 1+2  character codes packed into a bit stream -> one refinement radix sort of all suffixes by
+//              their first L_s characters -> the samples compacted out of that order, names from their
+//              group heads
 //   phase 3    names distinct: ranks directly; otherwise recursive string in (mod X, div X) order,
-//              discarding decision, recursion, ranks from the recursive SA
-//   phase 4    every suffix keyed by its packed (X-1)-character prefix -> refinement radix sort ->
-//              groups of equal prefixes ordered by the DC comparator on the sample ranks
+//              discarding decision, recursion, ranks from the recursive SA (permutation scatters)
+//   phase 4    groups of suffixes still tied after L_s characters ordered by the DC comparator on the
+//              sample ranks; the sorted order is the level's SA
 // All data stays in HBM; the host only reads back a handful of scalars per step.
 #include <algorithm>
 #include <chrono>

@@ -1,11 +1,13 @@
 // primitives.hpp -- device-wide building blocks of the DCX engine, written for gfx950 (wave64):
 //
-//  * LSD radix sort of (u64 key, u32 value) pairs, one kernel launch per 8-bit digit.  Each pass is
-//    a "onesweep" pass: a workgroup ranks a 7168-key tile with wavefront ballots (no per-key
-//    atomics), publishes its per-digit counts with a decoupled lookback over the preceding tiles,
-//    reorders the tile in LDS by digit and writes each digit run contiguously.  One upfront kernel
-//    builds the histograms of every pass from a single read of the keys.
-//  * single-pass device scan with decoupled lookback over a functor (flags -> positions, names ...).
+//  * LSD radix sort of (u64 key, u32 value) pairs (and u32 keys for permutation scatters), one kernel
+//    launch per 8-bit digit.  Each pass is a "onesweep" pass: a workgroup ranks a 7168-key tile with
+//    wavefront ballots (no per-key atomics), reorders it by digit in LDS, obtains its per-digit output
+//    offsets with a decoupled lookback over the preceding tiles and writes each digit run contiguously.
+//    One upfront kernel builds the histograms of every pass from a single read of the first-pass keys,
+//    which may come from a key source (e.g. windows of a packed character stream) instead of an array.
+//  * device scans over a functor (flags -> positions, names ...): two-pass reduce-then-scan for long
+//    inputs, single-pass decoupled lookback for short ones.
 //
 // These replace the reference's distributed sorts and exclusive prefix sums
 // (lightweight_dcx/src/sorters/sample_sort.hpp:37-171, lightweight_dcx/src/mpi/reduce.hpp ex_prefix_sum).
