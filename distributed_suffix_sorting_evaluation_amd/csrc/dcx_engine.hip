@@ -215,20 +215,8 @@ class Engine {
                 continue;
             }
             if (c == 0) throw Error(DCX_E_UNSUPPORTED, "character width too large for refinement");
-            DBuf<uint8_t> Ha;
-            {
-                DBuf<uint64_t> rka(cache, A), rkb(cache, A);
-                DBuf<uint32_t> rva(cache, A), rvb(cache, A);
-                k_keys_rr<<<grid(A), 256, 0, stream>>>(PT, bits, A, cur.act.get(), cur.gid.get(), order.get(), C, c * bits,
-                                                       gbits, rka.get(), rva.get());
-                uint64_t* x = rka.get();
-                uint64_t* x2 = rkb.get();
-                uint32_t* y = rva.get();
-                uint32_t* y2 = rvb.get();
-                prim.sort_pairs(x, x2, y, y2, A, 64 - gbits - c * bits, 64);
-                Ha = DBuf<uint8_t>(cache, (size_t)A + 1);
-                k_writeback<<<grid(A), 256, 0, stream>>>(x, y, cur.act.get(), A, order.get(), H.get(), 1, Ha.get());
-            }
+            DBuf<uint8_t> Ha(cache, (size_t)A + 1);
+            refine_round(PT, bits, C, c, cur, order, H, Ha);
             C += c;
             ActiveSet next;
             active_set(Ha.get(), cur.act.get(), A, next, true);
@@ -260,6 +248,37 @@ class Engine {
         uint32_t *ks = nullptr, *vs = nullptr;
         prim.sort_u32(dest, val, k1.get(), v1.get(), k2.get(), v2.get(), N, wbits, np, &ks, &vs);
         k_scatter_window<<<(unsigned)ceil_div((uint64_t)N, 1ull << wbits), 256, 0, stream>>>(ks, vs, N, add, wbits, out);
+    }
+
+    // One character round of the refinement: orders the members of every tied group of `cur` by their
+    // next c characters (after the first C), writes the new group starts into H and the compact heads of
+    // the list into Ha (A + 1 entries).  Small groups are sorted in place by one thread each; the members
+    // of large groups go through a radix sort keyed by (group, characters).
+    void refine_round(const uint64_t* PT, uint32_t bits, uint32_t C, uint32_t c, const ActiveSet& cur,
+                      DBuf<uint32_t>& order, DBuf<uint8_t>& H, DBuf<uint8_t>& Ha) {
+        const uint32_t A = cur.A, G = cur.G;
+        DBuf<uint32_t> gstart(cache, (size_t)G + 1);
+        k_group_starts<<<grid(A), 256, 0, stream>>>(cur.act.get(), cur.gid.get(), H.get(), A, gstart.get());
+        prim.fill_u32(gstart.get() + G, A, 1);
+        DBuf<uint8_t> large(cache, G);
+        k_refine_small<<<grid(G), 256, 0, stream>>>(gstart.get(), G, cur.act.get(), order.get(), H.get(), Ha.get(), PT,
+                                                    bits, C, c * bits, large.get());
+        prim.fill_u8(Ha.get() + A, 1, 1);
+        DBuf<uint32_t> slots(cache, A);
+        prim.scan(LargeMembersOp{large.get(), cur.gid.get(), slots.get()}, A, 1);
+        const uint32_t B = (uint32_t)prim.read_scalars(2)[1];
+        if (B == 0) return;
+        const uint32_t gbits = bit_width(G - 1);
+        DBuf<uint64_t> rka(cache, B), rkb(cache, B);
+        DBuf<uint32_t> rva(cache, B), rvb(cache, B);
+        k_keys_large<<<grid(B), 256, 0, stream>>>(PT, bits, B, slots.get(), cur.act.get(), cur.gid.get(), order.get(), C,
+                                                  c * bits, gbits, rka.get(), rva.get());
+        uint64_t* x = rka.get();
+        uint64_t* x2 = rkb.get();
+        uint32_t* y = rva.get();
+        uint32_t* y2 = rvb.get();
+        prim.sort_pairs(x, x2, y, y2, B, 64 - gbits - c * bits, 64);
+        k_writeback_large<<<grid(B), 256, 0, stream>>>(x, y, slots.get(), B, cur.act.get(), order.get(), H.get(), Ha.get());
     }
 
     // dense words only: boundary suffixes that share a key group are moved to its front (k_boundary_fix)

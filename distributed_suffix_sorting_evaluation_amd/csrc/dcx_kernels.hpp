@@ -590,6 +590,100 @@ __global__ __launch_bounds__(256) void k_group_starts(const uint32_t* __restrict
         if (H[act_idx[a]]) gstart[gid[a]] = a;
 }
 
+// ------------------------------------------------------------------------------------------------
+// Refinement rounds by group size: a tied group of at most RF_SMALL suffixes is ordered by one thread
+// (insertion sort of the members' next characters); the members of larger groups go through the radix
+// sort.  Both write the new group heads into H and the compact heads Ha of the active list.
+// ------------------------------------------------------------------------------------------------
+constexpr uint32_t RF_SMALL = 16;
+
+__global__ __launch_bounds__(256) void k_refine_small(const uint32_t* __restrict__ gstart, uint32_t G,
+                                                      const uint32_t* __restrict__ act_idx, uint32_t* __restrict__ order,
+                                                      uint8_t* __restrict__ H, uint8_t* __restrict__ Ha,
+                                                      const uint64_t* __restrict__ PT, uint32_t bits, uint32_t C,
+                                                      uint32_t cbits, uint8_t* __restrict__ large) {
+    const StreamKeys<false> chars{PT, bits, top_mask(cbits)};
+    for (uint32_t g = blockIdx.x * 256 + threadIdx.x; g < G; g += gridDim.x * 256) {
+        const uint32_t a0 = gstart[g], s = gstart[g + 1] - a0;
+        if (s > RF_SMALL) {
+            large[g] = 1;
+            continue;
+        }
+        large[g] = 0;
+        const uint32_t j0 = act_idx[a0];  // a group occupies consecutive slots of the order
+        uint64_t k[RF_SMALL];
+        uint32_t e[RF_SMALL];
+        for (uint32_t t = 0; t < s; t++) {
+            e[t] = order[j0 + t];
+            k[t] = chars((uint64_t)e[t] + C);
+        }
+        for (uint32_t t = 1; t < s; t++) {  // stable: equal keys keep their order
+            const uint64_t kx = k[t];
+            const uint32_t ex = e[t];
+            int u = (int)t - 1;
+            while (u >= 0 && kx < k[u]) {
+                k[u + 1] = k[u];
+                e[u + 1] = e[u];
+                u--;
+            }
+            k[u + 1] = kx;
+            e[u + 1] = ex;
+        }
+        for (uint32_t t = 0; t < s; t++) {
+            order[j0 + t] = e[t];
+            const bool head = t == 0 || k[t] != k[t - 1];
+            Ha[a0 + t] = head ? 1 : 0;
+            if (head && t > 0) H[j0 + t] = 1;
+        }
+    }
+}
+
+// members of the large groups of the active list: their list indices, in list order
+struct LargeMembersOp {
+    static constexpr bool kBlocked = true;
+    const uint8_t* large;
+    const uint32_t* gid;
+    uint32_t* slots;
+    __device__ __forceinline__ uint64_t value(uint64_t a) const { return large[gid[a]]; }
+    __device__ __forceinline__ void write(uint64_t a, uint64_t excl, uint64_t v) const {
+        if (v) slots[excl] = (uint32_t)a;
+    }
+};
+
+// radix keys of the large-group members: (group << (64 - gbits)) | next cbits bits of characters
+__global__ __launch_bounds__(256) void k_keys_large(const uint64_t* __restrict__ PT, uint32_t bits, uint32_t B,
+                                                    const uint32_t* __restrict__ slots,
+                                                    const uint32_t* __restrict__ act_idx,
+                                                    const uint32_t* __restrict__ gid, const uint32_t* __restrict__ order,
+                                                    uint32_t C, uint32_t cbits, uint32_t gbits,
+                                                    uint64_t* __restrict__ keys, uint32_t* __restrict__ vals) {
+    const StreamKeys<false> chars{PT, bits, top_mask(cbits)};
+    for (uint32_t b = blockIdx.x * 256 + threadIdx.x; b < B; b += gridDim.x * 256) {
+        const uint32_t a = slots[b];
+        const uint32_t p = order[act_idx[a]];
+        const uint64_t ch = chars((uint64_t)p + C);
+        keys[b] = gbits ? (((uint64_t)gid[a] << (64 - gbits)) | (ch >> gbits)) : ch;
+        vals[b] = p;
+    }
+}
+
+// sorted large-group members back into the order, with their heads (H and the compact Ha)
+__global__ __launch_bounds__(256) void k_writeback_large(const uint64_t* __restrict__ keys,
+                                                         const uint32_t* __restrict__ vals,
+                                                         const uint32_t* __restrict__ slots, uint32_t B,
+                                                         const uint32_t* __restrict__ act_idx,
+                                                         uint32_t* __restrict__ order, uint8_t* __restrict__ H,
+                                                         uint8_t* __restrict__ Ha) {
+    for (uint32_t b = blockIdx.x * 256 + threadIdx.x; b < B; b += gridDim.x * 256) {
+        const uint32_t a = slots[b];
+        const uint32_t j = act_idx[a];
+        order[j] = vals[b];
+        const bool head = b == 0 || keys[b] != keys[b - 1];  // the group id is in the key
+        Ha[a] = head ? 1 : 0;
+        if (head) H[j] = 1;
+    }
+}
+
 constexpr uint32_t TB_SMALL = 16;
 
 // groups of size <= TB_SMALL: one thread sorts its group by insertion sort with the DC comparator;
