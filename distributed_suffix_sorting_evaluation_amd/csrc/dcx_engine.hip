@@ -27,6 +27,11 @@ struct Options {
     uint64_t base_case_chars = 10000;
 };
 
+#ifndef DCX_ROUND0_SLACK
+#define DCX_ROUND0_SLACK 8
+#endif
+constexpr uint32_t ROUND0_SLACK = DCX_ROUND0_SLACK;  // key bits beyond log2(N) in round 0 of large sorts
+
 class Engine {
   public:
     int device = 0;
@@ -107,7 +112,7 @@ class Engine {
     // active sets: members of groups of size >= 2 of the heads array H, as positions and group ids
     // -------------------------------------------------------------------------------------------
     struct ActiveSet {
-        DBuf<uint32_t> act, gid;
+        DBuf<uint32_t> act, gid, gstart;  // gstart: G + 1 entries, the last one A
         uint32_t A = 0, G = 0;
     };
 
@@ -116,13 +121,15 @@ class Engine {
     void active_set(const uint8_t* H, const uint32_t* src, uint32_t Nsrc, ActiveSet& out, bool compact = false) {
         out.act = DBuf<uint32_t>(cache, Nsrc);
         out.gid = DBuf<uint32_t>(cache, Nsrc);
+        out.gstart = DBuf<uint32_t>(cache, (size_t)Nsrc / 2 + 1);  // groups have >= 2 members
         if (src && !compact)
-            prim.scan(ActiveScanOp<true>{H, src, out.act.get(), out.gid.get()}, Nsrc, 0);
+            prim.scan(ActiveScanOp<true>{H, src, out.act.get(), out.gid.get(), out.gstart.get()}, Nsrc, 0);
         else
-            prim.scan(ActiveScanOp<false>{H, src, out.act.get(), out.gid.get()}, Nsrc, 0);
+            prim.scan(ActiveScanOp<false>{H, src, out.act.get(), out.gid.get(), out.gstart.get()}, Nsrc, 0);
         const uint64_t tot = prim.read_scalars(1)[0];
         out.A = (uint32_t)tot;
         out.G = (uint32_t)(tot >> 32);
+        prim.fill_u32(out.gstart.get() + out.G, out.A, 1);
     }
 
     // -------------------------------------------------------------------------------------------
@@ -160,8 +167,9 @@ class Engine {
             k_heads_packed<<<grid(N + 1ull), 256, 0, stream>>>(k, N, kbits, (1ull << pbits) - 1, H.get(), order.get());
         } else {
             if (N >= (1u << 24)) {
-                // large sorts: round 0 takes about log2(N) + 16 bits in whole 8-bit digits
-                const uint32_t want = std::min<uint32_t>(64, (bit_width(N) + 16 + 7) / 8 * 8);
+                // large sorts: round 0 takes about log2(N) + ROUND0_SLACK bits in whole 8-bit digits; the
+                // positions still tied after it go through the cheaper group rounds
+                const uint32_t want = std::min<uint32_t>(64, (bit_width(N) + ROUND0_SLACK + 7) / 8 * 8);
                 c0 = std::max<uint32_t>(1, std::min<uint32_t>(c0, want / bits));
             }
             DBuf<uint64_t> ka(cache, N), kb(cache, N);
@@ -257,12 +265,9 @@ class Engine {
     void refine_round(const uint64_t* PT, uint32_t bits, uint32_t C, uint32_t c, const ActiveSet& cur,
                       DBuf<uint32_t>& order, DBuf<uint8_t>& H, DBuf<uint8_t>& Ha) {
         const uint32_t A = cur.A, G = cur.G;
-        DBuf<uint32_t> gstart(cache, (size_t)G + 1);
-        k_group_starts<<<grid(A), 256, 0, stream>>>(cur.act.get(), cur.gid.get(), H.get(), A, gstart.get());
-        prim.fill_u32(gstart.get() + G, A, 1);
         DBuf<uint8_t> large(cache, G);
-        k_refine_small<<<grid(G), 256, 0, stream>>>(gstart.get(), G, cur.act.get(), order.get(), H.get(), Ha.get(), PT,
-                                                    bits, C, c * bits, large.get());
+        k_refine_small<<<grid(G), 256, 0, stream>>>(cur.gstart.get(), G, cur.act.get(), order.get(), H.get(), Ha.get(),
+                                                    PT, bits, C, c * bits, large.get());
         prim.fill_u8(Ha.get() + A, 1, 1);
         DBuf<uint32_t> slots(cache, A);
         prim.scan(LargeMembersOp{large.get(), cur.gid.get(), slots.get()}, A, 1);
@@ -294,15 +299,13 @@ class Engine {
         if (hc > BOUNDARY_CAP) throw Error(DCX_E_HIP, "boundary suffix count exceeds its bound");
         std::vector<uint32_t> g(hc);
         DCX_HIP(hipMemcpyAsync(g.data(), found.get(), hc * 4, hipMemcpyDeviceToHost, stream));
-        DBuf<uint32_t> gstart(cache, (size_t)tied.G + 1);
-        k_group_starts<<<grid(tied.A), 256, 0, stream>>>(tied.act.get(), tied.gid.get(), H.get(), tied.A, gstart.get());
-        prim.fill_u32(gstart.get() + tied.G, tied.A, 1);
+        const uint32_t* gstart = tied.gstart.get();
         DCX_HIP(hipStreamSynchronize(stream));
         std::sort(g.begin(), g.end());
         g.erase(std::unique(g.begin(), g.end()), g.end());
         for (uint32_t gi : g) {
             uint32_t ab[2], j0 = 0;
-            DCX_HIP(hipMemcpyAsync(ab, gstart.get() + gi, 8, hipMemcpyDeviceToHost, stream));
+            DCX_HIP(hipMemcpyAsync(ab, gstart + gi, 8, hipMemcpyDeviceToHost, stream));
             DCX_HIP(hipStreamSynchronize(stream));
             DCX_HIP(hipMemcpyAsync(&j0, tied.act.get() + ab[0], 4, hipMemcpyDeviceToHost, stream));
             DCX_HIP(hipStreamSynchronize(stream));
@@ -320,11 +323,8 @@ class Engine {
     void tie_break(const DcDev& dc, DBuf<uint32_t>& order, DBuf<uint8_t>& H, ActiveSet& as, const uint32_t* ranks) {
         const uint32_t A = as.A, G = as.G;
         if (A == 0) return;
-        DBuf<uint32_t> gstart(cache, (size_t)G + 1);
-        k_group_starts<<<grid(A), 256, 0, stream>>>(as.act.get(), as.gid.get(), H.get(), A, gstart.get());
-        prim.fill_u32(gstart.get() + G, A, 1);
         DBuf<uint8_t> large(cache, G);
-        k_tiebreak_small<<<grid(G), 256, 0, stream>>>(gstart.get(), G, as.act.get(), order.get(), dc, d_cmp.get(),
+        k_tiebreak_small<<<grid(G), 256, 0, stream>>>(as.gstart.get(), G, as.act.get(), order.get(), dc, d_cmp.get(),
                                                       ranks, large.get());
         // large groups: merge sort with the composite (group, DC comparator) order
         DBuf<uint64_t> items(cache, A), items2(cache, A);

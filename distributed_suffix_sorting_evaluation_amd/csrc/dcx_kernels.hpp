@@ -293,6 +293,7 @@ struct ActiveScanOp {
     const uint32_t* src;
     uint32_t* act_idx;
     uint32_t* gid;
+    uint32_t* gstart;  // list index of each group's first member
     __device__ __forceinline__ uint64_t value(uint64_t a) const {
         const uint64_t j = GATHER ? src[a] : a;
         const uint32_t h = H[j], h1 = H[j + 1];  // both loads unconditional: mergeable
@@ -302,7 +303,9 @@ struct ActiveScanOp {
         if (v) {
             const uint32_t o = (uint32_t)excl;
             act_idx[o] = src ? src[a] : (uint32_t)a;
-            gid[o] = (uint32_t)(excl >> 32) + (uint32_t)(v >> 32) - 1u;
+            const uint32_t g = (uint32_t)(excl >> 32) + (uint32_t)(v >> 32) - 1u;
+            gid[o] = g;
+            if (v >> 32) gstart[g] = o;  // a head: the first member of its group
         }
     }
 };
@@ -584,18 +587,12 @@ __device__ __forceinline__ bool dc_rank_less(uint32_t a, uint32_t b, const DcDev
     return ra < rb;
 }
 
-__global__ __launch_bounds__(256) void k_group_starts(const uint32_t* __restrict__ act_idx, const uint32_t* __restrict__ gid,
-                                                      const uint8_t* __restrict__ H, uint32_t A, uint32_t* __restrict__ gstart) {
-    for (uint32_t a = blockIdx.x * 256 + threadIdx.x; a < A; a += gridDim.x * 256)
-        if (H[act_idx[a]]) gstart[gid[a]] = a;
-}
-
 // ------------------------------------------------------------------------------------------------
 // Refinement rounds by group size: a tied group of at most RF_SMALL suffixes is ordered by one thread
 // (insertion sort of the members' next characters); the members of larger groups go through the radix
 // sort.  Both write the new group heads into H and the compact heads Ha of the active list.
 // ------------------------------------------------------------------------------------------------
-constexpr uint32_t RF_SMALL = 16;
+constexpr uint32_t RF_SMALL = 8;
 
 __global__ __launch_bounds__(256) void k_refine_small(const uint32_t* __restrict__ gstart, uint32_t G,
                                                       const uint32_t* __restrict__ act_idx, uint32_t* __restrict__ order,
@@ -611,6 +608,18 @@ __global__ __launch_bounds__(256) void k_refine_small(const uint32_t* __restrict
         }
         large[g] = 0;
         const uint32_t j0 = act_idx[a0];  // a group occupies consecutive slots of the order
+        if (s == 2) {  // the common case (a copied segment and its source): one comparison
+            const uint32_t e0 = order[j0], e1 = order[j0 + 1];
+            const uint64_t k0 = chars((uint64_t)e0 + C), k1 = chars((uint64_t)e1 + C);
+            if (k1 < k0) {
+                order[j0] = e1;
+                order[j0 + 1] = e0;
+            }
+            Ha[a0] = 1;
+            Ha[a0 + 1] = k0 != k1 ? 1 : 0;
+            if (k0 != k1) H[j0 + 1] = 1;
+            continue;
+        }
         uint64_t k[RF_SMALL];
         uint32_t e[RF_SMALL];
         for (uint32_t t = 0; t < s; t++) {
