@@ -81,8 +81,8 @@ struct StreamKeys {
         const uint64_t o = i * bits;
         const uint64_t w = o >> 6;
         const uint32_t sh = (uint32_t)(o & 63);
-        uint64_t x = s[w] << sh;
-        if (sh) x |= s[w + 1] >> (64 - sh);
+        // branch-free funnel shift: (next >> 1) >> (63 - sh) is next >> (64 - sh), and 0 for sh = 0
+        const uint64_t x = (s[w] << sh) | ((s[w + 1] >> 1) >> (63 - sh));
         return PACKPOS ? ((x & mask) | i) : (x & mask);
     }
 };
