@@ -106,6 +106,41 @@ __global__ __launch_bounds__(256) void k_radix_hist(Src src, uint32_t N, uint32_
     }
 }
 
+// hw[d] = number of q in [0, N) whose byte-wide window (top 8 bits of the 64-bit window at bit q * bits)
+// is d, counted once per wave and digit
+__global__ __launch_bounds__(256) void k_window_hist(const uint64_t* __restrict__ s, uint32_t bits, uint32_t N,
+                                                     uint32_t* __restrict__ hw) {
+    __shared__ uint32_t s_h[RS_RADIX];
+    s_h[threadIdx.x] = 0;
+    __syncthreads();
+    const StreamKeys<false> win{s, bits, ~0ull};
+    const uint64_t stride = (uint64_t)gridDim.x * 256;
+    for (uint64_t q = blockIdx.x * 256ull + threadIdx.x; q < N; q += stride) {
+        const uint32_t d = (uint32_t)(win(q) >> 56);
+        atomicAdd(&s_h[d], 1u);
+    }
+    __syncthreads();
+    const uint32_t c = s_h[threadIdx.x];
+    if (c) atomicAdd(&hw[threadIdx.x], c);
+}
+
+// pass p's histogram: its digit is the byte window at character i + s_p, s_p = (56 - lo_bit - 8p) / bits,
+// so hist_p = hw - windows at [0, s_p) + windows at [N, N + s_p).  One workgroup per pass, thread d.
+__global__ __launch_bounds__(RS_RADIX) void k_window_hist_passes(const uint64_t* __restrict__ s, uint32_t bits,
+                                                                 uint32_t N, uint32_t lo_bit,
+                                                                 const uint32_t* __restrict__ hw,
+                                                                 uint32_t* __restrict__ hist) {
+    const uint32_t p = blockIdx.x, d = threadIdx.x;
+    const uint32_t sp = (56 - lo_bit - 8 * p) / bits;
+    const StreamKeys<false> win{s, bits, ~0ull};
+    int32_t c = (int32_t)hw[d];
+    for (uint32_t q = 0; q < sp; q++) {
+        c -= (uint32_t)(win(q) >> 56) == d;
+        c += (uint32_t)(win((uint64_t)N + q) >> 56) == d;
+    }
+    hist[p * RS_RADIX + d] = (uint32_t)c;
+}
+
 // keys[i] = src(i) for i < N (sorts of a single element)
 template <class Src>
 __global__ void k_materialize(Src src, uint32_t N, uint64_t* __restrict__ keys) {
@@ -614,7 +649,7 @@ struct Primitives {
         DCX_HIP(hipMemsetAsync(ticket, 0, sizeof(unsigned long long), stream));
         DCX_HIP(hipMalloc(&scalars, 64 * sizeof(uint64_t)));
         DCX_HIP(hipMemsetAsync(scalars, 0, 64 * sizeof(uint64_t), stream));
-        DCX_HIP(hipMalloc(&hist, 8 * RS_RADIX * sizeof(uint32_t)));
+        DCX_HIP(hipMalloc(&hist, 9 * RS_RADIX * sizeof(uint32_t)));  // 8 passes + a window histogram
         DCX_HIP(hipMalloc(&scan_part, 2 * SC_BLOCKS * sizeof(uint64_t)));
         DCX_HIP(hipMalloc(&scan_prefix, SC_BLOCKS * sizeof(uint64_t)));
         DCX_HIP(hipHostMalloc(&host_scalars, 64 * sizeof(uint64_t), hipHostMallocDefault));
@@ -684,6 +719,30 @@ struct Primitives {
         scan_tag = 0;
     }
 
+    // digit histograms of all passes into `hist` (zeroed)
+    template <class Src>
+    void radix_hist(const Src& src, uint32_t N, uint32_t lo_bit, uint32_t np) {
+        hipLaunchKernelGGL(k_radix_hist<Src>, dim3(grid_for(N, 256, 2048)), dim3(256), 0, stream, src, N, lo_bit, np,
+                           hist);
+    }
+    // Windows of a character stream whose width divides 8, with every digit on a character boundary:
+    // the digit of pass p of key i is the byte-wide window at character i + s_p, so every pass's
+    // histogram is one histogram of byte windows over [0, N) corrected at both ends.
+    void radix_hist(const StreamKeys<false>& src, uint32_t N, uint32_t lo_bit, uint32_t np) {
+        const uint32_t bits = src.bits;
+        const bool aligned = (8 % bits == 0) && lo_bit % 8 == 0 && lo_bit + 8 * np <= 64 &&
+                             (src.mask & (~0ull << lo_bit)) == (~0ull << lo_bit) && N > 64;
+        if (!aligned) {
+            hipLaunchKernelGGL(k_radix_hist<StreamKeys<false>>, dim3(grid_for(N, 256, 2048)), dim3(256), 0, stream, src,
+                               N, lo_bit, np, hist);
+            return;
+        }
+        hipLaunchKernelGGL(k_window_hist, dim3(grid_for(N, 256, 2048)), dim3(256), 0, stream, src.s, bits, N,
+                           hist + 8 * RS_RADIX);
+        hipLaunchKernelGGL(k_window_hist_passes, dim3(np), dim3(RS_RADIX), 0, stream, src.s, bits, N, lo_bit,
+                           (const uint32_t*)(hist + 8 * RS_RADIX), hist);
+    }
+
     // LSD radix sort of (u64 key, u32 value) pairs by key bits [lo_bit, hi_bit) (rounded up to whole
     // bytes); bits above hi_bit must be equal for all keys.  The first pass reads its keys from `first`
     // (ArrayKeys{keys}: the keys array itself); with ident_vals its values are the input indices (vals
@@ -704,9 +763,8 @@ struct Primitives {
         if (lo_bit + 8 * np > 64) lo_bit = 64 - 8 * np;
         uint64_t tiles = ceil_div(N, RS_TILE);
         ensure_sort_status(tiles);
-        fill_u32(hist, 0, 8 * RS_RADIX);
-        hipLaunchKernelGGL(k_radix_hist<Src>, dim3(grid_for(N, 256, 2048)), dim3(256), 0, stream, first, N, lo_bit, np,
-                           hist);
+        fill_u32(hist, 0, 9 * RS_RADIX);
+        radix_hist(first, N, lo_bit, np);
         hipLaunchKernelGGL(k_radix_hist_scan, dim3(np), dim3(256), 0, stream, hist);
         for (uint32_t p = 0; p < np; p++) {
             if (sort_tag >= LB_TAG_LIMIT) {
