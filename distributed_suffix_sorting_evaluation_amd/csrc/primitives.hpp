@@ -29,6 +29,11 @@ namespace dcx {
 constexpr int RS_THREADS = DCX_RS_THREADS;  // 256 or 512
 constexpr int RS_ITEMS = DCX_RS_ITEMS;  // keys per thread of one radix tile
 constexpr int RS_TILE = RS_THREADS * RS_ITEMS;  // keys per workgroup
+// 32-bit keys (permutation scatters) take less LDS and fewer registers per key: larger tiles
+#ifndef DCX_RS_ITEMS_U32
+#define DCX_RS_ITEMS_U32 16
+#endif
+constexpr int RS_ITEMS_U32 = DCX_RS_ITEMS_U32;
 constexpr int RS_WAVES = RS_THREADS / 64;
 constexpr int RS_RADIX = 256;
 #ifndef DCX_RS_LOOKBACK
@@ -159,14 +164,15 @@ __global__ __launch_bounds__(256) void k_radix_hist_scan(uint32_t* hist) {
 // One LSD pass over digit (key >> shift) & 255.  Stable.  Keys come from `src` (the previous pass's
 // output, or a first-pass key source) and go to kout as KeyT: 64-bit sort keys or 32-bit keys (radix
 // partitions by destination index).  IDENT: values are the input indices.
-template <bool VALS, bool IDENT, class Src, class KeyT = uint64_t>
+template <bool VALS, bool IDENT, class Src, class KeyT = uint64_t, int ITEMS = RS_ITEMS>
 __global__ __launch_bounds__(RS_THREADS) void k_onesweep(Src src, KeyT* __restrict__ kout,
                                                          const uint32_t* __restrict__ vin, uint32_t* __restrict__ vout,
                                                          uint32_t N, uint32_t shift, const uint32_t* __restrict__ dbase,
                                                          uint64_t* __restrict__ status, uint32_t tag,
                                                          unsigned long long* __restrict__ ticket,
                                                          unsigned long long ticket_base) {
-    __shared__ KeyT s_keys[RS_TILE];
+    constexpr int TILE = RS_THREADS * ITEMS;
+    __shared__ KeyT s_keys[TILE];
     __shared__ uint32_t s_wcnt[RS_WAVES][RS_RADIX];  // per-wave digit counts, then wave offsets
     __shared__ uint32_t s_start[RS_RADIX];
     __shared__ uint32_t s_gofs[RS_RADIX];
@@ -180,14 +186,14 @@ __global__ __launch_bounds__(RS_THREADS) void k_onesweep(Src src, KeyT* __restri
     for (int i = tid; i < RS_WAVES * RS_RADIX; i += RS_THREADS) (&s_wcnt[0][0])[i] = 0;
     __syncthreads();
     const uint32_t tile = s_tile;
-    const uint64_t tile_base = (uint64_t)tile * RS_TILE;
-    const uint64_t wbase = tile_base + (uint64_t)wave * (64 * RS_ITEMS);
+    const uint64_t tile_base = (uint64_t)tile * TILE;
+    const uint64_t wbase = tile_base + (uint64_t)wave * (64 * ITEMS);
 
-    uint64_t key[RS_ITEMS];
-    uint32_t val[RS_ITEMS];
-    uint32_t rk[RS_ITEMS];
+    KeyT key[ITEMS];  // 32-bit keys stay 32-bit in registers
+    uint32_t val[ITEMS];
+    uint32_t rk[ITEMS];
 #pragma unroll
-    for (int k = 0; k < RS_ITEMS; k++) {
+    for (int k = 0; k < ITEMS; k++) {
         uint64_t idx = wbase + k * 64 + lane;
         bool valid = idx < N;
         key[k] = valid ? src(idx) : 0ull;
@@ -195,7 +201,7 @@ __global__ __launch_bounds__(RS_THREADS) void k_onesweep(Src src, KeyT* __restri
     }
     // rank every key among the keys of its wave with the same digit, in tile order
 #pragma unroll
-    for (int k = 0; k < RS_ITEMS; k++) {
+    for (int k = 0; k < ITEMS; k++) {
         uint64_t idx = wbase + k * 64 + lane;
         bool valid = idx < N;
         uint32_t d = (uint32_t)(key[k] >> shift) & 0xFFu;
@@ -237,7 +243,7 @@ __global__ __launch_bounds__(RS_THREADS) void k_onesweep(Src src, KeyT* __restri
     __syncthreads();
     // reorder the tile by digit in LDS (the keys' registers are free for the lookback afterwards)
 #pragma unroll
-    for (int k = 0; k < RS_ITEMS; k++) {
+    for (int k = 0; k < ITEMS; k++) {
         uint64_t idx = wbase + k * 64 + lane;
         uint32_t dk = rk[k] & 0xFFu;
         if (idx < N) {
@@ -288,10 +294,10 @@ __global__ __launch_bounds__(RS_THREADS) void k_onesweep(Src src, KeyT* __restri
     __syncthreads();
     // write out: position i of the reordered tile goes to gofs[digit] + i; the digits of a thread's
     // positions stay in registers for the values
-    const uint32_t tile_n = (uint32_t)((N - tile_base) < (uint64_t)RS_TILE ? (N - tile_base) : RS_TILE);
-    uint32_t dg[RS_ITEMS];
+    const uint32_t tile_n = (uint32_t)((N - tile_base) < (uint64_t)TILE ? (N - tile_base) : TILE);
+    uint32_t dg[ITEMS];
 #pragma unroll
-    for (int r = 0; r < RS_ITEMS; r++) {
+    for (int r = 0; r < ITEMS; r++) {
         const uint32_t i = tid + r * RS_THREADS;
         dg[r] = 0;
         if (i < tile_n) {
@@ -304,14 +310,14 @@ __global__ __launch_bounds__(RS_THREADS) void k_onesweep(Src src, KeyT* __restri
         __syncthreads();
         uint32_t* s_vals = reinterpret_cast<uint32_t*>(s_keys);
 #pragma unroll
-        for (int k = 0; k < RS_ITEMS; k++) {
+        for (int k = 0; k < ITEMS; k++) {
             uint64_t idx = wbase + k * 64 + lane;
             uint32_t dk = rk[k] & 0xFFu;
             if (idx < N) s_vals[s_start[dk] + s_wcnt[wave][dk] + (rk[k] >> 8)] = val[k];
         }
         __syncthreads();
 #pragma unroll
-        for (int r = 0; r < RS_ITEMS; r++) {
+        for (int r = 0; r < ITEMS; r++) {
             const uint32_t i = tid + r * RS_THREADS;
             if (i < tile_n) vout[s_gofs[dg[r]] + i] = s_vals[i];
         }
@@ -834,7 +840,7 @@ struct Primitives {
         *keys_out = k1;
         *vals_out = v1;
         if (N == 0 || np == 0) return;
-        const uint64_t tiles = ceil_div(N, RS_TILE);
+        const uint64_t tiles = ceil_div(N, (uint64_t)RS_THREADS * RS_ITEMS_U32);
         ensure_sort_status(tiles);
         fill_u32(hist, 0, np * RS_RADIX);
         hipLaunchKernelGGL(k_radix_hist<ArrayKeys<uint32_t>>, dim3(grid_for(N, 256, 2048)), dim3(256), 0, stream,
@@ -848,11 +854,11 @@ struct Primitives {
             const uint32_t tag = ++sort_tag;
             const ArrayKeys<uint32_t> in{kin};
             if (vin)
-                hipLaunchKernelGGL((k_onesweep<true, false, ArrayKeys<uint32_t>, uint32_t>), dim3((unsigned)tiles),
+                hipLaunchKernelGGL((k_onesweep<true, false, ArrayKeys<uint32_t>, uint32_t, RS_ITEMS_U32>), dim3((unsigned)tiles),
                                    dim3(RS_THREADS), 0, stream, in, ko, vin, vo, N, lo_bit + 8 * p,
                                    (const uint32_t*)(hist + p * RS_RADIX), sort_status, tag, ticket, ticket_host);
             else
-                hipLaunchKernelGGL((k_onesweep<true, true, ArrayKeys<uint32_t>, uint32_t>), dim3((unsigned)tiles),
+                hipLaunchKernelGGL((k_onesweep<true, true, ArrayKeys<uint32_t>, uint32_t, RS_ITEMS_U32>), dim3((unsigned)tiles),
                                    dim3(RS_THREADS), 0, stream, in, ko, (const uint32_t*)nullptr, vo, N, lo_bit + 8 * p,
                                    (const uint32_t*)(hist + p * RS_RADIX), sort_status, tag, ticket, ticket_host);
             ticket_host += tiles;
