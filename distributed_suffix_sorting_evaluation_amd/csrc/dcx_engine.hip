@@ -362,6 +362,89 @@ class Engine {
         return ls;
     }
 
+    // phase 3 when the names are not all distinct: recursive string, discarding decision, recursion,
+    // sample ranks (handle_recursive_call, pdcx.hpp:145-180).  s_order: the samples in name order (their
+    // text-order sample indices); names: their names; Hs: their group heads; ranks: m + D entries out.
+    void recursion(const DcDev& dc, uint32_t n, uint32_t m, uint32_t M, int depth, DBuf<uint32_t>& s_order,
+                   DBuf<uint32_t>& names, DBuf<uint8_t>& Hs, DBuf<uint32_t>& ranks) {
+        dcx_level_stats* ls = st && depth < DCX_MAX_LEVELS ? &st->level[depth] : nullptr;
+        const double thr = opt.discarding_threshold;
+        // the kept set of discarding, from the non-unique samples: after >= NU and after <= 2 NU + 1
+        prim.scan(NonUniqueOp{Hs.get(), s_order.get(), dc, m, nullptr}, m, 0);
+        const uint32_t nu = (uint32_t)prim.read_scalars(1)[0];
+        DBuf<uint32_t> K;
+        uint32_t after = m;
+        if (n > 80 && (double)nu / (double)m <= thr) {
+            const uint32_t nc = 2 * nu + 1;
+            DBuf<uint32_t> cand(cache, nc);
+            prim.scan(NonUniqueOp{Hs.get(), s_order.get(), dc, m, cand.get()}, m, 0);
+            prim.fill_u32(cand.get() + 2 * nu, 0, 1);  // element 0 is always kept
+            DBuf<uint32_t> k1(cache, nc), v1(cache, nc), k2(cache, nc), v2(cache, nc);
+            uint32_t *ks = nullptr, *vs = nullptr;
+            const uint32_t cb = bit_width(m);
+            prim.sort_u32(cand.get(), nullptr, k1.get(), v1.get(), k2.get(), v2.get(), nc, 0, (cb + 7) / 8, &ks, &vs);
+            K = DBuf<uint32_t>(cache, nc);
+            prim.scan(DistinctOp{ks, K.get()}, nc, 0);
+            after = (uint32_t)prim.read_scalars(1)[0];
+        }
+        const bool use_disc = n > 80 && ((double)after / (double)m) <= thr;
+        if (!use_disc) {
+            // recursive string of all samples in (mod X, div X) order, recursion, ranks from its SA
+            K.reset();
+            DBuf<uint32_t> R(cache, m);
+            {
+                DBuf<uint32_t> dest(cache, m), word(cache, m);
+                k_rec_pairs<<<grid(m), 256, 0, stream>>>(s_order.get(), names.get(), m, dc, dest.get(), word.get());
+                names.reset();
+                scatter_u32(dest.get(), word.get(), m, 0, R.get());
+            }
+            DBuf<uint32_t> sar;
+            if (n <= 80) {
+                sar = DBuf<uint32_t>(cache, m);
+                base_case(R.get(), nullptr, m, sar.get());  // sequential_sa_on_local_ranks (pdcx.hpp:115-121)
+            } else {
+                level<uint32_t>(R.get(), nullptr, m, M, depth + 1, nullptr, nullptr, &sar);
+            }
+            DBuf<uint32_t> dest(cache, m);
+            k_sa_dest<<<grid(m), 256, 0, stream>>>(sar.get(), m, dc, dest.get());
+            sar.reset();
+            scatter_u32(dest.get(), nullptr, m, 1, ranks.get());
+            return;
+        }
+        if (ls) {
+            ls->discarding = 1;
+            ls->reduced = after;
+        }
+        // ranks in name order: final for the unique samples, patched below for the others
+        scatter_u32(s_order.get(), nullptr, m, 1, ranks.get());
+        DBuf<uint32_t> sar, isar(cache, after);
+        {
+            DBuf<uint32_t> Rr(cache, after);
+            k_reduced_string<<<grid(after), 256, 0, stream>>>(K.get(), after, dc, ranks.get(), names.get(), Rr.get());
+            names.reset();
+            level<uint32_t>(Rr.get(), nullptr, after, M, depth + 1, nullptr, nullptr, &sar);
+        }
+        scatter_u32(sar.get(), nullptr, after, 1, isar.get());
+        sar.reset();
+        // order of (name, rank2) = name order refined inside non-unique groups
+        ActiveSet stied;
+        active_set(Hs.get(), nullptr, m, stied);
+        const uint32_t A = stied.A, G = stied.G;
+        if (A == 0) return;
+        const uint32_t gb = bit_width(G - 1), r2b = bit_width(after);
+        DBuf<uint64_t> ka(cache, A), kb(cache, A);
+        DBuf<uint32_t> va(cache, A), vb(cache, A);
+        k_keys_rank2_kept<<<grid(A), 256, 0, stream>>>(A, stied.act.get(), stied.gid.get(), s_order.get(), dc, K.get(),
+                                                       after, isar.get(), r2b, ka.get(), va.get());
+        uint64_t* x = ka.get();
+        uint64_t* x2 = kb.get();
+        uint32_t* y = va.get();
+        uint32_t* y2 = vb.get();
+        prim.sort_pairs(x, x2, y, y2, A, 0, gb + r2b);
+        k_writeback<<<grid(A), 256, 0, stream>>>(x, y, stied.act.get(), A, s_order.get(), Hs.get(), 0, (uint8_t*)nullptr);
+        k_patch_ranks<<<grid(A), 256, 0, stream>>>(A, stied.act.get(), s_order.get(), ranks.get());
+    }
+
     // bits per character of the reference's packed variants (cli.cpp:488-519), extended by 9 bits
     static uint32_t ref_pack_bits(uint32_t sig) { return sig <= 7 ? 3 : sig <= 31 ? 5 : sig <= 255 ? 8 : 9; }
 
@@ -483,64 +566,7 @@ class Engine {
             } else {
                 if (ls) ls->recursed = 1;
                 if (M >= UNIQ_BIT) throw Error(DCX_E_UNSUPPORTED, "more than 2^31 distinct sample names");
-                DBuf<uint32_t> R(cache, m);
-                {
-                    DBuf<uint32_t> dest(cache, m), word(cache, m);
-                    k_rec_pairs<<<grid(m), 256, 0, stream>>>(s_order.get(), names.get(), Hs.get(), m, dc, dest.get(),
-                                                             word.get());
-                    names.reset();
-                    scatter_u32(dest.get(), word.get(), m, 0, R.get());
-                }
-                DBuf<uint32_t> Rr(cache, m), red(cache, m);
-                prim.scan(KeepOp{R.get(), Rr.get(), red.get()}, m, 0);
-                const uint32_t after = (uint32_t)prim.read_scalars(1)[0];
-                const bool use_disc = ((double)after / (double)m) <= opt.discarding_threshold && n > 80;
-                if (!use_disc) {
-                    Rr.reset();
-                    red.reset();
-                    k_strip_flags<<<grid(m), 256, 0, stream>>>(R.get(), m);
-                    DBuf<uint32_t> sar;
-                    if (n <= 80) {
-                        sar = DBuf<uint32_t>(cache, m);
-                        base_case(R.get(), nullptr, m, sar.get());  // sequential_sa_on_local_ranks (pdcx.hpp:115-121)
-                    } else {
-                        level<uint32_t>(R.get(), nullptr, m, M, depth + 1, nullptr, nullptr, &sar);
-                    }
-                    DBuf<uint32_t> dest(cache, m);
-                    k_sa_dest<<<grid(m), 256, 0, stream>>>(sar.get(), m, dc, dest.get());
-                    sar.reset();
-                    scatter_u32(dest.get(), nullptr, m, 1, ranks.get());
-                } else {
-                    if (ls) {
-                        ls->discarding = 1;
-                        ls->reduced = after;
-                    }
-                    R.reset();
-                    DBuf<uint32_t> sar, isar(cache, after);
-                    level<uint32_t>(Rr.get(), nullptr, after, M, depth + 1, nullptr, nullptr, &sar);
-                    scatter_u32(sar.get(), nullptr, after, 1, isar.get());
-                    sar.reset();
-                    Rr.reset();
-                    // order of (name, rank2) = sorted-by-name order refined inside non-unique groups
-                    ActiveSet stied;
-                    active_set(Hs.get(), nullptr, m, stied);
-                    const uint32_t A = stied.A, G = stied.G;
-                    if (A > 0) {
-                        const uint32_t gb = bit_width(G - 1), r2b = bit_width(after);
-                        DBuf<uint64_t> ka(cache, A), kb(cache, A);
-                        DBuf<uint32_t> va(cache, A), vb(cache, A);
-                        k_keys_rank2<<<grid(A), 256, 0, stream>>>(A, stied.act.get(), stied.gid.get(), s_order.get(),
-                                                                  dc, red.get(), isar.get(), r2b, ka.get(), va.get());
-                        uint64_t* x = ka.get();
-                        uint64_t* x2 = kb.get();
-                        uint32_t* y = va.get();
-                        uint32_t* y2 = vb.get();
-                        prim.sort_pairs(x, x2, y, y2, A, 0, gb + r2b);
-                        k_writeback<<<grid(A), 256, 0, stream>>>(x, y, stied.act.get(), A, s_order.get(), Hs.get(), 0,
-                                                                 (uint8_t*)nullptr);
-                    }
-                    scatter_u32(s_order.get(), nullptr, m, 1, ranks.get());
-                }
+                recursion(dc, n, m, M, depth, s_order, names, Hs, ranks);
             }
         }
         if (ev && depth == 0) DCX_HIP(hipEventRecord(ev[3], stream));

@@ -482,17 +482,16 @@ __global__ __launch_bounds__(1024) void k_boundary_fix(uint32_t* __restrict__ or
     for (uint32_t i = threadIdx.x; i < s - b; i += 1024) order[j0 + b + i] = tmp[i];
 }
 
-// recursive string in (mod X, div X) order with unique flags (handle_recursive_call,
-// pdcx.hpp:145-180; flag_unique_ranks, compute_ranks.hpp:113-134).  Names stay below 2^31 (checked on
-// the host), so the unique flag rides in bit 31 of the name: one scattered word per sample.
+// recursive string in (mod X, div X) order (handle_recursive_call, pdcx.hpp:145-180): the name of the j-th
+// sample of the name order goes to class_major(sample index), written by a partitioned scatter
+// (scatter_u32).  Names stay below 2^31 (checked on the host).
 constexpr uint32_t UNIQ_BIT = 0x80000000u;
-// its (destination, word) pairs, written to R by a partitioned scatter (scatter_u32)
 __global__ __launch_bounds__(256) void k_rec_pairs(const uint32_t* __restrict__ order, const uint32_t* __restrict__ names,
-                                                   const uint8_t* __restrict__ H, uint32_t m, DcDev dc,
-                                                   uint32_t* __restrict__ dest, uint32_t* __restrict__ word) {
+                                                   uint32_t m, DcDev dc, uint32_t* __restrict__ dest,
+                                                   uint32_t* __restrict__ word) {
     for (uint32_t j = blockIdx.x * 256 + threadIdx.x; j < m; j += gridDim.x * 256) {
         dest[j] = class_major(dc, order[j]);
-        word[j] = names[j] | ((H[j] && H[j + 1]) ? UNIQ_BIT : 0u);
+        word[j] = names[j];
     }
 }
 
@@ -528,44 +527,92 @@ __global__ __launch_bounds__(256) void k_scatter_pairs(const uint32_t* __restric
     for (uint32_t i = blockIdx.x * 256 + threadIdx.x; i < N; i += gridDim.x * 256) out[dest[i]] = val[i] + add;
 }
 
-// discarding (pdcx.hpp:215-246): keep c unless c and c-1 are both unique; element 0 is kept.  Also
-// removes the flags into Rr (the reduced string) and records every element's reduced index.
-struct KeepOp {
-    static constexpr bool kBlocked = false;  // value(0) must not read R[-1]
-    const uint32_t* R;
-    uint32_t* Rr;
-    uint32_t* red;
-    __device__ __forceinline__ uint64_t value(uint64_t c) const {
-        return (c == 0 || !((R[c] & UNIQ_BIT) && (R[c - 1] & UNIQ_BIT))) ? 1u : 0u;
+// Discarding from the non-unique samples alone.  Element c of the recursive string is kept unless c
+// and c - 1 are both unique (pdcx.hpp:215-246), so the kept set is {0} plus every non-unique position
+// and its successor.  NonUniqueOp counts the non-unique samples (sorted order j, unique iff H[j] &&
+// H[j+1]); with `cand` it also writes their class-major positions c and c + 1.
+struct NonUniqueOp {
+    static constexpr bool kBlocked = true;
+    const uint8_t* H;
+    const uint32_t* order;
+    DcDev dc;
+    uint32_t m;
+    uint32_t* cand;
+    __device__ __forceinline__ uint64_t value(uint64_t j) const {
+        const uint32_t h = H[j], h1 = H[j + 1];
+        return (h & h1) ? 0ull : 1ull;
     }
-    __device__ __forceinline__ void write(uint64_t c, uint64_t excl, uint64_t v) const {
-        red[c] = (uint32_t)excl;
-        if (v) Rr[excl] = R[c] & ~UNIQ_BIT;
+    __device__ __forceinline__ void write(uint64_t j, uint64_t excl, uint64_t v) const {
+        if (v && cand) {
+            const uint32_t c = class_major(dc, order[j]);
+            cand[2 * excl] = c;
+            cand[2 * excl + 1] = c + 1 < m ? c + 1 : c;
+        }
     }
 };
 
-__global__ __launch_bounds__(256) void k_strip_flags(uint32_t* __restrict__ R, uint32_t m) {
-    for (uint32_t j = blockIdx.x * 256 + threadIdx.x; j < m; j += gridDim.x * 256) R[j] &= ~UNIQ_BIT;
+// distinct values of a sorted array, in order
+struct DistinctOp {
+    static constexpr bool kBlocked = false;  // value(0) must not read a[-1]
+    const uint32_t* a;
+    uint32_t* out;
+    __device__ __forceinline__ uint64_t value(uint64_t i) const { return (i == 0 || a[i] != a[i - 1]) ? 1u : 0u; }
+    __device__ __forceinline__ void write(uint64_t i, uint64_t excl, uint64_t v) const {
+        if (v) out[excl] = a[i];
+    }
+};
+
+// the reduced string: the name of the sample at each kept class-major position K[i] (ranks0: 1 + position
+// of each sample in the name order)
+__global__ __launch_bounds__(256) void k_reduced_string(const uint32_t* __restrict__ K, uint32_t after, DcDev dc,
+                                                        const uint32_t* __restrict__ ranks0,
+                                                        const uint32_t* __restrict__ names, uint32_t* __restrict__ Rr) {
+    for (uint32_t i = blockIdx.x * 256 + threadIdx.x; i < after; i += gridDim.x * 256)
+        Rr[i] = names[ranks0[class_major_inv(dc, K[i])] - 1];
+}
+
+// index of c in the sorted array K[0..n) (c is present)
+__device__ __forceinline__ uint32_t sorted_index(const uint32_t* __restrict__ K, uint32_t n, uint32_t c) {
+    uint32_t lo = 0, hi = n;
+    while (lo < hi) {
+        const uint32_t mid = (lo + hi) >> 1;
+        if (K[mid] < c)
+            lo = mid + 1;
+        else
+            hi = mid;
+    }
+    return lo;
+}
+
+// (group, rank2) keys of the non-unique samples; rank2 = ISA of the reduced string at the sample's kept
+// index (RankRankIndex, pdcx.hpp:297-323)
+__global__ __launch_bounds__(256) void k_keys_rank2_kept(uint32_t A, const uint32_t* __restrict__ act_idx,
+                                                         const uint32_t* __restrict__ gid,
+                                                         const uint32_t* __restrict__ order, DcDev dc,
+                                                         const uint32_t* __restrict__ K, uint32_t after,
+                                                         const uint32_t* __restrict__ isar, uint32_t rbits,
+                                                         uint64_t* __restrict__ keys, uint32_t* __restrict__ vals) {
+    for (uint32_t a = blockIdx.x * 256 + threadIdx.x; a < A; a += gridDim.x * 256) {
+        const uint32_t k = order[act_idx[a]];
+        const uint32_t r2 = isar[sorted_index(K, after, class_major(dc, k))];
+        keys[a] = ((uint64_t)gid[a] << rbits) | r2;
+        vals[a] = k;
+    }
+}
+
+// ranks of the refined non-unique samples: 1 + their new position
+__global__ __launch_bounds__(256) void k_patch_ranks(uint32_t A, const uint32_t* __restrict__ act_idx,
+                                                     const uint32_t* __restrict__ order, uint32_t* __restrict__ ranks) {
+    for (uint32_t a = blockIdx.x * 256 + threadIdx.x; a < A; a += gridDim.x * 256) {
+        const uint32_t j = act_idx[a];
+        ranks[order[j]] = j + 1;
+    }
 }
 
 // recursive_call_direct (pdcx.hpp:182-212): the rank of map_back(SA_R[j]) is j + 1; dest[j] = map_back(SA_R[j])
 __global__ __launch_bounds__(256) void k_sa_dest(const uint32_t* __restrict__ sar, uint32_t m, DcDev dc,
                                                  uint32_t* __restrict__ dest) {
     for (uint32_t j = blockIdx.x * 256 + threadIdx.x; j < m; j += gridDim.x * 256) dest[j] = class_major_inv(dc, sar[j]);
-}
-
-// RankRankIndex keys (pdcx.hpp:297-323) restricted to the non-unique groups: (group, rank2)
-__global__ __launch_bounds__(256) void k_keys_rank2(uint32_t A, const uint32_t* __restrict__ act_idx,
-                                                    const uint32_t* __restrict__ gid, const uint32_t* __restrict__ order,
-                                                    DcDev dc, const uint32_t* __restrict__ red,
-                                                    const uint32_t* __restrict__ isar, uint32_t rbits,
-                                                    uint64_t* __restrict__ keys, uint32_t* __restrict__ vals) {
-    for (uint32_t a = blockIdx.x * 256 + threadIdx.x; a < A; a += gridDim.x * 256) {
-        uint32_t k = order[act_idx[a]];
-        uint32_t r2 = isar[red[class_major(dc, k)]];
-        keys[a] = ((uint64_t)gid[a] << rbits) | r2;
-        vals[a] = k;
-    }
 }
 
 // ------------------------------------------------------------------------------------------------
