@@ -37,9 +37,9 @@ constexpr int RS_ITEMS_U32 = DCX_RS_ITEMS_U32;
 constexpr int RS_WAVES = RS_THREADS / 64;
 constexpr int RS_RADIX = 256;
 #ifndef DCX_RS_LOOKBACK
-#define DCX_RS_LOOKBACK 16
+#define DCX_RS_LOOKBACK 8
 #endif
-constexpr int RS_LOOKBACK = DCX_RS_LOOKBACK;  // predecessors read per lookback round trip after the first
+constexpr int RS_LOOKBACK = DCX_RS_LOOKBACK;  // predecessors read per lookback round trip (8 measured best of 8/16/24)
 
 // lookback word: [63:42] tag  [41:40] flag  [39:0] count
 constexpr uint64_t LB_VALUE_MASK = (1ull << 40) - 1;
@@ -252,40 +252,35 @@ __global__ __launch_bounds__(RS_THREADS) void k_onesweep(Src src, KeyT* __restri
         }
     }
     if (dthr) {
-        // decoupled lookback: exclusive count of this digit over all preceding tiles.  Walks run to a
-        // dozen tiles and more while tiles stream through the machine, so after the nearest predecessor
-        // (polled alone: it is the one still likely to be unpublished) RS_LOOKBACK predecessors are read
-        // per memory round trip.
+        // decoupled lookback: exclusive count of this digit over all preceding tiles.  While tiles stream
+        // through the machine a walk passes some 30 predecessors that have only published their
+        // aggregates, and every read is a round trip to memory, so RS_LOOKBACK predecessors are read per
+        // round trip.
         uint64_t excl = 0;
         if (tile > 0) {
             int64_t j = (int64_t)tile - 1;  // next predecessor to add; tile 0 is always inclusive
             while (true) {
-                uint64_t w = ld_agent(status + (uint64_t)j * RS_RADIX + d);
-                if ((uint32_t)(w >> 42) != tag || (w & LB_FLAG_MASK) == 0) {
-                    __builtin_amdgcn_s_sleep(1);
-                    continue;
-                }
-                excl += w & LB_VALUE_MASK;
-                if (w & LB_INC) break;
-                --j;
                 const int cnt = (int)(j + 1 < RS_LOOKBACK ? j + 1 : RS_LOOKBACK);
                 uint64_t wb[RS_LOOKBACK];
 #pragma unroll
                 for (int b = 0; b < RS_LOOKBACK; b++)
                     wb[b] = b < cnt ? ld_agent(status + (uint64_t)(j - b) * RS_RADIX + d) : 0ull;
                 bool stop = false, inc = false;
+                int used = 0;
 #pragma unroll
                 for (int b = 0; b < RS_LOOKBACK; b++) {
                     if (stop || b >= cnt) continue;
                     if ((uint32_t)(wb[b] >> 42) != tag || (wb[b] & LB_FLAG_MASK) == 0) {
-                        stop = true;  // unpublished: polled alone at the top of the loop
+                        stop = true;  // not published yet: read again from here
                         continue;
                     }
                     excl += wb[b] & LB_VALUE_MASK;
-                    --j;
+                    used++;
                     if (wb[b] & LB_INC) inc = stop = true;
                 }
                 if (inc) break;
+                if (used == 0) __builtin_amdgcn_s_sleep(1);
+                j -= used;
             }
             st_agent(st, lb_pack(tag, LB_INC, excl + total));
         }
