@@ -445,6 +445,14 @@ class Engine {
         k_patch_ranks<<<grid(A), 256, 0, stream>>>(A, stied.act.get(), s_order.get(), ranks.get());
     }
 
+    // SA entries to the output width
+    void widen(const uint32_t* sa, uint64_t n, uint32_t width, uint8_t* out) {
+        if (width == 5 && (reinterpret_cast<uintptr_t>(out) & 15) == 0 && (reinterpret_cast<uintptr_t>(sa) & 15) == 0)
+            k_widen40<<<grid(ceil_div(n, 16)), 256, 0, stream>>>(sa, n, out);
+        else
+            k_widen<<<grid(n), 256, 0, stream>>>(sa, n, width, out);
+    }
+
     // bits per character of the reference's packed variants (cli.cpp:488-519), extended by 9 bits
     static uint32_t ref_pack_bits(uint32_t sig) { return sig <= 7 ? 3 : sig <= 31 ? 5 : sig <= 255 ? 8 : 9; }
 
@@ -818,12 +826,12 @@ class Engine {
         DCX_HIP(hipEventRecord(ev[4], stream));  // SA resident in HBM
         // output in the requested width
         if (o->sa_on_device) {
-            k_widen<<<grid(n), 256, 0, stream>>>(sa.get(), n, width, static_cast<uint8_t*>(sa_out));
+            widen(sa.get(), n, width, static_cast<uint8_t*>(sa_out));
         } else if (width == 4) {
             DCX_HIP(hipMemcpyAsync(sa_out, sa.get(), n * 4, hipMemcpyDeviceToHost, stream));
         } else {
             DBuf<uint8_t> wide(cache, n * width);
-            k_widen<<<grid(n), 256, 0, stream>>>(sa.get(), n, width, wide.get());
+            widen(sa.get(), n, width, wide.get());
             DCX_HIP(hipMemcpyAsync(sa_out, wide.get(), n * width, hipMemcpyDeviceToHost, stream));
             DCX_HIP(hipStreamSynchronize(stream));
         }

@@ -947,6 +947,41 @@ __global__ __launch_bounds__(256) void k_widen(const uint32_t* __restrict__ sa, 
     }
 }
 
+// 40-bit output with 16-byte aligned buffers: a thread turns 16 entries (four 16-byte loads) into 80 bytes
+// (five 16-byte stores) instead of five byte stores per entry; four entries make five little-endian words
+__device__ __forceinline__ void pack40x4(uint32_t a, uint32_t b, uint32_t c, uint32_t d, uint32_t* w) {
+    w[0] = a;
+    w[1] = b << 8;
+    w[2] = (b >> 24) | (c << 16);
+    w[3] = (c >> 16) | (d << 24);
+    w[4] = d >> 8;
+}
+__global__ __launch_bounds__(256) void k_widen40(const uint32_t* __restrict__ sa, uint64_t n, uint8_t* __restrict__ out) {
+    const uint64_t groups = n / 16;
+    for (uint64_t t = blockIdx.x * 256ull + threadIdx.x; t < groups; t += (uint64_t)gridDim.x * 256) {
+        const uint4* in = reinterpret_cast<const uint4*>(sa + 16 * t);
+        const uint4 q0 = in[0], q1 = in[1], q2 = in[2], q3 = in[3];
+        uint32_t w[20];
+        pack40x4(q0.x, q0.y, q0.z, q0.w, w);
+        pack40x4(q1.x, q1.y, q1.z, q1.w, w + 5);
+        pack40x4(q2.x, q2.y, q2.z, q2.w, w + 10);
+        pack40x4(q3.x, q3.y, q3.z, q3.w, w + 15);
+        uint4* o = reinterpret_cast<uint4*>(out + 80 * t);
+#pragma unroll
+        for (int k = 0; k < 5; k++) o[k] = make_uint4(w[4 * k], w[4 * k + 1], w[4 * k + 2], w[4 * k + 3]);
+    }
+    // the last n mod 16 entries
+    for (uint64_t j = groups * 16 + blockIdx.x * 256ull + threadIdx.x; j < n; j += (uint64_t)gridDim.x * 256) {
+        const uint32_t v = sa[j];
+        uint8_t* o = out + 5 * j;
+        o[0] = (uint8_t)v;
+        o[1] = (uint8_t)(v >> 8);
+        o[2] = (uint8_t)(v >> 16);
+        o[3] = (uint8_t)(v >> 24);
+        o[4] = 0;
+    }
+}
+
 // ------------------------------------------------------------------------------------------------
 // SA validity check (check_suffixarray, sa_check.hpp:56-184)
 // ------------------------------------------------------------------------------------------------

@@ -144,6 +144,22 @@ def test_sa_widths(engine):
     assert s5.tobytes() == oracle.uint40_bytes(ref)
 
 
+def test_width5_device_output_tails_and_alignment(engine):
+    """40-bit output written on the device: lengths that are not a multiple of 16 and an output pointer
+    that is not 16-byte aligned take the per-entry path, the rest the 16-entry vector path."""
+    import torch
+    for n in (100003, 16 * 4096 + 7, 16 * 5000):
+        t = random_text(n, 4, n % 1000)
+        want = oracle.uint40_bytes(oracle.dcx_sa(t, X=39))
+        dt = torch.from_numpy(t).cuda()
+        for off in (0, 3):
+            buf = torch.zeros(5 * n + 32, dtype=torch.uint8, device="cuda")
+            engine.build_sa_device(dt.data_ptr(), n, buf.data_ptr() + off, 5, X=39)
+            torch.cuda.synchronize()
+            assert buf[off:off + 5 * n].cpu().numpy().tobytes() == want, (n, off)
+            assert int(buf[:off].sum()) == 0 and int(buf[off + 5 * n:].sum()) == 0
+
+
 def test_device_checker(engine):
     t = copies_text(100000, 3)
     sa = engine.build_sa(t, sa_width=8, X=39)
