@@ -10,6 +10,7 @@
 //              sample ranks; the sorted order is the level's SA
 // All data stays in HBM; the host only reads back a handful of scalars per step.
 #include <algorithm>
+#include <cmath>
 #include <chrono>
 #include <cstdio>
 #include <cstring>
@@ -143,16 +144,18 @@ class Engine {
     // a word) or, for wide characters, double the covered prefix with the group ids of the positions C
     // ahead (exact: the last step takes only the L - C characters still missing).  Doubling needs the
     // sentinel coding (dense = false): the empty suffix past the end must be the smallest key.
-    void refine_sort(const uint64_t* PT, uint32_t N, uint32_t L, uint32_t bits, bool dense, DBuf<uint32_t>& order,
-                     DBuf<uint8_t>& H, ActiveSet& fin) {
+    // `info`: estimated bits of information per character (at most `bits`), which sizes round 0.
+    void refine_sort(const uint64_t* PT, uint32_t N, uint32_t L, uint32_t bits, bool dense, double info,
+                     DBuf<uint32_t>& order, DBuf<uint8_t>& H, ActiveSet& fin) {
         const uint32_t cpw = 64 / bits;
         uint32_t c0 = L < cpw ? L : cpw;
         const uint32_t pbits = bit_width(N - 1);
         const uint32_t cpack = std::min<uint32_t>(c0, (64 - pbits) / bits);
-        if (N >= (1u << 16) && cpack * bits >= bit_width(N) + 14) {
+        if (N >= (1u << 16) && cpack * info >= bit_width(N) + 14) {
             // round 0 sorts one word per position holding the first characters above the position itself
             // (16 instead of 24 bytes moved per pass) when the characters left beside the position still
-            // carry 14 bits more than log2(N): low-entropy texts would otherwise leave most positions tied
+            // carry 14 bits of information more than log2(N): low-entropy texts would otherwise leave most
+            // positions tied
             c0 = cpack;
             const uint32_t kbits = c0 * bits;
             const uint64_t hi = ~0ull << (64 - kbits);
@@ -167,10 +170,12 @@ class Engine {
             k_heads_packed<<<grid(N + 1ull), 256, 0, stream>>>(k, N, kbits, (1ull << pbits) - 1, H.get(), order.get());
         } else {
             if (N >= (1u << 24)) {
-                // large sorts: round 0 takes about log2(N) + ROUND0_SLACK bits in whole 8-bit digits; the
-                // positions still tied after it go through the cheaper group rounds
-                const uint32_t want = std::min<uint32_t>(64, (bit_width(N) + ROUND0_SLACK + 7) / 8 * 8);
-                c0 = std::max<uint32_t>(1, std::min<uint32_t>(c0, want / bits));
+                // large sorts: round 0 takes the characters that carry about log2(N) + ROUND0_SLACK bits of
+                // information, rounded up to whole 8-bit digits; the positions still tied after it go
+                // through the cheaper group rounds
+                const uint32_t want = (uint32_t)std::ceil((bit_width(N) + ROUND0_SLACK) / info);
+                const uint32_t digits = std::min<uint32_t>(8, (std::max<uint32_t>(1, want) * bits + 7) / 8);
+                c0 = std::max<uint32_t>(1, std::min<uint32_t>(c0, 8 * digits / bits));
             }
             DBuf<uint64_t> ka(cache, N), kb(cache, N);
             DBuf<uint32_t> va(cache, N), vb(cache, N);
@@ -460,6 +465,28 @@ class Engine {
     // one recursion level: SA of text[0..n) into sa_out (device, n entries).  Level 0 reads bytes
     // through the alphabet map `lut`; deeper levels read names (uint32, >= 1).
     // -------------------------------------------------------------------------------------------
+    // order-0 entropy (bits per character, within [1, bits]) of a byte text from a strided sample of its
+    // codes; names of deeper levels are taken at their full width
+    double info_bits(const uint8_t* text, const uint16_t* lut, uint32_t n, uint32_t bits) {
+        if (n < (1u << 24)) return bits;
+        const uint64_t stride = 61;  // odd: no alignment with periodic structure
+        DBuf<unsigned long long> hist(cache, 257);
+        DCX_HIP(hipMemsetAsync(hist.get(), 0, 257 * 8, stream));
+        k_code_hist<<<grid(ceil_div(n, stride * 256) * 256, 4096), 256, 0, stream>>>(text, n, lut, stride, hist.get());
+        unsigned long long h[257];
+        DCX_HIP(hipMemcpyAsync(h, hist.get(), sizeof(h), hipMemcpyDeviceToHost, stream));
+        DCX_HIP(hipStreamSynchronize(stream));
+        double total = 0, e = 0;
+        for (int i = 0; i < 257; i++) total += (double)h[i];
+        for (int i = 0; i < 257; i++)
+            if (h[i]) {
+                const double p = (double)h[i] / total;
+                e -= p * std::log2(p);
+            }
+        return std::min<double>(bits, std::max(1.0, e));
+    }
+    double info_bits(const uint32_t*, const uint16_t*, uint32_t, uint32_t bits) { return bits; }
+
     // the level's characters as a packed code stream (k_pack_text) of `chars` characters
     template <class CharT>
     void make_stream(const CharT* text, const uint16_t* lut, uint32_t n, uint32_t bits, uint32_t dense, uint64_t chars,
@@ -544,7 +571,7 @@ class Engine {
         {
             DBuf<uint64_t> PT;
             make_stream(text, lut, n, mbits, dense ? 1 : 0, (uint64_t)n + Ls + 2, PT);
-            refine_sort(PT.get(), n, Ls, mbits, dense, order, H, tied);
+            refine_sort(PT.get(), n, Ls, mbits, dense, info_bits(text, lut, n, mbits), order, H, tied);
         }
         if (dense && tied.A > 0) boundary_fixup(dc.bnd, order, H, tied);
 

@@ -174,6 +174,23 @@ __global__ __launch_bounds__(256) void k_byte_mask(const uint8_t* __restrict__ t
     if (threadIdx.x < 4 && s_m[threadIdx.x]) atomicOr(&mask[threadIdx.x], s_m[threadIdx.x]);
 }
 
+// histogram of the character codes lut[t[q]] at every stride-th position (codes 0..256): the order-0
+// entropy of a level-0 text decides how many characters its first sort round takes
+__global__ __launch_bounds__(256) void k_code_hist(const uint8_t* __restrict__ t, uint64_t n,
+                                                   const uint16_t* __restrict__ lut, uint64_t stride,
+                                                   unsigned long long* __restrict__ hist) {
+    __shared__ uint32_t s_h[257];
+    __shared__ uint16_t s_lut[256];
+    for (int i = threadIdx.x; i < 257; i += 256) s_h[i] = 0;
+    s_lut[threadIdx.x] = lut[threadIdx.x];
+    __syncthreads();
+    for (uint64_t q = (blockIdx.x * 256ull + threadIdx.x) * stride; q < n; q += (uint64_t)gridDim.x * 256 * stride)
+        atomicAdd(&s_h[s_lut[t[q]]], 1u);
+    __syncthreads();
+    for (int i = threadIdx.x; i < 257; i += 256)
+        if (s_h[i]) atomicAdd(&hist[i], (unsigned long long)s_h[i]);
+}
+
 // ------------------------------------------------------------------------------------------------
 // refinement sort: packed characters, heads, active compaction, later-round keys, write back
 // ------------------------------------------------------------------------------------------------
