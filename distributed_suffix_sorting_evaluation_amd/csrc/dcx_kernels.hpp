@@ -415,9 +415,10 @@ struct SampleScanOp {
         const uint32_t q = fast_div(p, dc.X, dc.inv_X);
         return (uint64_t)dc_in(dc, p - q * dc.X) | ((uint64_t)H[j] << 32);
     }
-    __device__ __forceinline__ void write(uint64_t j, uint64_t excl, uint64_t v) const {
+    __device__ __forceinline__ uint32_t payload(uint64_t j) const { return order[j]; }
+    __device__ __forceinline__ void write(uint64_t j, uint64_t excl, uint64_t v) const { write(j, excl, v, order[j]); }
+    __device__ __forceinline__ void write(uint64_t, uint64_t excl, uint64_t v, uint32_t p) const {
         if (v & 1u) {
-            const uint32_t p = order[j];
             const uint32_t q = fast_div(p, dc.X, dc.inv_X);
             const uint32_t t = (uint32_t)excl + shift;
             s_order[t] = q * dc.D + dc_next_rank(dc, p - q * dc.X);

@@ -454,17 +454,34 @@ constexpr uint32_t SC_BLOCKS = DCX_SC_BLOCKS;  // blocks of the two-pass scan
 constexpr uint64_t SC_SINGLE_PASS_TILES = DCX_SC_SINGLE_PASS_TILES;  // up to this many tiles: one pass
 static_assert(SC_BLOCKS >= 1 && SC_BLOCKS <= 4 * 1024, "k_scan_spine scans at most 4096 block sums");
 
+// Ops with a payload(i) (a 32-bit word that write() needs, typically loaded by value() anyway) get it
+// handed to write(i, excl, v, payload) from LDS instead of loading it a second time.
+template <class Op, class = void>
+struct ScanPayload {
+    static constexpr bool value = false;
+};
 template <class Op>
+struct ScanPayload<Op, decltype((void)&Op::payload)> {
+    static constexpr bool value = true;
+};
+
+template <bool PL = false, class Op>
 __device__ __forceinline__ void scan_load(const Op& op, uint64_t N, uint64_t tbase, uint32_t tid,
-                                          uint64_t (&v)[SC_ITEMS]) {
+                                          uint64_t (&v)[SC_ITEMS], uint32_t* pl = nullptr) {
     if (Op::kBlocked) {
         const uint64_t base = tbase + (uint64_t)tid * SC_ITEMS;
         if (tbase + SC_TILE <= N) {
 #pragma unroll
-            for (int k = 0; k < SC_ITEMS; k++) v[k] = op.value(base + k);
+            for (int k = 0; k < SC_ITEMS; k++) {
+                v[k] = op.value(base + k);
+                if constexpr (PL) pl[k] = op.payload(base + k);
+            }
         } else {
 #pragma unroll
-            for (int k = 0; k < SC_ITEMS; k++) v[k] = base + k < N ? op.value(base + k) : 0ull;
+            for (int k = 0; k < SC_ITEMS; k++) {
+                v[k] = base + k < N ? op.value(base + k) : 0ull;
+                if constexpr (PL) pl[k] = base + k < N ? op.payload(base + k) : 0u;
+            }
         }
     } else {
 #pragma unroll
@@ -543,9 +560,11 @@ __global__ __launch_bounds__(1024) void k_scan_spine(const uint64_t* __restrict_
 template <class Op>
 __global__ __launch_bounds__(SC_THREADS) void k_scan_down(Op op, uint64_t N, uint32_t tpb,
                                                           const uint64_t* __restrict__ prefix) {
+    constexpr bool PL = ScanPayload<Op>::value && Op::kBlocked;
     constexpr uint32_t PAD = SC_TILE + SC_TILE / 16;  // slot of the tile's end prefix
     __shared__ uint64_t s_tmp[SC_THREADS / 64];
     __shared__ uint64_t s_e[PAD + 1];  // padded to spread blocked accesses over banks
+    __shared__ uint32_t s_p[PL ? PAD : 1];
     const uint32_t tid = threadIdx.x;
     const uint64_t t0 = (uint64_t)blockIdx.x * tpb;
     uint64_t carry = prefix[blockIdx.x];
@@ -553,7 +572,8 @@ __global__ __launch_bounds__(SC_THREADS) void k_scan_down(Op op, uint64_t N, uin
         const uint64_t tbase = t * SC_TILE;
         if (tbase >= N) break;
         uint64_t v[SC_ITEMS];
-        scan_load(op, N, tbase, tid, v);
+        uint32_t pl[PL ? SC_ITEMS : 1];
+        scan_load<PL>(op, N, tbase, tid, v, pl);
         if (!Op::kBlocked) {
             // striped values -> blocked through LDS
 #pragma unroll
@@ -578,6 +598,7 @@ __global__ __launch_bounds__(SC_THREADS) void k_scan_down(Op op, uint64_t N, uin
             const uint32_t idx = tid * SC_ITEMS + k;
             s_e[idx + (idx >> 4)] = run;
             run += v[k];
+            if constexpr (PL) s_p[idx + (idx >> 4)] = pl[k];
         }
         if (tid == SC_THREADS - 1) s_e[PAD] = run;
         __syncthreads();
@@ -587,7 +608,11 @@ __global__ __launch_bounds__(SC_THREADS) void k_scan_down(Op op, uint64_t N, uin
             const uint32_t nx = idx + 1;
             const uint64_t i = tbase + idx;
             const uint64_t e = s_e[idx + (idx >> 4)];
-            if (i < N) op.write(i, e, s_e[nx + (nx >> 4)] - e);
+            if constexpr (PL) {
+                if (i < N) op.write(i, e, s_e[nx + (nx >> 4)] - e, s_p[idx + (idx >> 4)]);
+            } else {
+                if (i < N) op.write(i, e, s_e[nx + (nx >> 4)] - e);
+            }
         }
         carry += tile_total;
         __syncthreads();
