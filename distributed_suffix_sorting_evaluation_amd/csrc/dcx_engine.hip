@@ -32,6 +32,12 @@ struct Options {
 #define DCX_ROUND0_SLACK 8
 #endif
 constexpr uint32_t ROUND0_SLACK = DCX_ROUND0_SLACK;  // key bits beyond log2(N) in round 0 of large sorts
+#ifndef DCX_PACKED_SLACK
+#define DCX_PACKED_SLACK 14
+#endif
+// round 0 packs the position into the key when the characters beside it carry this many bits of
+// information beyond log2(N)
+constexpr uint32_t PACKED_SLACK = DCX_PACKED_SLACK;
 
 class Engine {
   public:
@@ -151,11 +157,11 @@ class Engine {
         uint32_t c0 = L < cpw ? L : cpw;
         const uint32_t pbits = bit_width(N - 1);
         const uint32_t cpack = std::min<uint32_t>(c0, (64 - pbits) / bits);
-        if (N >= (1u << 16) && cpack * info >= bit_width(N) + 14) {
+        if (N >= (1u << 16) && cpack * info >= bit_width(N) + PACKED_SLACK) {
             // round 0 sorts one word per position holding the first characters above the position itself
             // (16 instead of 24 bytes moved per pass) when the characters left beside the position still
-            // carry 14 bits of information more than log2(N): low-entropy texts would otherwise leave most
-            // positions tied
+            // carry PACKED_SLACK bits of information more than log2(N); the positions they leave tied go
+            // through the group rounds
             c0 = cpack;
             const uint32_t kbits = c0 * bits;
             const uint64_t hi = ~0ull << (64 - kbits);
