@@ -19,14 +19,15 @@ namespace dcx {
 
 // 512 threads x 14 keys: 7168-key tiles at two workgroups per CU.  Fewer, larger tiles mean fewer
 // lookbacks; measured on gfx950 for 64-bit keys against 512 x 8/10/12/13/15/16, 384 x 10/12, 256 x 12/16/24,
-// 768 x 8 and 1024 x 6/8 (15 and 16 keys per thread fall off sharply)
+// 768 x 8 and 1024 x 6/8/10/12/14 (15 and 16 keys per thread fall off sharply; 1024 x 14 at one
+// workgroup per CU wins on random 64-bit keys but not on the suffix-sort passes)
 #ifndef DCX_RS_ITEMS
 #define DCX_RS_ITEMS 14
 #endif
 #ifndef DCX_RS_THREADS
 #define DCX_RS_THREADS 512
 #endif
-constexpr int RS_THREADS = DCX_RS_THREADS;  // 256 or 512
+constexpr int RS_THREADS = DCX_RS_THREADS;  // a multiple of 64, at least 256 (one thread per digit)
 constexpr int RS_ITEMS = DCX_RS_ITEMS;  // keys per thread of one radix tile
 constexpr int RS_TILE = RS_THREADS * RS_ITEMS;  // keys per workgroup
 // 32-bit keys (permutation scatters) take less LDS and fewer registers per key: larger tiles
