@@ -119,7 +119,7 @@ class Engine {
     // active sets: members of groups of size >= 2 of the heads array H, as positions and group ids
     // -------------------------------------------------------------------------------------------
     struct ActiveSet {
-        DBuf<uint32_t> act, gid, gstart;  // gstart: G + 1 entries, the last one A
+        DBuf<uint32_t> act, gid, gstart, gfirst;  // gstart: G + 1 entries, the last one A; gfirst: G entries
         uint32_t A = 0, G = 0;
     };
 
@@ -129,10 +129,13 @@ class Engine {
         out.act = DBuf<uint32_t>(cache, Nsrc);
         out.gid = DBuf<uint32_t>(cache, Nsrc);
         out.gstart = DBuf<uint32_t>(cache, (size_t)Nsrc / 2 + 1);  // groups have >= 2 members
+        out.gfirst = DBuf<uint32_t>(cache, (size_t)Nsrc / 2 + 1);
         if (src && !compact)
-            prim.scan(ActiveScanOp<true>{H, src, out.act.get(), out.gid.get(), out.gstart.get()}, Nsrc, 0);
+            prim.scan(ActiveScanOp<true>{H, src, out.act.get(), out.gid.get(), out.gstart.get(), out.gfirst.get()},
+                      Nsrc, 0);
         else
-            prim.scan(ActiveScanOp<false>{H, src, out.act.get(), out.gid.get(), out.gstart.get()}, Nsrc, 0);
+            prim.scan(ActiveScanOp<false>{H, src, out.act.get(), out.gid.get(), out.gstart.get(), out.gfirst.get()},
+                      Nsrc, 0);
         const uint64_t tot = prim.read_scalars(1)[0];
         out.A = (uint32_t)tot;
         out.G = (uint32_t)(tot >> 32);
@@ -277,8 +280,8 @@ class Engine {
                       DBuf<uint32_t>& order, DBuf<uint8_t>& H, DBuf<uint8_t>& Ha) {
         const uint32_t A = cur.A, G = cur.G;
         DBuf<uint8_t> large(cache, G);
-        k_refine_small<<<grid(G), 256, 0, stream>>>(cur.gstart.get(), G, cur.act.get(), order.get(), H.get(), Ha.get(),
-                                                    PT, bits, C, c * bits, large.get());
+        k_refine_small<<<grid(G), 256, 0, stream>>>(cur.gstart.get(), G, cur.gfirst.get(), order.get(), H.get(),
+                                                    Ha.get(), PT, bits, C, c * bits, large.get());
         prim.fill_u8(Ha.get() + A, 1, 1);
         DBuf<uint32_t> slots(cache, A);
         prim.scan(LargeMembersOp{large.get(), cur.gid.get(), slots.get()}, A, 1);
@@ -335,7 +338,7 @@ class Engine {
         const uint32_t A = as.A, G = as.G;
         if (A == 0) return;
         DBuf<uint8_t> large(cache, G);
-        k_tiebreak_small<<<grid(G), 256, 0, stream>>>(as.gstart.get(), G, as.act.get(), order.get(), dc, d_cmp.get(),
+        k_tiebreak_small<<<grid(G), 256, 0, stream>>>(as.gstart.get(), G, as.gfirst.get(), order.get(), dc, d_cmp.get(),
                                                       ranks, large.get());
         // large groups: merge sort with the composite (group, DC comparator) order
         DBuf<uint64_t> items(cache, A), items2(cache, A);

@@ -311,6 +311,7 @@ struct ActiveScanOp {
     uint32_t* act_idx;
     uint32_t* gid;
     uint32_t* gstart;  // list index of each group's first member
+    uint32_t* gfirst;  // element index (order slot) of each group's first member
     __device__ __forceinline__ uint64_t value(uint64_t a) const {
         const uint64_t j = GATHER ? src[a] : a;
         const uint32_t h = H[j], h1 = H[j + 1];  // both loads unconditional: mergeable
@@ -319,10 +320,14 @@ struct ActiveScanOp {
     __device__ __forceinline__ void write(uint64_t a, uint64_t excl, uint64_t v) const {
         if (v) {
             const uint32_t o = (uint32_t)excl;
-            act_idx[o] = src ? src[a] : (uint32_t)a;
+            const uint32_t j = src ? src[a] : (uint32_t)a;
+            act_idx[o] = j;
             const uint32_t g = (uint32_t)(excl >> 32) + (uint32_t)(v >> 32) - 1u;
             gid[o] = g;
-            if (v >> 32) gstart[g] = o;  // a head: the first member of its group
+            if (v >> 32) {  // a head: the first member of its group
+                gstart[g] = o;
+                gfirst[g] = j;
+            }
         }
     }
 };
@@ -660,7 +665,7 @@ __device__ __forceinline__ bool dc_rank_less(uint32_t a, uint32_t b, const DcDev
 constexpr uint32_t RF_SMALL = 8;
 
 __global__ __launch_bounds__(256) void k_refine_small(const uint32_t* __restrict__ gstart, uint32_t G,
-                                                      const uint32_t* __restrict__ act_idx, uint32_t* __restrict__ order,
+                                                      const uint32_t* __restrict__ gfirst, uint32_t* __restrict__ order,
                                                       uint8_t* __restrict__ H, uint8_t* __restrict__ Ha,
                                                       const uint64_t* __restrict__ PT, uint32_t bits, uint32_t C,
                                                       uint32_t cbits, uint8_t* __restrict__ large) {
@@ -672,7 +677,7 @@ __global__ __launch_bounds__(256) void k_refine_small(const uint32_t* __restrict
             continue;
         }
         large[g] = 0;
-        const uint32_t j0 = act_idx[a0];  // a group occupies consecutive slots of the order
+        const uint32_t j0 = gfirst[g];  // a group occupies consecutive slots of the order
         if (s == 2) {  // the common case (a copied segment and its source): one comparison
             const uint32_t e0 = order[j0], e1 = order[j0 + 1];
             const uint64_t k0 = chars((uint64_t)e0 + C), k1 = chars((uint64_t)e1 + C);
@@ -763,7 +768,7 @@ constexpr uint32_t TB_SMALL = 16;
 // groups of size <= TB_SMALL: one thread sorts its group by insertion sort with the DC comparator;
 // larger groups are flagged for the merge-sort path
 __global__ __launch_bounds__(256) void k_tiebreak_small(const uint32_t* __restrict__ gstart, uint32_t G,
-                                                        const uint32_t* __restrict__ act_idx, uint32_t* __restrict__ order,
+                                                        const uint32_t* __restrict__ gfirst, uint32_t* __restrict__ order,
                                                         DcDev dc, const uint32_t* __restrict__ cmp,
                                                         const uint32_t* __restrict__ ranks, uint8_t* __restrict__ large) {
     for (uint32_t g = blockIdx.x * 256 + threadIdx.x; g < G; g += gridDim.x * 256) {
@@ -773,7 +778,7 @@ __global__ __launch_bounds__(256) void k_tiebreak_small(const uint32_t* __restri
             continue;
         }
         large[g] = 0;
-        uint32_t j0 = act_idx[a0];
+        uint32_t j0 = gfirst[g];
         uint32_t e[TB_SMALL];
         for (uint32_t t = 0; t < s; t++) e[t] = order[j0 + t];
         for (uint32_t t = 1; t < s; t++) {
