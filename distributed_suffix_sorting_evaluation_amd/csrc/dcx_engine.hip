@@ -192,11 +192,18 @@ class Engine {
             uint64_t* k2 = kb.get();
             uint32_t* v = va.get();
             uint32_t* v2 = vb.get();
-            // every position is an element: the first pass reads the packed words directly
-            prim.sort_pairs_from(StreamKeys<false>{PT, bits, top_mask(c0 * bits)}, k, k2, v, v2, N, 64 - c0 * bits, 64,
-                                 true);
+            // every position is an element: the first pass reads the character stream directly; the last
+            // pass writes only the 32 key bits below its digit, enough for the heads
+            DBuf<uint32_t> low(cache, N);
+            const uint32_t np = prim.sort_pairs_from(StreamKeys<false>{PT, bits, top_mask(c0 * bits)}, k, k2, v, v2, N,
+                                                     64 - c0 * bits, 64, true, low.get());
             H = DBuf<uint8_t>(cache, (size_t)N + 1);
-            k_heads<<<grid(N + 1ull), 256, 0, stream>>>(k, N, c0 * bits, H.get());
+            if (np) {
+                k_heads_low32<<<grid(N + 1ull), 256, 0, stream>>>(low.get(), N, H.get());
+                k_mark_runs<<<1, RS_RADIX, 0, stream>>>(prim.hist + (np - 1) * RS_RADIX, N, H.get());
+            } else {
+                k_heads<<<grid(N + 1ull), 256, 0, stream>>>(k, N, c0 * bits, H.get());
+            }
             if (v == va.get())
                 order = std::move(va);
             else

@@ -286,6 +286,19 @@ __global__ __launch_bounds__(256) void k_heads(const uint64_t* __restrict__ keys
         H[j] = (j == 0 || j == N) ? 1 : (top_bits(keys[j], kbits) != top_bits(keys[j - 1], kbits));
 }
 
+// heads from the low words written by the last radix pass (sort_pairs_from's low32): equal low words
+// within a run of the last digit; the run starts are marked by k_mark_runs
+__global__ __launch_bounds__(256) void k_heads_low32(const uint32_t* __restrict__ low, uint32_t N,
+                                                     uint8_t* __restrict__ H) {
+    for (uint32_t j = blockIdx.x * 256 + threadIdx.x; j <= N; j += gridDim.x * 256)
+        H[j] = (j == 0 || j == N) ? 1 : (low[j] != low[j - 1]);
+}
+__global__ __launch_bounds__(256) void k_mark_runs(const uint32_t* __restrict__ offsets, uint32_t N,
+                                                   uint8_t* __restrict__ H) {
+    const uint32_t o = offsets[threadIdx.x];  // exclusive digit offsets: every run starts at one
+    if (o < N) H[o] = 1;
+}
+
 // packed round 0: keys are (characters << pbits) | position; heads compare the characters, order gets the
 // positions
 __global__ __launch_bounds__(256) void k_heads_packed(const uint64_t* __restrict__ keys, uint32_t N, uint32_t kbits,

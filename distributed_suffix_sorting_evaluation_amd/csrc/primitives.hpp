@@ -165,13 +165,15 @@ __global__ __launch_bounds__(256) void k_radix_hist_scan(uint32_t* hist) {
 // One LSD pass over digit (key >> shift) & 255.  Stable.  Keys come from `src` (the previous pass's
 // output, or a first-pass key source) and go to kout as KeyT: 64-bit sort keys or 32-bit keys (radix
 // partitions by destination index).  IDENT: values are the input indices.
-template <bool VALS, bool IDENT, class Src, class KeyT = uint64_t, int ITEMS = RS_ITEMS>
-__global__ __launch_bounds__(RS_THREADS) void k_onesweep(Src src, KeyT* __restrict__ kout,
+// OutT (when not KeyT): the pass writes OutT(key >> out_shift) instead of the key (the last pass of a sort
+// whose remaining key bits fit 32 bits below its digit).
+template <bool VALS, bool IDENT, class Src, class KeyT = uint64_t, int ITEMS = RS_ITEMS, class OutT = KeyT>
+__global__ __launch_bounds__(RS_THREADS) void k_onesweep(Src src, OutT* __restrict__ kout,
                                                          const uint32_t* __restrict__ vin, uint32_t* __restrict__ vout,
                                                          uint32_t N, uint32_t shift, const uint32_t* __restrict__ dbase,
                                                          uint64_t* __restrict__ status, uint32_t tag,
                                                          unsigned long long* __restrict__ ticket,
-                                                         unsigned long long ticket_base) {
+                                                         unsigned long long ticket_base, uint32_t out_shift = 0) {
     constexpr int TILE = RS_THREADS * ITEMS;
     __shared__ KeyT s_keys[TILE];
     __shared__ uint32_t s_wcnt[RS_WAVES][RS_RADIX];  // per-wave digit counts, then wave offsets
@@ -299,7 +301,7 @@ __global__ __launch_bounds__(RS_THREADS) void k_onesweep(Src src, KeyT* __restri
         if (i < tile_n) {
             const KeyT kk = s_keys[i];
             dg[r] = (uint32_t)((uint64_t)kk >> shift) & 0xFFu;
-            kout[s_gofs[dg[r]] + i] = kk;
+            kout[s_gofs[dg[r]] + i] = (OutT)((uint64_t)kk >> out_shift);
         }
     }
     if (VALS) {
@@ -775,19 +777,27 @@ struct Primitives {
     // (ArrayKeys{keys}: the keys array itself); with ident_vals its values are the input indices (vals
     // is then only an output).  vals == nullptr sorts keys only.  On return keys/vals point at the
     // sorted data (the buffers may have been swapped with the alternates).
+    //
+    // low32 (optional, with values and at most five passes): the last pass writes the key bits below its
+    // digit, (key >> lo_bit) truncated to 32 bits, to low32 instead of the keys (keys then holds the input
+    // of that pass).  Two sorted keys are equal iff they lie in the same run of the last digit (the runs
+    // start at the last pass's digit offsets, hist + (np - 1) * 256) and their low32 words are equal.
+    // Returns the number of passes when low32 was written, else 0.
     template <class Src>
-    void sort_pairs_from(const Src& first, uint64_t*& keys, uint64_t*& keys_alt, uint32_t*& vals,
-                         uint32_t*& vals_alt, uint32_t N, uint32_t lo_bit, uint32_t hi_bit, bool ident_vals) {
+    uint32_t sort_pairs_from(const Src& first, uint64_t*& keys, uint64_t*& keys_alt, uint32_t*& vals,
+                         uint32_t*& vals_alt, uint32_t N, uint32_t lo_bit, uint32_t hi_bit, bool ident_vals,
+                         uint32_t* low32 = nullptr) {
         if (hi_bit <= lo_bit) hi_bit = lo_bit + 1;
         if (N <= 1) {
             // nothing to order: materialize the inputs in place
             if (N == 1) hipLaunchKernelGGL(k_materialize<Src>, dim3(1), dim3(64), 0, stream, first, N, keys);
             if (N == 1 && ident_vals && vals) fill_u32(vals, 0, 1);
-            return;
+            return 0;
         }
         uint32_t np = (hi_bit - lo_bit + 7) / 8;
         if (np > 8) np = 8;
         if (lo_bit + 8 * np > 64) lo_bit = 64 - 8 * np;
+        if (np > 5 || np < 2 || !vals) low32 = nullptr;  // the last pass must not be the first
         uint64_t tiles = ceil_div(N, RS_TILE);
         ensure_sort_status(tiles);
         fill_u32(hist, 0, 9 * RS_RADIX);
@@ -821,16 +831,21 @@ struct Primitives {
                     hipLaunchKernelGGL((k_onesweep<true, false, Src>), g, b, 0, stream, first, keys_alt,
                                        (const uint32_t*)vals, vals_alt, N, shift, dbase, sort_status, tag, ticket,
                                        ticket_host);
+            } else if (low32 && p == np - 1 && vals) {
+                const ArrayKeys<uint64_t> in{keys};
+                hipLaunchKernelGGL((k_onesweep<true, false, ArrayKeys<uint64_t>, uint64_t, RS_ITEMS, uint32_t>), g, b, 0,
+                                   stream, in, low32, (const uint32_t*)vals, vals_alt, N, shift, dbase, sort_status, tag,
+                                   ticket, ticket_host, lo_bit);
             } else {
                 const ArrayKeys<uint64_t> in{keys};
                 if (!vals)
                     hipLaunchKernelGGL((k_onesweep<false, false, ArrayKeys<uint64_t>>), g, b, 0, stream, in, keys_alt,
                                        (const uint32_t*)nullptr, (uint32_t*)nullptr, N, shift, dbase, sort_status, tag,
-                                       ticket, ticket_host);
+                                       ticket, ticket_host, 0u);
                 else
                     hipLaunchKernelGGL((k_onesweep<true, false, ArrayKeys<uint64_t>>), g, b, 0, stream, in, keys_alt,
                                        (const uint32_t*)vals, vals_alt, N, shift, dbase, sort_status, tag, ticket,
-                                       ticket_host);
+                                       ticket_host, 0u);
             }
             if (profile) {
                 DCX_HIP(hipEventRecord(e1, stream));
@@ -838,9 +853,10 @@ struct Primitives {
                 pass_items.push_back(N);
             }
             ticket_host += tiles;
-            std::swap(keys, keys_alt);
+            if (!(low32 && p == np - 1 && vals)) std::swap(keys, keys_alt);
             if (vals) std::swap(vals, vals_alt);
         }
+        return low32 ? np : 0;
     }
 
     // the same with the keys array as the first pass's key source
