@@ -795,7 +795,7 @@ class Engine {
             DCX_HIP(hipEventElapsedTime(&a, e0, e1));
             st->ms_device = a;
             st->alg_bytes = alg_bytes;
-            if (prim.profile) prim.collect_profile(&st->sort_passes, &st->sort_pass_items, &st->sort_pass_ms);
+            if (prim.profile) prim.collect_profile(&st->sort_passes, &st->sort_pass_items, &st->sort_pass_bytes, &st->sort_pass_ms);
             st->peak_device_bytes = cache.peak();
             st->ms_total = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
         }
@@ -896,7 +896,7 @@ class Engine {
             DCX_HIP(hipEventElapsedTime(&a, ev[4], ev[5]));
             st->ms_phase[DCX_PHASE_OUTPUT] = a;
             st->alg_bytes = alg_bytes;
-            if (prim.profile) prim.collect_profile(&st->sort_passes, &st->sort_pass_items, &st->sort_pass_ms);
+            if (prim.profile) prim.collect_profile(&st->sort_passes, &st->sort_pass_items, &st->sort_pass_bytes, &st->sort_pass_ms);
             st->peak_device_bytes = cache.peak();
             st->ms_total =
                 std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();

@@ -3,9 +3,10 @@
 //  * LSD radix sort of (u64 key, u32 value) pairs (and u32 keys for permutation scatters), one kernel
 //    launch per 8-bit digit.  Each pass is a "onesweep" pass: a workgroup ranks a 7168-key tile with
 //    wavefront ballots (no per-key atomics), reorders it by digit in LDS, obtains its per-digit output
-//    offsets with a decoupled lookback over the preceding tiles and writes each digit run contiguously.
-//    One upfront kernel builds the histograms of every pass from a single read of the first-pass keys,
-//    which may come from a key source (e.g. windows of a packed character stream) instead of an array.
+//    offsets with a decoupled lookback over the preceding tiles of its segment (one segment, and one
+//    lookback chain, per XCD) and writes each digit run contiguously.  One upfront kernel builds the
+//    joint (segment, digit) histograms of every pass from a single read of the first-pass keys, which
+//    may come from a key source (e.g. windows of a packed character stream) instead of an array.
 //  * device scans over a functor (flags -> positions, names ...): two-pass reduce-then-scan for long
 //    inputs, single-pass decoupled lookback for short ones.
 //
@@ -73,6 +74,7 @@ template <class KeyT>
 struct ArrayKeys {  // keys[i]
     const KeyT* k;
     __device__ __forceinline__ uint64_t operator()(uint64_t i) const { return (uint64_t)k[i]; }
+    uint64_t read_bytes(uint64_t n) const { return n * sizeof(KeyT); }  // bytes a pass reads for n keys
 };
 
 // The 64-bit window at bit offset i * bits of an MSB-first bit stream (one element of `bits` bits per
@@ -91,61 +93,8 @@ struct StreamKeys {
         const uint64_t x = (s[w] << sh) | ((s[w + 1] >> 1) >> (63 - sh));
         return PACKPOS ? ((x & mask) | i) : (x & mask);
     }
+    uint64_t read_bytes(uint64_t n) const { return (n * bits + 63) / 64 * 8; }  // the stream words of n keys
 };
-
-// Histograms of all digit passes [lo_bit, lo_bit + 8*npasses) from one read of the keys.
-template <class Src>
-__global__ __launch_bounds__(256) void k_radix_hist(Src src, uint32_t N, uint32_t lo_bit, uint32_t npasses,
-                                                    uint32_t* __restrict__ hist) {
-    __shared__ uint32_t s_h[8][RS_RADIX];
-    for (int i = threadIdx.x; i < 8 * RS_RADIX; i += 256) (&s_h[0][0])[i] = 0;
-    __syncthreads();
-    const uint64_t stride = (uint64_t)gridDim.x * 256;
-    for (uint64_t i = blockIdx.x * 256ull + threadIdx.x; i < N; i += stride) {
-        const uint64_t k = src(i);
-        for (uint32_t p = 0; p < npasses; p++) atomicAdd(&s_h[p][(k >> (lo_bit + 8 * p)) & 0xFF], 1u);
-    }
-    __syncthreads();
-    for (uint32_t p = 0; p < npasses; p++) {
-        uint32_t c = s_h[p][threadIdx.x];
-        if (c) atomicAdd(&hist[p * RS_RADIX + threadIdx.x], c);
-    }
-}
-
-// hw[d] = number of q in [0, N) whose byte-wide window (top 8 bits of the 64-bit window at bit q * bits)
-// is d, counted once per wave and digit
-__global__ __launch_bounds__(256) void k_window_hist(const uint64_t* __restrict__ s, uint32_t bits, uint32_t N,
-                                                     uint32_t* __restrict__ hw) {
-    __shared__ uint32_t s_h[RS_RADIX];
-    s_h[threadIdx.x] = 0;
-    __syncthreads();
-    const StreamKeys<false> win{s, bits, ~0ull};
-    const uint64_t stride = (uint64_t)gridDim.x * 256;
-    for (uint64_t q = blockIdx.x * 256ull + threadIdx.x; q < N; q += stride) {
-        const uint32_t d = (uint32_t)(win(q) >> 56);
-        atomicAdd(&s_h[d], 1u);
-    }
-    __syncthreads();
-    const uint32_t c = s_h[threadIdx.x];
-    if (c) atomicAdd(&hw[threadIdx.x], c);
-}
-
-// pass p's histogram: its digit is the byte window at character i + s_p, s_p = (56 - lo_bit - 8p) / bits,
-// so hist_p = hw - windows at [0, s_p) + windows at [N, N + s_p).  One workgroup per pass, thread d.
-__global__ __launch_bounds__(RS_RADIX) void k_window_hist_passes(const uint64_t* __restrict__ s, uint32_t bits,
-                                                                 uint32_t N, uint32_t lo_bit,
-                                                                 const uint32_t* __restrict__ hw,
-                                                                 uint32_t* __restrict__ hist) {
-    const uint32_t p = blockIdx.x, d = threadIdx.x;
-    const uint32_t sp = (56 - lo_bit - 8 * p) / bits;
-    const StreamKeys<false> win{s, bits, ~0ull};
-    int32_t c = (int32_t)hw[d];
-    for (uint32_t q = 0; q < sp; q++) {
-        c -= (uint32_t)(win(q) >> 56) == d;
-        c += (uint32_t)(win((uint64_t)N + q) >> 56) == d;
-    }
-    hist[p * RS_RADIX + d] = (uint32_t)c;
-}
 
 // keys[i] = src(i) for i < N (sorts of a single element)
 template <class Src>
@@ -153,13 +102,203 @@ __global__ void k_materialize(Src src, uint32_t N, uint64_t* __restrict__ keys) 
     for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < N; i += gridDim.x * blockDim.x) keys[i] = src(i);
 }
 
-// In-place exclusive scan of each pass's 256 counts -> global digit offsets.
-__global__ __launch_bounds__(256) void k_radix_hist_scan(uint32_t* hist) {
+// ------------------------------------------------------------------------------------------------
+// Segmented radix passes.  A pass cuts its input into RS_SEGS contiguous segments with one
+// decoupled-lookback chain each.  A workgroup takes the next tile of the segment numbered like the XCD
+// it runs on (the hardware XCC id), and the other segments' tiles once its own are all taken.  The
+// consecutive tiles of a segment then run on one XCD, so the partial cache lines at the ends of their
+// digit runs meet in that XCD's L2 and leave it as whole lines: 10% faster passes on gfx950 than one
+// chain whose consecutive tiles land on different XCDs.
+// Segments of pass 0: RS_SEGS equal ranges of tiles.  Of pass p >= 1, whose input is ordered by the
+// digit of pass p - 1: the keys whose pass p - 1 digit lies in [32 s, 32 s + 32).  One joint histogram
+// of the first-pass keys therefore gives every pass's (segment, digit) counts.  Within a segment the
+// tiles are handed out in order by the segment's own counter, so the chain predecessors of a tile are
+// always held by running workgroups.
+// ------------------------------------------------------------------------------------------------
+constexpr int RS_SEGS = 8;
+constexpr uint32_t RS_SEG_SHIFT = 5;             // a segment spans 256 / RS_SEGS digits of the previous pass
+constexpr int RS_SEG_BINS = RS_SEGS * RS_RADIX;  // (segment, digit) bins of one pass
+constexpr int RS_SEG_INFO = 2 * RS_SEGS + 2;     // per pass: segment starts [0, 8], tile prefixes [9, 17]
+constexpr uint32_t RS_HIST_KEYS = 1u << 16;      // histogram keys per workgroup
+constexpr uint32_t RS_HIST_WPS = 64;             // at most this many histogram workgroups per segment
+constexpr int RS_HIST_THREADS = 1024;            // 512 workgroups of 16 waves fill the machine
+
+// the XCD (accelerator complex die) this wave runs on: hardware register XCC_ID, bits [3:0]
+__device__ __forceinline__ uint32_t xcc_id() { return __builtin_amdgcn_s_getreg(20 | (0 << 6) | (3 << 11)); }
+
+// Joint histograms of every pass from one read of the first-pass keys.  Workgroup b covers a
+// contiguous share of pass-0 segment b / wps; part[b][p][s][d] counts its keys whose pass p digit is d
+// and that lie in segment s of pass p.  Workgroup 0 also clears h2 for k_hist_seg_reduce.
+template <class Src>
+__global__ __launch_bounds__(RS_HIST_THREADS) void k_radix_hist_seg(Src src, uint32_t N, uint32_t lo_bit, uint32_t np,
+                                                        uint32_t seg0_keys, uint32_t wps, uint32_t* __restrict__ part,
+                                                        uint32_t* __restrict__ h2) {
+    __shared__ uint32_t s_h[8 * RS_SEG_BINS];
+    const uint32_t bins = np * RS_SEG_BINS;
+    for (uint32_t i = threadIdx.x; i < bins; i += RS_HIST_THREADS) s_h[i] = 0;
+    if (blockIdx.x == 0)
+        for (uint32_t i = threadIdx.x; i < bins; i += RS_HIST_THREADS) h2[i] = 0;
+    __syncthreads();
+    const uint32_t sg0 = blockIdx.x / wps, c = blockIdx.x % wps;
+    const uint64_t a = (uint64_t)sg0 * seg0_keys < N ? (uint64_t)sg0 * seg0_keys : N;
+    const uint64_t e = (uint64_t)(sg0 + 1) * seg0_keys < N ? (uint64_t)(sg0 + 1) * seg0_keys : N;
+    const uint64_t len = e - a, chunk = (len + wps - 1) / wps;
+    const uint64_t lo = a + (c * chunk < len ? c * chunk : len);
+    const uint64_t hi = a + ((c + 1) * chunk < len ? (c + 1) * chunk : len);
+    for (uint64_t i = lo + threadIdx.x; i < hi; i += RS_HIST_THREADS) {
+        const uint64_t k = src(i);
+        uint32_t sg = sg0;
+        for (uint32_t p = 0; p < np; p++) {
+            const uint32_t dg = (uint32_t)(k >> (lo_bit + 8 * p)) & 0xFFu;
+            atomicAdd(&s_h[p * RS_SEG_BINS + sg * RS_RADIX + dg], 1u);
+            sg = dg >> RS_SEG_SHIFT;
+        }
+    }
+    __syncthreads();
+    uint32_t* out = part + (uint64_t)blockIdx.x * bins;
+    for (uint32_t i = threadIdx.x; i < bins; i += RS_HIST_THREADS) out[i] = s_h[i];
+}
+
+// Character streams whose width divides 8, with every digit on a character boundary.  The digit of
+// pass p of key q is the top byte of the stream window at character q + s_p, s_p = (56 - lo - 8 p) /
+// bits, and the pass p (segment, digit) pair, for p >= 1, is the top 11 bits of that window.  So one
+// histogram of the top 11 bits of the windows at [0, N), kept per pass-0 segment, gives all passes
+// after corrections at the ends.  part[b][w] counts the windows at the characters of histogram
+// workgroup b (a share of pass-0 segment b / wps) whose top 11 bits are w.
+__global__ __launch_bounds__(RS_HIST_THREADS) void k_window_hist_seg(const uint64_t* __restrict__ s, uint32_t bits, uint32_t N,
+                                                         uint32_t seg0_keys, uint32_t wps,
+                                                         uint32_t* __restrict__ part) {
+    __shared__ uint32_t s_h[RS_SEG_BINS];
+    for (uint32_t i = threadIdx.x; i < RS_SEG_BINS; i += RS_HIST_THREADS) s_h[i] = 0;
+    __syncthreads();
+    const uint32_t sg0 = blockIdx.x / wps, c = blockIdx.x % wps;
+    const uint64_t a = (uint64_t)sg0 * seg0_keys < N ? (uint64_t)sg0 * seg0_keys : N;
+    const uint64_t e = (uint64_t)(sg0 + 1) * seg0_keys < N ? (uint64_t)(sg0 + 1) * seg0_keys : N;
+    const uint64_t len = e - a, chunk = (len + wps - 1) / wps;
+    const uint64_t lo = a + (c * chunk < len ? c * chunk : len);
+    const uint64_t hi = a + ((c + 1) * chunk < len ? (c + 1) * chunk : len);
+    const StreamKeys<false> win{s, bits, ~0ull};
+    for (uint64_t q = lo + threadIdx.x; q < hi; q += RS_HIST_THREADS) atomicAdd(&s_h[(uint32_t)(win(q) >> 53)], 1u);
+    __syncthreads();
+    uint32_t* out = part + (uint64_t)blockIdx.x * RS_SEG_BINS;
+    for (uint32_t i = threadIdx.x; i < RS_SEG_BINS; i += RS_HIST_THREADS) out[i] = s_h[i];
+}
+
+// h11[s][w] = sum of part[b][w] over the histogram workgroups b of pass-0 segment s (grid 8 x RS_SEGS)
+__global__ __launch_bounds__(256) void k_window_seg_reduce(const uint32_t* __restrict__ part, uint32_t wps,
+                                                           uint32_t* __restrict__ h11) {
+    const uint32_t w = blockIdx.x * 256 + threadIdx.x, sg = blockIdx.y;
+    uint32_t c = 0;
+    for (uint32_t b = 0; b < wps; b++) c += part[(uint64_t)(sg * wps + b) * RS_SEG_BINS + w];
+    h11[sg * RS_SEG_BINS + w] = c;
+}
+
+// Workgroup p: the joint counts h2[p][s][d] of pass p from the window histograms.  Pass p >= 1:
+// windows at [s_p, N + s_p) = all windows - [0, s_p) + [N, N + s_p).  Pass 0, segment [a, b):
+// windows at [a + s_0, b + s_0) = the segment's windows - [a, a + s_0) + [b, b + s_0).
+__global__ __launch_bounds__(256) void k_window_seg_joint(const uint64_t* __restrict__ s, uint32_t bits, uint32_t N,
+                                                          uint32_t lo_bit, uint32_t seg0_keys,
+                                                          const uint32_t* __restrict__ h11,
+                                                          uint32_t* __restrict__ h2) {
+    __shared__ int s_c[RS_SEG_BINS];
+    const uint32_t p = blockIdx.x, d = threadIdx.x;
+    const uint32_t sp = (56 - lo_bit - 8 * p) / bits;
+    const StreamKeys<false> win{s, bits, ~0ull};
+    for (uint32_t i = d; i < RS_SEG_BINS; i += 256) s_c[i] = 0;
+    __syncthreads();
+    uint32_t* out = h2 + (uint64_t)p * RS_SEG_BINS;
+    if (p > 0) {
+        for (uint32_t t = d; t < sp; t += 256) {
+            atomicSub(&s_c[(uint32_t)(win(t) >> 53)], 1);
+            atomicAdd(&s_c[(uint32_t)(win((uint64_t)N + t) >> 53)], 1);
+        }
+        __syncthreads();
+#pragma unroll
+        for (int j = 0; j < RS_SEGS; j++) {
+            const uint32_t w = (d << 3) | j;
+            int c = s_c[w];
+#pragma unroll
+            for (int sg = 0; sg < RS_SEGS; sg++) c += (int)h11[sg * RS_SEG_BINS + w];
+            out[j * RS_RADIX + d] = (uint32_t)c;
+        }
+    } else {
+        // s_c[sg][d]: boundary corrections of segment sg
+        for (uint32_t t = d; t < RS_SEGS * sp; t += 256) {
+            const uint32_t sg = t / sp, k = t % sp;
+            const uint64_t a = (uint64_t)sg * seg0_keys < N ? (uint64_t)sg * seg0_keys : N;
+            const uint64_t b = (uint64_t)(sg + 1) * seg0_keys < N ? (uint64_t)(sg + 1) * seg0_keys : N;
+            atomicSub(&s_c[sg * RS_RADIX + (uint32_t)(win(a + k) >> 56)], 1);
+            atomicAdd(&s_c[sg * RS_RADIX + (uint32_t)(win(b + k) >> 56)], 1);
+        }
+        __syncthreads();
+#pragma unroll
+        for (int sg = 0; sg < RS_SEGS; sg++) {
+            int c = s_c[sg * RS_RADIX + d];
+#pragma unroll
+            for (int j = 0; j < RS_SEGS; j++) c += (int)h11[sg * RS_SEG_BINS + ((d << 3) | j)];
+            out[sg * RS_RADIX + d] = (uint32_t)c;
+        }
+    }
+}
+
+// h2[i] += the sum of part[b][i] over this workgroup's share of the histogram workgroups b
+__global__ __launch_bounds__(256) void k_hist_seg_reduce(const uint32_t* __restrict__ part, uint32_t blocks,
+                                                         uint32_t bins, uint32_t* __restrict__ h2) {
+    const uint32_t i = blockIdx.x * 256 + threadIdx.x;
+    if (i >= bins) return;
+    uint32_t c = 0;
+    for (uint32_t b = blockIdx.y; b < blocks; b += gridDim.y) c += part[(uint64_t)b * bins + i];
+    if (c) atomicAdd(&h2[i], c);
+}
+
+// Workgroup p plans pass p: hist[p][d] = global offset of digit d, segoff[p][s][d] = offset of the keys
+// of digit d in segment s, info[p] = segment starts and tile prefixes (tiles of `tile` keys); it also
+// clears the pass's tile counters.
+__global__ __launch_bounds__(256) void k_seg_setup(const uint32_t* __restrict__ h2, uint32_t N, uint32_t tile,
+                                                   uint32_t seg0_keys, uint32_t* __restrict__ hist,
+                                                   uint32_t* __restrict__ segoff, uint32_t* __restrict__ info,
+                                                   uint32_t* __restrict__ tick) {
     __shared__ uint32_t s_tmp[4];
-    uint32_t* h = hist + blockIdx.x * RS_RADIX;
-    uint32_t v = h[threadIdx.x];
-    uint32_t e = block_excl_scan<256>(v, s_tmp, nullptr);
-    h[threadIdx.x] = e;
+    __shared__ uint32_t s_prev[RS_RADIX];
+    const uint32_t p = blockIdx.x, d = threadIdx.x;
+    const uint32_t* hp = h2 + (uint64_t)p * RS_SEG_BINS;
+    uint32_t t = 0;
+#pragma unroll
+    for (int sg = 0; sg < RS_SEGS; sg++) t += hp[sg * RS_RADIX + d];
+    const uint32_t db = block_excl_scan<256>(t, s_tmp, nullptr);
+    hist[p * RS_RADIX + d] = db;
+    uint32_t run = db;
+#pragma unroll
+    for (int sg = 0; sg < RS_SEGS; sg++) {
+        segoff[p * RS_SEG_BINS + sg * RS_RADIX + d] = run;
+        run += hp[sg * RS_RADIX + d];
+    }
+    if (d < (uint32_t)RS_SEGS) tick[p * RS_SEGS + d] = 0;
+    if (p > 0) {
+        // segment starts: the digit offsets of the previous pass at multiples of 32
+        __syncthreads();
+        const uint32_t* hq = hp - RS_SEG_BINS;
+        uint32_t tq = 0;
+#pragma unroll
+        for (int sg = 0; sg < RS_SEGS; sg++) tq += hq[sg * RS_RADIX + d];
+        s_prev[d] = block_excl_scan<256>(tq, s_tmp, nullptr);
+    }
+    __syncthreads();
+    if (d == 0) {
+        uint32_t* in = info + p * RS_SEG_INFO;
+        for (int sg = 0; sg <= RS_SEGS; sg++) {
+            uint32_t a;
+            if (sg == RS_SEGS) a = N;
+            else if (p == 0) a = (uint64_t)sg * seg0_keys < N ? sg * seg0_keys : N;
+            else a = s_prev[sg << RS_SEG_SHIFT];
+            in[sg] = a;
+        }
+        uint32_t T = 0;
+        for (int sg = 0; sg <= RS_SEGS; sg++) {
+            in[RS_SEGS + 1 + sg] = T;
+            if (sg < RS_SEGS) T += (in[sg + 1] - in[sg] + tile - 1) / tile;
+        }
+    }
 }
 
 // One LSD pass over digit (key >> shift) & 255.  Stable.  Keys come from `src` (the previous pass's
@@ -170,26 +309,54 @@ __global__ __launch_bounds__(256) void k_radix_hist_scan(uint32_t* hist) {
 template <bool VALS, bool IDENT, class Src, class KeyT = uint64_t, int ITEMS = RS_ITEMS, class OutT = KeyT>
 __global__ __launch_bounds__(RS_THREADS) void k_onesweep(Src src, OutT* __restrict__ kout,
                                                          const uint32_t* __restrict__ vin, uint32_t* __restrict__ vout,
-                                                         uint32_t N, uint32_t shift, const uint32_t* __restrict__ dbase,
+                                                         uint32_t shift, const uint32_t* __restrict__ segoff,
+                                                         const uint32_t* __restrict__ info, uint32_t* __restrict__ tick,
                                                          uint64_t* __restrict__ status, uint32_t tag,
-                                                         unsigned long long* __restrict__ ticket,
-                                                         unsigned long long ticket_base, uint32_t out_shift = 0) {
+                                                         uint32_t out_shift = 0) {
     constexpr int TILE = RS_THREADS * ITEMS;
     __shared__ KeyT s_keys[TILE];
     __shared__ uint32_t s_wcnt[RS_WAVES][RS_RADIX];  // per-wave digit counts, then wave offsets
     __shared__ uint32_t s_start[RS_RADIX];
     __shared__ uint32_t s_gofs[RS_RADIX];
     __shared__ uint32_t s_scan[RS_WAVES];
-    __shared__ uint32_t s_tile;
+    __shared__ uint32_t s_tile[4];
 
     const uint32_t tid = threadIdx.x;
     const uint32_t lane = __lane_id();
     const uint32_t wave = tid / 64;
-    if (tid == 0) s_tile = (uint32_t)(atomicAdd(ticket, 1ull) - ticket_base);
+    if (tid == 0) {
+        // the next tile of this XCD's segment, else of the next segment with tiles left (the grid has a
+        // workgroup per tile, so every tile is taken)
+        const uint32_t* tp = info + RS_SEGS + 1;
+        const uint32_t x = xcc_id();
+        uint32_t sg = 0, i = 0, found = 0;
+        for (uint32_t v = 0; v < (uint32_t)RS_SEGS && !found; v++) {
+            sg = (x + v) & (RS_SEGS - 1);
+            const uint32_t T = tp[sg + 1] - tp[sg];
+            if (T) {
+                i = atomicAdd(&tick[sg], 1u);
+                found = i < T;
+            }
+        }
+        uint32_t n = 0, b = 0;
+        if (found) {
+            b = info[sg] + i * (uint32_t)TILE;
+            n = info[sg + 1] - b < (uint32_t)TILE ? info[sg + 1] - b : (uint32_t)TILE;
+        }
+        s_tile[0] = i;       // tile within the segment
+        s_tile[1] = tp[sg];  // status slot of the segment's first tile
+        s_tile[2] = b;       // first key
+        s_tile[3] = n | (sg << 29);
+    }
     for (int i = tid; i < RS_WAVES * RS_RADIX; i += RS_THREADS) (&s_wcnt[0][0])[i] = 0;
     __syncthreads();
-    const uint32_t tile = s_tile;
-    const uint64_t tile_base = (uint64_t)tile * TILE;
+    const uint32_t tile_n = s_tile[3] & ((1u << 29) - 1);
+    if (tile_n == 0) return;  // all tiles taken
+    const uint32_t tile = s_tile[0];
+    const uint32_t slot0 = s_tile[1];
+    const uint32_t sg = s_tile[3] >> 29;
+    const uint64_t tile_base = s_tile[2];
+    const uint64_t tile_end = tile_base + tile_n;
     const uint64_t wbase = tile_base + (uint64_t)wave * (64 * ITEMS);
 
     KeyT key[ITEMS];  // 32-bit keys stay 32-bit in registers
@@ -198,7 +365,7 @@ __global__ __launch_bounds__(RS_THREADS) void k_onesweep(Src src, OutT* __restri
 #pragma unroll
     for (int k = 0; k < ITEMS; k++) {
         uint64_t idx = wbase + k * 64 + lane;
-        bool valid = idx < N;
+        bool valid = idx < tile_end;
         key[k] = valid ? src(idx) : 0ull;
         if (VALS) val[k] = IDENT ? (uint32_t)idx : (valid ? vin[idx] : 0u);
     }
@@ -206,7 +373,7 @@ __global__ __launch_bounds__(RS_THREADS) void k_onesweep(Src src, OutT* __restri
 #pragma unroll
     for (int k = 0; k < ITEMS; k++) {
         uint64_t idx = wbase + k * 64 + lane;
-        bool valid = idx < N;
+        bool valid = idx < tile_end;
         uint32_t d = (uint32_t)(key[k] >> shift) & 0xFFu;
         uint64_t peers = __ballot(valid);
 #pragma unroll
@@ -237,10 +404,10 @@ __global__ __launch_bounds__(RS_THREADS) void k_onesweep(Src src, OutT* __restri
         }
     }
     uint32_t start = block_excl_scan<RS_THREADS>(dthr ? total : 0u, s_scan, nullptr);
-    uint64_t* st = status + (uint64_t)tile * RS_RADIX + d;
+    uint64_t* st = status + (uint64_t)(slot0 + tile) * RS_RADIX + d;
     if (dthr) {
         s_start[d] = start;
-        // publish this tile's count of the digit: inclusive right away for tile 0
+        // publish this tile's count of the digit: inclusive right away for the segment's first tile
         st_agent(st, lb_pack(tag, tile == 0 ? LB_INC : LB_AGG, total));
     }
     __syncthreads();
@@ -249,25 +416,23 @@ __global__ __launch_bounds__(RS_THREADS) void k_onesweep(Src src, OutT* __restri
     for (int k = 0; k < ITEMS; k++) {
         uint64_t idx = wbase + k * 64 + lane;
         uint32_t dk = rk[k] & 0xFFu;
-        if (idx < N) {
+        if (idx < tile_end) {
             uint32_t lp = s_start[dk] + s_wcnt[wave][dk] + (rk[k] >> 8);
             s_keys[lp] = (KeyT)key[k];
         }
     }
     if (dthr) {
-        // decoupled lookback: exclusive count of this digit over all preceding tiles.  While tiles stream
-        // through the machine a walk passes some 30 predecessors that have only published their
-        // aggregates, and every read is a round trip to memory, so RS_LOOKBACK predecessors are read per
-        // round trip.
+        // decoupled lookback: exclusive count of this digit over the preceding tiles of the segment.
+        // Every read is a round trip to memory, so RS_LOOKBACK predecessors are read per round trip.
         uint64_t excl = 0;
         if (tile > 0) {
-            int64_t j = (int64_t)tile - 1;  // next predecessor to add; tile 0 is always inclusive
+            int64_t j = (int64_t)tile - 1;  // next predecessor to add; the first tile is always inclusive
             while (true) {
                 const int cnt = (int)(j + 1 < RS_LOOKBACK ? j + 1 : RS_LOOKBACK);
                 uint64_t wb[RS_LOOKBACK];
 #pragma unroll
                 for (int b = 0; b < RS_LOOKBACK; b++)
-                    wb[b] = b < cnt ? ld_agent(status + (uint64_t)(j - b) * RS_RADIX + d) : 0ull;
+                    wb[b] = b < cnt ? ld_agent(status + (uint64_t)(slot0 + j - b) * RS_RADIX + d) : 0ull;
                 bool stop = false, inc = false;
                 int used = 0;
 #pragma unroll
@@ -287,12 +452,11 @@ __global__ __launch_bounds__(RS_THREADS) void k_onesweep(Src src, OutT* __restri
             }
             st_agent(st, lb_pack(tag, LB_INC, excl + total));
         }
-        s_gofs[d] = dbase[d] + (uint32_t)excl - start;
+        s_gofs[d] = segoff[sg * RS_RADIX + d] + (uint32_t)excl - start;
     }
     __syncthreads();
     // write out: position i of the reordered tile goes to gofs[digit] + i; the digits of a thread's
     // positions stay in registers for the values
-    const uint32_t tile_n = (uint32_t)((N - tile_base) < (uint64_t)TILE ? (N - tile_base) : TILE);
     uint32_t dg[ITEMS];
 #pragma unroll
     for (int r = 0; r < ITEMS; r++) {
@@ -311,7 +475,7 @@ __global__ __launch_bounds__(RS_THREADS) void k_onesweep(Src src, OutT* __restri
         for (int k = 0; k < ITEMS; k++) {
             uint64_t idx = wbase + k * 64 + lane;
             uint32_t dk = rk[k] & 0xFFu;
-            if (idx < N) s_vals[s_start[dk] + s_wcnt[wave][dk] + (rk[k] >> 8)] = val[k];
+            if (idx < tile_end) s_vals[s_start[dk] + s_wcnt[wave][dk] + (rk[k] >> 8)] = val[k];
         }
         __syncthreads();
 #pragma unroll
@@ -636,22 +800,32 @@ struct Primitives {
     uint64_t* scan_hi = nullptr;
     size_t scan_tiles_cap = 0;
     uint32_t scan_tag = 0;
-    unsigned long long* ticket = nullptr;  // device counter of tile tickets
+    unsigned long long* ticket = nullptr;  // device counter of scan tile tickets
     unsigned long long ticket_host = 0;
     uint64_t* scalars = nullptr;           // device scalars (scan totals)
     uint64_t* scan_part = nullptr;         // two-pass scan: block sums (2 per block) and block prefixes
     uint64_t* scan_prefix = nullptr;
-    uint32_t* hist = nullptr;              // 8 x 256
+    uint32_t* hist = nullptr;              // 8 passes x 256 global digit offsets of the last sort
+    // segmented radix passes (up to 8 passes): per-histogram-workgroup joint counts, their sums, the
+    // (segment, digit) offsets, segment starts / tile prefixes and the per-segment tile counters
+    uint32_t* seg_part = nullptr;
+    uint32_t* seg_h2 = nullptr;
+    uint32_t* seg_h11 = nullptr;  // window histograms per pass-0 segment
+    uint32_t* seg_off = nullptr;
+    uint32_t* seg_info = nullptr;
+    uint32_t* seg_tick = nullptr;
     uint64_t* host_scalars = nullptr;      // pinned mirror
     // optional per-pass timing of the radix passes
     bool profile = false;
     std::vector<std::pair<hipEvent_t, hipEvent_t>> pass_events;
     std::vector<uint32_t> pass_items;
+    std::vector<uint64_t> pass_bytes;  // algorithmic bytes of each recorded pass
 
     // sums the recorded pass times (synchronizes) and releases the events
-    void collect_profile(uint64_t* passes, uint64_t* items, double* ms) {
+    void collect_profile(uint64_t* passes, uint64_t* items, uint64_t* bytes, double* ms) {
         *passes = pass_events.size();
         *items = 0;
+        *bytes = 0;
         *ms = 0;
         for (size_t i = 0; i < pass_events.size(); i++) {
             float a = 0;
@@ -659,6 +833,7 @@ struct Primitives {
             DCX_HIP(hipEventElapsedTime(&a, pass_events[i].first, pass_events[i].second));
             *ms += a;
             *items += pass_items[i];
+            *bytes += pass_bytes[i];
         }
         drop_profile();
     }
@@ -669,6 +844,7 @@ struct Primitives {
         }
         pass_events.clear();
         pass_items.clear();
+        pass_bytes.clear();
     }
 
     void init(hipStream_t s, DeviceCache* c) {
@@ -678,7 +854,13 @@ struct Primitives {
         DCX_HIP(hipMemsetAsync(ticket, 0, sizeof(unsigned long long), stream));
         DCX_HIP(hipMalloc(&scalars, 64 * sizeof(uint64_t)));
         DCX_HIP(hipMemsetAsync(scalars, 0, 64 * sizeof(uint64_t), stream));
-        DCX_HIP(hipMalloc(&hist, 9 * RS_RADIX * sizeof(uint32_t)));  // 8 passes + a window histogram
+        DCX_HIP(hipMalloc(&hist, 8 * RS_RADIX * sizeof(uint32_t)));
+        DCX_HIP(hipMalloc(&seg_part, (size_t)RS_SEGS * RS_HIST_WPS * 8 * RS_SEG_BINS * sizeof(uint32_t)));
+        DCX_HIP(hipMalloc(&seg_h2, 8 * RS_SEG_BINS * sizeof(uint32_t)));
+        DCX_HIP(hipMalloc(&seg_h11, RS_SEGS * RS_SEG_BINS * sizeof(uint32_t)));
+        DCX_HIP(hipMalloc(&seg_off, 8 * RS_SEG_BINS * sizeof(uint32_t)));
+        DCX_HIP(hipMalloc(&seg_info, 8 * RS_SEG_INFO * sizeof(uint32_t)));
+        DCX_HIP(hipMalloc(&seg_tick, 8 * RS_SEGS * sizeof(uint32_t)));
         DCX_HIP(hipMalloc(&scan_part, 2 * SC_BLOCKS * sizeof(uint64_t)));
         DCX_HIP(hipMalloc(&scan_prefix, SC_BLOCKS * sizeof(uint64_t)));
         DCX_HIP(hipHostMalloc(&host_scalars, 64 * sizeof(uint64_t), hipHostMallocDefault));
@@ -691,6 +873,9 @@ struct Primitives {
         if (ticket) (void)hipFree(ticket);
         if (scalars) (void)hipFree(scalars);
         if (hist) (void)hipFree(hist);
+        for (uint32_t* q : {seg_part, seg_h2, seg_h11, seg_off, seg_info, seg_tick})
+            if (q) (void)hipFree(q);
+        seg_part = seg_h2 = seg_h11 = seg_off = seg_info = seg_tick = nullptr;
         if (scan_part) (void)hipFree(scan_part);
         if (scan_prefix) (void)hipFree(scan_prefix);
         scan_part = scan_prefix = nullptr;
@@ -748,28 +933,50 @@ struct Primitives {
         scan_tag = 0;
     }
 
-    // digit histograms of all passes into `hist` (zeroed)
+    // Key sources whose digits are byte windows of a character stream (histograms from windows)
     template <class Src>
-    void radix_hist(const Src& src, uint32_t N, uint32_t lo_bit, uint32_t np) {
-        hipLaunchKernelGGL(k_radix_hist<Src>, dim3(grid_for(N, 256, 2048)), dim3(256), 0, stream, src, N, lo_bit, np,
-                           hist);
+    static bool window_aligned(const Src&, uint32_t, uint32_t, uint32_t) {
+        return false;
     }
-    // Windows of a character stream whose width divides 8, with every digit on a character boundary:
-    // the digit of pass p of key i is the byte-wide window at character i + s_p, so every pass's
-    // histogram is one histogram of byte windows over [0, N) corrected at both ends.
-    void radix_hist(const StreamKeys<false>& src, uint32_t N, uint32_t lo_bit, uint32_t np) {
-        const uint32_t bits = src.bits;
-        const bool aligned = (8 % bits == 0) && lo_bit % 8 == 0 && lo_bit + 8 * np <= 64 &&
-                             (src.mask & (~0ull << lo_bit)) == (~0ull << lo_bit) && N > 64;
-        if (!aligned) {
-            hipLaunchKernelGGL(k_radix_hist<StreamKeys<false>>, dim3(grid_for(N, 256, 2048)), dim3(256), 0, stream, src,
-                               N, lo_bit, np, hist);
-            return;
+    static bool window_aligned(const StreamKeys<false>& src, uint32_t N, uint32_t lo_bit, uint32_t np) {
+        return (8 % src.bits == 0) && lo_bit % 8 == 0 && lo_bit + 8 * np <= 64 &&
+               (src.mask & (~0ull << lo_bit)) == (~0ull << lo_bit) && N > 64;
+    }
+    template <class Src>
+    void hist_windows(const Src&, uint32_t, uint32_t, uint32_t, uint32_t, uint32_t) {}
+    void hist_windows(const StreamKeys<false>& src, uint32_t N, uint32_t lo_bit, uint32_t np, uint32_t seg0_keys,
+                      uint32_t wps) {
+        hipLaunchKernelGGL(k_window_hist_seg, dim3(RS_SEGS * wps), dim3(RS_HIST_THREADS), 0, stream, src.s, src.bits, N, seg0_keys,
+                           wps, seg_part);
+        hipLaunchKernelGGL(k_window_seg_reduce, dim3(RS_SEG_BINS / 256, RS_SEGS), dim3(256), 0, stream,
+                           (const uint32_t*)seg_part, wps, seg_h11);
+        hipLaunchKernelGGL(k_window_seg_joint, dim3(np), dim3(256), 0, stream, src.s, src.bits, N, lo_bit, seg0_keys,
+                           (const uint32_t*)seg_h11, seg_h2);
+    }
+
+    // Plans the segmented passes of a sort of N keys over bits [lo_bit, lo_bit + 8 np) in tiles of
+    // `tile` keys from one read of the first-pass keys: hist, seg_off, seg_info, cleared seg_tick.
+    // Returns the grid of every pass: one workgroup per tile of all segments (at most N / tile + 8).
+    template <class Src>
+    uint64_t seg_plan(const Src& first, uint32_t N, uint32_t lo_bit, uint32_t np, uint32_t tile) {
+        const uint64_t tiles = ceil_div(N, tile);
+        const uint64_t seg0_keys = ceil_div(tiles, (uint64_t)RS_SEGS) * tile;
+        const uint64_t grid = tiles + RS_SEGS;
+        ensure_sort_status(grid);
+        uint64_t wps = ceil_div(seg0_keys, (uint64_t)RS_HIST_KEYS);
+        if (wps > RS_HIST_WPS) wps = RS_HIST_WPS;
+        const uint32_t bins = np * RS_SEG_BINS;
+        if (window_aligned(first, N, lo_bit, np)) {
+            hist_windows(first, N, lo_bit, np, (uint32_t)seg0_keys, (uint32_t)wps);
+        } else {
+            hipLaunchKernelGGL(k_radix_hist_seg<Src>, dim3((unsigned)(RS_SEGS * wps)), dim3(RS_HIST_THREADS), 0, stream, first, N,
+                               lo_bit, np, (uint32_t)seg0_keys, (uint32_t)wps, seg_part, seg_h2);
+            hipLaunchKernelGGL(k_hist_seg_reduce, dim3(bins / 256, 16), dim3(256), 0, stream,
+                               (const uint32_t*)seg_part, (uint32_t)(RS_SEGS * wps), bins, seg_h2);
         }
-        hipLaunchKernelGGL(k_window_hist, dim3(grid_for(N, 256, 2048)), dim3(256), 0, stream, src.s, bits, N,
-                           hist + 8 * RS_RADIX);
-        hipLaunchKernelGGL(k_window_hist_passes, dim3(np), dim3(RS_RADIX), 0, stream, src.s, bits, N, lo_bit,
-                           (const uint32_t*)(hist + 8 * RS_RADIX), hist);
+        hipLaunchKernelGGL(k_seg_setup, dim3(np), dim3(256), 0, stream, (const uint32_t*)seg_h2, N, tile,
+                           (uint32_t)seg0_keys, hist, seg_off, seg_info, seg_tick);
+        return grid;
     }
 
     // LSD radix sort of (u64 key, u32 value) pairs by key bits [lo_bit, hi_bit) (rounded up to whole
@@ -798,11 +1005,7 @@ struct Primitives {
         if (np > 8) np = 8;
         if (lo_bit + 8 * np > 64) lo_bit = 64 - 8 * np;
         if (np > 5 || np < 2 || !vals) low32 = nullptr;  // the last pass must not be the first
-        uint64_t tiles = ceil_div(N, RS_TILE);
-        ensure_sort_status(tiles);
-        fill_u32(hist, 0, 9 * RS_RADIX);
-        radix_hist(first, N, lo_bit, np);
-        hipLaunchKernelGGL(k_radix_hist_scan, dim3(np), dim3(256), 0, stream, hist);
+        const uint64_t tiles = seg_plan(first, N, lo_bit, np, RS_TILE);
         for (uint32_t p = 0; p < np; p++) {
             if (sort_tag >= LB_TAG_LIMIT) {
                 fill_u64(sort_status, 0, sort_status_words);
@@ -816,43 +1019,44 @@ struct Primitives {
                 DCX_HIP(hipEventRecord(e0, stream));
             }
             const uint32_t shift = lo_bit + 8 * p;
-            const uint32_t* dbase = hist + p * RS_RADIX;
+            const uint32_t* so = seg_off + p * RS_SEG_BINS;
+            const uint32_t* si = seg_info + p * RS_SEG_INFO;
+            uint32_t* tk = seg_tick + p * RS_SEGS;
             const dim3 g((unsigned)tiles), b(RS_THREADS);
             if (p == 0) {
                 if (!vals)
                     hipLaunchKernelGGL((k_onesweep<false, false, Src>), g, b, 0, stream, first, keys_alt,
-                                       (const uint32_t*)nullptr, (uint32_t*)nullptr, N, shift, dbase, sort_status, tag,
-                                       ticket, ticket_host);
+                                       (const uint32_t*)nullptr, (uint32_t*)nullptr, shift, so, si, tk, sort_status, tag);
                 else if (ident_vals)
                     hipLaunchKernelGGL((k_onesweep<true, true, Src>), g, b, 0, stream, first, keys_alt,
-                                       (const uint32_t*)nullptr, vals_alt, N, shift, dbase, sort_status, tag, ticket,
-                                       ticket_host);
+                                       (const uint32_t*)nullptr, vals_alt, shift, so, si, tk, sort_status, tag);
                 else
                     hipLaunchKernelGGL((k_onesweep<true, false, Src>), g, b, 0, stream, first, keys_alt,
-                                       (const uint32_t*)vals, vals_alt, N, shift, dbase, sort_status, tag, ticket,
-                                       ticket_host);
+                                       (const uint32_t*)vals, vals_alt, shift, so, si, tk, sort_status, tag);
             } else if (low32 && p == np - 1 && vals) {
                 const ArrayKeys<uint64_t> in{keys};
                 hipLaunchKernelGGL((k_onesweep<true, false, ArrayKeys<uint64_t>, uint64_t, RS_ITEMS, uint32_t>), g, b, 0,
-                                   stream, in, low32, (const uint32_t*)vals, vals_alt, N, shift, dbase, sort_status, tag,
-                                   ticket, ticket_host, lo_bit);
+                                   stream, in, low32, (const uint32_t*)vals, vals_alt, shift, so, si, tk, sort_status, tag, lo_bit);
             } else {
                 const ArrayKeys<uint64_t> in{keys};
                 if (!vals)
                     hipLaunchKernelGGL((k_onesweep<false, false, ArrayKeys<uint64_t>>), g, b, 0, stream, in, keys_alt,
-                                       (const uint32_t*)nullptr, (uint32_t*)nullptr, N, shift, dbase, sort_status, tag,
-                                       ticket, ticket_host, 0u);
+                                       (const uint32_t*)nullptr, (uint32_t*)nullptr, shift, so, si, tk, sort_status, tag, 0u);
                 else
                     hipLaunchKernelGGL((k_onesweep<true, false, ArrayKeys<uint64_t>>), g, b, 0, stream, in, keys_alt,
-                                       (const uint32_t*)vals, vals_alt, N, shift, dbase, sort_status, tag, ticket,
-                                       ticket_host, 0u);
+                                       (const uint32_t*)vals, vals_alt, shift, so, si, tk, sort_status, tag, 0u);
             }
             if (profile) {
                 DCX_HIP(hipEventRecord(e1, stream));
                 pass_events.emplace_back(e0, e1);
                 pass_items.push_back(N);
+                // keys in (the first pass: its key source) and out (a low32 last pass: 4 B), values in
+                // (not for identity values) and out
+                const bool out32 = low32 && p == np - 1;
+                uint64_t b = (p == 0 ? first.read_bytes(N) : 8ull * N) + (out32 ? 4ull : 8ull) * N;
+                if (vals) b += (p == 0 && ident_vals ? 4ull : 8ull) * N;
+                pass_bytes.push_back(b);
             }
-            ticket_host += tiles;
             if (!(low32 && p == np - 1 && vals)) std::swap(keys, keys_alt);
             if (vals) std::swap(vals, vals_alt);
         }
@@ -877,12 +1081,7 @@ struct Primitives {
         *keys_out = k1;
         *vals_out = v1;
         if (N == 0 || np == 0) return;
-        const uint64_t tiles = ceil_div(N, (uint64_t)RS_THREADS * RS_ITEMS_U32);
-        ensure_sort_status(tiles);
-        fill_u32(hist, 0, np * RS_RADIX);
-        hipLaunchKernelGGL(k_radix_hist<ArrayKeys<uint32_t>>, dim3(grid_for(N, 256, 2048)), dim3(256), 0, stream,
-                           ArrayKeys<uint32_t>{keys}, N, lo_bit, np, hist);
-        hipLaunchKernelGGL(k_radix_hist_scan, dim3(np), dim3(256), 0, stream, hist);
+        const uint64_t tiles = seg_plan(ArrayKeys<uint32_t>{keys}, N, lo_bit, np, RS_THREADS * RS_ITEMS_U32);
         for (uint32_t p = 0; p < np; p++) {
             if (sort_tag >= LB_TAG_LIMIT) {
                 fill_u64(sort_status, 0, sort_status_words);
@@ -892,13 +1091,14 @@ struct Primitives {
             const ArrayKeys<uint32_t> in{kin};
             if (vin)
                 hipLaunchKernelGGL((k_onesweep<true, false, ArrayKeys<uint32_t>, uint32_t, RS_ITEMS_U32>), dim3((unsigned)tiles),
-                                   dim3(RS_THREADS), 0, stream, in, ko, vin, vo, N, lo_bit + 8 * p,
-                                   (const uint32_t*)(hist + p * RS_RADIX), sort_status, tag, ticket, ticket_host);
+                                   dim3(RS_THREADS), 0, stream, in, ko, vin, vo, lo_bit + 8 * p,
+                                   seg_off + p * RS_SEG_BINS, seg_info + p * RS_SEG_INFO, seg_tick + p * RS_SEGS,
+                                   sort_status, tag);
             else
                 hipLaunchKernelGGL((k_onesweep<true, true, ArrayKeys<uint32_t>, uint32_t, RS_ITEMS_U32>), dim3((unsigned)tiles),
-                                   dim3(RS_THREADS), 0, stream, in, ko, (const uint32_t*)nullptr, vo, N, lo_bit + 8 * p,
-                                   (const uint32_t*)(hist + p * RS_RADIX), sort_status, tag, ticket, ticket_host);
-            ticket_host += tiles;
+                                   dim3(RS_THREADS), 0, stream, in, ko, (const uint32_t*)nullptr, vo, lo_bit + 8 * p,
+                                   seg_off + p * RS_SEG_BINS, seg_info + p * RS_SEG_INFO, seg_tick + p * RS_SEGS,
+                                   sort_status, tag);
             *keys_out = ko;
             *vals_out = vo;
             kin = ko;

@@ -83,6 +83,7 @@ class Stats(ctypes.Structure):
         ("sort_passes", ctypes.c_uint64),
         ("sort_pass_items", ctypes.c_uint64),
         ("sort_pass_ms", ctypes.c_double),
+        ("sort_pass_bytes", ctypes.c_uint64),
     ]
 
     def to_dict(self):
@@ -101,6 +102,7 @@ class Stats(ctypes.Structure):
             "sort_passes": int(self.sort_passes),
             "sort_pass_items": int(self.sort_pass_items),
             "sort_pass_ms": float(self.sort_pass_ms),
+            "sort_pass_bytes": int(self.sort_pass_bytes),
         }
 
 

@@ -78,6 +78,9 @@ typedef struct dcx_stats {
     uint64_t sort_passes;         /* number of pass launches                                  */
     uint64_t sort_pass_items;     /* sum over launches of the (key, value) pairs moved        */
     double sort_pass_ms;          /* sum over launches of the kernel time                     */
+    uint64_t sort_pass_bytes;     /* sum over launches of the algorithmic bytes: keys and values
+                                     read and written (a first pass keyed by the character stream
+                                     reads the stream; identity values are not read)          */
 } dcx_stats;
 
 /* ms_phase indices; the names follow the reference's timer keys. */
