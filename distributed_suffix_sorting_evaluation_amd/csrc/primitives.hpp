@@ -632,6 +632,18 @@ struct ScanPayload<Op, decltype((void)&Op::payload)> {
     static constexpr bool value = true;
 };
 
+// Ops with kDirect write in the blocked order they load in: each thread writes its own elements from
+// its running prefix, with no LDS transpose to striped order.  Pays for sparse writes to one array
+// (sample compaction); writes to several arrays, or to group-indexed ones, want the striped order.
+template <class Op, class = void>
+struct ScanDirect {
+    static constexpr bool value = false;
+};
+template <class Op>
+struct ScanDirect<Op, decltype((void)Op::kDirect)> {
+    static constexpr bool value = Op::kDirect;
+};
+
 template <bool PL = false, class Op>
 __device__ __forceinline__ void scan_load(const Op& op, uint64_t N, uint64_t tbase, uint32_t tid,
                                           uint64_t (&v)[SC_ITEMS], uint32_t* pl = nullptr) {
@@ -728,10 +740,11 @@ template <class Op>
 __global__ __launch_bounds__(SC_THREADS) void k_scan_down(Op op, uint64_t N, uint32_t tpb,
                                                           const uint64_t* __restrict__ prefix) {
     constexpr bool PL = ScanPayload<Op>::value && Op::kBlocked;
+    constexpr bool DIRECT = ScanDirect<Op>::value && Op::kBlocked;
     constexpr uint32_t PAD = SC_TILE + SC_TILE / 16;  // slot of the tile's end prefix
     __shared__ uint64_t s_tmp[SC_THREADS / 64];
-    __shared__ uint64_t s_e[PAD + 1];  // padded to spread blocked accesses over banks
-    __shared__ uint32_t s_p[PL ? PAD : 1];
+    __shared__ uint64_t s_e[DIRECT ? 1 : PAD + 1];  // padded to spread blocked accesses over banks
+    __shared__ uint32_t s_p[PL && !DIRECT ? PAD : 1];
     const uint32_t tid = threadIdx.x;
     const uint64_t t0 = (uint64_t)blockIdx.x * tpb;
     uint64_t carry = prefix[blockIdx.x];
@@ -741,7 +754,7 @@ __global__ __launch_bounds__(SC_THREADS) void k_scan_down(Op op, uint64_t N, uin
         uint64_t v[SC_ITEMS];
         uint32_t pl[PL ? SC_ITEMS : 1];
         scan_load<PL>(op, N, tbase, tid, v, pl);
-        if (!Op::kBlocked) {
+        if constexpr (!Op::kBlocked) {
             // striped values -> blocked through LDS
 #pragma unroll
             for (int k = 0; k < SC_ITEMS; k++) {
@@ -760,25 +773,38 @@ __global__ __launch_bounds__(SC_THREADS) void k_scan_down(Op op, uint64_t N, uin
         for (int k = 0; k < SC_ITEMS; k++) sum += v[k];
         uint64_t tile_total = 0;
         uint64_t run = carry + block_excl_scan64<SC_THREADS>(sum, s_tmp, &tile_total);  // synchronizes
+        if constexpr (DIRECT) {
+            const uint64_t base = tbase + (uint64_t)tid * SC_ITEMS;
 #pragma unroll
-        for (int k = 0; k < SC_ITEMS; k++) {
-            const uint32_t idx = tid * SC_ITEMS + k;
-            s_e[idx + (idx >> 4)] = run;
-            run += v[k];
-            if constexpr (PL) s_p[idx + (idx >> 4)] = pl[k];
-        }
-        if (tid == SC_THREADS - 1) s_e[PAD] = run;
-        __syncthreads();
+            for (int k = 0; k < SC_ITEMS; k++) {
+                if constexpr (PL) {
+                    if (base + k < N) op.write(base + k, run, v[k], pl[k]);
+                } else {
+                    if (base + k < N) op.write(base + k, run, v[k]);
+                }
+                run += v[k];
+            }
+        } else {
 #pragma unroll
-        for (int k = 0; k < SC_ITEMS; k++) {
-            const uint32_t idx = k * SC_THREADS + tid;
-            const uint32_t nx = idx + 1;
-            const uint64_t i = tbase + idx;
-            const uint64_t e = s_e[idx + (idx >> 4)];
-            if constexpr (PL) {
-                if (i < N) op.write(i, e, s_e[nx + (nx >> 4)] - e, s_p[idx + (idx >> 4)]);
-            } else {
-                if (i < N) op.write(i, e, s_e[nx + (nx >> 4)] - e);
+            for (int k = 0; k < SC_ITEMS; k++) {
+                const uint32_t idx = tid * SC_ITEMS + k;
+                s_e[idx + (idx >> 4)] = run;
+                run += v[k];
+                if constexpr (PL) s_p[idx + (idx >> 4)] = pl[k];
+            }
+            if (tid == SC_THREADS - 1) s_e[PAD] = run;
+            __syncthreads();
+#pragma unroll
+            for (int k = 0; k < SC_ITEMS; k++) {
+                const uint32_t idx = k * SC_THREADS + tid;
+                const uint32_t nx = idx + 1;
+                const uint64_t i = tbase + idx;
+                const uint64_t e = s_e[idx + (idx >> 4)];
+                if constexpr (PL) {
+                    if (i < N) op.write(i, e, s_e[nx + (nx >> 4)] - e, s_p[idx + (idx >> 4)]);
+                } else {
+                    if (i < N) op.write(i, e, s_e[nx + (nx >> 4)] - e);
+                }
             }
         }
         carry += tile_total;
