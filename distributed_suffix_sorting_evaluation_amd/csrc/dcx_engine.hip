@@ -39,6 +39,11 @@ constexpr uint32_t ROUND0_SLACK = DCX_ROUND0_SLACK;  // key bits beyond log2(N) 
 // information beyond log2(N)
 constexpr uint32_t PACKED_SLACK = DCX_PACKED_SLACK;
 
+// Longest text (and longest joined batch, plus one separator) of this build: positions, counts and the
+// grid-stride loops of the kernels are 32-bit, and a loop of at most 8192 x 256 threads stays clear of
+// wrap-around while the element count plus 2^21 is below 2^32.
+constexpr uint64_t MAX_TEXT = (1ull << 32) - (1ull << 22);
+
 class Engine {
   public:
     int device = 0;
@@ -681,12 +686,12 @@ class Engine {
         const uint64_t base = offsets[0], total = offsets[count] - offsets[0];
         for (uint32_t k = 0; k < count; k++) {
             if (offsets[k + 1] < offsets[k]) throw Error(DCX_E_ARG, "offsets must be non-decreasing");
-            if (offsets[k + 1] - offsets[k] >= (1ull << 32) - (1ull << 16))
+            if (offsets[k + 1] - offsets[k] > MAX_TEXT)
                 throw Error(DCX_E_UNSUPPORTED, "a text of the batch is too long for 32-bit positions");
         }
         if (total && (!texts || !sa_out)) throw Error(DCX_E_ARG, "null argument");
         uint64_t limit = o->batch_max_chars ? o->batch_max_chars : (1ull << 31);
-        if (limit > (1ull << 32) - (1ull << 17)) limit = (1ull << 32) - (1ull << 17);
+        if (limit > MAX_TEXT) limit = MAX_TEXT;  // the joined string, its separators included
         if (budget) {
             const uint64_t by_budget = budget > (64ull << 20) ? (budget - (64ull << 20)) / 64 : 0;
             if (by_budget < limit) limit = by_budget;
@@ -809,8 +814,8 @@ class Engine {
         if (!o || (!text && n) || (!sa_out && n)) throw Error(DCX_E_ARG, "null argument");
         if (width != 4 && width != 5 && width != 8) throw Error(DCX_E_ARG, "sa_width_bytes must be 4, 5 or 8");
         set_options(o);
-        if (n >= (1ull << 32) - (1ull << 16))
-            throw Error(DCX_E_UNSUPPORTED, "texts of 2^32 - 2^16 characters or more are not supported by this build");
+        if (n > MAX_TEXT)
+            throw Error(DCX_E_UNSUPPORTED, "texts longer than 2^32 - 2^22 characters are not supported by this build");
         if (width == 4 && n > (1ull << 32)) throw Error(DCX_E_WIDTH, "32-bit SA cannot index this text");
         if (budget) {
             uint64_t need = estimate_bytes(n);

@@ -15,11 +15,18 @@ _LIB = None
 
 KINDS = {"uniform_acgt": 0, "genome": 1, "zipf256": 2, "acgtn": 3, "uniform_sigma": 4}
 
+# Longest text the engine takes (32-bit positions): 4 GiB - 4 MiB
+MAX_TEXT = (1 << 32) - (1 << 22)
+
 # The benchmark workloads (sizes and seeds) -- plain DCX on i.i.d. DNA, the genome-like text and the
-# byte-alphabet text.
+# byte-alphabet text (Zipf(1.1) over 256 values plus a 64 MiB period-17 block) at the longest supported
+# length, for three difference covers.
 WORKLOADS = {
     "dna_8mib_dc39": {"kind": "uniform_acgt", "n": 8 << 20, "seed": 1, "X": 39, "sa_width": 8},
     "genome_1gib_dc39": {"kind": "genome", "n": 1 << 30, "seed": 2, "X": 39, "sa_width": 5},
+    "zipf_4gib_dc21": {"kind": "zipf256", "n": MAX_TEXT, "seed": 3, "X": 21, "sa_width": 8},
+    "zipf_4gib_dc39": {"kind": "zipf256", "n": MAX_TEXT, "seed": 3, "X": 39, "sa_width": 8},
+    "zipf_4gib_dc133": {"kind": "zipf256", "n": MAX_TEXT, "seed": 3, "X": 133, "sa_width": 8},
 }
 
 # workloads whose libsais digests are committed in tests/golden/sa_digests.json
