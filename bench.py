@@ -134,7 +134,9 @@ def bench_serve(args):
             ("continuous_batching_burst", dict(workers=2, batching=True), np.zeros(count)),
             ("plain_burst", dict(workers=1, batching=False), np.zeros(count)))
     for name, kwargs, arr in runs:
-        srv = SAServer(hbm_budget=80 * 10 ** 9, X=39, **kwargs)
+        # every server first builds throw-away SAs of every size class (outside the timed run), so its
+        # engines' device caches are sized before the first request
+        srv = SAServer(hbm_budget=80 * 10 ** 9, X=39, warm_chars=256 << 20, **kwargs)
         try:
             res = run_open_loop(srv, texts, arr, keep_results=True)
             reqs = res.pop("requests_list")

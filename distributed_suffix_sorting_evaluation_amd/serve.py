@@ -58,7 +58,7 @@ class SAServer:
 
     def __init__(self, workers: int = 2, hbm_budget: int = 80 * 10 ** 9, max_batch_texts: int = 1024,
                  max_batch_chars: int = 64 << 20, solo_chars: int = 4 << 20, device: int = 0,
-                 batching: bool = True, **build_options):
+                 batching: bool = True, warm_chars: int = 0, **build_options):
         import torch
         self.torch = torch
         self.budget = int(hbm_budget)
@@ -74,9 +74,37 @@ class SAServer:
         self.next_id = 0
         self.batches = 0
         self.engines = [Engine(device) for _ in range(workers)]
+        if warm_chars:
+            self._warm_up(int(warm_chars))
         self.threads = [threading.Thread(target=self._worker, args=(i,), daemon=True) for i in range(workers)]
         for t in self.threads:
             t.start()
+
+    def _warm_up(self, max_chars: int):
+        """Builds throw-away suffix arrays of every power-of-two size up to max_chars (4- and 256-letter
+        random texts, alone and batched) on every engine, so that the engines' device caches already hold
+        blocks of every size class before the first request: a build that has to grow its cache calls
+        hipMalloc and hipFree, which stall the other worker's work too."""
+        torch = self.torch
+        g = torch.Generator(device="cpu").manual_seed(12345)
+        sizes = []
+        n = 1 << 16
+        while n <= max_chars:
+            sizes.append(n)
+            n <<= 1
+        for sigma in (4, 256):
+            big = torch.randint(0, sigma, (max(sizes),), generator=g, dtype=torch.uint8).cuda()
+            out = torch.empty(max(sizes), dtype=torch.int32, device="cuda")
+            for eng in self.engines:
+                for n in reversed(sizes):
+                    eng.build_sa_device(big.data_ptr(), n, out.data_ptr(), 4, **self.opts)
+                if self.batching:
+                    for n in reversed([x for x in sizes if x <= self.max_chars]):
+                        k = max(1, min(self.max_texts, n // (1 << 16)))
+                        off = np.linspace(0, n, k + 1).astype(np.uint64)
+                        eng.build_sa_batch_device(big.data_ptr(), off, out.data_ptr(), **self.opts)
+            del big, out
+        torch.cuda.synchronize()
 
     # -- client side ---------------------------------------------------------------------------------
     def submit(self, text) -> Request:

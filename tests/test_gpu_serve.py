@@ -22,7 +22,8 @@ def test_results_equal_alone_and_in_the_mix(engine):
     from distributed_suffix_sorting_evaluation_amd.serve import SAServer, poisson_arrivals, run_open_loop
     texts = _texts(40, 3)
     dev = [torch.from_numpy(t).cuda() for t in texts]
-    srv = SAServer(workers=2, hbm_budget=8 * 10 ** 9, X=39)
+    # warmed engines (throw-away builds of every size class up to 256 KiB first) serve the same results
+    srv = SAServer(workers=2, hbm_budget=8 * 10 ** 9, X=39, warm_chars=1 << 18)
     try:
         res = run_open_loop(srv, dev, poisson_arrivals(len(dev), 2000.0, 5))
     finally:
