@@ -127,18 +127,26 @@ def bench_serve(args):
     starts = [int(rng.integers(0, pool_n - int(s) + 1)) for s in sizes]
     texts = [pools[bool(b)][o:o + int(s)] for b, o, s in zip(sig256, starts, sizes)]
     arrivals = poisson_arrivals(count, rate, 5)
+    classes = np.where(sig256, 3, 0)  # the client knows its alphabet: 256 letters or ACGT
     log(f"[bench] serve: {count} requests at {rate}/s, {int(sizes.sum()) / 2 ** 30:.2f} GiB offered")
+    # an untimed pass first: process-wide first-use costs (allocator growth, code loading) are paid here
+    # and not by whichever timed run comes first
+    srv = SAServer(hbm_budget=80 * 10 ** 9, X=39, warm_chars=256 << 20, reserve_bytes=64 << 30)
+    try:
+        run_open_loop(srv, texts[:100], np.zeros(100), keep_results=False, sigma_classes=classes[:100])
+    finally:
+        srv.close()
     out = {}
     runs = (("continuous_batching", dict(workers=2, batching=True), arrivals),
             ("plain", dict(workers=1, batching=False), arrivals),
             ("continuous_batching_burst", dict(workers=2, batching=True), np.zeros(count)),
             ("plain_burst", dict(workers=1, batching=False), np.zeros(count)))
     for name, kwargs, arr in runs:
-        # every server first builds throw-away SAs of every size class (outside the timed run), so its
-        # engines' device caches are sized before the first request
-        srv = SAServer(hbm_budget=80 * 10 ** 9, X=39, warm_chars=256 << 20, **kwargs)
+        # every server first builds throw-away SAs of every size class and reserves one block in torch's
+        # allocator (outside the timed run), so no request waits for hipMalloc / hipFree
+        srv = SAServer(hbm_budget=80 * 10 ** 9, X=39, warm_chars=256 << 20, reserve_bytes=64 << 30, **kwargs)
         try:
-            res = run_open_loop(srv, texts, arr, keep_results=True)
+            res = run_open_loop(srv, texts, arr, keep_results=True, sigma_classes=classes)
             reqs = res.pop("requests_list")
             from distributed_suffix_sorting_evaluation_amd import Engine
             with Engine(0) as chk:

@@ -4,8 +4,10 @@ Continuous batching: requests arrive at any time and wait in a FIFO queue.  Each
 (its own HIP stream and device-memory cache).  A free worker admits queued requests in arrival order for as long
 as their estimated HBM footprint fits the budget that is still free, joins their texts on the device and builds
 all their suffix arrays with one ``dcx_build_sa_batch`` call.  Requests that arrive while batches run are
-admitted by the next free worker; with two workers the batches of both overlap on the device.  Results are the
-per-text 32-bit suffix arrays, bit-identical to building each text alone (the batched call is exact).
+admitted by the next free worker; with two workers the batches of both overlap on the device.  A large request
+(it fills the GPU by itself) is built alone, and at most one large build runs at a time, so it is as fast as
+on an idle GPU while the other worker serves the small requests.  Results are the per-text 32-bit suffix
+arrays, bit-identical to building each text alone (the batched call is exact).
 
 The reference has no serving layer (it builds one SA per ``mpiexec`` run, lightweight_dcx/src/executables/cli.cpp);
 this module serves callers with a stream of requests.
@@ -20,6 +22,9 @@ from dataclasses import dataclass, field
 import numpy as np
 
 from .engine import DCX_E_ADMISSION, DCX_E_CANCELLED, DCXError, Engine
+
+# per device: [stream, bytes reserved in torch's allocator for it] of the worker streams, shared by servers
+_STREAMS: dict = {}
 
 # device bytes per text character while a request is in flight: engine working set (see the engine's
 # estimate) plus the 4-byte SA and the joined copy of the text
@@ -42,6 +47,7 @@ class Request:
     error: int = 0
     sigma_class: int = -1           # packing width class of the alphabet (3/5/8/9-bit characters)
     batch: int = -1
+    large: bool = False             # built alone as a large request
     event: threading.Event = field(default_factory=threading.Event)
 
     @property
@@ -58,7 +64,7 @@ class SAServer:
 
     def __init__(self, workers: int = 2, hbm_budget: int = 80 * 10 ** 9, max_batch_texts: int = 1024,
                  max_batch_chars: int = 64 << 20, solo_chars: int = 4 << 20, device: int = 0,
-                 batching: bool = True, warm_chars: int = 0, **build_options):
+                 batching: bool = True, warm_chars: int = 0, reserve_bytes: int = 0, **build_options):
         import torch
         self.torch = torch
         self.budget = int(hbm_budget)
@@ -73,12 +79,32 @@ class SAServer:
         self.stop = False
         self.next_id = 0
         self.batches = 0
+        self.large_running = 0
         self.engines = [Engine(device) for _ in range(workers)]
+        # worker streams are kept per process and reused by later servers: torch's caching allocator hands
+        # a freed block only to allocations on the stream that freed it, so the blocks cached for these
+        # streams (outputs, joined texts) serve every later server too
+        pool = _STREAMS.setdefault(device, [])
+        while len(pool) < workers:
+            pool.append([torch.cuda.Stream(device=device), 0])
+        self.streams = [pool[i][0] for i in range(workers)]
         if warm_chars:
             self._warm_up(int(warm_chars))
+        if reserve_bytes:
+            # one large block per worker stream (reserve_bytes each, once per process and stream): the
+            # per-request outputs and joined texts are carved from it instead of calling hipMalloc while
+            # requests are in flight
+            for i in range(workers):
+                if pool[i][1] == 0:
+                    with torch.cuda.stream(pool[i][0]):
+                        torch.empty(int(reserve_bytes), dtype=torch.uint8, device="cuda")
+                    pool[i][1] = int(reserve_bytes)
+            torch.cuda.synchronize()
         self.threads = [threading.Thread(target=self._worker, args=(i,), daemon=True) for i in range(workers)]
         for t in self.threads:
             t.start()
+        if warm_chars:
+            self._prime(int(warm_chars))
 
     def _warm_up(self, max_chars: int):
         """Builds throw-away suffix arrays of every power-of-two size up to max_chars (4- and 256-letter
@@ -106,6 +132,22 @@ class SAServer:
             del big, out
         torch.cuda.synchronize()
 
+    def _prime(self, max_chars: int):
+        """Runs a few throw-away requests through the worker threads (small ones of both alphabet classes,
+        batched, and large ones alone) so that their first-use costs are paid before real requests."""
+        torch = self.torch
+        g = torch.Generator(device="cpu").manual_seed(54321)
+        texts = []
+        for sigma in (4, 256):
+            texts += [torch.randint(0, sigma, (1 << 16,), generator=g, dtype=torch.uint8).cuda() for _ in range(4)]
+            big = min(max_chars, max(self.solo_chars, 1 << 16))
+            texts += [torch.randint(0, sigma, (big,), generator=g, dtype=torch.uint8).cuda() for _ in range(2)]
+        reqs = [self.submit(t) for t in texts]
+        for r in reqs:
+            r.event.wait()
+        with self.cv:
+            self.batches = 0
+
     # -- client side ---------------------------------------------------------------------------------
     def submit(self, text) -> Request:
         r = text if isinstance(text, Request) else Request(text=text)
@@ -127,7 +169,8 @@ class SAServer:
 
     def _sigma_class(self, text) -> int:
         # texts are only batched with texts of the same packing width: the joined alphabet decides the
-        # width of every character of the batch
+        # width of every character of the batch.  Counting the alphabet is GPU work and a device-to-host
+        # read on the submitting thread; clients that know their alphabet pass the class instead.
         torch = self.torch
         if text.numel() == 0:
             return 0
@@ -156,15 +199,34 @@ class SAServer:
 
     # -- workers -------------------------------------------------------------------------------------
     def _take(self) -> list[Request]:
-        """Admits the queue head; a small head is joined by later small requests of the same alphabet class,
-        in arrival order, while their footprint fits the free budget and the batch limits."""
+        """Admits work in arrival order.  A large request (solo_chars or more) is built alone, and with
+        batching at most one large build runs at a time: it gets the whole GPU, as in the plain path, while
+        the other workers take small requests.  A small request is joined by later small requests of the
+        same alphabet class while their footprint fits the free budget and the batch limits; small requests
+        may pass a large queue head that waits for the running large build."""
+        if not self.queue:
+            return []
         head = self.queue[0]
+        if head.n >= self.solo_chars:
+            if not (self.batching and self.large_running):
+                fp = footprint(head.n)
+                if fp > self.free:
+                    return []
+                self.queue.remove(head)
+                self.free -= fp
+                self.large_running += 1
+                head.large = True
+                return [head]
+            smalls = [r for r in self.queue if r.n < self.solo_chars]
+            if not smalls:
+                return []
+            head = smalls[0]
         fp = footprint(head.n)
         if fp > self.free:
             return []
         batch, chars, need = [head], head.n + 1, fp
-        if self.batching and head.n < self.solo_chars:
-            for r in self.queue[1:]:
+        if self.batching:
+            for r in self.queue[self.queue.index(head) + 1:]:
                 if len(batch) >= self.max_texts:
                     break
                 if r.n >= self.solo_chars or r.sigma_class != head.sigma_class:
@@ -183,18 +245,18 @@ class SAServer:
     def _worker(self, wid: int):
         torch = self.torch
         eng = self.engines[wid]
-        stream = torch.cuda.Stream()
+        stream = self.streams[wid]
         while True:
             with self.cv:
-                while not self.stop and not (self.queue and footprint(self.queue[0].n) <= self.free):
+                while True:
+                    if self.stop and not self.queue:
+                        return
+                    batch = self._take()
+                    if batch:
+                        break
                     self.cv.wait(0.05)
-                if self.stop and not self.queue:
-                    return
-                batch = self._take()
                 bid = self.batches
                 self.batches += 1
-            if not batch:
-                continue
             now = time.perf_counter()
             for r in batch:
                 r.admitted = now
@@ -223,6 +285,8 @@ class SAServer:
             done = time.perf_counter()
             with self.cv:
                 self.free += sum(footprint(r.n) for r in batch)
+                if batch[0].large:
+                    self.large_running -= 1
                 self.cv.notify_all()
             for r in batch:
                 r.error = err
@@ -240,15 +304,20 @@ def log_uniform_sizes(count: int, lo: int, hi: int, seed: int) -> np.ndarray:
     return np.exp(rng.uniform(math.log(lo), math.log(hi + 1), count)).astype(np.int64).clip(lo, hi)
 
 
-def run_open_loop(server: SAServer, texts: list, arrivals: np.ndarray, keep_results: bool = True) -> dict:
-    """Submits texts at the given arrival offsets (seconds) and waits for all; returns latency statistics."""
+def run_open_loop(server: SAServer, texts: list, arrivals: np.ndarray, keep_results: bool = True,
+                  sigma_classes=None) -> dict:
+    """Submits texts at the given arrival offsets (seconds) and waits for all; returns latency statistics.
+    sigma_classes (optional): the alphabet class of every text as the client knows it (see
+    SAServer._sigma_class), so that submitting does not have to count the alphabet on the GPU."""
     t0 = time.perf_counter()
     reqs = []
-    for text, at in zip(texts, arrivals):
+    for k, (text, at) in enumerate(zip(texts, arrivals)):
         delay = t0 + at - time.perf_counter()
         if delay > 0:
             time.sleep(delay)
         r = Request(text=text, arrival=t0 + at)
+        if sigma_classes is not None:
+            r.sigma_class = int(sigma_classes[k])
         reqs.append(server.submit(r))
     for r in reqs:
         r.event.wait()

@@ -105,3 +105,47 @@ def test_serving_workload_generators():
     # log-uniform: the median is near the geometric mean of the bounds
     assert abs(np.log(np.median(s)) - 0.5 * (np.log(64 << 10) + np.log(256 << 20))) < 0.1
     assert footprint(10) < footprint(11)
+
+
+def test_serving_admission_order():
+    """The admission policy of the serving loop, without a GPU: one large build at a time, small requests
+    batched by alphabet class and allowed past a waiting large head, the budget respected."""
+    import threading
+    from distributed_suffix_sorting_evaluation_amd.serve import Request, SAServer, footprint
+
+    class Text:
+        def __init__(self, n):
+            self.n = n
+
+        def numel(self):
+            return self.n
+
+    def server(batching=True, free=10 ** 12):
+        s = object.__new__(SAServer)
+        s.batching, s.solo_chars, s.max_texts, s.max_chars = batching, 4 << 20, 1024 if batching else 1, 64 << 20
+        s.free, s.budget, s.large_running, s.queue, s.cv = free, free, 0, [], threading.Condition()
+        return s
+
+    def req(n, cls=0):
+        return Request(text=Text(n), sigma_class=cls)
+
+    A, b, C, d, e = req(8 << 20), req(1 << 20), req(16 << 20), req(2 << 20), req(1 << 20, 1)
+    s = server()
+    s.queue = [A, b, C, d, e]
+    assert s._take() == [A] and A.large and s.large_running == 1
+    assert s._take() == [b, d]          # small requests pass the large head C while A runs
+    assert s.queue == [C, e]
+    assert s._take() == [e]
+    assert s._take() == [] and s.queue == [C]  # C waits for A
+    s.large_running = 0
+    assert s._take() == [C]
+    # plain mode: strict arrival order, one request per build
+    p = server(batching=False)
+    p.queue = [req(8 << 20), req(1 << 20), req(16 << 20)]
+    got = [p._take()[0].n for _ in range(3)]
+    assert got == [8 << 20, 1 << 20, 16 << 20]
+    # budget: a head that does not fit waits, nothing passes it
+    q = server(free=footprint(1 << 20))
+    big, small = req(2 << 20), req(1 << 10)
+    q.queue = [big, small]
+    assert q._take() == [] and q.queue == [big, small]
