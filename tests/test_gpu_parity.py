@@ -298,6 +298,29 @@ def test_batch_matches_per_text(engine):
                 assert sa.dtype == np.uint32 and np.array_equal(sa.astype(np.uint64), ref), (X, limit, k)
 
 
+def test_autotuned_x_batch(engine):
+    """The X autotuner times real batched builds and returns one of its candidates; the batch built with
+    the chosen X equals the oracle per text (every X gives the same SAs)."""
+    import torch
+    from distributed_suffix_sorting_evaluation_amd.autotune import XTuner, choose_x_batch
+    texts = [np.frombuffer(t, dtype=np.uint8) if isinstance(t, bytes) else np.asarray(t, dtype=np.uint8)
+             for t in _batch_cases()]
+    off = np.zeros(len(texts) + 1, dtype=np.uint64)
+    off[1:] = np.cumsum([t.size for t in texts])
+    dt = torch.from_numpy(np.concatenate(texts)).cuda()
+    ds = torch.empty(int(off[-1]), dtype=torch.int32, device="cuda")
+    say("autotune X")
+    x, ms = choose_x_batch(engine, dt.data_ptr(), off, ds.data_ptr(), candidates=(7, 21, 39), reps=1)
+    assert x in (7, 21, 39) and set(ms) == {7, 21, 39} and ms[x] == min(ms.values())
+    tuner = XTuner(engine, candidates=(7, 39), reps=1)
+    assert tuner.x_for(("acgt", 16), dt.data_ptr(), off, ds.data_ptr()) in (7, 39)
+    assert ("acgt", 16) in tuner.choice
+    engine.build_sa_batch_device(dt.data_ptr(), off, ds.data_ptr(), X=x)
+    got = ds.cpu().numpy().view(np.uint32)
+    for k, t in enumerate(texts):
+        assert np.array_equal(got[int(off[k]):int(off[k + 1])].astype(np.uint64), oracle.dcx_sa(t, X=39)), k
+
+
 def test_batch_byte_alphabet(engine):
     # all 256 byte values: joined characters need a 32-bit text inside the engine
     texts = [random_text(20000, 256, 5), bytes(range(256)) * 30, random_text(5000, 3, 6), b"\x00\x00\x00"]
