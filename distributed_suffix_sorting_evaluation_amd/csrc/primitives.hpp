@@ -119,7 +119,7 @@ constexpr int RS_SEGS = 8;
 constexpr uint32_t RS_SEG_SHIFT = 5;             // a segment spans 256 / RS_SEGS digits of the previous pass
 constexpr int RS_SEG_BINS = RS_SEGS * RS_RADIX;  // (segment, digit) bins of one pass
 constexpr int RS_SEG_INFO = 2 * RS_SEGS + 2;     // per pass: segment starts [0, 8], tile prefixes [9, 17]
-constexpr uint32_t RS_HIST_KEYS = 1u << 16;      // histogram keys per workgroup
+constexpr uint32_t RS_HIST_KEYS = 1u << 13;      // histogram keys per workgroup (at least)
 constexpr uint32_t RS_HIST_WPS = 64;             // at most this many histogram workgroups per segment
 constexpr int RS_HIST_THREADS = 1024;            // 512 workgroups of 16 waves fill the machine
 
