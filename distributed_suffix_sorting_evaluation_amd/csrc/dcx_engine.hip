@@ -204,7 +204,7 @@ class Engine {
                                                      64 - c0 * bits, 64, true, low.get());
             H = DBuf<uint8_t>(cache, (size_t)N + 1);
             if (np) {
-                k_heads_low32<<<grid(N + 1ull), 256, 0, stream>>>(low.get(), N, H.get());
+                k_heads_low32<<<grid(N / 16 + 1ull), 256, 0, stream>>>(low.get(), N, H.get());
                 k_mark_runs<<<1, RS_RADIX, 0, stream>>>(prim.hist + (np - 1) * RS_RADIX, N, H.get());
             } else {
                 k_heads<<<grid(N + 1ull), 256, 0, stream>>>(k, N, c0 * bits, H.get());

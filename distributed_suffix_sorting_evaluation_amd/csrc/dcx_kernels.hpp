@@ -131,47 +131,38 @@ struct TextU32 {
 
 // ------------------------------------------------------------------------------------------------
 // byte alphabet (compress_alphabet, cli.cpp:417-463): which byte values occur, as a 256-bit mask.  Only
-// presence matters for the alphabet map, and a register mask per thread avoids the same-address LDS
-// atomics that per-byte counting of a small alphabet serializes on.
+// presence matters for the alphabet map, so every byte just stores a 1 into its LDS flag (plain stores:
+// lanes that hit the same flag all write the same value; no atomics, no per-byte register arithmetic --
+// 2.9x faster than or-ing each byte into per-thread register masks).  A workgroup reads 16 KB per step,
+// four 16-byte loads per thread in flight.
 // ------------------------------------------------------------------------------------------------
-__device__ __forceinline__ void mark_byte(uint32_t c, uint64_t (&m)[4]) {
-    const uint64_t bit = 1ull << (c & 63);
-    const uint32_t g = c >> 6;
-    m[0] |= g == 0 ? bit : 0ull;
-    m[1] |= g == 1 ? bit : 0ull;
-    m[2] |= g == 2 ? bit : 0ull;
-    m[3] |= g == 3 ? bit : 0ull;
-}
-
 __global__ __launch_bounds__(256) void k_byte_mask(const uint8_t* __restrict__ t, uint64_t n,
                                                    unsigned long long* __restrict__ mask) {
-    __shared__ unsigned long long s_m[4];
-    if (threadIdx.x < 4) s_m[threadIdx.x] = 0;
+    __shared__ uint32_t s_f[256];
+    s_f[threadIdx.x] = 0;
     __syncthreads();
-    uint64_t m[4] = {0, 0, 0, 0};
-    const uint64_t stride = (uint64_t)gridDim.x * 256 * 16;
-    for (uint64_t base = (blockIdx.x * 256ull + threadIdx.x) * 16; base < n; base += stride) {
-        if (base + 16 <= n && ((reinterpret_cast<uintptr_t>(t + base) & 15) == 0)) {
-            const uint4 v = *reinterpret_cast<const uint4*>(t + base);
-            const uint32_t x[4] = {v.x, v.y, v.z, v.w};
+    constexpr uint64_t chunk = 256ull * 16 * 4;
+    for (uint64_t cb = blockIdx.x * chunk; cb < n; cb += (uint64_t)gridDim.x * chunk) {
+        if (cb + chunk <= n && ((reinterpret_cast<uintptr_t>(t + cb) & 15) == 0)) {
+            uint4 v[4];
 #pragma unroll
-            for (int q = 0; q < 4; q++)
+            for (int k = 0; k < 4; k++)
+                v[k] = *reinterpret_cast<const uint4*>(t + cb + (k * 256ull + threadIdx.x) * 16);
 #pragma unroll
-                for (int b = 0; b < 4; b++) mark_byte((x[q] >> (8 * b)) & 0xFF, m);
+            for (int k = 0; k < 4; k++) {
+                const uint32_t x[4] = {v[k].x, v[k].y, v[k].z, v[k].w};
+#pragma unroll
+                for (int q = 0; q < 4; q++)
+#pragma unroll
+                    for (int b = 0; b < 4; b++) s_f[(x[q] >> (8 * b)) & 0xFF] = 1;
+            }
         } else {
-            for (uint64_t i = base; i < base + 16 && i < n; i++) mark_byte(t[i], m);
+            for (uint64_t i = cb + threadIdx.x; i < cb + chunk && i < n; i += 256) s_f[t[i]] = 1;
         }
     }
-#pragma unroll
-    for (int g = 0; g < 4; g++) {
-#pragma unroll
-        for (int d = 1; d < 64; d <<= 1) m[g] |= __shfl_xor(m[g], d, 64);
-    }
-    if (__lane_id() == 0)
-        for (int g = 0; g < 4; g++)
-            if (m[g]) atomicOr(&s_m[g], (unsigned long long)m[g]);
     __syncthreads();
-    if (threadIdx.x < 4 && s_m[threadIdx.x]) atomicOr(&mask[threadIdx.x], s_m[threadIdx.x]);
+    const uint64_t seen = __ballot(s_f[threadIdx.x] != 0);  // wave w covers byte values [64 w, 64 w + 64)
+    if (__lane_id() == 0 && seen) atomicOr(&mask[threadIdx.x >> 6], (unsigned long long)seen);
 }
 
 // histogram of the character codes lut[t[q]] at every stride-th position (codes 0..256): the order-0
@@ -288,10 +279,32 @@ __global__ __launch_bounds__(256) void k_heads(const uint64_t* __restrict__ keys
 
 // heads from the low words written by the last radix pass (sort_pairs_from's low32): equal low words
 // within a run of the last digit; the run starts are marked by k_mark_runs
+// (a thread per 16 consecutive positions: four 16-byte loads, one 16-byte store; low and H are 256-byte
+// aligned device buffers)
 __global__ __launch_bounds__(256) void k_heads_low32(const uint32_t* __restrict__ low, uint32_t N,
                                                      uint8_t* __restrict__ H) {
-    for (uint32_t j = blockIdx.x * 256 + threadIdx.x; j <= N; j += gridDim.x * 256)
-        H[j] = (j == 0 || j == N) ? 1 : (low[j] != low[j - 1]);
+    const uint32_t groups = N / 16 + 1;  // covers positions 0..N
+    for (uint32_t g = blockIdx.x * 256 + threadIdx.x; g < groups; g += gridDim.x * 256) {
+        const uint32_t j0 = g * 16;
+        if (j0 + 16 <= N) {
+            uint32_t w[17];
+            w[0] = j0 ? low[j0 - 1] : ~low[0];  // position 0 is always a head
+#pragma unroll
+            for (int v = 0; v < 4; v++) {
+                const uint4 x = reinterpret_cast<const uint4*>(low + j0)[v];
+                w[1 + 4 * v] = x.x;
+                w[2 + 4 * v] = x.y;
+                w[3 + 4 * v] = x.z;
+                w[4 + 4 * v] = x.w;
+            }
+            uint32_t h[4] = {0, 0, 0, 0};
+#pragma unroll
+            for (int k = 0; k < 16; k++) h[k >> 2] |= (uint32_t)(w[k + 1] != w[k]) << (8 * (k & 3));
+            reinterpret_cast<uint4*>(H + j0)[0] = make_uint4(h[0], h[1], h[2], h[3]);
+        } else {
+            for (uint32_t j = j0; j < j0 + 16 && j <= N; j++) H[j] = (j == 0 || j == N) ? 1 : (low[j] != low[j - 1]);
+        }
+    }
 }
 __global__ __launch_bounds__(256) void k_mark_runs(const uint32_t* __restrict__ offsets, uint32_t N,
                                                    uint8_t* __restrict__ H) {
