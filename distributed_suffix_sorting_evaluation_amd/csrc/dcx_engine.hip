@@ -248,7 +248,10 @@ class Engine {
                 cur = std::move(next);
                 continue;
             }
-            if (c == 0) throw Error(DCX_E_UNSUPPORTED, "character width too large for refinement");
+            // a character round takes as many characters as one stream window holds: small groups compare
+            // the characters alone, and large groups whose rank and characters do not fit one key word
+            // take two stable sorts
+            c = std::min<uint32_t>(cpw, L - C);
             DBuf<uint8_t> Ha(cache, (size_t)A + 1);
             refine_round(PT, bits, C, c, cur, order, H, Ha);
             C += c;
@@ -285,9 +288,10 @@ class Engine {
     }
 
     // One character round of the refinement: orders the members of every tied group of `cur` by their
-    // next c characters (after the first C), writes the new group starts into H and the compact heads of
-    // the list into Ha (A + 1 entries).  Small groups are sorted in place by one thread each; the members
-    // of large groups go through a radix sort keyed by (group, characters).
+    // next c characters (after the first C, c * bits <= 64), writes the new group starts into H and the
+    // compact heads of the list into Ha (A + 1 entries).  Small groups are sorted in place by one thread
+    // each; the members of large groups go through a radix sort keyed by (rank among the large groups,
+    // characters), or two stable sorts when those do not fit 64 bits.
     void refine_round(const uint64_t* PT, uint32_t bits, uint32_t C, uint32_t c, const ActiveSet& cur,
                       DBuf<uint32_t>& order, DBuf<uint8_t>& H, DBuf<uint8_t>& Ha) {
         const uint32_t A = cur.A, G = cur.G;
@@ -295,21 +299,37 @@ class Engine {
         k_refine_small<<<grid(G), 256, 0, stream>>>(cur.gstart.get(), G, cur.gfirst.get(), order.get(), H.get(),
                                                     Ha.get(), PT, bits, C, c * bits, large.get());
         prim.fill_u8(Ha.get() + A, 1, 1);
-        DBuf<uint32_t> slots(cache, A);
-        prim.scan(LargeMembersOp{large.get(), cur.gid.get(), slots.get()}, A, 1);
-        const uint32_t B = (uint32_t)prim.read_scalars(2)[1];
+        DBuf<uint32_t> slots(cache, A), lrank(cache, A);
+        prim.scan(LargeMembersOp{large.get(), cur.gid.get(), cur.gstart.get(), slots.get(), lrank.get()}, A, 1);
+        const uint64_t tot = prim.read_scalars(2)[1];
+        const uint32_t B = (uint32_t)tot, nlarge = (uint32_t)(tot >> 32);
         if (B == 0) return;
-        const uint32_t gbits = bit_width(G - 1);
+        const uint32_t lbits = bit_width(nlarge - 1);
+        const uint32_t cbits = c * bits;
         DBuf<uint64_t> rka(cache, B), rkb(cache, B);
         DBuf<uint32_t> rva(cache, B), rvb(cache, B);
-        k_keys_large<<<grid(B), 256, 0, stream>>>(PT, bits, B, slots.get(), cur.act.get(), cur.gid.get(), order.get(), C,
-                                                  c * bits, gbits, rka.get(), rva.get());
         uint64_t* x = rka.get();
         uint64_t* x2 = rkb.get();
         uint32_t* y = rva.get();
         uint32_t* y2 = rvb.get();
-        prim.sort_pairs(x, x2, y, y2, B, 64 - gbits - c * bits, 64);
-        k_writeback_large<<<grid(B), 256, 0, stream>>>(x, y, slots.get(), B, cur.act.get(), order.get(), H.get(), Ha.get());
+        if (lbits + cbits <= 64) {
+            k_keys_large<<<grid(B), 256, 0, stream>>>(PT, bits, B, slots.get(), cur.act.get(), lrank.get(),
+                                                      order.get(), C, cbits, lbits, x, y);
+            prim.sort_pairs(x, x2, y, y2, B, 64 - lbits - cbits, 64);
+            k_writeback_large<<<grid(B), 256, 0, stream>>>(x, y, slots.get(), B, cur.act.get(), order.get(), H.get(),
+                                                           Ha.get());
+            return;
+        }
+        // characters first, then a stable sort by the group rank (the members are identified by index)
+        DBuf<uint32_t> P(cache, B);
+        k_keys_large_chars<<<grid(B), 256, 0, stream>>>(PT, bits, B, slots.get(), cur.act.get(), order.get(), C,
+                                                        cbits, x, P.get());
+        prim.sort_pairs_from(ArrayKeys<uint64_t>{x}, x, x2, y, y2, B, 64 - cbits, 64, true);
+        k_keys_large_rank<<<grid(B), 256, 0, stream>>>(y, lrank.get(), B, x2);
+        std::swap(x, x2);
+        prim.sort_pairs(x, x2, y, y2, B, 0, lbits);
+        k_writeback_large2<<<grid(B), 256, 0, stream>>>(x, y, P.get(), PT, bits, C, cbits, slots.get(), B,
+                                                        cur.act.get(), order.get(), H.get(), Ha.get());
     }
 
     // dense words only: boundary suffixes that share a key group are moved to its front (k_boundary_fix)

@@ -745,35 +745,47 @@ __global__ __launch_bounds__(256) void k_refine_small(const uint32_t* __restrict
 }
 
 // members of the large groups of the active list: their list indices, in list order
+// members of the large groups of the active list: their list indices, in list order, and the rank of
+// their group among the large groups (the high counter counts the first members of large groups)
 struct LargeMembersOp {
     static constexpr bool kBlocked = true;
     const uint8_t* large;
     const uint32_t* gid;
+    const uint32_t* gstart;
     uint32_t* slots;
-    __device__ __forceinline__ uint64_t value(uint64_t a) const { return large[gid[a]]; }
+    uint32_t* lrank;
+    __device__ __forceinline__ uint64_t value(uint64_t a) const {
+        const uint32_t g = gid[a];
+        return large[g] ? (1ull | ((uint64_t)(gstart[g] == (uint32_t)a) << 32)) : 0ull;
+    }
     __device__ __forceinline__ void write(uint64_t a, uint64_t excl, uint64_t v) const {
-        if (v) slots[excl] = (uint32_t)a;
+        if (v) {
+            slots[(uint32_t)excl] = (uint32_t)a;
+            lrank[(uint32_t)excl] = (uint32_t)(excl >> 32) + (uint32_t)(v >> 32) - 1u;
+        }
     }
 };
 
-// radix keys of the large-group members: (group << (64 - gbits)) | next cbits bits of characters
+// radix keys of the large-group members: (large-group rank << (64 - lbits)) | next cbits bits of characters
+// (lbits + cbits <= 64); values are the suffixes
 __global__ __launch_bounds__(256) void k_keys_large(const uint64_t* __restrict__ PT, uint32_t bits, uint32_t B,
                                                     const uint32_t* __restrict__ slots,
                                                     const uint32_t* __restrict__ act_idx,
-                                                    const uint32_t* __restrict__ gid, const uint32_t* __restrict__ order,
-                                                    uint32_t C, uint32_t cbits, uint32_t gbits,
-                                                    uint64_t* __restrict__ keys, uint32_t* __restrict__ vals) {
+                                                    const uint32_t* __restrict__ lrank,
+                                                    const uint32_t* __restrict__ order, uint32_t C, uint32_t cbits,
+                                                    uint32_t lbits, uint64_t* __restrict__ keys,
+                                                    uint32_t* __restrict__ vals) {
     const StreamKeys<false> chars{PT, bits, top_mask(cbits)};
     for (uint32_t b = blockIdx.x * 256 + threadIdx.x; b < B; b += gridDim.x * 256) {
-        const uint32_t a = slots[b];
-        const uint32_t p = order[act_idx[a]];
+        const uint32_t p = order[act_idx[slots[b]]];
         const uint64_t ch = chars((uint64_t)p + C);
-        keys[b] = gbits ? (((uint64_t)gid[a] << (64 - gbits)) | (ch >> gbits)) : ch;
+        keys[b] = lbits ? (((uint64_t)lrank[b] << (64 - lbits)) | (ch >> lbits)) : ch;
         vals[b] = p;
     }
 }
 
-// sorted large-group members back into the order, with their heads (H and the compact Ha)
+// sorted large-group members back into `order` (a group's members keep their slots); new group starts
+// into H and the compact heads Ha
 __global__ __launch_bounds__(256) void k_writeback_large(const uint64_t* __restrict__ keys,
                                                          const uint32_t* __restrict__ vals,
                                                          const uint32_t* __restrict__ slots, uint32_t B,
@@ -784,7 +796,48 @@ __global__ __launch_bounds__(256) void k_writeback_large(const uint64_t* __restr
         const uint32_t a = slots[b];
         const uint32_t j = act_idx[a];
         order[j] = vals[b];
-        const bool head = b == 0 || keys[b] != keys[b - 1];  // the group id is in the key
+        const bool head = b == 0 || keys[b] != keys[b - 1];  // the group rank is in the key
+        Ha[a] = head ? 1 : 0;
+        if (head) H[j] = 1;
+    }
+}
+
+// The same when rank and characters do not fit one word: an LSD sort by the characters (values: member
+// indices b, suffixes kept in P), then a stable sort by the rank.
+__global__ __launch_bounds__(256) void k_keys_large_chars(const uint64_t* __restrict__ PT, uint32_t bits, uint32_t B,
+                                                          const uint32_t* __restrict__ slots,
+                                                          const uint32_t* __restrict__ act_idx,
+                                                          const uint32_t* __restrict__ order, uint32_t C,
+                                                          uint32_t cbits, uint64_t* __restrict__ keys,
+                                                          uint32_t* __restrict__ P) {
+    const StreamKeys<false> chars{PT, bits, top_mask(cbits)};
+    for (uint32_t b = blockIdx.x * 256 + threadIdx.x; b < B; b += gridDim.x * 256) {
+        const uint32_t p = order[act_idx[slots[b]]];
+        keys[b] = chars((uint64_t)p + C);
+        P[b] = p;
+    }
+}
+__global__ __launch_bounds__(256) void k_keys_large_rank(const uint32_t* __restrict__ members,
+                                                         const uint32_t* __restrict__ lrank, uint32_t B,
+                                                         uint64_t* __restrict__ keys) {
+    for (uint32_t i = blockIdx.x * 256 + threadIdx.x; i < B; i += gridDim.x * 256) keys[i] = lrank[members[i]];
+}
+__global__ __launch_bounds__(256) void k_writeback_large2(const uint64_t* __restrict__ rkeys,
+                                                          const uint32_t* __restrict__ members,
+                                                          const uint32_t* __restrict__ P,
+                                                          const uint64_t* __restrict__ PT, uint32_t bits, uint32_t C,
+                                                          uint32_t cbits, const uint32_t* __restrict__ slots, uint32_t B,
+                                                          const uint32_t* __restrict__ act_idx,
+                                                          uint32_t* __restrict__ order, uint8_t* __restrict__ H,
+                                                          uint8_t* __restrict__ Ha) {
+    const StreamKeys<false> chars{PT, bits, top_mask(cbits)};
+    for (uint32_t i = blockIdx.x * 256 + threadIdx.x; i < B; i += gridDim.x * 256) {
+        const uint32_t a = slots[i];
+        const uint32_t j = act_idx[a];
+        const uint32_t p = P[members[i]];
+        order[j] = p;
+        const bool head = i == 0 || rkeys[i] != rkeys[i - 1] ||
+                          chars((uint64_t)p + C) != chars((uint64_t)P[members[i - 1]] + C);
         Ha[a] = head ? 1 : 0;
         if (head) H[j] = 1;
     }
