@@ -416,7 +416,7 @@ class Engine {
         dcx_level_stats* ls = st && depth < DCX_MAX_LEVELS ? &st->level[depth] : nullptr;
         const double thr = opt.discarding_threshold;
         // the kept set of discarding, from the non-unique samples: after >= NU and after <= 2 NU + 1
-        prim.scan(NonUniqueOp{Hs.get(), s_order.get(), dc, m, nullptr}, m, 0);
+        prim.total(NonUniqueOp{Hs.get(), s_order.get(), dc, m, nullptr}, m, 0);
         const uint32_t nu = (uint32_t)prim.read_scalars(1)[0];
         DBuf<uint32_t> K;
         uint32_t after = m;

@@ -177,8 +177,20 @@ __global__ __launch_bounds__(RS_HIST_THREADS) void k_window_hist_seg(const uint6
     const uint64_t len = e - a, chunk = (len + wps - 1) / wps;
     const uint64_t lo = a + (c * chunk < len ? c * chunk : len);
     const uint64_t hi = a + ((c + 1) * chunk < len ? (c + 1) * chunk : len);
-    const StreamKeys<false> win{s, bits, ~0ull};
-    for (uint64_t q = lo + threadIdx.x; q < hi; q += RS_HIST_THREADS) atomicAdd(&s_h[(uint32_t)(win(q) >> 53)], 1u);
+    // a thread per stream word: its 64 / bits windows from the word and the next one, two loads in all
+    // (one load per window pair made the address units the bound)
+    const uint32_t ppw = 64 / bits;
+    const uint64_t w_lo = lo / ppw, w_hi = (hi + ppw - 1) / ppw;
+    for (uint64_t w = w_lo + threadIdx.x; w < w_hi; w += RS_HIST_THREADS) {
+        const uint64_t x0 = s[w], x1 = s[w + 1], q0 = w * ppw;
+        const uint32_t k0 = q0 < lo ? (uint32_t)(lo - q0) : 0u;
+        const uint32_t k1 = q0 + ppw > hi ? (uint32_t)(hi - q0) : ppw;
+        for (uint32_t k = k0; k < k1; k++) {
+            const uint32_t sh = k * bits;
+            const uint64_t x = (x0 << sh) | ((x1 >> 1) >> (63 - sh));
+            atomicAdd(&s_h[(uint32_t)(x >> 53)], 1u);
+        }
+    }
     __syncthreads();
     uint32_t* out = part + (uint64_t)blockIdx.x * RS_SEG_BINS;
     for (uint32_t i = threadIdx.x; i < RS_SEG_BINS; i += RS_HIST_THREADS) out[i] = s_h[i];
@@ -1162,6 +1174,21 @@ struct Primitives {
         hipLaunchKernelGGL(k_scan<Op>, dim3((unsigned)tiles), dim3(SC_THREADS), 0, stream, op, N, scan_lo, scan_hi,
                            tag, ticket, ticket_host, scalars + slot);
         ticket_host += tiles;
+    }
+
+    // The grand total of a scan only (into scalar slot `slot`): long inputs skip the down pass.
+    template <class Op>
+    void total(const Op& op, uint64_t N, int slot) {
+        const uint64_t tiles = ceil_div(N, SC_TILE);
+        if (N == 0 || tiles <= SC_SINGLE_PASS_TILES) {
+            scan(op, N, slot);
+            return;
+        }
+        const uint32_t tpb = (uint32_t)ceil_div(tiles, (uint64_t)SC_BLOCKS);
+        const uint32_t nb = (uint32_t)ceil_div(tiles, (uint64_t)tpb);
+        hipLaunchKernelGGL(k_scan_reduce<Op>, dim3(nb), dim3(SC_THREADS), 0, stream, op, N, tpb, scan_part);
+        hipLaunchKernelGGL(k_scan_spine, dim3(1), dim3(1024), 0, stream, (const uint64_t*)scan_part, nb, scan_prefix,
+                           scalars + slot);
     }
 
     // Copies `count` device scalars to the host (synchronizes the stream).
