@@ -204,7 +204,7 @@ class Engine {
                                                      64 - c0 * bits, 64, true, low.get());
             H = DBuf<uint8_t>(cache, (size_t)N + 1);
             if (np) {
-                k_heads_low32<<<grid(N / 16 + 1ull), 256, 0, stream>>>(low.get(), N, H.get());
+                k_heads_u32<<<grid(N / 16 + 1ull), 256, 0, stream>>>(low.get(), N, H.get());
                 k_mark_runs<<<1, RS_RADIX, 0, stream>>>(prim.hist + (np - 1) * RS_RADIX, N, H.get());
             } else {
                 k_heads<<<grid(N + 1ull), 256, 0, stream>>>(k, N, c0 * bits, H.get());
@@ -628,7 +628,7 @@ class Engine {
             }
             prim.scan(SampleScanOp{order.get(), H.get(), dc, dummy ? 1u : 0u, s_order.get(), s_gid.get()}, n, 0);
             DBuf<uint8_t> Hs(cache, (size_t)m + 1);
-            k_sample_heads<<<grid(m + 1ull), 256, 0, stream>>>(s_gid.get(), m, Hs.get());
+            k_heads_u32<<<grid(m / 16 + 1ull), 256, 0, stream>>>(s_gid.get(), m, Hs.get());
             s_gid.reset();
             DBuf<uint32_t> names(cache, m);
             prim.scan(NamesOp{Hs.get(), names.get()}, m, 0);

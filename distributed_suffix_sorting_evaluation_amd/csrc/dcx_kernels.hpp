@@ -277,12 +277,12 @@ __global__ __launch_bounds__(256) void k_heads(const uint64_t* __restrict__ keys
         H[j] = (j == 0 || j == N) ? 1 : (top_bits(keys[j], kbits) != top_bits(keys[j - 1], kbits));
 }
 
-// heads from the low words written by the last radix pass (sort_pairs_from's low32): equal low words
-// within a run of the last digit; the run starts are marked by k_mark_runs
-// (a thread per 16 consecutive positions: four 16-byte loads, one 16-byte store; low and H are 256-byte
-// aligned device buffers)
-__global__ __launch_bounds__(256) void k_heads_low32(const uint32_t* __restrict__ low, uint32_t N,
-                                                     uint8_t* __restrict__ H) {
+// H[j] = 1 where a[j] != a[j - 1], and at 0 and N.  Used for the heads from the low words written by the
+// last radix pass (sort_pairs_from's low32: equal low words within a run of the last digit; the run starts
+// are marked by k_mark_runs) and for the sample groups (group ids in sorted order).  A thread per 16
+// consecutive positions: four 16-byte loads, one 16-byte store (a and H are 256-byte aligned buffers).
+__global__ __launch_bounds__(256) void k_heads_u32(const uint32_t* __restrict__ low, uint32_t N,
+                                                   uint8_t* __restrict__ H) {
     const uint32_t groups = N / 16 + 1;  // covers positions 0..N
     for (uint32_t g = blockIdx.x * 256 + threadIdx.x; g < groups; g += gridDim.x * 256) {
         const uint32_t j0 = g * 16;
@@ -459,11 +459,6 @@ struct SampleScanOp {
     }
 };
 
-__global__ __launch_bounds__(256) void k_sample_heads(const uint32_t* __restrict__ s_gid, uint32_t m,
-                                                      uint8_t* __restrict__ H) {
-    for (uint32_t t = blockIdx.x * 256 + threadIdx.x; t <= m; t += gridDim.x * 256)
-        H[t] = (t == 0 || t == m) ? 1 : (s_gid[t] != s_gid[t - 1]);
-}
 
 // ------------------------------------------------------------------------------------------------
 // Dense character codes read the padding like the smallest character, so a suffix whose key window
