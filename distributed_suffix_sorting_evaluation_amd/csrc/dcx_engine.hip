@@ -300,7 +300,7 @@ class Engine {
                                                     Ha.get(), PT, bits, C, c * bits, large.get());
         prim.fill_u8(Ha.get() + A, 1, 1);
         DBuf<uint32_t> slots(cache, A), lrank(cache, A);
-        prim.scan(LargeMembersOp{large.get(), cur.gid.get(), cur.gstart.get(), slots.get(), lrank.get()}, A, 1);
+        prim.scan(LargeMembersOp{large.get(), cur.gid.get(), slots.get(), lrank.get()}, A, 1);
         const uint64_t tot = prim.read_scalars(2)[1];
         const uint32_t B = (uint32_t)tot, nlarge = (uint32_t)(tot >> 32);
         if (B == 0) return;

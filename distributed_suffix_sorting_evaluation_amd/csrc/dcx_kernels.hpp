@@ -741,17 +741,18 @@ __global__ __launch_bounds__(256) void k_refine_small(const uint32_t* __restrict
 
 // members of the large groups of the active list: their list indices, in list order
 // members of the large groups of the active list: their list indices, in list order, and the rank of
-// their group among the large groups (the high counter counts the first members of large groups)
+// their group among the large groups (the high counter counts the first members of large groups; a
+// group's members are consecutive in the list)
 struct LargeMembersOp {
     static constexpr bool kBlocked = true;
     const uint8_t* large;
     const uint32_t* gid;
-    const uint32_t* gstart;
     uint32_t* slots;
     uint32_t* lrank;
     __device__ __forceinline__ uint64_t value(uint64_t a) const {
         const uint32_t g = gid[a];
-        return large[g] ? (1ull | ((uint64_t)(gstart[g] == (uint32_t)a) << 32)) : 0ull;
+        const bool first = a == 0 || gid[a - 1] != g;
+        return large[g] ? (1ull | ((uint64_t)first << 32)) : 0ull;
     }
     __device__ __forceinline__ void write(uint64_t a, uint64_t excl, uint64_t v) const {
         if (v) {
