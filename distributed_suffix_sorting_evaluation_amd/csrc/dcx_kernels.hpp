@@ -332,6 +332,7 @@ __global__ __launch_bounds__(256) void k_heads_packed(const uint64_t* __restrict
 template <bool GATHER>
 struct ActiveScanOp {
     static constexpr bool kBlocked = true;
+    static constexpr bool kCompact = true;  // writes only where the low half is 1
     const uint8_t* H;
     const uint32_t* src;
     uint32_t* act_idx;
@@ -578,6 +579,7 @@ __global__ __launch_bounds__(256) void k_scatter_pairs(const uint32_t* __restric
 // H[j+1]); with `cand` it also writes their class-major positions c and c + 1.
 struct NonUniqueOp {
     static constexpr bool kBlocked = true;
+    static constexpr bool kCompact = true;  // writes only where the low half is 1
     const uint8_t* H;
     const uint32_t* order;
     DcDev dc;
@@ -745,6 +747,7 @@ __global__ __launch_bounds__(256) void k_refine_small(const uint32_t* __restrict
 // group's members are consecutive in the list)
 struct LargeMembersOp {
     static constexpr bool kBlocked = true;
+    static constexpr bool kCompact = true;  // writes only where the low half is 1
     const uint8_t* large;
     const uint32_t* gid;
     uint32_t* slots;
@@ -873,6 +876,7 @@ __global__ __launch_bounds__(256) void k_tiebreak_small(const uint32_t* __restri
 // elements of the large groups: (group << 32 | suffix) plus the order slot they came from
 struct LargeScanOp {
     static constexpr bool kBlocked = true;
+    static constexpr bool kCompact = true;  // writes only where the low half is 1
     const uint8_t* large;
     const uint32_t* gid;
     const uint32_t* act_idx;

@@ -656,6 +656,19 @@ struct ScanDirect<Op, decltype((void)Op::kDirect)> {
     static constexpr bool value = Op::kDirect;
 };
 
+// Ops with kCompact write only for elements whose low value half is 1 (the low half is 0 or 1, the high
+// half 0 or 1): the down pass stages just those elements (their offset in the tile and high prefix) in
+// LDS, in output order, and hands them to write() striped over the compacted list, so consecutive threads
+// write consecutive outputs and LDS traffic follows the element count, not the tile size.
+template <class Op, class = void>
+struct ScanCompact {
+    static constexpr bool value = false;
+};
+template <class Op>
+struct ScanCompact<Op, decltype((void)Op::kCompact)> {
+    static constexpr bool value = Op::kCompact;
+};
+
 template <bool PL = false, class Op>
 __device__ __forceinline__ void scan_load(const Op& op, uint64_t N, uint64_t tbase, uint32_t tid,
                                           uint64_t (&v)[SC_ITEMS], uint32_t* pl = nullptr) {
@@ -753,10 +766,13 @@ __global__ __launch_bounds__(SC_THREADS) void k_scan_down(Op op, uint64_t N, uin
                                                           const uint64_t* __restrict__ prefix) {
     constexpr bool PL = ScanPayload<Op>::value && Op::kBlocked;
     constexpr bool DIRECT = ScanDirect<Op>::value && Op::kBlocked;
+    constexpr bool COMPACT = ScanCompact<Op>::value && Op::kBlocked && !DIRECT && !PL;
     constexpr uint32_t PAD = SC_TILE + SC_TILE / 16;  // slot of the tile's end prefix
     __shared__ uint64_t s_tmp[SC_THREADS / 64];
-    __shared__ uint64_t s_e[DIRECT ? 1 : PAD + 1];  // padded to spread blocked accesses over banks
+    __shared__ uint64_t s_e[DIRECT || COMPACT ? 1 : PAD + 1];  // padded to spread blocked accesses over banks
     __shared__ uint32_t s_p[PL && !DIRECT ? PAD : 1];
+    __shared__ uint32_t s_ci[COMPACT ? SC_TILE : 1];   // COMPACT: tile offset | high value bit << 31
+    __shared__ uint32_t s_chi[COMPACT ? SC_TILE : 1];  // COMPACT: high prefix relative to the tile's
     const uint32_t tid = threadIdx.x;
     const uint64_t t0 = (uint64_t)blockIdx.x * tpb;
     uint64_t carry = prefix[blockIdx.x];
@@ -795,6 +811,24 @@ __global__ __launch_bounds__(SC_THREADS) void k_scan_down(Op op, uint64_t N, uin
                     if (base + k < N) op.write(base + k, run, v[k]);
                 }
                 run += v[k];
+            }
+        } else if constexpr (COMPACT) {
+            const uint32_t lo0 = (uint32_t)carry, hi0 = (uint32_t)(carry >> 32);
+#pragma unroll
+            for (int k = 0; k < SC_ITEMS; k++) {
+                if ((uint32_t)v[k]) {
+                    const uint32_t e = (uint32_t)run - lo0;
+                    s_ci[e] = (tid * SC_ITEMS + k) | ((uint32_t)(v[k] >> 32) << 31);
+                    s_chi[e] = (uint32_t)(run >> 32) - hi0;
+                }
+                run += v[k];
+            }
+            __syncthreads();
+            const uint32_t cnt = (uint32_t)tile_total;
+            for (uint32_t e = tid; e < cnt; e += SC_THREADS) {
+                const uint32_t ci = s_ci[e];
+                const uint64_t excl = (uint64_t)(lo0 + e) | ((uint64_t)(hi0 + s_chi[e]) << 32);
+                op.write(tbase + (ci & 0x7FFFFFFFu), excl, 1ull | ((uint64_t)(ci >> 31) << 32));
             }
         } else {
 #pragma unroll
