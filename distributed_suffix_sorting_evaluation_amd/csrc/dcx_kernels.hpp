@@ -436,7 +436,7 @@ struct NamesOp {
 // ------------------------------------------------------------------------------------------------
 struct SampleScanOp {
     static constexpr bool kBlocked = true;
-    static constexpr bool kDirect = true;  // measured: 9% faster down pass than the striped writes
+    static constexpr bool kCompact = true;  // writes only samples (low half 1)
     const uint32_t* order;
     const uint8_t* H;
     DcDev dc;

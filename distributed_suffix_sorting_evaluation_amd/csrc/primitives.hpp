@@ -644,18 +644,6 @@ struct ScanPayload<Op, decltype((void)&Op::payload)> {
     static constexpr bool value = true;
 };
 
-// Ops with kDirect write in the blocked order they load in: each thread writes its own elements from
-// its running prefix, with no LDS transpose to striped order.  Pays for sparse writes to one array
-// (sample compaction); writes to several arrays, or to group-indexed ones, want the striped order.
-template <class Op, class = void>
-struct ScanDirect {
-    static constexpr bool value = false;
-};
-template <class Op>
-struct ScanDirect<Op, decltype((void)Op::kDirect)> {
-    static constexpr bool value = Op::kDirect;
-};
-
 // Ops with kCompact write only for elements whose low value half is 1 (the low half is 0 or 1, the high
 // half 0 or 1): the down pass stages just those elements (their offset in the tile and high prefix) in
 // LDS, in output order, and hands them to write() striped over the compacted list, so consecutive threads
@@ -765,14 +753,14 @@ template <class Op>
 __global__ __launch_bounds__(SC_THREADS) void k_scan_down(Op op, uint64_t N, uint32_t tpb,
                                                           const uint64_t* __restrict__ prefix) {
     constexpr bool PL = ScanPayload<Op>::value && Op::kBlocked;
-    constexpr bool DIRECT = ScanDirect<Op>::value && Op::kBlocked;
-    constexpr bool COMPACT = ScanCompact<Op>::value && Op::kBlocked && !DIRECT && !PL;
+    constexpr bool COMPACT = ScanCompact<Op>::value && Op::kBlocked;
     constexpr uint32_t PAD = SC_TILE + SC_TILE / 16;  // slot of the tile's end prefix
     __shared__ uint64_t s_tmp[SC_THREADS / 64];
-    __shared__ uint64_t s_e[DIRECT || COMPACT ? 1 : PAD + 1];  // padded to spread blocked accesses over banks
-    __shared__ uint32_t s_p[PL && !DIRECT ? PAD : 1];
+    __shared__ uint64_t s_e[COMPACT ? 1 : PAD + 1];  // padded to spread blocked accesses over banks
+    __shared__ uint32_t s_p[PL && !COMPACT ? PAD : 1];
     __shared__ uint32_t s_ci[COMPACT ? SC_TILE : 1];   // COMPACT: tile offset | high value bit << 31
     __shared__ uint32_t s_chi[COMPACT ? SC_TILE : 1];  // COMPACT: high prefix relative to the tile's
+    __shared__ uint32_t s_cp[COMPACT && PL ? SC_TILE : 1];  // COMPACT: payloads
     const uint32_t tid = threadIdx.x;
     const uint64_t t0 = (uint64_t)blockIdx.x * tpb;
     uint64_t carry = prefix[blockIdx.x];
@@ -801,18 +789,7 @@ __global__ __launch_bounds__(SC_THREADS) void k_scan_down(Op op, uint64_t N, uin
         for (int k = 0; k < SC_ITEMS; k++) sum += v[k];
         uint64_t tile_total = 0;
         uint64_t run = carry + block_excl_scan64<SC_THREADS>(sum, s_tmp, &tile_total);  // synchronizes
-        if constexpr (DIRECT) {
-            const uint64_t base = tbase + (uint64_t)tid * SC_ITEMS;
-#pragma unroll
-            for (int k = 0; k < SC_ITEMS; k++) {
-                if constexpr (PL) {
-                    if (base + k < N) op.write(base + k, run, v[k], pl[k]);
-                } else {
-                    if (base + k < N) op.write(base + k, run, v[k]);
-                }
-                run += v[k];
-            }
-        } else if constexpr (COMPACT) {
+        if constexpr (COMPACT) {
             const uint32_t lo0 = (uint32_t)carry, hi0 = (uint32_t)(carry >> 32);
 #pragma unroll
             for (int k = 0; k < SC_ITEMS; k++) {
@@ -820,6 +797,7 @@ __global__ __launch_bounds__(SC_THREADS) void k_scan_down(Op op, uint64_t N, uin
                     const uint32_t e = (uint32_t)run - lo0;
                     s_ci[e] = (tid * SC_ITEMS + k) | ((uint32_t)(v[k] >> 32) << 31);
                     s_chi[e] = (uint32_t)(run >> 32) - hi0;
+                    if constexpr (PL) s_cp[e] = pl[k];
                 }
                 run += v[k];
             }
@@ -828,7 +806,11 @@ __global__ __launch_bounds__(SC_THREADS) void k_scan_down(Op op, uint64_t N, uin
             for (uint32_t e = tid; e < cnt; e += SC_THREADS) {
                 const uint32_t ci = s_ci[e];
                 const uint64_t excl = (uint64_t)(lo0 + e) | ((uint64_t)(hi0 + s_chi[e]) << 32);
-                op.write(tbase + (ci & 0x7FFFFFFFu), excl, 1ull | ((uint64_t)(ci >> 31) << 32));
+                const uint64_t vv = 1ull | ((uint64_t)(ci >> 31) << 32);
+                if constexpr (PL)
+                    op.write(tbase + (ci & 0x7FFFFFFFu), excl, vv, s_cp[e]);
+                else
+                    op.write(tbase + (ci & 0x7FFFFFFFu), excl, vv);
             }
         } else {
 #pragma unroll
