@@ -248,10 +248,20 @@ class Engine {
                 cur = std::move(next);
                 continue;
             }
-            // a character round takes as many characters as one stream window holds: small groups compare
-            // the characters alone, and large groups whose rank and characters do not fit one key word
-            // take two stable sorts
+            // a character round takes as many characters as one stream window holds, as far as the large
+            // groups' keys (rank among the large groups, characters) still fit one word; small groups
+            // compare the characters alone
             c = std::min<uint32_t>(cpw, L - C);
+            {
+                DBuf<uint32_t> cnt(cache, 1);
+                prim.fill_u32(cnt.get(), 0, 1);
+                k_count_large<<<grid(cur.G, 2048), 256, 0, stream>>>(cur.gstart.get(), cur.G, cnt.get());
+                uint32_t nlarge = 0;
+                DCX_HIP(hipMemcpyAsync(&nlarge, cnt.get(), 4, hipMemcpyDeviceToHost, stream));
+                DCX_HIP(hipStreamSynchronize(stream));
+                const uint32_t lbits = nlarge > 1 ? bit_width(nlarge - 1) : 0;
+                if (lbits + c * bits > 64 && 64 - lbits >= bits) c = (64 - lbits) / bits;
+            }
             DBuf<uint8_t> Ha(cache, (size_t)A + 1);
             refine_round(PT, bits, C, c, cur, order, H, Ha);
             C += c;

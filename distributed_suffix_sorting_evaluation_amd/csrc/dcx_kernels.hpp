@@ -742,6 +742,16 @@ __global__ __launch_bounds__(256) void k_refine_small(const uint32_t* __restrict
 }
 
 // members of the large groups of the active list: their list indices, in list order
+// number of groups with more than RF_SMALL members (they take the radix path of a group round)
+__global__ __launch_bounds__(256) void k_count_large(const uint32_t* __restrict__ gstart, uint32_t G,
+                                                     uint32_t* __restrict__ count) {
+    uint32_t c = 0;
+    for (uint32_t g = blockIdx.x * 256 + threadIdx.x; g < G; g += gridDim.x * 256)
+        c += gstart[g + 1] - gstart[g] > RF_SMALL;
+    for (int d = 32; d >= 1; d >>= 1) c += __shfl_xor(c, d, 64);
+    if (__lane_id() == 0 && c) atomicAdd(count, c);
+}
+
 // members of the large groups of the active list: their list indices, in list order, and the rank of
 // their group among the large groups (the high counter counts the first members of large groups; a
 // group's members are consecutive in the list)
