@@ -785,13 +785,19 @@ class Engine {
             DCX_HIP(hipMemcpyAsync(d_co.get(), co.data(), (cnt + 1) * 8, hipMemcpyHostToDevice, stream));
             const uint32_t N = (uint32_t)joined;
             const uint64_t chars = offsets[k1] - offsets[k0];
+            // owner tables of the input and of the joined positions
+            const uint64_t in_chunks = (chars >> OWNER_SHIFT) + 1, co_chunks = ((uint64_t)N >> OWNER_SHIFT) + 1;
+            DBuf<uint32_t> t_in(cache, in_chunks + 1), t_co(cache, co_chunks + 1);
+            k_owner_table<<<grid(in_chunks + 1), 256, 0, stream>>>(d_in.get(), cnt, in_off[0], in_chunks, t_in.get());
+            k_owner_table<<<grid(co_chunks + 1), 256, 0, stream>>>(d_co.get(), cnt, 0, co_chunks, t_co.get());
             DBuf<uint32_t> sa;
             sigma = maxchar;
             if (maxchar <= 255) {
                 DBuf<uint8_t> jt(cache, (size_t)N + 16);
                 if (chars)
                     k_batch_text<uint8_t><<<grid(chars), 256, 0, stream>>>(d_texts, in_off[0], in_off[cnt], d_in.get(),
-                                                                            d_co.get(), cnt, dlut.get(), jt.get());
+                                                                            d_co.get(), cnt, dlut.get(), t_in.get(),
+                                                                            jt.get());
                 k_batch_seps<uint8_t><<<grid(cnt), 256, 0, stream>>>(d_co.get(), cnt, jt.get());
                 level<uint8_t>(jt.get(), ident.get(), N, maxchar, 0, nullptr, nullptr, &sa);
             } else {
@@ -799,14 +805,14 @@ class Engine {
                 if (chars)
                     k_batch_text<uint32_t><<<grid(chars), 256, 0, stream>>>(d_texts, in_off[0], in_off[cnt],
                                                                              d_in.get(), d_co.get(), cnt, dlut.get(),
-                                                                             jt.get());
+                                                                             t_in.get(), jt.get());
                 k_batch_seps<uint32_t><<<grid(cnt), 256, 0, stream>>>(d_co.get(), cnt, jt.get());
                 level<uint32_t>(jt.get(), nullptr, N, maxchar, 0, nullptr, nullptr, &sa);
             }
             // stable partition of the joined SA by text; separators sort last
             DBuf<uint64_t> ka(cache, N), kb(cache, N);
             DBuf<uint32_t> va(cache, N), vb(cache, N);
-            k_batch_keys<<<grid(N), 256, 0, stream>>>(sa.get(), N, d_co.get(), cnt, ka.get(), va.get());
+            k_batch_keys<<<grid(N), 256, 0, stream>>>(sa.get(), N, d_co.get(), cnt, t_co.get(), ka.get(), va.get());
             sa.reset();
             uint64_t* x = ka.get();
             uint64_t* x2 = kb.get();

@@ -1148,13 +1148,37 @@ __device__ __forceinline__ uint32_t find_owner(const uint64_t* __restrict__ star
     return lo;
 }
 
+// Owner lookup table of a text space [base, base + len) cut into chunks of 2^shift positions:
+// table[c] = owner of the chunk's first position (c <= len >> shift).  With texts of a kilobyte and more a
+// chunk meets one or two texts, so a lookup searches a range of one or two starts instead of all of them.
+constexpr uint32_t OWNER_SHIFT = 10;
+__global__ __launch_bounds__(256) void k_owner_table(const uint64_t* __restrict__ starts, uint32_t count,
+                                                     uint64_t base, uint64_t chunks, uint32_t* __restrict__ table) {
+    for (uint64_t c = blockIdx.x * 256ull + threadIdx.x; c <= chunks; c += (uint64_t)gridDim.x * 256)
+        table[c] = find_owner(starts, count, base + (c << OWNER_SHIFT));
+}
+__device__ __forceinline__ uint32_t find_owner_tab(const uint64_t* __restrict__ starts, uint32_t count,
+                                                   const uint32_t* __restrict__ table, uint64_t base, uint64_t pos) {
+    const uint64_t c = (pos - base) >> OWNER_SHIFT;
+    uint32_t lo = table[c], hi = table[c + 1] + 1;  // the owner lies in [owner(chunk), owner(next chunk)]
+    if (hi > count) hi = count;
+    while (hi - lo > 1) {
+        const uint32_t mid = (lo + hi) >> 1;
+        if (starts[mid] <= pos)
+            lo = mid;
+        else
+            hi = mid;
+    }
+    return lo;
+}
+
 template <class OutT>
 __global__ __launch_bounds__(256) void k_batch_text(const uint8_t* __restrict__ texts, uint64_t begin, uint64_t end,
                                                     const uint64_t* __restrict__ in_off, const uint64_t* __restrict__ co,
                                                     uint32_t count, const uint16_t* __restrict__ lut,
-                                                    OutT* __restrict__ out) {
+                                                    const uint32_t* __restrict__ table, OutT* __restrict__ out) {
     for (uint64_t i = begin + blockIdx.x * 256ull + threadIdx.x; i < end; i += (uint64_t)gridDim.x * 256) {
-        uint32_t k = find_owner(in_off, count, i);
+        uint32_t k = find_owner_tab(in_off, count, table, begin, i);
         out[co[k] + (i - in_off[k])] = (OutT)(lut[texts[i]] + 1u);
     }
 }
@@ -1168,10 +1192,11 @@ __global__ __launch_bounds__(256) void k_batch_seps(const uint64_t* __restrict__
 // (text, local position) of every suffix of the joined text in SA order; separators go to bucket `count`
 __global__ __launch_bounds__(256) void k_batch_keys(const uint32_t* __restrict__ sa, uint32_t N,
                                                     const uint64_t* __restrict__ co, uint32_t count,
-                                                    uint64_t* __restrict__ keys, uint32_t* __restrict__ vals) {
+                                                    const uint32_t* __restrict__ table, uint64_t* __restrict__ keys,
+                                                    uint32_t* __restrict__ vals) {
     for (uint32_t j = blockIdx.x * 256 + threadIdx.x; j < N; j += gridDim.x * 256) {
         uint32_t p = sa[j];
-        uint32_t k = find_owner(co, count, p);
+        uint32_t k = find_owner_tab(co, count, table, 0, p);
         uint32_t local = (uint32_t)(p - co[k]);
         bool sep = (uint64_t)p == co[k + 1] - 1;
         keys[j] = sep ? count : k;

@@ -25,12 +25,22 @@ def test_results_equal_alone_and_in_the_mix(engine):
     # warmed engines (throw-away builds of every size class up to 256 KiB first) serve the same results
     srv = SAServer(workers=2, hbm_budget=8 * 10 ** 9, X=39, warm_chars=1 << 18)
     try:
-        res = run_open_loop(srv, dev, poisson_arrivals(len(dev), 2000.0, 5))
+        # the first half arrives while the workers are held off (the queue lock is held), so it is batched
+        # for certain; the rest arrives as an open Poisson stream
+        half = len(dev) // 2
+        with srv.cv:
+            first = [srv.submit(t) for t in dev[:half]]
+        res = run_open_loop(srv, dev[half:], poisson_arrivals(len(dev) - half, 2000.0, 5))
+        for r in first:
+            r.event.wait()
+        batches = srv.batches
     finally:
         srv.close()
-    assert res["completed"] == len(texts)
-    assert res["batches"] < len(texts)  # requests were batched together
-    for t, r in zip(texts, res["requests_list"]):
+    reqs = first + res["requests_list"]
+    assert all(r.error == 0 for r in reqs) and len(reqs) == len(texts)
+    assert batches < len(texts)  # requests were batched together
+    assert len({r.batch for r in first}) < half
+    for t, r in zip(texts, reqs):
         alone = engine.build_sa(t, sa_width=4, X=39)
         got = r.sa.cpu().numpy().view(np.uint32)
         assert np.array_equal(got, alone), r.rid
