@@ -29,7 +29,7 @@ struct Options {
 };
 
 #ifndef DCX_ROUND0_SLACK
-#define DCX_ROUND0_SLACK 8
+#define DCX_ROUND0_SLACK 5
 #endif
 constexpr uint32_t ROUND0_SLACK = DCX_ROUND0_SLACK;  // key bits beyond log2(N) in round 0 of large sorts
 #ifndef DCX_PACKED_SLACK
