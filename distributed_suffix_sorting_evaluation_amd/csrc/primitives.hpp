@@ -42,6 +42,8 @@ constexpr int RS_RADIX = 256;
 #define DCX_RS_LOOKBACK 8
 #endif
 constexpr int RS_LOOKBACK = DCX_RS_LOOKBACK;  // predecessors read per lookback round trip (8 measured best of 8/16/24)
+// keys per batch of the ranking atomics (7: two batches of a 14-key tile fit 128 VGPRs; 14 does not)
+constexpr int RS_RANK_BATCH = 7;
 
 // lookback word: [63:42] tag  [41:40] flag  [39:0] count
 constexpr uint64_t LB_VALUE_MASK = (1ull << 40) - 1;
@@ -381,26 +383,37 @@ __global__ __launch_bounds__(RS_THREADS) void k_onesweep(Src src, OutT* __restri
         key[k] = valid ? src(idx) : 0ull;
         if (VALS) val[k] = IDENT ? (uint32_t)idx : (valid ? vin[idx] : 0u);
     }
-    // rank every key among the keys of its wave with the same digit, in tile order
+    // rank every key among the keys of its wave with the same digit, in tile order: the lowest lane of
+    // each digit adds the digit's lane count to the wave's counter (LDS atomics of one wave apply in
+    // order, so the returned count is the number of earlier keys of that digit), the others read it from
+    // that lane; RS_RANK_BATCH keys per batch keep the atomics in flight together
 #pragma unroll
-    for (int k = 0; k < ITEMS; k++) {
-        uint64_t idx = wbase + k * 64 + lane;
-        bool valid = idx < tile_end;
-        uint32_t d = (uint32_t)(key[k] >> shift) & 0xFFu;
-        uint64_t peers = __ballot(valid);
+    for (int k0 = 0; k0 < ITEMS; k0 += RS_RANK_BATCH) {
+        uint32_t old[RS_RANK_BATCH];
 #pragma unroll
-        for (int b = 0; b < 8; b++) {
-            uint32_t bit = (d >> b) & 1u;
-            uint64_t bb = __ballot(bit);
-            peers &= bit ? bb : ~bb;
+        for (int k = k0; k < k0 + RS_RANK_BATCH && k < ITEMS; k++) {
+            uint64_t idx = wbase + k * 64 + lane;
+            bool valid = idx < tile_end;
+            uint32_t d = (uint32_t)(key[k] >> shift) & 0xFFu;
+            uint64_t peers = __ballot(valid);
+#pragma unroll
+            for (int b = 0; b < 8; b++) {
+                uint32_t bit = (d >> b) & 1u;
+                uint64_t bb = __ballot(bit);
+                peers &= bit ? bb : ~bb;
+            }
+            uint32_t below = (uint32_t)__popcll(peers & lanemask_lt());
+            uint32_t lead = peers ? (uint32_t)__builtin_ctzll(peers) : lane;
+            old[k - k0] = 0;
+            if (valid && below == 0) old[k - k0] = atomicAdd(&s_wcnt[wave][d], (uint32_t)__popcll(peers));
+            rk[k] = d | (below << 8) | (lead << 16);
         }
-        uint32_t below = (uint32_t)__popcll(peers & lanemask_lt());
-        uint32_t cnt = 0;
-        if (valid) cnt = s_wcnt[wave][d];
-        __builtin_amdgcn_wave_barrier();
-        if (valid && below == 0) s_wcnt[wave][d] = cnt + (uint32_t)__popcll(peers);
-        __builtin_amdgcn_wave_barrier();
-        rk[k] = d | ((cnt + below) << 8);
+#pragma unroll
+        for (int k = k0; k < k0 + RS_RANK_BATCH && k < ITEMS; k++) {
+            const uint32_t lead = rk[k] >> 16;
+            const uint32_t cnt = (uint32_t)__builtin_amdgcn_ds_bpermute((int)(lead << 2), (int)old[k - k0]);
+            rk[k] = (rk[k] & 0xFFu) | ((cnt + ((rk[k] >> 8) & 0xFFu)) << 8);
+        }
     }
     __syncthreads();
     // per digit (threads 0..255): offsets of the waves and the tile total
@@ -626,7 +639,7 @@ __global__ __launch_bounds__(SC_THREADS) void k_scan(Op op, uint64_t N, uint64_t
 #ifndef DCX_SC_BLOCKS
 #define DCX_SC_BLOCKS 2048
 #endif
-constexpr uint32_t SC_BLOCKS = DCX_SC_BLOCKS;  // blocks of the two-pass scan
+constexpr uint32_t SC_BLOCKS = DCX_SC_BLOCKS;  // most blocks of the two-pass scan
 #ifndef DCX_SC_SINGLE_PASS_TILES
 #define DCX_SC_SINGLE_PASS_TILES 256
 #endif
@@ -869,6 +882,7 @@ struct Primitives {
     uint32_t* seg_info = nullptr;
     uint32_t* seg_tick = nullptr;
     uint64_t* host_scalars = nullptr;      // pinned mirror
+    uint32_t cus = 1;                      // compute units of the device
     // optional per-pass timing of the radix passes
     bool profile = false;
     std::vector<std::pair<hipEvent_t, hipEvent_t>> pass_events;
@@ -904,6 +918,10 @@ struct Primitives {
     void init(hipStream_t s, DeviceCache* c) {
         stream = s;
         cache = c;
+        int dev = 0, ncu = 0;
+        DCX_HIP(hipGetDevice(&dev));
+        DCX_HIP(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
+        cus = ncu > 0 ? (uint32_t)ncu : 1u;
         DCX_HIP(hipMalloc(&ticket, sizeof(unsigned long long)));
         DCX_HIP(hipMemsetAsync(ticket, 0, sizeof(unsigned long long), stream));
         DCX_HIP(hipMalloc(&scalars, 64 * sizeof(uint64_t)));
@@ -1163,6 +1181,24 @@ struct Primitives {
     }
 
     // Launches a scan; the grand total lands in device scalar slot `slot` (read with read_scalars).
+    // Blocks of the two-pass scan of Op. Every block scans the same number of tiles, so a round of
+    // resident blocks that is only partly filled takes as long as a full one: the count is a whole number
+    // of rounds of both kernels (the reduce kernel usually holds more blocks per compute unit than the
+    // down kernel; m = d * floor(r / d) blocks per unit is one round of the first, m / d of the second).
+    template <class Op>
+    uint32_t scan_blocks(uint64_t tiles) const {
+        static const uint32_t per_cu = [] {
+            int r = 0, d = 0;
+            DCX_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&r, k_scan_reduce<Op>, SC_THREADS, 0));
+            DCX_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&d, k_scan_down<Op>, SC_THREADS, 0));
+            r = std::max(r, 1);
+            d = std::max(d, 1);
+            return (uint32_t)(r >= d ? d * (r / d) : r * (d / r));
+        }();
+        const uint64_t want = std::min<uint64_t>(SC_BLOCKS, (uint64_t)per_cu * cus);
+        return (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>(tiles, want));
+    }
+
     template <class Op>
     void scan(const Op& op, uint64_t N, int slot) {
         if (N == 0) {
@@ -1171,7 +1207,7 @@ struct Primitives {
         }
         uint64_t tiles = ceil_div(N, SC_TILE);
         if (tiles > SC_SINGLE_PASS_TILES) {
-            const uint32_t tpb = (uint32_t)ceil_div(tiles, (uint64_t)SC_BLOCKS);
+            const uint32_t tpb = (uint32_t)ceil_div(tiles, (uint64_t)scan_blocks<Op>(tiles));
             const uint32_t nb = (uint32_t)ceil_div(tiles, (uint64_t)tpb);
             hipLaunchKernelGGL(k_scan_reduce<Op>, dim3(nb), dim3(SC_THREADS), 0, stream, op, N, tpb, scan_part);
             hipLaunchKernelGGL(k_scan_spine, dim3(1), dim3(1024), 0, stream, (const uint64_t*)scan_part, nb,
@@ -1200,7 +1236,7 @@ struct Primitives {
             scan(op, N, slot);
             return;
         }
-        const uint32_t tpb = (uint32_t)ceil_div(tiles, (uint64_t)SC_BLOCKS);
+        const uint32_t tpb = (uint32_t)ceil_div(tiles, (uint64_t)scan_blocks<Op>(tiles));
         const uint32_t nb = (uint32_t)ceil_div(tiles, (uint64_t)tpb);
         hipLaunchKernelGGL(k_scan_reduce<Op>, dim3(nb), dim3(SC_THREADS), 0, stream, op, N, tpb, scan_part);
         hipLaunchKernelGGL(k_scan_spine, dim3(1), dim3(1024), 0, stream, (const uint64_t*)scan_part, nb, scan_prefix,
