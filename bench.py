@@ -275,7 +275,7 @@ def main():
     roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": round(achieved / HBM_PEAK_GBS, 4),
                 "traffic": traffic.get("bytes_per_launch") if traffic else None,
-                "kernel": "k_onesweep (LSD radix pass, u64 key + u32 value)",
+                "kernel": "k_onesweep (LSD radix pass over 64-bit keys: u32 values, or split indices with byte values)",
                 "launches": passes, "avg_launch_ms": round(avg_ms, 4), "avg_pairs_per_launch": int(avg_items),
                 "alg_bytes_per_launch": int(avg_bytes)}
 

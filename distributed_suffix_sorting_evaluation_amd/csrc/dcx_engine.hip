@@ -200,8 +200,13 @@ class Engine {
             // every position is an element: the first pass reads the character stream directly; the last
             // pass writes only the 32 key bits below its digit, enough for the heads
             DBuf<uint32_t> low(cache, N);
+            DBuf<uint8_t> ia, ib;  // keys of at most 40 bits leave room for the low 24 bits of a position
+            if (c0 * bits <= 40) {
+                ia = DBuf<uint8_t>(cache, N);
+                ib = DBuf<uint8_t>(cache, N);
+            }
             const uint32_t np = prim.sort_pairs_from(StreamKeys<false>{PT, bits, top_mask(c0 * bits)}, k, k2, v, v2, N,
-                                                     64 - c0 * bits, 64, true, low.get());
+                                                     64 - c0 * bits, 64, true, low.get(), ia.get(), ib.get());
             H = DBuf<uint8_t>(cache, (size_t)N + 1);
             if (np) {
                 k_heads_u32<<<grid(N / 16 + 1ull), 256, 0, stream>>>(low.get(), N, H.get());

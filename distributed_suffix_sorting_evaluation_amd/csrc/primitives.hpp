@@ -320,15 +320,23 @@ __global__ __launch_bounds__(256) void k_seg_setup(const uint32_t* __restrict__ 
 // partitions by destination index).  IDENT: values are the input indices.
 // OutT (when not KeyT): the pass writes OutT(key >> out_shift) instead of the key (the last pass of a sort
 // whose remaining key bits fit 32 bits below its digit).
-template <bool VALS, bool IDENT, class Src, class KeyT = uint64_t, int ITEMS = RS_ITEMS, class OutT = KeyT>
+// Byte values (VinT or VoutT uint8_t, 64-bit keys whose digits lie at or above bit kpos): an index is
+// carried as its low kpos bits in the key's free low bits and its high bits in a byte, 9 bytes per
+// element instead of 12.  An IDENT pass splits the index; a pass with VoutT = uint32_t joins it again.
+template <bool VALS, bool IDENT, class Src, class KeyT = uint64_t, int ITEMS = RS_ITEMS, class OutT = KeyT,
+          class VinT = uint32_t, class VoutT = VinT>
 __global__ __launch_bounds__(RS_THREADS) void k_onesweep(Src src, OutT* __restrict__ kout,
-                                                         const uint32_t* __restrict__ vin, uint32_t* __restrict__ vout,
+                                                         const VinT* __restrict__ vin, VoutT* __restrict__ vout,
                                                          uint32_t shift, const uint32_t* __restrict__ segoff,
                                                          const uint32_t* __restrict__ info, uint32_t* __restrict__ tick,
                                                          uint64_t* __restrict__ status, uint32_t tag,
-                                                         uint32_t out_shift = 0) {
+                                                         uint32_t out_shift = 0, uint32_t kpos = 0) {
     constexpr int TILE = RS_THREADS * ITEMS;
+    constexpr bool SPLIT = VALS && (sizeof(VinT) == 1 || sizeof(VoutT) == 1);
+    static_assert(!SPLIT || sizeof(KeyT) == 8, "split indices need 64-bit keys");
+    const uint64_t kmask = (1ull << kpos) - 1;  // kpos <= 24
     __shared__ KeyT s_keys[TILE];
+    __shared__ uint8_t s_vb[SPLIT ? TILE : 1];  // split indices: the high bytes, reordered with the keys
     __shared__ uint32_t s_wcnt[RS_WAVES][RS_RADIX];  // per-wave digit counts, then wave offsets
     __shared__ uint32_t s_start[RS_RADIX];
     __shared__ uint32_t s_gofs[RS_RADIX];
@@ -381,7 +389,13 @@ __global__ __launch_bounds__(RS_THREADS) void k_onesweep(Src src, OutT* __restri
         uint64_t idx = wbase + k * 64 + lane;
         bool valid = idx < tile_end;
         key[k] = valid ? src(idx) : 0ull;
-        if (VALS) val[k] = IDENT ? (uint32_t)idx : (valid ? vin[idx] : 0u);
+        if constexpr (SPLIT && IDENT) {
+            const uint32_t i32 = (uint32_t)idx;  // indices of a sort stay below 2^32
+            key[k] |= i32 & (uint32_t)kmask;
+            val[k] = i32 >> kpos;
+        } else if constexpr (VALS) {
+            val[k] = IDENT ? (uint32_t)idx : (valid ? (uint32_t)vin[idx] : 0u);
+        }
     }
     // rank every key among the keys of its wave with the same digit, in tile order: the lowest lane of
     // each digit adds the digit's lane count to the wave's counter (LDS atomics of one wave apply in
@@ -444,6 +458,7 @@ __global__ __launch_bounds__(RS_THREADS) void k_onesweep(Src src, OutT* __restri
         if (idx < tile_end) {
             uint32_t lp = s_start[dk] + s_wcnt[wave][dk] + (rk[k] >> 8);
             s_keys[lp] = (KeyT)key[k];
+            if constexpr (SPLIT) s_vb[lp] = (uint8_t)val[k];
         }
     }
     if (dthr) {
@@ -490,10 +505,17 @@ __global__ __launch_bounds__(RS_THREADS) void k_onesweep(Src src, OutT* __restri
         if (i < tile_n) {
             const KeyT kk = s_keys[i];
             dg[r] = (uint32_t)((uint64_t)kk >> shift) & 0xFFu;
-            kout[s_gofs[dg[r]] + i] = (OutT)((uint64_t)kk >> out_shift);
+            const uint32_t o = s_gofs[dg[r]] + i;
+            kout[o] = (OutT)((uint64_t)kk >> out_shift);
+            if constexpr (SPLIT) {
+                if constexpr (sizeof(VoutT) == 1)
+                    vout[o] = s_vb[i];
+                else
+                    vout[o] = (VoutT)(((uint64_t)s_vb[i] << kpos) | ((uint64_t)kk & kmask));
+            }
         }
     }
-    if (VALS) {
+    if constexpr (VALS && !SPLIT) {
         __syncthreads();
         uint32_t* s_vals = reinterpret_cast<uint32_t*>(s_keys);
 #pragma unroll
@@ -1062,10 +1084,14 @@ struct Primitives {
     // of that pass).  Two sorted keys are equal iff they lie in the same run of the last digit (the runs
     // start at the last pass's digit offsets, hist + (np - 1) * 256) and their low32 words are equal.
     // Returns the number of passes when low32 was written, else 0.
+    //
+    // idx_bytes (two arrays of N bytes, optional, with identity values and low32): when an index fits
+    // the key bits below the digits plus one byte, the passes carry it split (low bits in the key, high
+    // byte in idx_bytes: 9 instead of 12 bytes per element); the last pass joins it into vals.
     template <class Src>
     uint32_t sort_pairs_from(const Src& first, uint64_t*& keys, uint64_t*& keys_alt, uint32_t*& vals,
                          uint32_t*& vals_alt, uint32_t N, uint32_t lo_bit, uint32_t hi_bit, bool ident_vals,
-                         uint32_t* low32 = nullptr) {
+                         uint32_t* low32 = nullptr, uint8_t* idx_bytes = nullptr, uint8_t* idx_bytes_alt = nullptr) {
         if (hi_bit <= lo_bit) hi_bit = lo_bit + 1;
         if (N <= 1) {
             // nothing to order: materialize the inputs in place
@@ -1077,6 +1103,8 @@ struct Primitives {
         if (np > 8) np = 8;
         if (lo_bit + 8 * np > 64) lo_bit = 64 - 8 * np;
         if (np > 5 || np < 2 || !vals) low32 = nullptr;  // the last pass must not be the first
+        const uint32_t kpos = std::min<uint32_t>(lo_bit, 24);
+        const bool split = low32 && ident_vals && idx_bytes && idx_bytes_alt && bit_width(N - 1) <= kpos + 8;
         const uint64_t tiles = seg_plan(first, N, lo_bit, np, RS_TILE);
         for (uint32_t p = 0; p < np; p++) {
             if (sort_tag >= LB_TAG_LIMIT) {
@@ -1095,7 +1123,21 @@ struct Primitives {
             const uint32_t* si = seg_info + p * RS_SEG_INFO;
             uint32_t* tk = seg_tick + p * RS_SEGS;
             const dim3 g((unsigned)tiles), b(RS_THREADS);
-            if (p == 0) {
+            if (split) {
+                const ArrayKeys<uint64_t> in{keys};
+                if (p == 0)
+                    hipLaunchKernelGGL((k_onesweep<true, true, Src, uint64_t, RS_ITEMS, uint64_t, uint8_t, uint8_t>), g, b, 0,
+                                       stream, first, keys_alt, (const uint8_t*)nullptr, idx_bytes_alt, shift, so, si, tk,
+                                       sort_status, tag, 0u, kpos);
+                else if (p < np - 1)
+                    hipLaunchKernelGGL((k_onesweep<true, false, ArrayKeys<uint64_t>, uint64_t, RS_ITEMS, uint64_t, uint8_t, uint8_t>),
+                                       g, b, 0, stream, in, keys_alt, (const uint8_t*)idx_bytes, idx_bytes_alt, shift, so, si,
+                                       tk, sort_status, tag, 0u, kpos);
+                else
+                    hipLaunchKernelGGL((k_onesweep<true, false, ArrayKeys<uint64_t>, uint64_t, RS_ITEMS, uint32_t, uint8_t, uint32_t>),
+                                       g, b, 0, stream, in, low32, (const uint8_t*)idx_bytes, vals_alt, shift, so, si, tk,
+                                       sort_status, tag, lo_bit, kpos);
+            } else if (p == 0) {
                 if (!vals)
                     hipLaunchKernelGGL((k_onesweep<false, false, Src>), g, b, 0, stream, first, keys_alt,
                                        (const uint32_t*)nullptr, (uint32_t*)nullptr, shift, so, si, tk, sort_status, tag);
@@ -1126,11 +1168,19 @@ struct Primitives {
                 // (not for identity values) and out
                 const bool out32 = low32 && p == np - 1;
                 uint64_t b = (p == 0 ? first.read_bytes(N) : 8ull * N) + (out32 ? 4ull : 8ull) * N;
-                if (vals) b += (p == 0 && ident_vals ? 4ull : 8ull) * N;
+                if (split)  // index bytes in (not in the first pass) and out (the last pass: 4-byte indices)
+                    b += (p == 0 ? 0ull : 1ull) * N + (p == np - 1 ? 4ull : 1ull) * N;
+                else if (vals)
+                    b += (p == 0 && ident_vals ? 4ull : 8ull) * N;
                 pass_bytes.push_back(b);
             }
             if (!(low32 && p == np - 1 && vals)) std::swap(keys, keys_alt);
-            if (vals) std::swap(vals, vals_alt);
+            if (split) {
+                if (p < np - 1) std::swap(idx_bytes, idx_bytes_alt);
+                else std::swap(vals, vals_alt);
+            } else if (vals) {
+                std::swap(vals, vals_alt);
+            }
         }
         return low32 ? np : 0;
     }
