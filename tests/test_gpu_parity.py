@@ -256,6 +256,25 @@ def test_genome_1gib_digest(engine):
     assert hashlib.sha256(sa.tobytes()).hexdigest() == rec["sa_u64_sha256"]
 
 
+def test_genome_max_text_device_check(engine):
+    """The longest text the build takes (4 GiB - 4 MiB) over ACGT: round 0 then sorts 40-bit keys whose
+    positions need all 32 bits (24 in the free key bits, 8 in the byte beside).  The 40-bit SA stays in
+    HBM and is checked there (permutation, suffix order by first character and the rank of the next)."""
+    import torch
+    from distributed_suffix_sorting_evaluation_amd import textgen
+    n = textgen.MAX_TEXT
+    t = textgen.generate("genome", n, 7)
+    say(f"genome max text n={n}")
+    dt = torch.from_numpy(t).to("cuda")
+    del t
+    ds = torch.empty(n * 5, dtype=torch.uint8, device="cuda")
+    st = engine.build_sa_device(dt.data_ptr(), n, ds.data_ptr(), 5, X=39)
+    assert st["levels"][0]["n"] == n
+    assert engine.check_sa_device(dt.data_ptr(), n, ds.data_ptr(), 5)
+    del dt, ds
+    torch.cuda.empty_cache()
+
+
 def test_dense_codes_text_end(engine):
     # alphabets of 2, 4 and 256 symbols make the merge use dense character codes, where the padding
     # reads like the smallest character; texts ending in long runs of that character exercise the
