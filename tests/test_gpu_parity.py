@@ -160,6 +160,31 @@ def test_width5_device_output_tails_and_alignment(engine):
             assert int(buf[:off].sum()) == 0 and int(buf[off + 5 * n:].sum()) == 0
 
 
+def test_device_output_all_widths_two_pass(engine):
+    """A 3 MiB text with copies and a periodic run (small and large tied groups for the merge) built into
+    device outputs of widths 4, 5 and 8, aligned and misaligned, against the oracle; the bytes around
+    the output stay untouched."""
+    import torch
+    n = (3 << 20) + 13
+    t = copies_text(n - 300007, 11)
+    t = np.concatenate([t[: n // 2], periodic_text(300007, 7, 4, 3), t[n // 2:]])[:n].copy()
+    assert len(t) == n
+    ref = oracle.dcx_sa(t, X=39)
+    dt = torch.from_numpy(t).cuda()
+    for width in (4, 5, 8):
+        if width == 5:
+            want = oracle.uint40_bytes(ref)
+        else:
+            want = ref.astype("<u4" if width == 4 else "<u8").tobytes()
+        for off in (0, 3):
+            say(f"device output width={width} off={off}")
+            buf = torch.zeros(width * n + 32, dtype=torch.uint8, device="cuda")
+            engine.build_sa_device(dt.data_ptr(), n, buf.data_ptr() + off, width, X=39)
+            torch.cuda.synchronize()
+            assert buf[off:off + width * n].cpu().numpy().tobytes() == want, (width, off)
+            assert int(buf[:off].sum()) == 0 and int(buf[off + width * n:].sum()) == 0
+
+
 def test_device_checker(engine):
     t = copies_text(100000, 3)
     sa = engine.build_sa(t, sa_width=8, X=39)
