@@ -905,15 +905,17 @@ struct Primitives {
     uint32_t* seg_tick = nullptr;
     uint64_t* host_scalars = nullptr;      // pinned mirror
     uint32_t cus = 1;                      // compute units of the device
-    // optional per-pass timing of the radix passes
+    // optional timing of the 64-bit radix passes: one event pair around the passes of each sort (they run
+    // back to back; an event per pass would put a gap of a few microseconds after every pass)
     bool profile = false;
     std::vector<std::pair<hipEvent_t, hipEvent_t>> pass_events;
-    std::vector<uint32_t> pass_items;
-    std::vector<uint64_t> pass_bytes;  // algorithmic bytes of each recorded pass
+    std::vector<uint32_t> pass_launches;  // passes of each recorded sort
+    std::vector<uint64_t> pass_items;     // elements summed over those passes
+    std::vector<uint64_t> pass_bytes;     // algorithmic bytes summed over those passes
 
-    // sums the recorded pass times (synchronizes) and releases the events
+    // sums the recorded sort times (synchronizes) and releases the events
     void collect_profile(uint64_t* passes, uint64_t* items, uint64_t* bytes, double* ms) {
-        *passes = pass_events.size();
+        *passes = 0;
         *items = 0;
         *bytes = 0;
         *ms = 0;
@@ -922,6 +924,7 @@ struct Primitives {
             DCX_HIP(hipEventSynchronize(pass_events[i].second));
             DCX_HIP(hipEventElapsedTime(&a, pass_events[i].first, pass_events[i].second));
             *ms += a;
+            *passes += pass_launches[i];
             *items += pass_items[i];
             *bytes += pass_bytes[i];
         }
@@ -933,6 +936,7 @@ struct Primitives {
             (void)hipEventDestroy(e.second);
         }
         pass_events.clear();
+        pass_launches.clear();
         pass_items.clear();
         pass_bytes.clear();
     }
@@ -1106,18 +1110,26 @@ struct Primitives {
         const uint32_t kpos = std::min<uint32_t>(lo_bit, 24);
         const bool split = low32 && ident_vals && idx_bytes && idx_bytes_alt && bit_width(N - 1) <= kpos + 8;
         const uint64_t tiles = seg_plan(first, N, lo_bit, np, RS_TILE);
+        uint64_t span_bytes = 0;
+        if (profile) {  // the pair is listed at once, so drop_profile releases it whatever happens next
+            hipEvent_t e0 = nullptr, e1 = nullptr;
+            DCX_HIP(hipEventCreate(&e0));
+            if (hipEventCreate(&e1) != hipSuccess) {
+                (void)hipEventDestroy(e0);
+                throw Error(-4, "hipEventCreate failed");
+            }
+            pass_events.emplace_back(e0, e1);
+            pass_launches.push_back(0);
+            pass_items.push_back(0);
+            pass_bytes.push_back(0);
+            DCX_HIP(hipEventRecord(e0, stream));
+        }
         for (uint32_t p = 0; p < np; p++) {
             if (sort_tag >= LB_TAG_LIMIT) {
                 fill_u64(sort_status, 0, sort_status_words);
                 sort_tag = 0;
             }
             uint32_t tag = ++sort_tag;
-            hipEvent_t e0 = nullptr, e1 = nullptr;
-            if (profile) {
-                DCX_HIP(hipEventCreate(&e0));
-                DCX_HIP(hipEventCreate(&e1));
-                DCX_HIP(hipEventRecord(e0, stream));
-            }
             const uint32_t shift = lo_bit + 8 * p;
             const uint32_t* so = seg_off + p * RS_SEG_BINS;
             const uint32_t* si = seg_info + p * RS_SEG_INFO;
@@ -1161,9 +1173,6 @@ struct Primitives {
                                        (const uint32_t*)vals, vals_alt, shift, so, si, tk, sort_status, tag, 0u);
             }
             if (profile) {
-                DCX_HIP(hipEventRecord(e1, stream));
-                pass_events.emplace_back(e0, e1);
-                pass_items.push_back(N);
                 // keys in (the first pass: its key source) and out (a low32 last pass: 4 B), values in
                 // (not for identity values) and out
                 const bool out32 = low32 && p == np - 1;
@@ -1172,7 +1181,7 @@ struct Primitives {
                     b += (p == 0 ? 0ull : 1ull) * N + (p == np - 1 ? 4ull : 1ull) * N;
                 else if (vals)
                     b += (p == 0 && ident_vals ? 4ull : 8ull) * N;
-                pass_bytes.push_back(b);
+                span_bytes += b;
             }
             if (!(low32 && p == np - 1 && vals)) std::swap(keys, keys_alt);
             if (split) {
@@ -1181,6 +1190,12 @@ struct Primitives {
             } else if (vals) {
                 std::swap(vals, vals_alt);
             }
+        }
+        if (profile) {
+            DCX_HIP(hipEventRecord(pass_events.back().second, stream));
+            pass_launches.back() = np;
+            pass_items.back() = (uint64_t)N * np;
+            pass_bytes.back() = span_bytes;
         }
         return low32 ? np : 0;
     }
