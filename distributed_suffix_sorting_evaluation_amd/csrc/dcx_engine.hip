@@ -38,6 +38,10 @@ constexpr uint32_t ROUND0_SLACK = DCX_ROUND0_SLACK;  // key bits beyond log2(N) 
 // round 0 packs the position into the key when the characters beside it carry this many bits of
 // information beyond log2(N)
 constexpr uint32_t PACKED_SLACK = DCX_PACKED_SLACK;
+// large-group lists from this many members, in groups of this many on average, try the partition around
+// each group's first key (long periodic runs make groups of millions; chance repeats, groups of tens)
+constexpr uint32_t LARGE_PARTITION_MIN = 1u << 16;
+constexpr uint32_t LARGE_PARTITION_AVG = 64;
 
 // Longest text (and longest joined batch, plus one separator) of this build: positions, counts and the
 // grid-stride loops of the kernels are 32-bit, and a loop of at most 8192 x 256 threads stays clear of
@@ -314,11 +318,12 @@ class Engine {
         k_refine_small<<<grid(G), 256, 0, stream>>>(cur.gstart.get(), G, cur.gfirst.get(), order.get(), H.get(),
                                                     Ha.get(), PT, bits, C, c * bits, large.get());
         prim.fill_u8(Ha.get() + A, 1, 1);
-        DBuf<uint32_t> slots(cache, A), lrank(cache, A);
-        prim.scan(LargeMembersOp{large.get(), cur.gid.get(), slots.get(), lrank.get()}, A, 1);
+        DBuf<uint32_t> slots(cache, A), lrank(cache, A), lfirst(cache, (size_t)G + 1);
+        prim.scan(LargeMembersOp{large.get(), cur.gid.get(), slots.get(), lrank.get(), lfirst.get()}, A, 1);
         const uint64_t tot = prim.read_scalars(2)[1];
         const uint32_t B = (uint32_t)tot, nlarge = (uint32_t)(tot >> 32);
         if (B == 0) return;
+        prim.fill_u32(lfirst.get() + nlarge, B, 1);
         const uint32_t lbits = bit_width(nlarge - 1);
         const uint32_t cbits = c * bits;
         DBuf<uint64_t> rka(cache, B), rkb(cache, B);
@@ -328,8 +333,14 @@ class Engine {
         uint32_t* y = rva.get();
         uint32_t* y2 = rvb.get();
         if (lbits + cbits <= 64) {
+            const bool part = B >= LARGE_PARTITION_MIN && B >= (uint64_t)LARGE_PARTITION_AVG * nlarge;
+            DBuf<uint8_t> cls(cache, part ? B : 0);
             k_keys_large<<<grid(B), 256, 0, stream>>>(PT, bits, B, slots.get(), cur.act.get(), lrank.get(),
-                                                      order.get(), C, cbits, lbits, x, y);
+                                                      order.get(), C, cbits, lbits, x, y, lfirst.get(),
+                                                      part ? cls.get() : nullptr);
+            if (part && partition_large(cur, slots.get(), lrank.get(), lfirst.get(), cls.get(), B, 64 - lbits - cbits,
+                                        x, y, x2, y2, order, H, Ha))
+                return;
             prim.sort_pairs(x, x2, y, y2, B, 64 - lbits - cbits, 64);
             k_writeback_large<<<grid(B), 256, 0, stream>>>(x, y, slots.get(), B, cur.act.get(), order.get(), H.get(),
                                                            Ha.get());
@@ -345,6 +356,38 @@ class Engine {
         prim.sort_pairs(x, x2, y, y2, B, 0, lbits);
         k_writeback_large2<<<grid(B), 256, 0, stream>>>(x, y, P.get(), PT, bits, C, cbits, slots.get(), B,
                                                         cur.act.get(), order.get(), H.get(), Ha.get());
+    }
+
+    // Large groups by a stable three-way partition around each group's first key (PivotScanOp), when at
+    // most half of their members differ from it: only that minority is sorted.  x, y: the members' keys
+    // and suffixes; x2, y2 (B entries each) take the compacted minority.  Returns false (nothing written,
+    // x and y untouched) when the caller should sort all members instead.
+    bool partition_large(const ActiveSet& cur, const uint32_t* slots, const uint32_t* lrank, const uint32_t* lfirst,
+                         const uint8_t* cls, uint32_t B, uint32_t lo_bit, const uint64_t* x, const uint32_t* y,
+                         uint64_t* x2, uint32_t* y2, DBuf<uint32_t>& order, DBuf<uint8_t>& H, DBuf<uint8_t>& Ha) {
+        DBuf<uint32_t> minpref(cache, (size_t)B + 1), c0pref(cache, (size_t)B + 1);
+        prim.scan(PivotScanOp{cls, x, minpref.get(), c0pref.get(), x2, y2}, B, 1);
+        const uint64_t tot = prim.read_scalars(2)[1];
+        const uint32_t Bm = (uint32_t)tot;
+        if ((uint64_t)Bm * 2 > B) return false;
+        if (Bm == 0) {
+            k_uniform_heads<<<grid(B), 256, 0, stream>>>(B, slots, lrank, lfirst, Ha.get());
+            return true;
+        }
+        prim.fill_u32(minpref.get() + B, Bm, 1);
+        prim.fill_u32(c0pref.get() + B, (uint32_t)(tot >> 32), 1);
+        DBuf<uint64_t> mk2(cache, Bm);
+        DBuf<uint32_t> mi2(cache, Bm);
+        uint64_t* mk = x2;
+        uint64_t* mka = mk2.get();
+        uint32_t* mi = y2;
+        uint32_t* mia = mi2.get();
+        prim.sort_pairs(mk, mka, mi, mia, Bm, lo_bit, 64);
+        k_place_equal<<<grid(B), 256, 0, stream>>>(B, y, slots, lrank, lfirst, minpref.get(), c0pref.get(),
+                                                   cur.act.get(), order.get(), H.get(), Ha.get());
+        k_place_minority<<<grid(Bm), 256, 0, stream>>>(Bm, mk, mi, y, slots, lrank, lfirst, minpref.get(), c0pref.get(),
+                                                       cur.act.get(), order.get(), H.get(), Ha.get());
+        return true;
     }
 
     // dense words only: boundary suffixes that share a key group are moved to its front (k_boundary_fix)

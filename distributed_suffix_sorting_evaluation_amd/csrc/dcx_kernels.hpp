@@ -762,6 +762,7 @@ struct LargeMembersOp {
     const uint32_t* gid;
     uint32_t* slots;
     uint32_t* lrank;
+    uint32_t* lfirst;  // per large group: its first member in the list
     __device__ __forceinline__ uint64_t value(uint64_t a) const {
         const uint32_t g = gid[a];
         const bool first = a == 0 || gid[a - 1] != g;
@@ -769,27 +770,37 @@ struct LargeMembersOp {
     }
     __device__ __forceinline__ void write(uint64_t a, uint64_t excl, uint64_t v) const {
         if (v) {
+            const uint32_t r = (uint32_t)(excl >> 32) + (uint32_t)(v >> 32) - 1u;
             slots[(uint32_t)excl] = (uint32_t)a;
-            lrank[(uint32_t)excl] = (uint32_t)(excl >> 32) + (uint32_t)(v >> 32) - 1u;
+            lrank[(uint32_t)excl] = r;
+            if (v >> 32) lfirst[r] = (uint32_t)excl;
         }
     }
 };
 
 // radix keys of the large-group members: (large-group rank << (64 - lbits)) | next cbits bits of characters
-// (lbits + cbits <= 64); values are the suffixes
+// (lbits + cbits <= 64); values are the suffixes.  cls (optional): the member's key against its group's
+// first member's, 0 smaller, 1 equal, 2 larger (that key recomputed: one stream window per group, cached)
 __global__ __launch_bounds__(256) void k_keys_large(const uint64_t* __restrict__ PT, uint32_t bits, uint32_t B,
                                                     const uint32_t* __restrict__ slots,
                                                     const uint32_t* __restrict__ act_idx,
                                                     const uint32_t* __restrict__ lrank,
                                                     const uint32_t* __restrict__ order, uint32_t C, uint32_t cbits,
                                                     uint32_t lbits, uint64_t* __restrict__ keys,
-                                                    uint32_t* __restrict__ vals) {
+                                                    uint32_t* __restrict__ vals, const uint32_t* __restrict__ lfirst,
+                                                    uint8_t* __restrict__ cls) {
     const StreamKeys<false> chars{PT, bits, top_mask(cbits)};
     for (uint32_t b = blockIdx.x * 256 + threadIdx.x; b < B; b += gridDim.x * 256) {
         const uint32_t p = order[act_idx[slots[b]]];
         const uint64_t ch = chars((uint64_t)p + C);
         keys[b] = lbits ? (((uint64_t)lrank[b] << (64 - lbits)) | (ch >> lbits)) : ch;
         vals[b] = p;
+        if (cls) {
+            const uint32_t fb = lfirst[lrank[b]];
+            const uint64_t cf = chars((uint64_t)order[act_idx[slots[fb]]] + C);
+            const uint64_t x = ch >> lbits, pv = cf >> lbits;  // the same group rank above both
+            cls[b] = x < pv ? 0 : (x == pv ? 1 : 2);
+        }
     }
 }
 
@@ -809,6 +820,106 @@ __global__ __launch_bounds__(256) void k_writeback_large(const uint64_t* __restr
         Ha[a] = head ? 1 : 0;
         if (head) H[j] = 1;
     }
+}
+
+// Large groups whose members mostly share one key (a periodic run: all members but the few near the
+// run's end have the same next characters) are ordered by a stable three-way partition around the key of
+// their first member instead of a sort of all members: the members with another key (the minority) are
+// compacted and sorted, the members equal to it keep their list order between the smaller and the larger
+// ones.  The result is that of the stable sort.  Members b of the large list are consecutive per group;
+// fb, fe are the group's first and end member (lfirst by large-group rank).
+struct LargeGroupSpan {
+    uint32_t fb, fe;
+};
+__device__ __forceinline__ LargeGroupSpan large_span(uint32_t b, const uint32_t* __restrict__ lrank,
+                                                     const uint32_t* __restrict__ lfirst) {
+    const uint32_t r = lrank[b];
+    return {lfirst[r], lfirst[r + 1]};  // lfirst holds B after the last group
+}
+
+// per member (its class from k_keys_large): minority (another key than the group's first) in the low
+// half, smaller in the high half; writes both exclusive prefixes of every member and compacts the
+// minority (key, member)
+struct PivotScanOp {
+    static constexpr bool kBlocked = true;
+    const uint8_t* cls;
+    const uint64_t* keys;
+    uint32_t* minpref;
+    uint32_t* c0pref;
+    uint64_t* mkeys;
+    uint32_t* midx;
+    __device__ __forceinline__ uint64_t value(uint64_t b) const {
+        const uint32_t c = cls[b];
+        return (uint64_t)(c != 1) | ((uint64_t)(c == 0) << 32);
+    }
+    __device__ __forceinline__ void write(uint64_t b, uint64_t excl, uint64_t v) const {
+        minpref[b] = (uint32_t)excl;
+        c0pref[b] = (uint32_t)(excl >> 32);
+        if (v & 1u) {
+            mkeys[(uint32_t)excl] = keys[b];
+            midx[(uint32_t)excl] = (uint32_t)b;
+        }
+    }
+};
+
+// member value `pos` to member slot `dest` of the large list, with its head flag
+__device__ __forceinline__ void large_put(uint32_t dest, uint32_t pos, bool head, const uint32_t* __restrict__ slots,
+                                          const uint32_t* __restrict__ act_idx, uint32_t* __restrict__ order,
+                                          uint8_t* __restrict__ H, uint8_t* __restrict__ Ha) {
+    const uint32_t ad = slots[dest];
+    const uint32_t j = act_idx[ad];
+    order[j] = pos;
+    Ha[ad] = head ? 1 : 0;
+    if (head) H[j] = 1;
+}
+
+// the members equal to their group's first key: after the group's smaller minority, in list order
+__global__ __launch_bounds__(256) void k_place_equal(uint32_t B, const uint32_t* __restrict__ vals,
+                                                     const uint32_t* __restrict__ slots,
+                                                     const uint32_t* __restrict__ lrank,
+                                                     const uint32_t* __restrict__ lfirst,
+                                                     const uint32_t* __restrict__ minpref,
+                                                     const uint32_t* __restrict__ c0pref,
+                                                     const uint32_t* __restrict__ act_idx, uint32_t* __restrict__ order,
+                                                     uint8_t* __restrict__ H, uint8_t* __restrict__ Ha) {
+    for (uint32_t b = blockIdx.x * 256 + threadIdx.x; b < B; b += gridDim.x * 256) {
+        if (minpref[b + 1] != minpref[b]) continue;  // minority
+        const LargeGroupSpan sp = large_span(b, lrank, lfirst);
+        const uint32_t n0 = c0pref[sp.fe] - c0pref[sp.fb];
+        const uint32_t r1 = (b - sp.fb) - (minpref[b] - minpref[sp.fb]);  // rank among the equal members
+        large_put(sp.fb + n0 + r1, vals[b], r1 == 0, slots, act_idx, order, H, Ha);
+    }
+}
+
+// the sorted minority: the smaller members first in their group, the larger ones after the equal ones
+__global__ __launch_bounds__(256) void k_place_minority(uint32_t Bm, const uint64_t* __restrict__ mkeys,
+                                                        const uint32_t* __restrict__ midx,
+                                                        const uint32_t* __restrict__ vals,
+                                                        const uint32_t* __restrict__ slots,
+                                                        const uint32_t* __restrict__ lrank,
+                                                        const uint32_t* __restrict__ lfirst,
+                                                        const uint32_t* __restrict__ minpref,
+                                                        const uint32_t* __restrict__ c0pref,
+                                                        const uint32_t* __restrict__ act_idx, uint32_t* __restrict__ order,
+                                                        uint8_t* __restrict__ H, uint8_t* __restrict__ Ha) {
+    for (uint32_t k = blockIdx.x * 256 + threadIdx.x; k < Bm; k += gridDim.x * 256) {
+        const uint32_t b = midx[k];
+        const LargeGroupSpan sp = large_span(b, lrank, lfirst);
+        const uint32_t n0 = c0pref[sp.fe] - c0pref[sp.fb];
+        const uint32_t nmin = minpref[sp.fe] - minpref[sp.fb];
+        const uint32_t local = k - minpref[sp.fb];  // the group's minority starts there in the sorted list
+        const uint32_t dest = local < n0 ? sp.fb + local : sp.fb + (sp.fe - sp.fb - nmin) + local;
+        const bool head = local == 0 || local == n0 || mkeys[k] != mkeys[k - 1];
+        large_put(dest, vals[b], head, slots, act_idx, order, H, Ha);
+    }
+}
+
+// groups whose members all share the first member's key: only the compact heads change
+__global__ __launch_bounds__(256) void k_uniform_heads(uint32_t B, const uint32_t* __restrict__ slots,
+                                                       const uint32_t* __restrict__ lrank,
+                                                       const uint32_t* __restrict__ lfirst, uint8_t* __restrict__ Ha) {
+    for (uint32_t b = blockIdx.x * 256 + threadIdx.x; b < B; b += gridDim.x * 256)
+        Ha[slots[b]] = b == lfirst[lrank[b]] ? 1 : 0;
 }
 
 // The same when rank and characters do not fit one word: an LSD sort by the characters (values: member

@@ -300,6 +300,21 @@ def test_genome_max_text_device_check(engine):
     torch.cuda.empty_cache()
 
 
+def test_large_groups_partition(engine):
+    """Large tied groups whose members share most continuations: a period-13 run (its groups split off only
+    the members near the run's end in each round) and 30,000 copies of one 60-character string behind
+    random 12-character prefixes (every member of a group has the same next characters for several
+    rounds).  Both take the three-way partition around each group's first key; against the oracle."""
+    rng = np.random.default_rng(21)
+    s60 = rng.integers(0, 4, 60, dtype=np.uint8)
+    copies = np.concatenate([np.concatenate([rng.integers(0, 4, 12, dtype=np.uint8), s60]) for _ in range(30000)])
+    t = np.concatenate([random_text(2 << 20, 4, 5), periodic_text(1 << 20, 13, 4, 6), copies,
+                        random_text(1 << 20, 4, 7)])
+    say(f"large groups partition n={len(t)}")
+    sa = engine.build_sa(t, sa_width=8, X=39)
+    assert np.array_equal(sa, oracle.dcx_sa(t, X=39))
+
+
 def test_dense_codes_text_end(engine):
     # alphabets of 2, 4 and 256 symbols make the merge use dense character codes, where the padding
     # reads like the smallest character; texts ending in long runs of that character exercise the
