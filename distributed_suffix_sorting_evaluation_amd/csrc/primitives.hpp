@@ -425,8 +425,28 @@ __global__ __launch_bounds__(RS_THREADS) void k_onesweep(Src src, OutT* __restri
             const uint32_t i32 = (uint32_t)idx;  // indices of a sort stay below 2^32
             key[k] |= i32 & (uint32_t)kmask;
             val[k] = i32 >> kpos;
+        } else if constexpr (SPLIT) {
+            val[k] = 0;  // from the staged bytes below
         } else if constexpr (VALS) {
             val[k] = IDENT ? (uint32_t)idx : (valid ? (uint32_t)vin[idx] : 0u);
+        }
+    }
+    if constexpr (SPLIT && !IDENT) {
+        // the wave's index bytes by 16-byte loads through LDS (the key area is free until the reorder): a
+        // byte load per key would take as many address-unit cycles as a key load for an eighth of the data.
+        // The chunks start at a multiple of 16 and end at most 15 bytes past the last byte, inside the
+        // buffer (allocations are whole multiples of 256 bytes).
+        uint8_t* w = reinterpret_cast<uint8_t*>(s_keys) + wave * 1024;
+        const uint64_t wend = wbase + 64 * ITEMS < tile_end ? wbase + 64 * ITEMS : tile_end;
+        const uint64_t b16 = wbase & ~15ull;
+        const uint32_t nc = wbase < wend ? (uint32_t)((wend - b16 + 15) >> 4) : 0u;  // at most 57
+        if (lane < nc)
+            reinterpret_cast<uint4*>(w)[lane] = reinterpret_cast<const uint4*>(vin + b16)[lane];
+        __builtin_amdgcn_wave_barrier();
+#pragma unroll
+        for (int k = 0; k < ITEMS; k++) {
+            const uint64_t idx = wbase + k * 64 + lane;
+            val[k] = idx < tile_end ? (uint32_t)w[idx - b16] : 0u;
         }
     }
     // rank every key among the keys of its wave with the same digit, in tile order: the lowest lane of
