@@ -1143,7 +1143,9 @@ struct Primitives {
     //
     // idx_bytes (two arrays of N bytes, optional, with identity values and low32): when an index fits
     // the key bits below the digits plus one byte, the passes carry it split (low bits in the key, high
-    // byte in idx_bytes: 9 instead of 12 bytes per element); the last pass joins it into vals.
+    // byte in idx_bytes: 9 instead of 12 bytes per element); the last pass joins it into vals.  The
+    // passes read idx_bytes in aligned 16-byte chunks: both arrays must be 16-byte aligned and readable
+    // up to N rounded up to 16 (DeviceCache blocks are).
     template <class Src>
     uint32_t sort_pairs_from(const Src& first, uint64_t*& keys, uint64_t*& keys_alt, uint32_t*& vals,
                          uint32_t*& vals_alt, uint32_t N, uint32_t lo_bit, uint32_t hi_bit, bool ident_vals,
