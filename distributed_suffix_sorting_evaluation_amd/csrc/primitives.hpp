@@ -845,8 +845,12 @@ __global__ __launch_bounds__(SC_THREADS) void k_scan_down(Op op, uint64_t N, uin
     __shared__ uint64_t s_tmp[SC_THREADS / 64];
     __shared__ uint64_t s_e[COMPACT ? 1 : PAD + 1];  // padded to spread blocked accesses over banks
     __shared__ uint32_t s_p[PL && !COMPACT ? PAD : 1];
-    __shared__ uint32_t s_ci[COMPACT ? SC_TILE : 1];   // COMPACT: tile offset | high value bit << 31
-    __shared__ uint32_t s_chi[COMPACT ? SC_TILE : 1];  // COMPACT: high prefix relative to the tile's
+    // COMPACT, 16-bit entries (a tile offset and a count within a tile fit 13 bits; half the LDS of 32-bit
+    // entries lets more workgroups share a compute unit): tile offset | high value bit << 15, and the high
+    // prefix relative to the tile's
+    static_assert(SC_TILE <= (1 << 15), "compaction entries hold tile offsets in 15 bits");
+    __shared__ uint16_t s_ci[COMPACT ? SC_TILE : 1];
+    __shared__ uint16_t s_chi[COMPACT ? SC_TILE : 1];
     __shared__ uint32_t s_cp[COMPACT && PL ? SC_TILE : 1];  // COMPACT: payloads
     const uint32_t tid = threadIdx.x;
     const uint64_t t0 = (uint64_t)blockIdx.x * tpb;
@@ -882,8 +886,8 @@ __global__ __launch_bounds__(SC_THREADS) void k_scan_down(Op op, uint64_t N, uin
             for (int k = 0; k < SC_ITEMS; k++) {
                 if ((uint32_t)v[k]) {
                     const uint32_t e = (uint32_t)run - lo0;
-                    s_ci[e] = (tid * SC_ITEMS + k) | ((uint32_t)(v[k] >> 32) << 31);
-                    s_chi[e] = (uint32_t)(run >> 32) - hi0;
+                    s_ci[e] = (uint16_t)((tid * SC_ITEMS + k) | ((uint32_t)(v[k] >> 32) << 15));
+                    s_chi[e] = (uint16_t)((uint32_t)(run >> 32) - hi0);
                     if constexpr (PL) s_cp[e] = pl[k];
                 }
                 run += v[k];
@@ -893,11 +897,11 @@ __global__ __launch_bounds__(SC_THREADS) void k_scan_down(Op op, uint64_t N, uin
             for (uint32_t e = tid; e < cnt; e += SC_THREADS) {
                 const uint32_t ci = s_ci[e];
                 const uint64_t excl = (uint64_t)(lo0 + e) | ((uint64_t)(hi0 + s_chi[e]) << 32);
-                const uint64_t vv = 1ull | ((uint64_t)(ci >> 31) << 32);
+                const uint64_t vv = 1ull | ((uint64_t)(ci >> 15) << 32);
                 if constexpr (PL)
-                    op.write(tbase + (ci & 0x7FFFFFFFu), excl, vv, s_cp[e]);
+                    op.write(tbase + (ci & 0x7FFFu), excl, vv, s_cp[e]);
                 else
-                    op.write(tbase + (ci & 0x7FFFFFFFu), excl, vv);
+                    op.write(tbase + (ci & 0x7FFFu), excl, vv);
             }
         } else {
 #pragma unroll
