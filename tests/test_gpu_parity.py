@@ -11,7 +11,8 @@ import numpy as np
 import pytest
 
 import oracle
-from helpers import ALL_X, GOLDEN, brute_sa, copies_text, edge_cases, periodic_text, random_text, read
+from helpers import (ALL_X, GOLDEN, brute_sa, copies_text, edge_cases, periodic_text, random_text, read,
+                     repeat_words_text)
 
 pytestmark = pytest.mark.gpu
 
@@ -311,6 +312,19 @@ def test_large_groups_partition(engine):
     t = np.concatenate([random_text(2 << 20, 4, 5), periodic_text(1 << 20, 13, 4, 6), copies,
                         random_text(1 << 20, 4, 7)])
     say(f"large groups partition n={len(t)}")
+    sa = engine.build_sa(t, sa_width=8, X=39)
+    assert np.array_equal(sa, oracle.dcx_sa(t, X=39))
+
+
+def test_large_groups_mixed_continuations(engine):
+    """Four 50-character words repeated 40,000 times in random order: the large groups of a word's
+    occurrences share their next characters inside the word (the partition's all-equal case) and split
+    four ways at its end, where most members differ from the first (the partition declines and the
+    members are sorted); against the oracle."""
+    rng = np.random.default_rng(33)
+    words = [rng.integers(0, 4, 50, dtype=np.uint8).tobytes() for _ in range(4)]
+    t = np.concatenate([repeat_words_text(words, 40000, 34), random_text(1 << 20, 4, 35)])
+    say(f"large groups mixed continuations n={len(t)}")
     sa = engine.build_sa(t, sa_width=8, X=39)
     assert np.array_equal(sa, oracle.dcx_sa(t, X=39))
 
