@@ -391,29 +391,29 @@ class Engine {
     }
 
     // dense words only: boundary suffixes that share a key group are moved to its front (k_boundary_fix)
-    void boundary_fixup(uint32_t bnd, DBuf<uint32_t>& order, DBuf<uint8_t>& H, ActiveSet& tied) {
-        DBuf<uint32_t> found(cache, BOUNDARY_CAP), cnt(cache, 1);
+    // The key groups (after the refinement over L characters of the dense stream PT) that hold boundary
+    // suffixes, found by searching the sorted order for each boundary suffix's key (k_boundary_find).
+    void boundary_fixup(const uint64_t* PT, uint32_t bits, uint32_t L, uint32_t n, uint32_t bnd,
+                        DBuf<uint32_t>& order, DBuf<uint8_t>& H, ActiveSet& tied) {
+        if (bnd >= n) return;
+        const uint32_t nb = n - bnd;  // boundary suffixes: at most L - 1
+        DBuf<uint32_t> found(cache, 2 * (size_t)nb), cnt(cache, 1);
         prim.fill_u32(cnt.get(), 0, 1);
-        k_boundary_collect<<<grid(tied.A), 256, 0, stream>>>(tied.act.get(), tied.gid.get(), order.get(), tied.A, bnd,
-                                                             found.get(), cnt.get(), BOUNDARY_CAP);
+        k_boundary_find<<<(nb + 63) / 64, 64, 0, stream>>>(PT, bits, L, order.get(), n, bnd, found.get(), cnt.get(), nb);
         uint32_t hc = 0;
         DCX_HIP(hipMemcpyAsync(&hc, cnt.get(), 4, hipMemcpyDeviceToHost, stream));
         DCX_HIP(hipStreamSynchronize(stream));
         if (hc == 0) return;
-        if (hc > BOUNDARY_CAP) throw Error(DCX_E_HIP, "boundary suffix count exceeds its bound");
-        std::vector<uint32_t> g(hc);
-        DCX_HIP(hipMemcpyAsync(g.data(), found.get(), hc * 4, hipMemcpyDeviceToHost, stream));
-        const uint32_t* gstart = tied.gstart.get();
+        if (hc > nb) throw Error(DCX_E_HIP, "boundary group count exceeds the boundary suffixes");
+        std::vector<uint32_t> rg(2 * (size_t)hc);
+        DCX_HIP(hipMemcpyAsync(rg.data(), found.get(), 8 * (size_t)hc, hipMemcpyDeviceToHost, stream));
         DCX_HIP(hipStreamSynchronize(stream));
+        std::vector<std::pair<uint32_t, uint32_t>> g;
+        for (uint32_t i = 0; i < hc; i++) g.emplace_back(rg[2 * i], rg[2 * i + 1]);
         std::sort(g.begin(), g.end());
         g.erase(std::unique(g.begin(), g.end()), g.end());
-        for (uint32_t gi : g) {
-            uint32_t ab[2], j0 = 0;
-            DCX_HIP(hipMemcpyAsync(ab, gstart + gi, 8, hipMemcpyDeviceToHost, stream));
-            DCX_HIP(hipStreamSynchronize(stream));
-            DCX_HIP(hipMemcpyAsync(&j0, tied.act.get() + ab[0], 4, hipMemcpyDeviceToHost, stream));
-            DCX_HIP(hipStreamSynchronize(stream));
-            const uint32_t sz = ab[1] - ab[0];
+        for (const auto& r : g) {
+            const uint32_t j0 = r.first, sz = r.second - r.first;
             DBuf<uint32_t> tmp(cache, sz);
             k_boundary_fix<<<1, 1024, 0, stream>>>(order.get(), H.get(), j0, sz, bnd, tmp.get());
         }
@@ -671,8 +671,8 @@ class Engine {
             DBuf<uint64_t> PT;
             make_stream(text, lut, n, mbits, dense ? 1 : 0, (uint64_t)n + Ls + 2, PT);
             refine_sort(PT.get(), n, Ls, mbits, dense, info_bits(text, lut, n, mbits), order, H, tied);
+            if (dense && tied.A > 0) boundary_fixup(PT.get(), mbits, Ls, n, dc.bnd, order, H, tied);
         }
-        if (dense && tied.A > 0) boundary_fixup(dc.bnd, order, H, tied);
 
         DBuf<uint32_t> ranks(cache, (size_t)m + D);
         prim.fill_u32(ranks.get() + m, 0, D);

@@ -468,19 +468,55 @@ struct SampleScanOp {
 // earlier one (its window is a proper prefix); every boundary suffix has a key of its own.  These kernels
 // move them to the front of their group, in that order, as singleton groups.
 // ------------------------------------------------------------------------------------------------
-__global__ __launch_bounds__(256) void k_boundary_collect(const uint32_t* __restrict__ act,
-                                                          const uint32_t* __restrict__ gid,
-                                                          const uint32_t* __restrict__ order, uint32_t A,
-                                                          uint32_t bnd, uint32_t* __restrict__ out,
-                                                          uint32_t* __restrict__ count, uint32_t cap) {
-    for (uint32_t a = blockIdx.x * 256 + threadIdx.x; a < A; a += gridDim.x * 256)
-        if (order[act[a]] >= bnd) {
-            uint32_t i = atomicAdd(count, 1u);
-            if (i < cap) out[i] = gid[a];
-        }
-}
 
 constexpr uint32_t BOUNDARY_CAP = 256;
+
+// The groups that hold boundary suffixes, found from the boundary suffixes themselves (at most L - 1 of
+// them) instead of a pass over the tied list: the order is sorted by the L-character keys of the dense
+// stream, so each boundary suffix's group is the range of its key there (two binary searches).  Ranges of
+// two or more slots go to `ranges` as (first, end).
+__device__ __forceinline__ int cmp_key_chars(const uint64_t* __restrict__ PT, uint32_t bits, uint32_t L, uint64_t a,
+                                             uint64_t b) {
+    const uint32_t cpw = 64 / bits;
+    for (uint32_t c = 0; c < L; c += cpw) {
+        const uint32_t k = L - c < cpw ? L - c : cpw;
+        const StreamKeys<false> w{PT, bits, top_mask(k * bits)};
+        const uint64_t x = w(a + c), y = w(b + c);
+        if (x != y) return x < y ? -1 : 1;
+    }
+    return 0;
+}
+__global__ __launch_bounds__(64) void k_boundary_find(const uint64_t* __restrict__ PT, uint32_t bits, uint32_t L,
+                                                      const uint32_t* __restrict__ order, uint32_t n, uint32_t bnd,
+                                                      uint32_t* __restrict__ ranges, uint32_t* __restrict__ count,
+                                                      uint32_t cap) {
+    const uint32_t p = bnd + blockIdx.x * 64 + threadIdx.x;
+    if (p >= n) return;
+    uint32_t lo = 0, hi = n;
+    while (lo < hi) {  // first slot whose key is not below p's
+        const uint32_t mid = lo + (hi - lo) / 2;
+        if (cmp_key_chars(PT, bits, L, order[mid], p) < 0)
+            lo = mid + 1;
+        else
+            hi = mid;
+    }
+    const uint32_t first = lo;
+    hi = n;
+    while (lo < hi) {  // first slot whose key is above p's
+        const uint32_t mid = lo + (hi - lo) / 2;
+        if (cmp_key_chars(PT, bits, L, order[mid], p) <= 0)
+            lo = mid + 1;
+        else
+            hi = mid;
+    }
+    if (lo - first >= 2) {
+        const uint32_t i = atomicAdd(count, 1u);
+        if (i < cap) {
+            ranges[2 * i] = first;
+            ranges[2 * i + 1] = lo;
+        }
+    }
+}
 
 // one workgroup per affected group order[j0 .. j0+s)
 __global__ __launch_bounds__(1024) void k_boundary_fix(uint32_t* __restrict__ order, uint8_t* __restrict__ H,
