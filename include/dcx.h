@@ -114,8 +114,8 @@ int dcx_build_sa(dcx_handle* h, const uint8_t* text, uint64_t n, const dcx_optio
 
 /* Suffix arrays of `count` independent texts in one call (batched mode).  Text k is
  * texts[offsets[k] .. offsets[k+1]) (offsets has count + 1 entries, host memory, non-decreasing, each text
- * at most 2^32 - 2^22 characters); its SA is written to sa_out[offsets[k] .. offsets[k+1]) as 32-bit positions local to
- * the text.  The texts are joined with a separator symbol smaller than every character and run through one
+ * at most 2^32 - 2^22 characters); its SA is written to sa_out[offsets[k] - offsets[0] .. offsets[k+1] - offsets[0])
+ * as 32-bit positions local to the text (sa_out holds offsets[count] - offsets[0] entries).  The texts are joined with a separator symbol smaller than every character and run through one
  * DCX pipeline per sub-batch (at most batch_max_chars characters); a stable radix partition by text then
  * yields the per-text suffix arrays.  No counterpart in the reference: it builds one SA per cli run
  * (cli.cpp:747-820); this entry point serves callers with many texts at once. */

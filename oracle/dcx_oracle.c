@@ -537,3 +537,182 @@ void oracle_write_uint40(const uint64_t *SA, uint64_t n, uint8_t *out) {
     for (uint64_t k = 0; k < n; k++)
         for (int b = 0; b < 5; b++) out[5 * k + b] = (uint8_t)(SA[k] >> (8 * b));
 }
+
+/* ------------------------------------------------------------------------------------------
+ * The C ABI of include/dcx.h restated on the CPU under a dcx_host_ prefix: the same arguments,
+ * the same status codes for the same argument errors, and the same output encodings, so a test
+ * drives the HIP engine and this restatement with identical calls and compares the bytes.
+ * Differences, all because there is no device here: the handle holds no HBM budget (admission
+ * never rejects), device-pointer flags are rejected with DCX_E_ARG, and stats carry the level
+ * counts, sigma and the host wall time only.
+ * ---------------------------------------------------------------------------------------- */
+#include <time.h>
+#include "dcx.h"
+
+#define HOST_MAX_TEXT ((1ull << 32) - (1ull << 22)) /* the engine's 32-bit position limit */
+#define HOST_BASE_CASE_MAX 16384ull                 /* the engine's base-case limit       */
+
+struct dcx_handle {
+    char err[256];
+};
+
+static int host_fail(dcx_handle *h, int code, const char *msg) {
+    if (h) snprintf(h->err, sizeof(h->err), "%s", msg);
+    return code;
+}
+
+void dcx_host_default_options(dcx_options *opt) {
+    if (!opt) return;
+    memset(opt, 0, sizeof(*opt));
+    opt->X = 39;
+    opt->pack_samples = 1;
+    opt->pack_merging = 1;
+    opt->discarding_threshold = 0.7;
+    opt->base_case_chars = 10000;
+}
+
+int dcx_host_create(dcx_handle **out, int device, uint64_t hbm_budget_bytes) {
+    (void)device;
+    (void)hbm_budget_bytes;
+    if (!out) return DCX_E_ARG;
+    *out = (dcx_handle *)calloc(1, sizeof(dcx_handle));
+    return *out ? DCX_OK : DCX_E_NOMEM;
+}
+
+int dcx_host_destroy(dcx_handle *h) {
+    if (!h) return DCX_E_ARG;
+    free(h);
+    return DCX_OK;
+}
+
+/* the option checks of the engine, in its order: threshold, X, base case */
+static int host_check_options(dcx_handle *h, const dcx_options *o) {
+    if (!(o->discarding_threshold >= 0.0 && o->discarding_threshold <= 1.0))
+        return host_fail(h, DCX_E_ARG, "discarding_threshold must lie in [0, 1]");
+    int known = 0;
+    for (size_t i = 0; i < sizeof(DC_SETS) / sizeof(DC_SETS[0]); i++) known |= DC_SETS[i][0] == o->X;
+    if (!known) return host_fail(h, DCX_E_INVALID_X, "X is not a supported difference cover");
+    if (o->base_case_chars > HOST_BASE_CASE_MAX) return host_fail(h, DCX_E_ARG, "base_case_chars must be <= 16384");
+    return DCX_OK;
+}
+
+static void host_store(const uint64_t *SA, uint64_t n, void *out, uint32_t width) {
+    uint8_t *o = (uint8_t *)out;
+    for (uint64_t k = 0; k < n; k++)
+        for (uint32_t b = 0; b < width; b++) o[width * k + b] = (uint8_t)(SA[k] >> (8 * b));
+}
+
+int dcx_host_build_sa(dcx_handle *h, const uint8_t *text, uint64_t n, const dcx_options *o, void *sa_out,
+                      uint32_t width, dcx_stats *stats) {
+    struct timespec t0, t1;
+    clock_gettime(CLOCK_MONOTONIC, &t0);
+    if (!h) return DCX_E_ARG;
+    if (!o || (!text && n) || (!sa_out && n)) return host_fail(h, DCX_E_ARG, "null argument");
+    if (width != 4 && width != 5 && width != 8) return host_fail(h, DCX_E_ARG, "sa_width_bytes must be 4, 5 or 8");
+    if (o->text_on_device || o->sa_on_device) return host_fail(h, DCX_E_ARG, "device pointers on the host path");
+    int rc0 = host_check_options(h, o);
+    if (rc0 != DCX_OK) return rc0;
+    if (n > HOST_MAX_TEXT) return host_fail(h, DCX_E_UNSUPPORTED, "text too long for 32-bit positions");
+    if (stats) memset(stats, 0, sizeof(*stats));
+    if (n == 0) return DCX_OK;
+    uint64_t *SA = (uint64_t *)malloc(sizeof(uint64_t) * n);
+    oracle_stats *st = (oracle_stats *)calloc(1, sizeof(oracle_stats));
+    if (!SA || !st) {
+        free(SA);
+        free(st);
+        return host_fail(h, DCX_E_NOMEM, "host allocation failed");
+    }
+    int rc = oracle_dcx_sa(text, n, o->X, o->pack_samples != 0, o->pack_merging != 0, o->discarding_threshold,
+                           o->base_case_chars, SA, st);
+    if (rc == 0) host_store(SA, n, sa_out, width);
+    if (rc == 0 && stats) {
+        stats->sigma = st->sigma;
+        stats->levels = st->levels < DCX_MAX_LEVELS ? st->levels : DCX_MAX_LEVELS;
+        for (uint32_t l = 0; l < stats->levels; l++) {
+            dcx_level_stats *d = &stats->level[l];
+            const oracle_level_stat *s = &st->lv[l];
+            d->n = s->n;
+            d->samples = s->samples;
+            d->names = s->names;
+            d->reduced = s->reduced;
+            d->base_case = s->base_case;
+            d->recursed = s->recursed;
+            d->discarding = s->discarding;
+            d->key_chars = s->key_chars;
+            d->merge_chars = s->merge_chars;
+            d->bits_per_char = s->bits_per_char;
+        }
+        clock_gettime(CLOCK_MONOTONIC, &t1);
+        stats->ms_total = (double)(t1.tv_sec - t0.tv_sec) * 1e3 + (double)(t1.tv_nsec - t0.tv_nsec) * 1e-6;
+    }
+    free(SA);
+    free(st);
+    return rc == 0 ? DCX_OK : host_fail(h, DCX_E_NOMEM, "host allocation failed");
+}
+
+/* dcx_build_sa_batch on the CPU: each text's SA on its own (the joined-string pipeline of the engine
+ * must give exactly these), written as 32-bit positions local to the text at sa_out + offsets[k] - offsets[0]. */
+int dcx_host_build_sa_batch(dcx_handle *h, const uint8_t *texts, const uint64_t *offsets, uint32_t count,
+                            const dcx_options *o, uint32_t *sa_out, dcx_stats *stats) {
+    if (!h) return DCX_E_ARG;
+    if (!o || !offsets) return host_fail(h, DCX_E_ARG, "null argument");
+    if (o->text_on_device || o->sa_on_device) return host_fail(h, DCX_E_ARG, "device pointers on the host path");
+    int rc0 = host_check_options(h, o);
+    if (rc0 != DCX_OK) return rc0;
+    if (count == 0) {
+        if (stats) memset(stats, 0, sizeof(*stats));
+        return DCX_OK;
+    }
+    const uint64_t base = offsets[0], total = offsets[count] - offsets[0];
+    for (uint32_t k = 0; k < count; k++) {
+        if (offsets[k + 1] < offsets[k]) return host_fail(h, DCX_E_ARG, "offsets must be non-decreasing");
+        if (offsets[k + 1] - offsets[k] > HOST_MAX_TEXT)
+            return host_fail(h, DCX_E_UNSUPPORTED, "a text of the batch is too long for 32-bit positions");
+    }
+    if (total && (!texts || !sa_out)) return host_fail(h, DCX_E_ARG, "null argument");
+    for (uint32_t k = 0; k < count; k++) {
+        const uint64_t n = offsets[k + 1] - offsets[k];
+        int rc = dcx_host_build_sa(h, texts + offsets[k], n, o, sa_out + (offsets[k] - base), 4, NULL);
+        if (rc != DCX_OK) return rc;
+    }
+    if (stats) memset(stats, 0, sizeof(*stats));
+    return DCX_OK;
+}
+
+int dcx_host_check_sa(dcx_handle *h, const uint8_t *text, uint64_t n, const void *sa, uint32_t width,
+                      int on_device, int *ok) {
+    if (!h) return DCX_E_ARG;
+    if (!ok || (n && (!text || !sa))) return host_fail(h, DCX_E_ARG, "null argument");
+    if (width != 4 && width != 5 && width != 8) return host_fail(h, DCX_E_ARG, "sa_width_bytes must be 4, 5 or 8");
+    if (on_device) return host_fail(h, DCX_E_ARG, "device pointers on the host path");
+    if (n >= (1ull << 32) - 1) return host_fail(h, DCX_E_UNSUPPORTED, "text too long for this build");
+    *ok = 0;
+    uint64_t *SA = (uint64_t *)malloc(sizeof(uint64_t) * (n ? n : 1));
+    if (!SA) return host_fail(h, DCX_E_NOMEM, "host allocation failed");
+    const uint8_t *b = (const uint8_t *)sa;
+    for (uint64_t k = 0; k < n; k++) {
+        uint64_t v = 0;
+        for (uint32_t i = 0; i < width; i++) v |= (uint64_t)b[width * k + i] << (8 * i);
+        SA[k] = v;
+    }
+    int r = oracle_check_sa(text, n, SA);
+    free(SA);
+    if (r < 0) return host_fail(h, DCX_E_NOMEM, "host allocation failed");
+    *ok = r;
+    return DCX_OK;
+}
+
+const char *dcx_host_strerror(int code) {
+    switch (code) {
+        case DCX_OK: return "ok";
+        case DCX_E_INVALID_X: return "unsupported difference cover X";
+        case DCX_E_ALPHABET: return "alphabet cannot be represented";
+        case DCX_E_NOMEM: return "host memory exhausted";
+        case DCX_E_ARG: return "invalid argument";
+        case DCX_E_WIDTH: return "SA width too small for the text length";
+        case DCX_E_UNSUPPORTED: return "unsupported size";
+        default: return "unknown error";
+    }
+}
+
+const char *dcx_host_last_error(dcx_handle *h) { return h ? h->err : "no handle"; }

@@ -407,3 +407,44 @@ def test_batch_errors(engine):
     with pytest.raises(DCXError):
         engine.build_sa_batch([b"abc"], X=5)
     assert engine.build_sa_batch([], X=39) == []
+
+
+def test_same_calls_as_host_abi(engine):
+    """dcx_build_sa and dcx_build_sa_batch against the oracle's dcx_host_* restatement of the same C ABI, with
+    identical arguments: the same status codes and the same output bytes in every SA width."""
+    import ctypes
+
+    from helpers import abi_batch, abi_build, host_abi
+    H = host_abi()
+    hh = ctypes.c_void_p()
+    assert H.dcx_host_create(ctypes.byref(hh), 0, 0) == 0
+    try:
+        cases = [("three_brothers", read("the_three_brothers.txt")), ("mississippi", b"mississippi"),
+                 ("copies", copies_text(50000, 31)), ("periodic", periodic_text(20000, 17, 4, 32)),
+                 ("bytes", random_text(30000, 256, 33))]
+        for name, t in cases:
+            for X, width in ((3, 4), (21, 5), (39, 8), (133, 5)):
+                say(f"host-abi case {name} X={X} width={width}")
+                o = engine.make_options(X=X, base_case_chars=0)
+                g = abi_build(engine._L, "dcx_", engine._h, t, o, width)
+                c = abi_build(H, "dcx_host_", hh, t, o, width)
+                assert g[0] == c[0] == 0
+                assert g[1] == c[1], (name, X, width)
+        for kw, width in (({"X": 40}, 8), ({}, 6), ({"discarding_threshold": 1.5}, 8),
+                          ({"base_case_chars": 16385}, 8)):
+            o = engine.make_options(**kw)
+            say(f"host-abi error case {kw} width={width}")
+            assert abi_build(engine._L, "dcx_", engine._h, b"abracadabra", o, width)[0] == \
+                abi_build(H, "dcx_host_", hh, b"abracadabra", o, width)[0] != 0
+        texts = [random_text(n, 4, 40 + n) for n in (0, 1, 7, 300, 2000, 70000)]
+        joined = np.concatenate([np.frombuffer(b"head", dtype=np.uint8)] + texts)
+        offsets = np.cumsum([4] + [x.size for x in texts]).astype(np.uint64)
+        for X in (7, 39):
+            say(f"host-abi batch X={X} offsets[0]=4")
+            o = engine.make_options(X=X)
+            g = abi_batch(engine._L, "dcx_", engine._h, joined, offsets, o)
+            c = abi_batch(H, "dcx_host_", hh, joined, offsets, o)
+            assert g[0] == c[0] == 0
+            assert np.array_equal(g[1], c[1])
+    finally:
+        H.dcx_host_destroy(hh)

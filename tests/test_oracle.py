@@ -181,3 +181,100 @@ def test_uint40_writer():
     b = oracle.uint40_bytes(sa)
     assert len(b) == 20
     assert b[10:15] == bytes([5, 0, 0, 0, 1])
+
+
+# --- the C ABI restated on the CPU (dcx_host_*) ------------------------------------------------------------
+
+def _host_handle(L):
+    import ctypes
+    h = ctypes.c_void_p()
+    assert L.dcx_host_create(ctypes.byref(h), 0, 0) == 0
+    return h
+
+
+def _host_opts(L, **kw):
+    import ctypes
+
+    from distributed_suffix_sorting_evaluation_amd import engine as eng
+    o = eng.Options()
+    L.dcx_host_default_options(ctypes.byref(o))
+    for k, v in kw.items():
+        setattr(o, k, v)
+    return o
+
+
+def test_host_abi_mirrors_every_declared_entry_point():
+    import re
+
+    from helpers import host_abi
+    L = host_abi()
+    src = re.sub(r"/\*.*?\*/", "", open(os.path.join(os.path.dirname(GOLDEN), "..", "include", "dcx.h")).read(),
+                 flags=re.S)
+    for name in sorted(set(re.findall(r"\b(dcx_[a-z_0-9]+)\s*\(", src))):
+        assert hasattr(L, name.replace("dcx_", "dcx_host_", 1)), name
+
+
+def test_host_abi_fixtures_in_every_width():
+    from helpers import abi_build, host_abi
+    L = host_abi()
+    h = _host_handle(L)
+    cases = [(read("abracadabra4"), np.frombuffer(read("abracadabra4.sa"), dtype="<u4").astype(np.uint64)),
+             (read("the_three_brothers.txt"), np.frombuffer(read("the_three_brothers_size_t_sa"), dtype="<u8")),
+             (b"mississippi", np.array([10, 7, 4, 1, 0, 9, 8, 6, 3, 5, 2], dtype=np.uint64))]
+    for text, expect in cases:
+        for X in (3, 39, 133):
+            for width in (4, 5, 8):
+                rc, out, st = abi_build(L, "dcx_host_", h, text, _host_opts(L, X=X, base_case_chars=0), width)
+                assert rc == 0, L.dcx_host_last_error(h)
+                got = np.frombuffer(out, dtype=np.uint8).reshape(-1, width)
+                vals = sum(got[:, b].astype(np.uint64) << np.uint64(8 * b) for b in range(width))
+                assert np.array_equal(vals, expect), (X, width)
+                assert st.levels >= 1 and st.level[0].n == len(text)
+    # the check entry point accepts the fixture and rejects a swapped pair
+    import ctypes
+    text, expect = cases[1]
+    t = np.frombuffer(text, dtype=np.uint8)
+    for good in (True, False):
+        sa = expect.astype("<u8").copy()
+        if not good:
+            sa[[5, 6]] = sa[[6, 5]]
+        ok = ctypes.c_int(-1)
+        assert L.dcx_host_check_sa(h, t.ctypes.data, t.size, sa.ctypes.data, 8, 0, ctypes.byref(ok)) == 0
+        assert ok.value == int(good)
+    L.dcx_host_destroy(h)
+
+
+def test_host_abi_argument_errors():
+    from helpers import abi_build, host_abi
+    L = host_abi()
+    h = _host_handle(L)
+    t = b"abracadabra"
+    assert abi_build(L, "dcx_host_", h, t, _host_opts(L, X=40), 8)[0] == -1
+    assert abi_build(L, "dcx_host_", h, t, _host_opts(L), 6)[0] == -6
+    assert abi_build(L, "dcx_host_", h, t, _host_opts(L, discarding_threshold=1.5), 8)[0] == -6
+    assert abi_build(L, "dcx_host_", h, t, _host_opts(L, base_case_chars=16385), 8)[0] == -6
+    assert abi_build(L, "dcx_host_", h, t, _host_opts(L, sa_on_device=1), 8)[0] == -6
+    rc, out, _ = abi_build(L, "dcx_host_", h, b"", _host_opts(L), 8)
+    assert rc == 0 and out == b""
+    assert L.dcx_host_strerror(-1) == b"unsupported difference cover X"
+    L.dcx_host_destroy(h)
+
+
+def test_host_abi_batch_is_per_text_sa():
+    from helpers import abi_batch, host_abi, random_text
+    L = host_abi()
+    h = _host_handle(L)
+    texts = [random_text(n, 4, 10 + n) for n in (0, 1, 7, 300, 2000)]
+    pad = np.frombuffer(b"xyz", dtype=np.uint8)
+    joined = np.concatenate([pad] + texts)
+    offsets = np.cumsum([3] + [t.size for t in texts]).astype(np.uint64)
+    offsets = np.concatenate([[3], offsets[1:]]).astype(np.uint64)
+    rc, sa = abi_batch(L, "dcx_host_", h, joined, offsets, _host_opts(L, X=7))
+    assert rc == 0
+    for k, t in enumerate(texts):
+        a, b = int(offsets[k] - offsets[0]), int(offsets[k + 1] - offsets[0])
+        assert np.array_equal(sa[a:b].astype(np.uint64), brute_sa(t)), k
+    bad = offsets.copy()
+    bad[2] = bad[1] - 1
+    assert abi_batch(L, "dcx_host_", h, joined, bad, _host_opts(L, X=7))[0] == -6
+    L.dcx_host_destroy(h)
