@@ -10,11 +10,13 @@
 //              sample ranks; the sorted order is the level's SA
 // All data stays in HBM; the host only reads back a handful of scalars per step.
 #include <algorithm>
+#include <atomic>
 #include <cmath>
 #include <chrono>
 #include <cstdio>
 #include <cstring>
 #include <string>
+#include <thread>
 
 #include "../../include/dcx.h"
 #include "dcx_kernels.hpp"
@@ -67,6 +69,78 @@ class Engine {
     uint64_t alg_bytes = 0;
     uint32_t sa_width = 8;
 
+    // pinned staging ring for downloads into pageable host memory (allocated on the first large download)
+    static constexpr int kStages = 4;
+    static constexpr size_t kStageBytes = 64ull << 20;
+    uint8_t* h_stage[kStages] = {};
+    hipEvent_t stage_ev[kStages] = {};
+
+    // Synchronous copy of `bytes` from device memory to pageable host memory, ordered after the work on the
+    // engine's stream.  The DMA engine writes pinned memory at about 57 GB/s and pageable memory at about
+    // 17 GB/s (5 GiB: 94 against 308 ms), so large copies go in 64 MiB chunks through a ring of pinned
+    // buffers; host threads copy each chunk out (each its own slice) while the next chunks download.
+    void download(void* dst, const void* src, size_t bytes) {
+        if (bytes < (16ull << 20)) {
+            DCX_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, stream));
+            DCX_HIP(hipStreamSynchronize(stream));
+            return;
+        }
+        if (!h_stage[0]) {
+            for (int s = 0; s < kStages; s++) {
+                DCX_HIP(hipHostMalloc((void**)&h_stage[s], kStageBytes, hipHostMallocDefault));
+                DCX_HIP(hipEventCreateWithFlags(&stage_ev[s], hipEventDisableTiming));
+            }
+        }
+        const size_t nch = (bytes + kStageBytes - 1) / kStageBytes;
+        const unsigned hw = std::thread::hardware_concurrency();
+        const unsigned T = std::min(8u, std::max(1u, hw / 2));
+        std::atomic<size_t> posted{0};                // chunks enqueued (their event recorded)
+        std::atomic<unsigned> done[kStages];          // workers finished with the stage's current chunk
+        std::atomic<int> failed{0};
+        std::atomic<int> fail_code{0};
+        for (int s = 0; s < kStages; s++) done[s] = T;
+        auto chunk_len = [&](size_t k) { return std::min(kStageBytes, bytes - k * kStageBytes); };
+        std::vector<std::thread> workers;
+        workers.reserve(T);
+        for (unsigned t = 0; t < T; t++) {
+            workers.emplace_back([&, t]() {
+                for (size_t k = 0; k < nch; k++) {
+                    while (posted.load() <= k && !failed.load()) std::this_thread::yield();
+                    if (failed.load()) return;
+                    const int s = (int)(k % kStages);
+                    const hipError_t e = hipEventSynchronize(stage_ev[s]);
+                    if (e != hipSuccess) {
+                        fail_code = (int)e;
+                        failed = 1;
+                        return;
+                    }
+                    const size_t len = chunk_len(k);
+                    const size_t a = len * t / T, b = len * (t + 1) / T;
+                    memcpy(static_cast<uint8_t*>(dst) + k * kStageBytes + a, h_stage[s] + a, b - a);
+                    done[s].fetch_add(1);
+                }
+            });
+        }
+        for (size_t k = 0; k < nch && !failed.load(); k++) {
+            const int s = (int)(k % kStages);
+            while (done[s].load() < T && !failed.load()) std::this_thread::yield();
+            if (failed.load()) break;
+            done[s] = 0;
+            hipError_t e = hipMemcpyAsync(h_stage[s], static_cast<const uint8_t*>(src) + k * kStageBytes,
+                                          chunk_len(k), hipMemcpyDeviceToHost, stream);
+            if (e == hipSuccess) e = hipEventRecord(stage_ev[s], stream);
+            if (e != hipSuccess) {
+                fail_code = (int)e;
+                failed = 1;
+                break;
+            }
+            posted = k + 1;
+        }
+        for (auto& w : workers) w.join();
+        if (failed.load())
+            throw Error(DCX_E_HIP, std::string("staged download: ") + hipGetErrorString((hipError_t)fail_code.load()));
+    }
+
     explicit Engine(int dev) : device(dev) {
         DCX_HIP(hipSetDevice(device));
         DCX_HIP(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
@@ -75,6 +149,10 @@ class Engine {
     ~Engine() {
         (void)hipSetDevice(device);
         if (stream) (void)hipStreamSynchronize(stream);
+        for (int k = 0; k < kStages; k++) {
+            if (h_stage[k]) (void)hipHostFree(h_stage[k]);
+            if (stage_ev[k]) (void)hipEventDestroy(stage_ev[k]);
+        }
         d_cmp.reset();
         prim.destroy();
         cache.release_all();
@@ -872,7 +950,7 @@ class Engine {
                 if (o->sa_on_device)
                     DCX_HIP(hipMemcpyAsync(dst, y, chars * 4, hipMemcpyDeviceToDevice, stream));
                 else
-                    DCX_HIP(hipMemcpyAsync(dst, y, chars * 4, hipMemcpyDeviceToHost, stream));
+                    download(dst, y, chars * 4);
             }
             DCX_HIP(hipStreamSynchronize(stream));
             k0 = k1;
@@ -960,12 +1038,11 @@ class Engine {
         if (o->sa_on_device) {
             widen(sa.get(), n, width, static_cast<uint8_t*>(sa_out));
         } else if (width == 4) {
-            DCX_HIP(hipMemcpyAsync(sa_out, sa.get(), n * 4, hipMemcpyDeviceToHost, stream));
+            download(sa_out, sa.get(), n * 4);
         } else {
             DBuf<uint8_t> wide(cache, n * width);
             widen(sa.get(), n, width, wide.get());
-            DCX_HIP(hipMemcpyAsync(sa_out, wide.get(), n * width, hipMemcpyDeviceToHost, stream));
-            DCX_HIP(hipStreamSynchronize(stream));
+            download(sa_out, wide.get(), n * width);
         }
         DCX_HIP(hipEventRecord(ev[5], stream));
         DCX_HIP(hipStreamSynchronize(stream));

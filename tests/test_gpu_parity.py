@@ -448,3 +448,30 @@ def test_same_calls_as_host_abi(engine):
             assert np.array_equal(g[1], c[1])
     finally:
         H.dcx_host_destroy(hh)
+
+
+def test_staged_host_download_all_widths(engine):
+    """Host outputs large enough for the pinned-ring download (64 MiB chunks, several host threads per chunk,
+    a partial last chunk) equal the device output of the same build, byte for byte, in widths 4, 5 and 8,
+    into a fresh buffer and again into the same buffer; a batch whose sub-batch output crosses chunks
+    equals its per-text builds."""
+    import torch
+    n = 20_000_003  # 80, 100 and 160 MB of output: partial last chunks, slices of uneven length
+    t = copies_text(n, 21, sigma=4, copy_frac=0.05, max_len=5000)
+    dt = torch.from_numpy(t).cuda()
+    for width in (4, 5, 8):
+        say(f"staged download n={n} width={width}")
+        buf = torch.zeros(width * n, dtype=torch.uint8, device="cuda")
+        engine.build_sa_device(dt.data_ptr(), n, buf.data_ptr(), width, X=39)
+        torch.cuda.synchronize()
+        want = buf.cpu().numpy().tobytes()
+        got = engine.build_sa(t, sa_width=width, X=39)
+        assert got.tobytes() == want, width
+        again = engine.build_sa(t, sa_width=width, X=39)
+        assert again.tobytes() == want, width
+    assert engine.check_sa(t, np.frombuffer(want, dtype="<u8"), sa_width=8)
+    texts = [copies_text(m, 50 + m % 97, sigma=4, copy_frac=0.05) for m in (6_000_001, 4_500_000, 7_000_017)]
+    say("staged download batch of 3 texts, 17.5 M chars")
+    sas = engine.build_sa_batch(texts, batch_max_chars=1 << 31)
+    for k, x in enumerate(texts):
+        assert np.array_equal(np.asarray(sas[k]).astype(np.uint64), engine.build_sa(x, sa_width=8, X=39)), k
