@@ -85,12 +85,7 @@ class Engine {
             DCX_HIP(hipStreamSynchronize(stream));
             return;
         }
-        if (!h_stage[0]) {
-            for (int s = 0; s < kStages; s++) {
-                DCX_HIP(hipHostMalloc((void**)&h_stage[s], kStageBytes, hipHostMallocDefault));
-                DCX_HIP(hipEventCreateWithFlags(&stage_ev[s], hipEventDisableTiming));
-            }
-        }
+        alloc_stages();
         const size_t nch = (bytes + kStageBytes - 1) / kStageBytes;
         const unsigned hw = std::thread::hardware_concurrency();
         const unsigned T = std::min(8u, std::max(1u, hw / 2));
@@ -139,6 +134,82 @@ class Engine {
         for (auto& w : workers) w.join();
         if (failed.load())
             throw Error(DCX_E_HIP, std::string("staged download: ") + hipGetErrorString((hipError_t)fail_code.load()));
+    }
+
+    // Copy of `bytes` from pageable host memory to device memory, enqueued on the engine's stream (the
+    // kernels after it see the data by stream order).  Large copies go through the same pinned ring: host
+    // threads fill a stage while the DMA engine uploads the previous ones.  Returns once every stage is
+    // free again, so the caller's buffer and the ring can be reused.
+    void upload(void* dst, const void* src, size_t bytes) {
+        if (bytes < (16ull << 20)) {
+            DCX_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, stream));
+            DCX_HIP(hipStreamSynchronize(stream));
+            return;
+        }
+        alloc_stages();
+        const size_t nch = (bytes + kStageBytes - 1) / kStageBytes;
+        const unsigned hw = std::thread::hardware_concurrency();
+        const unsigned T = std::min(8u, std::max(1u, hw / 2));
+        std::atomic<size_t> released{0};             // chunks whose upload has completed
+        std::atomic<unsigned> filled[kStages];        // workers done filling the stage's current chunk
+        std::atomic<int> failed{0};
+        int fail_code = 0;
+        for (int s = 0; s < kStages; s++) filled[s] = 0;
+        auto chunk_len = [&](size_t k) { return std::min(kStageBytes, bytes - k * kStageBytes); };
+        std::vector<std::thread> workers;
+        workers.reserve(T);
+        for (unsigned t = 0; t < T; t++) {
+            workers.emplace_back([&, t]() {
+                for (size_t k = 0; k < nch; k++) {
+                    // the stage's previous chunk (k - kStages) must have left it
+                    while (k >= (size_t)kStages && released.load() < k - kStages + 1 && !failed.load())
+                        std::this_thread::yield();
+                    if (failed.load()) return;
+                    const int s = (int)(k % kStages);
+                    const size_t len = chunk_len(k);
+                    const size_t a = len * t / T, b = len * (t + 1) / T;
+                    memcpy(h_stage[s] + a, static_cast<const uint8_t*>(src) + k * kStageBytes + a, b - a);
+                    filled[s].fetch_add(1);
+                }
+            });
+        }
+        size_t next_release = 0;
+        for (size_t k = 0; k < nch && !failed.load(); k++) {
+            const int s = (int)(k % kStages);
+            while (filled[s].load() < T) std::this_thread::yield();
+            filled[s] = 0;
+            hipError_t e = hipMemcpyAsync(static_cast<uint8_t*>(dst) + k * kStageBytes, h_stage[s], chunk_len(k),
+                                          hipMemcpyHostToDevice, stream);
+            if (e == hipSuccess) e = hipEventRecord(stage_ev[s], stream);
+            // free the stage the workers need next: chunk k + 1 reuses the stage of chunk k + 1 - kStages
+            if (e == hipSuccess && k + 1 >= (size_t)kStages && k + 1 < nch) {
+                e = hipEventSynchronize(stage_ev[(k + 1) % kStages]);
+                next_release = k + 2 - kStages;
+                released = next_release;
+            }
+            if (e != hipSuccess) {
+                fail_code = (int)e;
+                failed = 1;
+            }
+        }
+        for (auto& w : workers) w.join();
+        if (!failed.load()) {
+            const hipError_t e = hipStreamSynchronize(stream);
+            if (e != hipSuccess) {
+                fail_code = (int)e;
+                failed = 1;
+            }
+        }
+        if (failed.load())
+            throw Error(DCX_E_HIP, std::string("staged upload: ") + hipGetErrorString((hipError_t)fail_code));
+    }
+
+    void alloc_stages() {
+        if (h_stage[0]) return;
+        for (int s = 0; s < kStages; s++) {
+            DCX_HIP(hipHostMalloc((void**)&h_stage[s], kStageBytes, hipHostMallocDefault));
+            DCX_HIP(hipEventCreateWithFlags(&stage_ev[s], hipEventDisableTiming));
+        }
     }
 
     explicit Engine(int dev) : device(dev) {
@@ -876,7 +947,7 @@ class Engine {
         const uint8_t* d_texts = texts ? texts + base : nullptr;
         if (!o->text_on_device && total) {
             dtext = DBuf<uint8_t>(cache, total + 16);
-            DCX_HIP(hipMemcpyAsync(dtext.get(), texts + base, total, hipMemcpyHostToDevice, stream));
+            upload(dtext.get(), texts + base, total);
             d_texts = dtext.get();
         }
         DCX_HIP(hipEventRecord(e0, stream));
@@ -1020,7 +1091,7 @@ class Engine {
         const uint8_t* d_text = text;
         if (!o->text_on_device) {
             dtext = DBuf<uint8_t>(cache, n + 16);
-            DCX_HIP(hipMemcpyAsync(dtext.get(), text, n, hipMemcpyHostToDevice, stream));
+            upload(dtext.get(), text, n);
             d_text = dtext.get();
         }
         DCX_HIP(hipEventRecord(ev[1], stream));  // text resident

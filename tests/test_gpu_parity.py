@@ -451,8 +451,8 @@ def test_same_calls_as_host_abi(engine):
 
 
 def test_staged_host_download_all_widths(engine):
-    """Host outputs large enough for the pinned-ring download (64 MiB chunks, several host threads per chunk,
-    a partial last chunk) equal the device output of the same build, byte for byte, in widths 4, 5 and 8,
+    """Host texts and outputs large enough for the pinned-ring upload and download (64 MiB chunks, several host
+    threads per chunk, a partial last chunk) give outputs equal the device output of the same build, byte for byte, in widths 4, 5 and 8,
     into a fresh buffer and again into the same buffer; a batch whose sub-batch output crosses chunks
     equals its per-text builds."""
     import torch
