@@ -14,6 +14,7 @@
 #include <cmath>
 #include <chrono>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <string>
 #include <thread>
@@ -69,33 +70,56 @@ class Engine {
     uint64_t alg_bytes = 0;
     uint32_t sa_width = 8;
 
-    // pinned staging ring for downloads into pageable host memory (allocated on the first large download)
+    // pinned staging ring for copies between device memory and pageable host memory (allocated on the
+    // first large copy).  The stage size is 64 MiB; the environment variable DCX_STAGE_BYTES (read when
+    // the handle is created) sets another one, so that tests can make small copies wrap the ring.  Copies
+    // of at least a quarter of a stage go through the ring.
     static constexpr int kStages = 4;
-    static constexpr size_t kStageBytes = 64ull << 20;
+    size_t stage_bytes = 64ull << 20;
     uint8_t* h_stage[kStages] = {};
     hipEvent_t stage_ev[kStages] = {};
+
+    // Joins the copy threads of a staged transfer however the transfer ends.  If it ends while threads
+    // still run (an exception while they were being started), `failed` tells them to stop first, so a
+    // joinable std::thread is never destroyed (which would terminate the process).
+    struct Joiner {
+        std::vector<std::thread>& threads;
+        std::atomic<int>& failed;
+        ~Joiner() {
+            bool early = false;
+            for (auto& t : threads) early = early || t.joinable();
+            if (early) failed = 1;
+            for (auto& t : threads)
+                if (t.joinable()) t.join();
+        }
+    };
+    static unsigned copy_threads() {
+        const unsigned hw = std::thread::hardware_concurrency();
+        return std::min(8u, std::max(1u, hw / 2));
+    }
 
     // Synchronous copy of `bytes` from device memory to pageable host memory, ordered after the work on the
     // engine's stream.  The DMA engine writes pinned memory at about 57 GB/s and pageable memory at about
     // 17 GB/s (5 GiB: 94 against 308 ms), so large copies go in 64 MiB chunks through a ring of pinned
     // buffers; host threads copy each chunk out (each its own slice) while the next chunks download.
     void download(void* dst, const void* src, size_t bytes) {
-        if (bytes < (16ull << 20)) {
+        if (bytes < stage_bytes / 4) {
             DCX_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, stream));
             DCX_HIP(hipStreamSynchronize(stream));
             return;
         }
         alloc_stages();
-        const size_t nch = (bytes + kStageBytes - 1) / kStageBytes;
-        const unsigned hw = std::thread::hardware_concurrency();
-        const unsigned T = std::min(8u, std::max(1u, hw / 2));
+        const size_t SB = stage_bytes;
+        const size_t nch = (bytes + SB - 1) / SB;
+        const unsigned T = copy_threads();
         std::atomic<size_t> posted{0};                // chunks enqueued (their event recorded)
         std::atomic<unsigned> done[kStages];          // workers finished with the stage's current chunk
         std::atomic<int> failed{0};
         std::atomic<int> fail_code{0};
         for (int s = 0; s < kStages; s++) done[s] = T;
-        auto chunk_len = [&](size_t k) { return std::min(kStageBytes, bytes - k * kStageBytes); };
+        auto chunk_len = [&](size_t k) { return std::min(SB, bytes - k * SB); };
         std::vector<std::thread> workers;
+        Joiner joiner{workers, failed};
         workers.reserve(T);
         for (unsigned t = 0; t < T; t++) {
             workers.emplace_back([&, t]() {
@@ -111,7 +135,7 @@ class Engine {
                     }
                     const size_t len = chunk_len(k);
                     const size_t a = len * t / T, b = len * (t + 1) / T;
-                    memcpy(static_cast<uint8_t*>(dst) + k * kStageBytes + a, h_stage[s] + a, b - a);
+                    memcpy(static_cast<uint8_t*>(dst) + k * SB + a, h_stage[s] + a, b - a);
                     done[s].fetch_add(1);
                 }
             });
@@ -121,7 +145,7 @@ class Engine {
             while (done[s].load() < T && !failed.load()) std::this_thread::yield();
             if (failed.load()) break;
             done[s] = 0;
-            hipError_t e = hipMemcpyAsync(h_stage[s], static_cast<const uint8_t*>(src) + k * kStageBytes,
+            hipError_t e = hipMemcpyAsync(h_stage[s], static_cast<const uint8_t*>(src) + k * SB,
                                           chunk_len(k), hipMemcpyDeviceToHost, stream);
             if (e == hipSuccess) e = hipEventRecord(stage_ev[s], stream);
             if (e != hipSuccess) {
@@ -141,22 +165,23 @@ class Engine {
     // threads fill a stage while the DMA engine uploads the previous ones.  Returns once every stage is
     // free again, so the caller's buffer and the ring can be reused.
     void upload(void* dst, const void* src, size_t bytes) {
-        if (bytes < (16ull << 20)) {
+        if (bytes < stage_bytes / 4) {
             DCX_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, stream));
             DCX_HIP(hipStreamSynchronize(stream));
             return;
         }
         alloc_stages();
-        const size_t nch = (bytes + kStageBytes - 1) / kStageBytes;
-        const unsigned hw = std::thread::hardware_concurrency();
-        const unsigned T = std::min(8u, std::max(1u, hw / 2));
+        const size_t SB = stage_bytes;
+        const size_t nch = (bytes + SB - 1) / SB;
+        const unsigned T = copy_threads();
         std::atomic<size_t> released{0};             // chunks whose upload has completed
         std::atomic<unsigned> filled[kStages];        // workers done filling the stage's current chunk
         std::atomic<int> failed{0};
         int fail_code = 0;
         for (int s = 0; s < kStages; s++) filled[s] = 0;
-        auto chunk_len = [&](size_t k) { return std::min(kStageBytes, bytes - k * kStageBytes); };
+        auto chunk_len = [&](size_t k) { return std::min(SB, bytes - k * SB); };
         std::vector<std::thread> workers;
+        Joiner joiner{workers, failed};
         workers.reserve(T);
         for (unsigned t = 0; t < T; t++) {
             workers.emplace_back([&, t]() {
@@ -168,7 +193,7 @@ class Engine {
                     const int s = (int)(k % kStages);
                     const size_t len = chunk_len(k);
                     const size_t a = len * t / T, b = len * (t + 1) / T;
-                    memcpy(h_stage[s] + a, static_cast<const uint8_t*>(src) + k * kStageBytes + a, b - a);
+                    memcpy(h_stage[s] + a, static_cast<const uint8_t*>(src) + k * SB + a, b - a);
                     filled[s].fetch_add(1);
                 }
             });
@@ -178,7 +203,7 @@ class Engine {
             const int s = (int)(k % kStages);
             while (filled[s].load() < T) std::this_thread::yield();
             filled[s] = 0;
-            hipError_t e = hipMemcpyAsync(static_cast<uint8_t*>(dst) + k * kStageBytes, h_stage[s], chunk_len(k),
+            hipError_t e = hipMemcpyAsync(static_cast<uint8_t*>(dst) + k * SB, h_stage[s], chunk_len(k),
                                           hipMemcpyHostToDevice, stream);
             if (e == hipSuccess) e = hipEventRecord(stage_ev[s], stream);
             // free the stage the workers need next: chunk k + 1 reuses the stage of chunk k + 1 - kStages
@@ -207,12 +232,16 @@ class Engine {
     void alloc_stages() {
         if (h_stage[0]) return;
         for (int s = 0; s < kStages; s++) {
-            DCX_HIP(hipHostMalloc((void**)&h_stage[s], kStageBytes, hipHostMallocDefault));
+            DCX_HIP(hipHostMalloc((void**)&h_stage[s], stage_bytes, hipHostMallocDefault));
             DCX_HIP(hipEventCreateWithFlags(&stage_ev[s], hipEventDisableTiming));
         }
     }
 
     explicit Engine(int dev) : device(dev) {
+        if (const char* e = getenv("DCX_STAGE_BYTES")) {
+            const unsigned long long v = strtoull(e, nullptr, 10);
+            if (v >= 4096 && v <= (1ull << 30)) stage_bytes = (size_t)(v & ~4095ull);
+        }
         DCX_HIP(hipSetDevice(device));
         DCX_HIP(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
         prim.init(stream, &cache);

@@ -21,7 +21,7 @@ from dataclasses import dataclass, field
 
 import numpy as np
 
-from .engine import DCX_E_ADMISSION, DCX_E_CANCELLED, DCXError, Engine
+from .engine import DCX_E_ADMISSION, DCX_E_CANCELLED, DCX_E_HIP, DCXError, Engine
 
 # per device: [stream, bytes reserved in torch's allocator for it] of the worker streams, shared by servers
 _STREAMS: dict = {}
@@ -43,7 +43,7 @@ class Request:
     arrival: float = 0.0
     admitted: float = math.nan
     done: float = math.nan
-    sa: "object" = None             # torch.int32 CUDA tensor (uint32 positions) when finished
+    sa: "object" = None             # CUDA tensor of positions when finished: int32 (sa_width 4) or int64 (8)
     error: int = 0
     sigma_class: int = -1           # packing width class of the alphabet (3/5/8/9-bit characters)
     batch: int = -1
@@ -64,9 +64,18 @@ class SAServer:
 
     def __init__(self, workers: int = 2, hbm_budget: int = 80 * 10 ** 9, max_batch_texts: int = 1024,
                  max_batch_chars: int = 64 << 20, solo_chars: int = 4 << 20, device: int = 0,
-                 batching: bool = True, warm_chars: int = 0, reserve_bytes: int = 0, **build_options):
+                 batching: bool = True, warm_chars: int = 0, reserve_bytes: int = 0, sa_width: int = 4,
+                 batch_window_ms: float = 0.0, **build_options):
         import torch
+        if sa_width not in (4, 8):
+            raise ValueError("sa_width must be 4 (uint32 positions) or 8 (uint64 positions)")
         self.torch = torch
+        self.sa_width = int(sa_width)
+        # while another build runs, a small queue head waits up to this long for more small requests to
+        # join its batch (0: admitted at once)
+        self.window = float(batch_window_ms) / 1e3
+        self.worker_errors: list[str] = []  # unexpected exceptions of failed builds (their requests get DCX_E_HIP)
+        self.running = 0  # builds in flight
         self.budget = int(hbm_budget)
         self.free = int(hbm_budget)
         self.max_texts = max_batch_texts if batching else 1
@@ -203,27 +212,30 @@ class SAServer:
         batching at most one large build runs at a time: it gets the whole GPU, as in the plain path, while
         the other workers take small requests.  A small request is joined by later small requests of the
         same alphabet class while their footprint fits the free budget and the batch limits; small requests
-        may pass a large queue head that waits for the running large build."""
+        may pass a large queue head that waits for the running large build.  With a batch window, a small
+        head that is not yet a full batch waits while another build runs, until it has waited the window:
+        the requests arriving meanwhile join it.  Returns the batch (empty: nothing to admit) and how long
+        the worker may sleep before looking again."""
         if not self.queue:
-            return []
+            return [], 0.05
         head = self.queue[0]
         if head.n >= self.solo_chars:
             if not (self.batching and self.large_running):
                 fp = footprint(head.n)
                 if fp > self.free:
-                    return []
+                    return [], 0.05
                 self.queue.remove(head)
                 self.free -= fp
                 self.large_running += 1
                 head.large = True
-                return [head]
+                return [head], 0.0
             smalls = [r for r in self.queue if r.n < self.solo_chars]
             if not smalls:
-                return []
+                return [], 0.05
             head = smalls[0]
         fp = footprint(head.n)
         if fp > self.free:
-            return []
+            return [], 0.05
         batch, chars, need = [head], head.n + 1, fp
         if self.batching:
             for r in self.queue[self.queue.index(head) + 1:]:
@@ -237,10 +249,14 @@ class SAServer:
                 batch.append(r)
                 chars += r.n + 1
                 need += fr
+            if self.window > 0 and self.running > 0 and not self.stop and len(batch) < self.max_texts:
+                wait = head.arrival + self.window - time.perf_counter()
+                if wait > 0 and 2 * chars < self.max_chars:
+                    return [], wait
         for r in batch:
             self.queue.remove(r)
         self.free -= need
-        return batch
+        return batch, 0.0
 
     def _worker(self, wid: int):
         torch = self.torch
@@ -251,47 +267,69 @@ class SAServer:
                 while True:
                     if self.stop and not self.queue:
                         return
-                    batch = self._take()
+                    batch, wait = self._take()
                     if batch:
                         break
-                    self.cv.wait(0.05)
+                    self.cv.wait(wait)
                 bid = self.batches
                 self.batches += 1
+                self.running += 1
             now = time.perf_counter()
             for r in batch:
                 r.admitted = now
                 r.batch = bid
             err = 0
             try:
-                with torch.cuda.stream(stream):
-                    if self.batching and len(batch) > 1:
-                        joined = torch.cat([r.text for r in batch])
-                        total = int(joined.numel())
-                        out = torch.empty(max(total, 1), dtype=torch.int32, device=joined.device)
-                        offsets = np.zeros(len(batch) + 1, dtype=np.uint64)
-                        offsets[1:] = np.cumsum([r.n for r in batch])
-                        stream.synchronize()
-                        eng.build_sa_batch_device(joined.data_ptr(), offsets, out.data_ptr(), **self.opts)
-                        for k, r in enumerate(batch):
-                            r.sa = out[int(offsets[k]):int(offsets[k + 1])]
-                    else:
-                        r = batch[0]
-                        out = torch.empty(max(r.n, 1), dtype=torch.int32, device=r.text.device)
-                        stream.synchronize()
-                        eng.build_sa_device(r.text.data_ptr(), r.n, out.data_ptr(), 4, **self.opts)
-                        r.sa = out[:r.n]
+                self._build(eng, stream, batch)
             except DCXError as e:
                 err = e.code
-            done = time.perf_counter()
-            with self.cv:
-                self.free += sum(footprint(r.n) for r in batch)
-                if batch[0].large:
-                    self.large_running -= 1
-                self.cv.notify_all()
-            for r in batch:
-                r.error = err
-                r.done = done
-                r.event.set()
+            except Exception as e:  # torch OOM, ctypes errors ...: the request fails, the server lives on
+                err = DCX_E_HIP
+                self.worker_errors.append(repr(e))
+            finally:
+                # the budget, the large-build slot and the waiting clients are released whatever happened
+                done = time.perf_counter()
+                with self.cv:
+                    self.free += sum(footprint(r.n) for r in batch)
+                    self.running -= 1
+                    if batch[0].large:
+                        self.large_running -= 1
+                    self.cv.notify_all()
+                for r in batch:
+                    if err:
+                        r.sa = None
+                    r.error = err
+                    r.done = done
+                    r.event.set()
+
+    def _build(self, eng: Engine, stream, batch: list[Request]):
+        """Builds the suffix arrays of one admitted batch on the worker's engine and stream."""
+        torch = self.torch
+        with torch.cuda.stream(stream):
+            if self.batching and len(batch) > 1:
+                joined = torch.cat([r.text for r in batch])
+                total = int(joined.numel())
+                out = torch.empty(max(total, 1), dtype=torch.int32, device=joined.device)
+                offsets = np.zeros(len(batch) + 1, dtype=np.uint64)
+                offsets[1:] = np.cumsum([r.n for r in batch])
+                stream.synchronize()
+                eng.build_sa_batch_device(joined.data_ptr(), offsets, out.data_ptr(), **self.opts)
+                if self.sa_width == 8:
+                    # per-text positions are below 2^32: zero extension on the worker's stream
+                    wide = out[:total].to(torch.int64) & 0xFFFFFFFF
+                    for k, r in enumerate(batch):
+                        r.sa = wide[int(offsets[k]):int(offsets[k + 1])]
+                    stream.synchronize()
+                else:
+                    for k, r in enumerate(batch):
+                        r.sa = out[int(offsets[k]):int(offsets[k + 1])]
+            else:
+                r = batch[0]
+                dt = torch.int64 if self.sa_width == 8 else torch.int32
+                out = torch.empty(max(r.n, 1), dtype=dt, device=r.text.device)
+                stream.synchronize()
+                eng.build_sa_device(r.text.data_ptr(), r.n, out.data_ptr(), self.sa_width, **self.opts)
+                r.sa = out[:r.n]
 
 
 def poisson_arrivals(count: int, rate: float, seed: int) -> np.ndarray:

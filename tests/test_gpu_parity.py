@@ -452,9 +452,9 @@ def test_same_calls_as_host_abi(engine):
 
 def test_staged_host_download_all_widths(engine):
     """Host texts and outputs large enough for the pinned-ring upload and download (64 MiB chunks, several host
-    threads per chunk, a partial last chunk) give outputs equal the device output of the same build, byte for byte, in widths 4, 5 and 8,
-    into a fresh buffer and again into the same buffer; a batch whose sub-batch output crosses chunks
-    equals its per-text builds."""
+    threads per chunk, a partial last chunk) give outputs equal to the device output of the same build, byte
+    for byte, in widths 4, 5 and 8, twice in a row; a batch whose sub-batch output crosses chunks equals its
+    per-text builds."""
     import torch
     n = 20_000_003  # 80, 100 and 160 MB of output: partial last chunks, slices of uneven length
     t = copies_text(n, 21, sigma=4, copy_frac=0.05, max_len=5000)
@@ -475,3 +475,46 @@ def test_staged_host_download_all_widths(engine):
     sas = engine.build_sa_batch(texts, batch_max_chars=1 << 31)
     for k, x in enumerate(texts):
         assert np.array_equal(np.asarray(sas[k]).astype(np.uint64), engine.build_sa(x, sa_width=8, X=39)), k
+
+
+def test_staged_ring_wraps_many_times():
+    """With 64 KiB stages (DCX_STAGE_BYTES, read when a handle is created) a 1 MB text goes up in 16 chunks
+    and its SA comes down in 62-123 chunks: both rings of four stages wrap many times, every stage is reused,
+    and the host threads copy slices of uneven length (n * width is not a multiple of the stage).  The
+    outputs, written twice into the same host buffer, equal the device output of the same build."""
+    import ctypes
+    import os
+
+    import torch
+    from distributed_suffix_sorting_evaluation_amd import Engine
+    from distributed_suffix_sorting_evaluation_amd import engine as eng_mod
+    os.environ["DCX_STAGE_BYTES"] = "65536"
+    try:
+        eng = Engine(0)
+    finally:
+        del os.environ["DCX_STAGE_BYTES"]
+    try:
+        n = 1_000_003
+        t = copies_text(n, 7, sigma=4, copy_frac=0.05, max_len=3000)
+        dt = torch.from_numpy(t).cuda()
+        for width in (4, 5, 8):
+            say(f"staged ring 64 KiB n={n} width={width}")
+            buf = torch.zeros(width * n, dtype=torch.uint8, device="cuda")
+            eng.build_sa_device(dt.data_ptr(), n, buf.data_ptr(), width, X=39)
+            torch.cuda.synchronize()
+            want = buf.cpu().numpy().tobytes()
+            out = np.full(width * n, 0xAB, dtype=np.uint8)
+            for _ in range(2):
+                o = eng.make_options(X=39)
+                rc = eng._L.dcx_build_sa(eng._h, t.ctypes.data, n, ctypes.byref(o), out.ctypes.data, width,
+                                         ctypes.byref(eng_mod.Stats()))
+                assert rc == 0
+                assert out.tobytes() == want, width
+                out[::4097] ^= 0x5A  # disturb the buffer before the second call
+        say("staged ring 64 KiB batch")
+        texts = [copies_text(m, 60 + m % 13, sigma=4, copy_frac=0.05, max_len=2000) for m in (300_001, 250_000, 410_007)]
+        sas = eng.build_sa_batch(texts, batch_max_chars=1 << 31)
+        for k, x in enumerate(texts):
+            assert np.array_equal(np.asarray(sas[k]).astype(np.uint64), oracle.libsais_sa(x)), k
+    finally:
+        eng.close()

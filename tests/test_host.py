@@ -124,7 +124,11 @@ def test_serving_admission_order():
         s = object.__new__(SAServer)
         s.batching, s.solo_chars, s.max_texts, s.max_chars = batching, 4 << 20, 1024 if batching else 1, 64 << 20
         s.free, s.budget, s.large_running, s.queue, s.cv = free, free, 0, [], threading.Condition()
+        s.window, s.running, s.stop = 0.0, 0, False
         return s
+
+    def take(s):
+        return s._take()[0]
 
     def req(n, cls=0):
         return Request(text=Text(n), sigma_class=cls)
@@ -132,20 +136,100 @@ def test_serving_admission_order():
     A, b, C, d, e = req(8 << 20), req(1 << 20), req(16 << 20), req(2 << 20), req(1 << 20, 1)
     s = server()
     s.queue = [A, b, C, d, e]
-    assert s._take() == [A] and A.large and s.large_running == 1
-    assert s._take() == [b, d]          # small requests pass the large head C while A runs
+    assert take(s) == [A] and A.large and s.large_running == 1
+    assert take(s) == [b, d]          # small requests pass the large head C while A runs
     assert s.queue == [C, e]
-    assert s._take() == [e]
-    assert s._take() == [] and s.queue == [C]  # C waits for A
+    assert take(s) == [e]
+    assert take(s) == [] and s.queue == [C]  # C waits for A
     s.large_running = 0
-    assert s._take() == [C]
+    assert take(s) == [C]
     # plain mode: strict arrival order, one request per build
     p = server(batching=False)
     p.queue = [req(8 << 20), req(1 << 20), req(16 << 20)]
-    got = [p._take()[0].n for _ in range(3)]
+    got = [take(p)[0].n for _ in range(3)]
     assert got == [8 << 20, 1 << 20, 16 << 20]
     # budget: a head that does not fit waits, nothing passes it
     q = server(free=footprint(1 << 20))
     big, small = req(2 << 20), req(1 << 10)
     q.queue = [big, small]
-    assert q._take() == [] and q.queue == [big, small]
+    assert take(q) == [] and q.queue == [big, small]
+
+
+def test_serving_batch_window():
+    """With a batch window, a small head waits while another build runs (so later small requests join it),
+    and is admitted at once on an idle GPU or when the window has passed."""
+    import threading
+    import time
+    from distributed_suffix_sorting_evaluation_amd.serve import Request, SAServer
+
+    class Text:
+        def __init__(self, n):
+            self.n = n
+
+        def numel(self):
+            return self.n
+
+    s = object.__new__(SAServer)
+    s.batching, s.solo_chars, s.max_texts, s.max_chars = True, 4 << 20, 1024, 64 << 20
+    s.free, s.budget, s.large_running, s.queue, s.cv = 10 ** 12, 10 ** 12, 0, [], threading.Condition()
+    s.window, s.running, s.stop = 0.010, 1, False
+    now = time.perf_counter()
+    a = Request(text=Text(1 << 16), sigma_class=0, arrival=now)
+    s.queue = [a]
+    batch, wait = s._take()
+    assert batch == [] and 0 < wait <= 0.010 and s.queue == [a]
+    b = Request(text=Text(1 << 16), sigma_class=0, arrival=now)
+    s.queue.append(b)
+    s.running = 0                      # the GPU went idle: admitted at once, with the request that joined
+    assert s._take() == ([a, b], 0.0)
+    c = Request(text=Text(1 << 16), sigma_class=0, arrival=now - 1.0)
+    s.queue, s.running = [c], 1        # waited longer than the window already
+    assert s._take()[0] == [c]
+
+
+def test_serving_worker_survives_unexpected_errors():
+    """A build that raises something other than DCXError (a torch OOM, a ctypes error) fails its requests
+    with DCX_E_HIP, returns their budget and frees the large-build slot; the worker keeps serving."""
+    import threading
+    from distributed_suffix_sorting_evaluation_amd.engine import DCX_E_HIP
+    from distributed_suffix_sorting_evaluation_amd.serve import Request, SAServer
+
+    class Text:
+        def __init__(self, n):
+            self.n = n
+
+        def numel(self):
+            return self.n
+
+    s = object.__new__(SAServer)
+    s.batching, s.solo_chars, s.max_texts, s.max_chars = True, 4 << 20, 1024, 64 << 20
+    s.free, s.budget, s.large_running, s.queue, s.cv = 10 ** 12, 10 ** 12, 0, [], threading.Condition()
+    s.window, s.running, s.stop, s.batches, s.worker_errors = 0.0, 0, False, 0, []
+    s.torch, s.engines, s.streams = None, [None], [None]
+    calls = []
+
+    def build(eng, stream, batch):
+        calls.append([r.n for r in batch])
+        if len(calls) == 1:
+            raise RuntimeError("HIP out of memory")
+        for r in batch:
+            r.sa = "ok"
+
+    s._build = build
+    big, small = Request(text=Text(8 << 20)), Request(text=Text(1 << 16), sigma_class=0)
+    s.queue = [big]
+    t = threading.Thread(target=s._worker, args=(0,))
+    t.start()
+    big.event.wait(5)
+    with s.cv:
+        s.queue.append(small)
+        s.cv.notify_all()
+    small.event.wait(5)
+    with s.cv:
+        s.stop = True
+        s.cv.notify_all()
+    t.join(5)
+    assert not t.is_alive()
+    assert big.error == DCX_E_HIP and big.sa is None and "out of memory" in s.worker_errors[0]
+    assert small.error == 0 and small.sa == "ok"
+    assert s.free == s.budget and s.large_running == 0 and s.running == 0
