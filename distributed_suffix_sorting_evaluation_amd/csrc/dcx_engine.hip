@@ -46,10 +46,10 @@ constexpr uint32_t PACKED_SLACK = DCX_PACKED_SLACK;
 constexpr uint32_t LARGE_PARTITION_MIN = 1u << 16;
 constexpr uint32_t LARGE_PARTITION_AVG = 64;
 
-// Longest text (and longest joined batch, plus one separator) of this build: positions, counts and the
-// grid-stride loops of the kernels are 32-bit, and a loop of at most 8192 x 256 threads stays clear of
-// wrap-around while the element count plus 2^21 is below 2^32.
-constexpr uint64_t MAX_TEXT = (1ull << 32) - (1ull << 22);
+// Longest text (and longest joined batch, separators included) of this build: positions and the values of
+// the sorts are 32-bit (a text of 2^32 characters has positions 0 .. 2^32 - 1), while counts, offsets and
+// the grid-stride loops of the kernels are 64-bit.
+constexpr uint64_t MAX_TEXT = 1ull << 32;
 
 class Engine {
   public:
@@ -305,14 +305,19 @@ class Engine {
     // -------------------------------------------------------------------------------------------
     // active sets: members of groups of size >= 2 of the heads array H, as positions and group ids
     // -------------------------------------------------------------------------------------------
+    // List indices, group starts and group ids are 32-bit: a list holds at most 2^32 entries (gstart[G] = A
+    // is then stored as 0; group sizes are differences of starts modulo 2^32, and no group holds all 2^32
+    // positions: round 0 tells the last position's key, which reaches past the text end, from the others')
+    // and at most 2^31 groups.
     struct ActiveSet {
         DBuf<uint32_t> act, gid, gstart, gfirst;  // gstart: G + 1 entries, the last one A; gfirst: G entries
-        uint32_t A = 0, G = 0;
+        uint64_t A = 0;
+        uint32_t G = 0;
     };
 
     // restricted to the previous active list `src` (its superset) or to all N positions (src == nullptr).
     // H: the heads of all positions, or with `compact` the heads of the list entries (k_writeback's Ha).
-    void active_set(const uint8_t* H, const uint32_t* src, uint32_t Nsrc, ActiveSet& out, bool compact = false) {
+    void active_set(const uint8_t* H, const uint32_t* src, uint64_t Nsrc, ActiveSet& out, bool compact = false) {
         out.act = DBuf<uint32_t>(cache, Nsrc);
         out.gid = DBuf<uint32_t>(cache, Nsrc);
         out.gstart = DBuf<uint32_t>(cache, (size_t)Nsrc / 2 + 1);  // groups have >= 2 members
@@ -323,10 +328,10 @@ class Engine {
         else
             prim.scan(ActiveScanOp<false>{H, src, out.act.get(), out.gid.get(), out.gstart.get(), out.gfirst.get()},
                       Nsrc, 0);
-        const uint64_t tot = prim.read_scalars(1)[0];
-        out.A = (uint32_t)tot;
-        out.G = (uint32_t)(tot >> 32);
-        prim.fill_u32(out.gstart.get() + out.G, out.A, 1);
+        const auto tot = prim.totals(0);
+        out.A = tot.lo;
+        out.G = (uint32_t)tot.hi;
+        prim.fill_u32(out.gstart.get() + out.G, (uint32_t)out.A, 1);
     }
 
     // -------------------------------------------------------------------------------------------
@@ -341,7 +346,7 @@ class Engine {
     // ahead (exact: the last step takes only the L - C characters still missing).  Doubling needs the
     // sentinel coding (dense = false): the empty suffix past the end must be the smallest key.
     // `info`: estimated bits of information per character (at most `bits`), which sizes round 0.
-    void refine_sort(const uint64_t* PT, uint32_t N, uint32_t L, uint32_t bits, bool dense, double info,
+    void refine_sort(const uint64_t* PT, uint64_t N, uint32_t L, uint32_t bits, bool dense, double info,
                      DBuf<uint32_t>& order, DBuf<uint8_t>& H, ActiveSet& fin) {
         const uint32_t cpw = 64 / bits;
         uint32_t c0 = L < cpw ? L : cpw;
@@ -406,7 +411,7 @@ class Engine {
         active_set(H.get(), nullptr, N, cur);
         DBuf<uint32_t> G;
         while (C < L && cur.A > 0) {
-            const uint32_t A = cur.A;
+            const uint64_t A = cur.A;
             const uint32_t gbits = bit_width(cur.G - 1);
             uint32_t c = (64 - gbits) / bits;
             if (c > L - C) c = L - C;
@@ -415,7 +420,7 @@ class Engine {
                 // doubling step
                 if (!G.get()) G = DBuf<uint32_t>(cache, N);
                 prim.scan(GroupScatterOp{H.get(), order.get(), G.get()}, N, 1);
-                const uint32_t groups = (uint32_t)prim.read_scalars(2)[1];
+                const uint64_t groups = prim.totals(1).lo;
                 const uint32_t off = std::min(C, L - C);
                 const uint32_t gvb = bit_width(groups);
                 DBuf<uint64_t> rka(cache, A), rkb(cache, A);
@@ -463,7 +468,7 @@ class Engine {
     // are i).  Word-sized writes to random places cost a cache line each; instead an LSD radix sort by the
     // high bits of dest (all but the low SCATTER_WINDOW_BITS or fewer) orders the pairs by destination
     // window, and each window is assembled in LDS and written out whole.
-    void scatter_u32(const uint32_t* dest, const uint32_t* val, uint32_t N, uint32_t add, uint32_t* out) {
+    void scatter_u32(const uint32_t* dest, const uint32_t* val, uint64_t N, uint32_t add, uint32_t* out) {
         if (N == 0) return;
         const uint32_t bw = bit_width(N - 1);
         if (N < SCATTER_PARTITION_MIN || bw <= SCATTER_WINDOW_BITS) {
@@ -491,17 +496,19 @@ class Engine {
     // characters), or two stable sorts when those do not fit 64 bits.
     void refine_round(const uint64_t* PT, uint32_t bits, uint32_t C, uint32_t c, const ActiveSet& cur,
                       DBuf<uint32_t>& order, DBuf<uint8_t>& H, DBuf<uint8_t>& Ha) {
-        const uint32_t A = cur.A, G = cur.G;
+        const uint64_t A = cur.A;
+        const uint32_t G = cur.G;
         DBuf<uint8_t> large(cache, G);
         k_refine_small<<<grid(G), 256, 0, stream>>>(cur.gstart.get(), G, cur.gfirst.get(), order.get(), H.get(),
                                                     Ha.get(), PT, bits, C, c * bits, large.get());
         prim.fill_u8(Ha.get() + A, 1, 1);
         DBuf<uint32_t> slots(cache, A), lrank(cache, A), lfirst(cache, (size_t)G + 1);
         prim.scan(LargeMembersOp{large.get(), cur.gid.get(), slots.get(), lrank.get(), lfirst.get()}, A, 1);
-        const uint64_t tot = prim.read_scalars(2)[1];
-        const uint32_t B = (uint32_t)tot, nlarge = (uint32_t)(tot >> 32);
+        const auto tot = prim.totals(1);
+        const uint64_t B = tot.lo;
+        const uint32_t nlarge = (uint32_t)tot.hi;
         if (B == 0) return;
-        prim.fill_u32(lfirst.get() + nlarge, B, 1);
+        prim.fill_u32(lfirst.get() + nlarge, (uint32_t)B, 1);  // B = 2^32 reads as 0: spans are differences
         const uint32_t lbits = bit_width(nlarge - 1);
         const uint32_t cbits = c * bits;
         DBuf<uint64_t> rka(cache, B), rkb(cache, B);
@@ -541,19 +548,19 @@ class Engine {
     // and suffixes; x2, y2 (B entries each) take the compacted minority.  Returns false (nothing written,
     // x and y untouched) when the caller should sort all members instead.
     bool partition_large(const ActiveSet& cur, const uint32_t* slots, const uint32_t* lrank, const uint32_t* lfirst,
-                         const uint8_t* cls, uint32_t B, uint32_t lo_bit, const uint64_t* x, const uint32_t* y,
+                         const uint8_t* cls, uint64_t B, uint32_t lo_bit, const uint64_t* x, const uint32_t* y,
                          uint64_t* x2, uint32_t* y2, DBuf<uint32_t>& order, DBuf<uint8_t>& H, DBuf<uint8_t>& Ha) {
         DBuf<uint32_t> minpref(cache, (size_t)B + 1), c0pref(cache, (size_t)B + 1);
         prim.scan(PivotScanOp{cls, x, minpref.get(), c0pref.get(), x2, y2}, B, 1);
-        const uint64_t tot = prim.read_scalars(2)[1];
-        const uint32_t Bm = (uint32_t)tot;
-        if ((uint64_t)Bm * 2 > B) return false;
+        const auto tot = prim.totals(1);
+        const uint64_t Bm = tot.lo;
+        if (Bm * 2 > B) return false;
         if (Bm == 0) {
             k_uniform_heads<<<grid(B), 256, 0, stream>>>(B, slots, lrank, lfirst, Ha.get());
             return true;
         }
-        prim.fill_u32(minpref.get() + B, Bm, 1);
-        prim.fill_u32(c0pref.get() + B, (uint32_t)(tot >> 32), 1);
+        prim.fill_u32(minpref.get() + B, (uint32_t)Bm, 1);
+        prim.fill_u32(c0pref.get() + B, (uint32_t)tot.hi, 1);
         DBuf<uint64_t> mk2(cache, Bm);
         DBuf<uint32_t> mi2(cache, Bm);
         uint64_t* mk = x2;
@@ -571,10 +578,10 @@ class Engine {
     // dense words only: boundary suffixes that share a key group are moved to its front (k_boundary_fix)
     // The key groups (after the refinement over L characters of the dense stream PT) that hold boundary
     // suffixes, found by searching the sorted order for each boundary suffix's key (k_boundary_find).
-    void boundary_fixup(const uint64_t* PT, uint32_t bits, uint32_t L, uint32_t n, uint32_t bnd,
+    void boundary_fixup(const uint64_t* PT, uint32_t bits, uint32_t L, uint64_t n, uint64_t bnd,
                         DBuf<uint32_t>& order, DBuf<uint8_t>& H, ActiveSet& tied) {
         if (bnd >= n) return;
-        const uint32_t nb = n - bnd;  // boundary suffixes: at most L - 1
+        const uint32_t nb = (uint32_t)(n - bnd);  // boundary suffixes: at most L - 1
         DBuf<uint32_t> found(cache, 2 * (size_t)nb), cnt(cache, 1);
         prim.fill_u32(cnt.get(), 0, 1);
         k_boundary_find<<<(nb + 63) / 64, 64, 0, stream>>>(PT, bits, L, order.get(), n, bnd, found.get(), cnt.get(), nb);
@@ -586,12 +593,13 @@ class Engine {
         std::vector<uint32_t> rg(2 * (size_t)hc);
         DCX_HIP(hipMemcpyAsync(rg.data(), found.get(), 8 * (size_t)hc, hipMemcpyDeviceToHost, stream));
         DCX_HIP(hipStreamSynchronize(stream));
-        std::vector<std::pair<uint32_t, uint32_t>> g;
+        std::vector<std::pair<uint32_t, uint32_t>> g;  // (first slot, size)
         for (uint32_t i = 0; i < hc; i++) g.emplace_back(rg[2 * i], rg[2 * i + 1]);
         std::sort(g.begin(), g.end());
         g.erase(std::unique(g.begin(), g.end()), g.end());
         for (const auto& r : g) {
-            const uint32_t j0 = r.first, sz = r.second - r.first;
+            const uint64_t j0 = r.first;
+            const uint32_t sz = r.second;
             DBuf<uint32_t> tmp(cache, sz);
             k_boundary_fix<<<1, 1024, 0, stream>>>(order.get(), H.get(), j0, sz, bnd, tmp.get());
         }
@@ -603,7 +611,8 @@ class Engine {
     // groups of equal merge prefixes -> DC comparator on the sample ranks (tie_break_ranks,
     // merge_samples.hpp:265-333)
     void tie_break(const DcDev& dc, DBuf<uint32_t>& order, DBuf<uint8_t>& H, ActiveSet& as, const uint32_t* ranks) {
-        const uint32_t A = as.A, G = as.G;
+        const uint64_t A = as.A;
+        const uint32_t G = as.G;
         if (A == 0) return;
         DBuf<uint8_t> large(cache, G);
         k_tiebreak_small<<<grid(G), 256, 0, stream>>>(as.gstart.get(), G, as.gfirst.get(), order.get(), dc, d_cmp.get(),
@@ -613,13 +622,13 @@ class Engine {
         DBuf<uint32_t> slots(cache, A);
         LargeScanOp lop{large.get(), as.gid.get(), as.act.get(), order.get(), items.get(), slots.get()};
         prim.scan(lop, A, 1);
-        const uint32_t B = (uint32_t)prim.read_scalars(2)[1];
+        const uint64_t B = prim.totals(1).lo;
         if (B == 0) return;
         k_tb_tile_sort<<<(unsigned)ceil_div(B, TB_TILE), 256, 0, stream>>>(items.get(), B, dc, d_cmp.get(), ranks);
         uint64_t* in = items.get();
         uint64_t* out = items2.get();
         for (uint64_t w = TB_TILE; w < B; w *= 2) {
-            k_tb_merge<<<grid(B), 256, 0, stream>>>(in, out, B, (uint32_t)w, dc, d_cmp.get(), ranks);
+            k_tb_merge<<<grid(B), 256, 0, stream>>>(in, out, B, w, dc, d_cmp.get(), ranks);
             std::swap(in, out);
         }
         k_tb_scatter<<<grid(B), 256, 0, stream>>>(in, slots.get(), B, order.get());
@@ -647,13 +656,13 @@ class Engine {
     // phase 3 when the names are not all distinct: recursive string, discarding decision, recursion,
     // sample ranks (handle_recursive_call, pdcx.hpp:145-180).  s_order: the samples in name order (their
     // text-order sample indices); names: their names; Hs: their group heads; ranks: m + D entries out.
-    void recursion(const DcDev& dc, uint32_t n, uint32_t m, uint32_t M, int depth, DBuf<uint32_t>& s_order,
+    void recursion(const DcDev& dc, uint64_t n, uint32_t m, uint32_t M, int depth, DBuf<uint32_t>& s_order,
                    DBuf<uint32_t>& names, DBuf<uint8_t>& Hs, DBuf<uint32_t>& ranks) {
         dcx_level_stats* ls = st && depth < DCX_MAX_LEVELS ? &st->level[depth] : nullptr;
         const double thr = opt.discarding_threshold;
         // the kept set of discarding, from the non-unique samples: after >= NU and after <= 2 NU + 1
         prim.total(NonUniqueOp{Hs.get(), s_order.get(), dc, m, nullptr}, m, 0);
-        const uint32_t nu = (uint32_t)prim.read_scalars(1)[0];
+        const uint32_t nu = (uint32_t)prim.totals(0).lo;
         DBuf<uint32_t> K;
         uint32_t after = m;
         if (n > 80 && (double)nu / (double)m <= thr) {
@@ -667,7 +676,7 @@ class Engine {
             prim.sort_u32(cand.get(), nullptr, k1.get(), v1.get(), k2.get(), v2.get(), nc, 0, (cb + 7) / 8, &ks, &vs);
             K = DBuf<uint32_t>(cache, nc);
             prim.scan(DistinctOp{ks, K.get()}, nc, 0);
-            after = (uint32_t)prim.read_scalars(1)[0];
+            after = (uint32_t)prim.totals(0).lo;
         }
         const bool use_disc = n > 80 && ((double)after / (double)m) <= thr;
         if (!use_disc) {
@@ -711,7 +720,8 @@ class Engine {
         // order of (name, rank2) = name order refined inside non-unique groups
         ActiveSet stied;
         active_set(Hs.get(), nullptr, m, stied);
-        const uint32_t A = stied.A, G = stied.G;
+        const uint64_t A = stied.A;
+        const uint32_t G = stied.G;
         if (A == 0) return;
         const uint32_t gb = bit_width(G - 1), r2b = bit_width(after);
         DBuf<uint64_t> ka(cache, A), kb(cache, A);
@@ -744,7 +754,7 @@ class Engine {
     // -------------------------------------------------------------------------------------------
     // order-0 entropy (bits per character, within [1, bits]) of a byte text from a strided sample of its
     // codes; names of deeper levels are taken at their full width
-    double info_bits(const uint8_t* text, const uint16_t* lut, uint32_t n, uint32_t bits) {
+    double info_bits(const uint8_t* text, const uint16_t* lut, uint64_t n, uint32_t bits) {
         if (n < (1u << 24)) return bits;
         const uint64_t stride = 61;  // odd: no alignment with periodic structure
         DBuf<unsigned long long> hist(cache, 257);
@@ -762,11 +772,11 @@ class Engine {
             }
         return std::min<double>(bits, std::max(1.0, e));
     }
-    double info_bits(const uint32_t*, const uint16_t*, uint32_t, uint32_t bits) { return bits; }
+    double info_bits(const uint32_t*, const uint16_t*, uint64_t, uint32_t bits) { return bits; }
 
     // the level's characters as a packed code stream (k_pack_text) of `chars` characters
     template <class CharT>
-    void make_stream(const CharT* text, const uint16_t* lut, uint32_t n, uint32_t bits, uint32_t dense, uint64_t chars,
+    void make_stream(const CharT* text, const uint16_t* lut, uint64_t n, uint32_t bits, uint32_t dense, uint64_t chars,
                      DBuf<uint64_t>& PT) {
         const uint64_t nwords = stream_words(chars, bits);
         PT = DBuf<uint64_t>(cache, nwords);
@@ -786,7 +796,7 @@ class Engine {
     // characters of the level are 1..maxchar (0 is the padding)
     template <class CharT>
     // The SA goes to sa_out, or with sa_take != nullptr into *sa_take (the level's own buffer, no copy).
-    void level(const CharT* text, const uint16_t* lut, uint32_t n, uint32_t maxchar, int depth, uint32_t* sa_out,
+    void level(const CharT* text, const uint16_t* lut, uint64_t n, uint32_t maxchar, int depth, uint32_t* sa_out,
                hipEvent_t* ev, DBuf<uint32_t>* sa_take = nullptr) {
         const uint32_t X = dcs->X, D = dcs->D;
         const uint32_t bits = bit_width(maxchar);
@@ -803,7 +813,7 @@ class Engine {
                 *sa_take = DBuf<uint32_t>(cache, n > 0 ? n : 1);
                 sa_out = sa_take->get();
             }
-            base_case(text, lut, n, sa_out);
+            base_case(text, lut, (uint32_t)n, sa_out);
             return;
         }
         uint64_t m64 = 0;
@@ -823,7 +833,7 @@ class Engine {
         }
         const uint32_t Ls = (uint32_t)((double)X * (ps ? ratio : 1.0));
         const uint32_t Lm = (uint32_t)((double)X * (pm ? ratio : 1.0)) - 1;
-        dc.bnd = n - Ls + 1;
+        dc.bnd = (uint32_t)(n - Ls + 1);  // n <= 2^32 and Ls >= 2
         if (ls) {
             ls->samples = m;
             ls->key_chars = Ls;
@@ -835,7 +845,7 @@ class Engine {
             const uint64_t c = depth == 0 ? 1 : (bits + 7) / 8;
             const uint64_t K1 = 8 * ceil_div((uint64_t)X * b, 64), K4 = 8 * ceil_div((uint64_t)(X - 1) * b, 64);
             const uint64_t I = 4, S = depth == 0 ? sa_width : I;
-            alg_bytes += (uint64_t)n * c + 2ull * m * (K1 + I) + 2ull * n * (K4 + I) + (uint64_t)n * S;
+            alg_bytes += n * c + 2ull * m * (K1 + I) + 2ull * n * (K4 + I) + n * S;
         }
 
         // ---------------- phase 1 + 2: all suffixes sorted by their first L_s characters ----------------
@@ -847,7 +857,7 @@ class Engine {
         ActiveSet tied;
         {
             DBuf<uint64_t> PT;
-            make_stream(text, lut, n, mbits, dense ? 1 : 0, (uint64_t)n + Ls + 2, PT);
+            make_stream(text, lut, n, mbits, dense ? 1 : 0, n + Ls + 2, PT);
             refine_sort(PT.get(), n, Ls, mbits, dense, info_bits(text, lut, n, mbits), order, H, tied);
             if (dense && tied.A > 0) boundary_fixup(PT.get(), mbits, Ls, n, dc.bnd, order, H, tied);
         }
@@ -868,7 +878,7 @@ class Engine {
             s_gid.reset();
             DBuf<uint32_t> names(cache, m);
             prim.scan(NamesOp{Hs.get(), names.get()}, m, 0);
-            const uint32_t M = (uint32_t)prim.read_scalars(1)[0];
+            const uint32_t M = (uint32_t)prim.totals(0).lo;
             if (ls) ls->names = M;
             if (ev && depth == 0) DCX_HIP(hipEventRecord(ev[2], stream));
 
@@ -890,7 +900,7 @@ class Engine {
         if (sa_take)
             *sa_take = std::move(order);
         else
-            DCX_HIP(hipMemcpyAsync(sa_out, order.get(), (size_t)n * 4, hipMemcpyDeviceToDevice, stream));
+            DCX_HIP(hipMemcpyAsync(sa_out, order.get(), n * 4, hipMemcpyDeviceToDevice, stream));
     }
 
     // -------------------------------------------------------------------------------------------
@@ -942,7 +952,7 @@ class Engine {
         const uint64_t base = offsets[0], total = offsets[count] - offsets[0];
         for (uint32_t k = 0; k < count; k++) {
             if (offsets[k + 1] < offsets[k]) throw Error(DCX_E_ARG, "offsets must be non-decreasing");
-            if (offsets[k + 1] - offsets[k] > MAX_TEXT)
+            if (offsets[k + 1] - offsets[k] + 1 > MAX_TEXT)  // with its separator
                 throw Error(DCX_E_UNSUPPORTED, "a text of the batch is too long for 32-bit positions");
         }
         if (total && (!texts || !sa_out)) throw Error(DCX_E_ARG, "null argument");
@@ -1009,7 +1019,7 @@ class Engine {
             DBuf<uint64_t> d_in(cache, cnt + 1), d_co(cache, cnt + 1);
             DCX_HIP(hipMemcpyAsync(d_in.get(), in_off.data(), (cnt + 1) * 8, hipMemcpyHostToDevice, stream));
             DCX_HIP(hipMemcpyAsync(d_co.get(), co.data(), (cnt + 1) * 8, hipMemcpyHostToDevice, stream));
-            const uint32_t N = (uint32_t)joined;
+            const uint64_t N = joined;
             const uint64_t chars = offsets[k1] - offsets[k0];
             // owner tables of the input and of the joined positions
             const uint64_t in_chunks = (chars >> OWNER_SHIFT) + 1, co_chunks = ((uint64_t)N >> OWNER_SHIFT) + 1;
@@ -1077,7 +1087,7 @@ class Engine {
         if (width != 4 && width != 5 && width != 8) throw Error(DCX_E_ARG, "sa_width_bytes must be 4, 5 or 8");
         set_options(o);
         if (n > MAX_TEXT)
-            throw Error(DCX_E_UNSUPPORTED, "texts longer than 2^32 - 2^22 characters are not supported by this build");
+            throw Error(DCX_E_UNSUPPORTED, "texts longer than 2^32 characters are not supported by this build");
         if (width == 4 && n > (1ull << 32)) throw Error(DCX_E_WIDTH, "32-bit SA cannot index this text");
         if (budget) {
             uint64_t need = estimate_bytes(n);
@@ -1132,7 +1142,7 @@ class Engine {
         // phase boundaries default to "empty" (a base-case level never reaches them)
         DCX_HIP(hipEventRecord(ev[2], stream));
         DCX_HIP(hipEventRecord(ev[3], stream));
-        level<uint8_t>(d_text, dlut.get(), (uint32_t)n, sigma, 0, nullptr, ev, &sa);
+        level<uint8_t>(d_text, dlut.get(), n, sigma, 0, nullptr, ev, &sa);
         DCX_HIP(hipEventRecord(ev[4], stream));  // SA resident in HBM
         // output in the requested width
         if (o->sa_on_device) {
@@ -1179,7 +1189,7 @@ class Engine {
         DCX_HIP(hipSetDevice(device));
         if (!ok || (n && (!text || !sa))) throw Error(DCX_E_ARG, "null argument");
         if (width != 4 && width != 5 && width != 8) throw Error(DCX_E_ARG, "sa_width_bytes must be 4, 5 or 8");
-        if (n >= (1ull << 32) - 1) throw Error(DCX_E_UNSUPPORTED, "text too long for this build");
+        if (n > MAX_TEXT) throw Error(DCX_E_UNSUPPORTED, "text too long for this build");
         *ok = 1;
         if (n == 0) return DCX_OK;
         DBuf<uint8_t> dt, ds;
@@ -1193,11 +1203,13 @@ class Engine {
             d_text = dt.get();
             d_sa = ds.get();
         }
-        DBuf<uint32_t> rank(cache, n + 1);
+        // inverse SA (0-based ranks: n may be 2^32) and a bit per position for the permutation test
+        DBuf<uint32_t> rank(cache, n);
+        DBuf<uint32_t> seen(cache, n / 32 + 1);
         DBuf<uint32_t> bad(cache, 1);
-        prim.fill_u32(rank.get(), 0, n + 1);
+        prim.fill_u32(seen.get(), 0, n / 32 + 1);
         prim.fill_u32(bad.get(), 0, 1);
-        k_check_perm<<<grid(n), 256, 0, stream>>>(d_sa, width, n, rank.get(), bad.get());
+        k_check_perm<<<grid(n), 256, 0, stream>>>(d_sa, width, n, rank.get(), seen.get(), bad.get());
         k_check_order<<<grid(n), 256, 0, stream>>>(d_sa, width, n, d_text, rank.get(), bad.get());
         uint32_t hb = 0;
         DCX_HIP(hipMemcpyAsync(&hb, bad.get(), 4, hipMemcpyDeviceToHost, stream));

@@ -195,7 +195,7 @@ __global__ __launch_bounds__(256) void k_code_hist(const uint8_t* __restrict__ t
 // characters (a whole number of words for every width) staged in LDS.
 constexpr uint32_t PK_CHARS = 8192;
 template <class CharT>
-__global__ __launch_bounds__(256) void k_pack_text(const CharT* __restrict__ t, uint32_t n,
+__global__ __launch_bounds__(256) void k_pack_text(const CharT* __restrict__ t, uint64_t n,
                                                    const uint16_t* __restrict__ lut, uint32_t bits, uint32_t dense,
                                                    uint64_t nwords, uint64_t* __restrict__ PT) {
     __shared__ uint32_t s_c[PK_CHARS];
@@ -234,7 +234,7 @@ __global__ __launch_bounds__(256) void k_pack_text(const CharT* __restrict__ t, 
 // the same for widths that divide 64 (no character straddles two words): a thread packs the 64 / bits
 // characters of one word straight from global memory (consecutive loads the compiler merges)
 template <class CharT, uint32_t BITS>
-__global__ __launch_bounds__(256) void k_pack_aligned(const CharT* __restrict__ t, uint32_t n,
+__global__ __launch_bounds__(256) void k_pack_aligned(const CharT* __restrict__ t, uint64_t n,
                                                       const uint16_t* __restrict__ lut, uint32_t dense, uint64_t nwords,
                                                       uint64_t* __restrict__ PT) {
     constexpr uint32_t CPW = 64 / BITS;
@@ -271,9 +271,9 @@ __device__ __forceinline__ uint64_t top_bits(uint64_t w, uint32_t nbits) {
 }
 
 // group heads of sorted keys, comparing only the top kbits
-__global__ __launch_bounds__(256) void k_heads(const uint64_t* __restrict__ keys, uint32_t N, uint32_t kbits,
+__global__ __launch_bounds__(256) void k_heads(const uint64_t* __restrict__ keys, uint64_t N, uint32_t kbits,
                                                uint8_t* __restrict__ H) {
-    for (uint32_t j = blockIdx.x * 256 + threadIdx.x; j <= N; j += gridDim.x * 256)
+    for (uint64_t j = blockIdx.x * 256ull + threadIdx.x; j <= N; j += gridDim.x * 256ull)
         H[j] = (j == 0 || j == N) ? 1 : (top_bits(keys[j], kbits) != top_bits(keys[j - 1], kbits));
 }
 
@@ -281,11 +281,11 @@ __global__ __launch_bounds__(256) void k_heads(const uint64_t* __restrict__ keys
 // last radix pass (sort_pairs_from's low32: equal low words within a run of the last digit; the run starts
 // are marked by k_mark_runs) and for the sample groups (group ids in sorted order).  A thread per 16
 // consecutive positions: four 16-byte loads, one 16-byte store (a and H are 256-byte aligned buffers).
-__global__ __launch_bounds__(256) void k_heads_u32(const uint32_t* __restrict__ low, uint32_t N,
+__global__ __launch_bounds__(256) void k_heads_u32(const uint32_t* __restrict__ low, uint64_t N,
                                                    uint8_t* __restrict__ H) {
-    const uint32_t groups = N / 16 + 1;  // covers positions 0..N
-    for (uint32_t g = blockIdx.x * 256 + threadIdx.x; g < groups; g += gridDim.x * 256) {
-        const uint32_t j0 = g * 16;
+    const uint64_t groups = N / 16 + 1;  // covers positions 0..N
+    for (uint64_t g = blockIdx.x * 256ull + threadIdx.x; g < groups; g += gridDim.x * 256ull) {
+        const uint64_t j0 = g * 16;
         if (j0 + 16 <= N) {
             uint32_t w[17];
             w[0] = j0 ? low[j0 - 1] : ~low[0];  // position 0 is always a head
@@ -302,22 +302,22 @@ __global__ __launch_bounds__(256) void k_heads_u32(const uint32_t* __restrict__ 
             for (int k = 0; k < 16; k++) h[k >> 2] |= (uint32_t)(w[k + 1] != w[k]) << (8 * (k & 3));
             reinterpret_cast<uint4*>(H + j0)[0] = make_uint4(h[0], h[1], h[2], h[3]);
         } else {
-            for (uint32_t j = j0; j < j0 + 16 && j <= N; j++) H[j] = (j == 0 || j == N) ? 1 : (low[j] != low[j - 1]);
+            for (uint64_t j = j0; j < j0 + 16 && j <= N; j++) H[j] = (j == 0 || j == N) ? 1 : (low[j] != low[j - 1]);
         }
     }
 }
-__global__ __launch_bounds__(256) void k_mark_runs(const uint32_t* __restrict__ offsets, uint32_t N,
+__global__ __launch_bounds__(256) void k_mark_runs(const uint64_t* __restrict__ offsets, uint64_t N,
                                                    uint8_t* __restrict__ H) {
-    const uint32_t o = offsets[threadIdx.x];  // exclusive digit offsets: every run starts at one
+    const uint64_t o = offsets[threadIdx.x];  // exclusive digit offsets: every run starts at one
     if (o < N) H[o] = 1;
 }
 
 // packed round 0: keys are (characters << pbits) | position; heads compare the characters, order gets the
 // positions
-__global__ __launch_bounds__(256) void k_heads_packed(const uint64_t* __restrict__ keys, uint32_t N, uint32_t kbits,
+__global__ __launch_bounds__(256) void k_heads_packed(const uint64_t* __restrict__ keys, uint64_t N, uint32_t kbits,
                                                       uint64_t pmask, uint8_t* __restrict__ H,
                                                       uint32_t* __restrict__ order) {
-    for (uint32_t j = blockIdx.x * 256 + threadIdx.x; j <= N; j += gridDim.x * 256) {
+    for (uint64_t j = blockIdx.x * 256ull + threadIdx.x; j <= N; j += gridDim.x * 256ull) {
         if (j < N) order[j] = (uint32_t)(keys[j] & pmask);
         H[j] = (j == 0 || j == N) ? 1 : (top_bits(keys[j], kbits) != top_bits(keys[j - 1], kbits));
     }
@@ -359,12 +359,12 @@ struct ActiveScanOp {
     }
 };
 
-__global__ __launch_bounds__(256) void k_keys_rr(const uint64_t* __restrict__ PT, uint32_t bits, uint32_t A,
+__global__ __launch_bounds__(256) void k_keys_rr(const uint64_t* __restrict__ PT, uint32_t bits, uint64_t A,
                                                  const uint32_t* __restrict__ act_idx, const uint32_t* __restrict__ gid,
                                                  const uint32_t* __restrict__ order, uint32_t C, uint32_t cbits,
                                                  uint32_t gbits, uint64_t* __restrict__ keys, uint32_t* __restrict__ vals) {
     const StreamKeys<false> chars{PT, bits, top_mask(cbits)};
-    for (uint32_t a = blockIdx.x * 256 + threadIdx.x; a < A; a += gridDim.x * 256) {
+    for (uint64_t a = blockIdx.x * 256ull + threadIdx.x; a < A; a += gridDim.x * 256ull) {
         uint32_t p = order[act_idx[a]];
         uint64_t ch = chars((uint64_t)p + C);
         keys[a] = gbits ? (((uint64_t)gid[a] << (64 - gbits)) | (ch >> gbits)) : ch;
@@ -375,10 +375,10 @@ __global__ __launch_bounds__(256) void k_keys_rr(const uint64_t* __restrict__ PT
 // sorted active list back into `order`; with `mark`, new group starts into H and, if Ha is given, the
 // compact heads of the list (Ha[a] for entry a, Ha[A] = 1) for the next active set
 __global__ __launch_bounds__(256) void k_writeback(const uint64_t* __restrict__ keys, const uint32_t* __restrict__ vals,
-                                                   const uint32_t* __restrict__ act_idx, uint32_t A,
+                                                   const uint32_t* __restrict__ act_idx, uint64_t A,
                                                    uint32_t* __restrict__ order, uint8_t* __restrict__ H, int mark,
                                                    uint8_t* __restrict__ Ha) {
-    for (uint32_t a = blockIdx.x * 256 + threadIdx.x; a < A; a += gridDim.x * 256) {
+    for (uint64_t a = blockIdx.x * 256ull + threadIdx.x; a < A; a += gridDim.x * 256ull) {
         const uint32_t j = act_idx[a];
         order[j] = vals[a];
         if (mark) {
@@ -404,12 +404,12 @@ struct GroupScatterOp {
 };
 
 // key of a tied position for a doubling step: (group among tied groups, group of the position `off` ahead)
-__global__ __launch_bounds__(256) void k_keys_double(const uint32_t* __restrict__ G, uint32_t n, uint32_t A,
+__global__ __launch_bounds__(256) void k_keys_double(const uint32_t* __restrict__ G, uint64_t n, uint64_t A,
                                                      const uint32_t* __restrict__ act_idx,
                                                      const uint32_t* __restrict__ gid,
                                                      const uint32_t* __restrict__ order, uint32_t off, uint32_t gvb,
                                                      uint64_t* __restrict__ keys, uint32_t* __restrict__ vals) {
-    for (uint32_t a = blockIdx.x * 256 + threadIdx.x; a < A; a += gridDim.x * 256) {
+    for (uint64_t a = blockIdx.x * 256ull + threadIdx.x; a < A; a += gridDim.x * 256ull) {
         uint32_t p = order[act_idx[a]];
         uint64_t q = (uint64_t)p + off;
         uint32_t g2 = q < n ? G[q] : 0u;  // past the end: the empty suffix, smallest
@@ -474,7 +474,7 @@ constexpr uint32_t BOUNDARY_CAP = 256;
 // The groups that hold boundary suffixes, found from the boundary suffixes themselves (at most L - 1 of
 // them) instead of a pass over the tied list: the order is sorted by the L-character keys of the dense
 // stream, so each boundary suffix's group is the range of its key there (two binary searches).  Ranges of
-// two or more slots go to `ranges` as (first, end).
+// two or more slots go to `ranges` as (first, size).
 __device__ __forceinline__ int cmp_key_chars(const uint64_t* __restrict__ PT, uint32_t bits, uint32_t L, uint64_t a,
                                              uint64_t b) {
     const uint32_t cpw = 64 / bits;
@@ -487,40 +487,40 @@ __device__ __forceinline__ int cmp_key_chars(const uint64_t* __restrict__ PT, ui
     return 0;
 }
 __global__ __launch_bounds__(64) void k_boundary_find(const uint64_t* __restrict__ PT, uint32_t bits, uint32_t L,
-                                                      const uint32_t* __restrict__ order, uint32_t n, uint32_t bnd,
+                                                      const uint32_t* __restrict__ order, uint64_t n, uint64_t bnd,
                                                       uint32_t* __restrict__ ranges, uint32_t* __restrict__ count,
                                                       uint32_t cap) {
-    const uint32_t p = bnd + blockIdx.x * 64 + threadIdx.x;
+    const uint64_t p = bnd + blockIdx.x * 64ull + threadIdx.x;
     if (p >= n) return;
-    uint32_t lo = 0, hi = n;
+    uint64_t lo = 0, hi = n;
     while (lo < hi) {  // first slot whose key is not below p's
-        const uint32_t mid = lo + (hi - lo) / 2;
+        const uint64_t mid = lo + (hi - lo) / 2;
         if (cmp_key_chars(PT, bits, L, order[mid], p) < 0)
             lo = mid + 1;
         else
             hi = mid;
     }
-    const uint32_t first = lo;
+    const uint64_t first = lo;
     hi = n;
     while (lo < hi) {  // first slot whose key is above p's
-        const uint32_t mid = lo + (hi - lo) / 2;
+        const uint64_t mid = lo + (hi - lo) / 2;
         if (cmp_key_chars(PT, bits, L, order[mid], p) <= 0)
             lo = mid + 1;
         else
             hi = mid;
     }
-    if (lo - first >= 2) {
+    if (lo - first >= 2) {  // as (first slot, size): both below 2^32
         const uint32_t i = atomicAdd(count, 1u);
         if (i < cap) {
-            ranges[2 * i] = first;
-            ranges[2 * i + 1] = lo;
+            ranges[2 * i] = (uint32_t)first;
+            ranges[2 * i + 1] = (uint32_t)(lo - first);
         }
     }
 }
 
 // one workgroup per affected group order[j0 .. j0+s)
 __global__ __launch_bounds__(1024) void k_boundary_fix(uint32_t* __restrict__ order, uint8_t* __restrict__ H,
-                                                       uint32_t j0, uint32_t s, uint32_t bnd,
+                                                       uint64_t j0, uint32_t s, uint64_t bnd,
                                                        uint32_t* __restrict__ tmp) {
     __shared__ uint32_t sB[BOUNDARY_CAP];
     __shared__ uint32_t nB;
@@ -531,7 +531,7 @@ __global__ __launch_bounds__(1024) void k_boundary_fix(uint32_t* __restrict__ or
     for (uint32_t c = 0; c < s; c += 1024) {
         const uint32_t i = c + threadIdx.x;
         const uint32_t p = i < s ? order[j0 + i] : 0u;
-        const bool isB = i < s && p >= bnd;
+        const bool isB = i < s && (uint64_t)p >= bnd;
         const bool keep = i < s && !isB;
         if (isB) {
             uint32_t x = atomicAdd(&nB, 1u);
@@ -571,7 +571,7 @@ constexpr uint32_t UNIQ_BIT = 0x80000000u;
 __global__ __launch_bounds__(256) void k_rec_pairs(const uint32_t* __restrict__ order, const uint32_t* __restrict__ names,
                                                    uint32_t m, DcDev dc, uint32_t* __restrict__ dest,
                                                    uint32_t* __restrict__ word) {
-    for (uint32_t j = blockIdx.x * 256 + threadIdx.x; j < m; j += gridDim.x * 256) {
+    for (uint64_t j = blockIdx.x * 256ull + threadIdx.x; j < m; j += gridDim.x * 256ull) {
         dest[j] = class_major(dc, order[j]);
         word[j] = names[j];
     }
@@ -580,8 +580,8 @@ __global__ __launch_bounds__(256) void k_rec_pairs(const uint32_t* __restrict__ 
 // scatters of fewer words than this go straight to their destinations
 constexpr uint32_t SCATTER_PARTITION_MIN = 1u << 20;
 
-__global__ __launch_bounds__(256) void k_iota(uint32_t* __restrict__ out, uint32_t N) {
-    for (uint32_t i = blockIdx.x * 256 + threadIdx.x; i < N; i += gridDim.x * 256) out[i] = i;
+__global__ __launch_bounds__(256) void k_iota(uint32_t* __restrict__ out, uint64_t N) {
+    for (uint64_t i = blockIdx.x * 256ull + threadIdx.x; i < N; i += gridDim.x * 256ull) out[i] = (uint32_t)i;
 }
 
 // out[dest[i]] = val[i] + add for a permutation dest of [0, N) whose pairs are ordered by dest >> wbits:
@@ -589,7 +589,7 @@ __global__ __launch_bounds__(256) void k_iota(uint32_t* __restrict__ out, uint32
 // assembles in LDS and writes out whole (coalesced stores instead of one cache line per word).
 constexpr uint32_t SCATTER_WINDOW_BITS = 13;  // at most 8192 words (32 KB of LDS) per window
 __global__ __launch_bounds__(256) void k_scatter_window(const uint32_t* __restrict__ dest,
-                                                        const uint32_t* __restrict__ val, uint32_t N, uint32_t add,
+                                                        const uint32_t* __restrict__ val, uint64_t N, uint32_t add,
                                                         uint32_t wbits, uint32_t* __restrict__ out) {
     __shared__ uint32_t s_w[1u << SCATTER_WINDOW_BITS];
     const uint64_t beg = (uint64_t)blockIdx.x << wbits;
@@ -605,8 +605,8 @@ __global__ __launch_bounds__(256) void k_scatter_window(const uint32_t* __restri
 
 // out[dest[i]] = val[i] + add
 __global__ __launch_bounds__(256) void k_scatter_pairs(const uint32_t* __restrict__ dest, const uint32_t* __restrict__ val,
-                                                       uint32_t N, uint32_t add, uint32_t* __restrict__ out) {
-    for (uint32_t i = blockIdx.x * 256 + threadIdx.x; i < N; i += gridDim.x * 256) out[dest[i]] = val[i] + add;
+                                                       uint64_t N, uint32_t add, uint32_t* __restrict__ out) {
+    for (uint64_t i = blockIdx.x * 256ull + threadIdx.x; i < N; i += gridDim.x * 256ull) out[dest[i]] = val[i] + add;
 }
 
 // Discarding from the non-unique samples alone.  Element c of the recursive string is kept unless c
@@ -650,7 +650,7 @@ struct DistinctOp {
 __global__ __launch_bounds__(256) void k_reduced_string(const uint32_t* __restrict__ K, uint32_t after, DcDev dc,
                                                         const uint32_t* __restrict__ ranks0,
                                                         const uint32_t* __restrict__ names, uint32_t* __restrict__ Rr) {
-    for (uint32_t i = blockIdx.x * 256 + threadIdx.x; i < after; i += gridDim.x * 256)
+    for (uint64_t i = blockIdx.x * 256ull + threadIdx.x; i < after; i += gridDim.x * 256ull)
         Rr[i] = names[ranks0[class_major_inv(dc, K[i])] - 1];
 }
 
@@ -669,13 +669,13 @@ __device__ __forceinline__ uint32_t sorted_index(const uint32_t* __restrict__ K,
 
 // (group, rank2) keys of the non-unique samples; rank2 = ISA of the reduced string at the sample's kept
 // index (RankRankIndex, pdcx.hpp:297-323)
-__global__ __launch_bounds__(256) void k_keys_rank2_kept(uint32_t A, const uint32_t* __restrict__ act_idx,
+__global__ __launch_bounds__(256) void k_keys_rank2_kept(uint64_t A, const uint32_t* __restrict__ act_idx,
                                                          const uint32_t* __restrict__ gid,
                                                          const uint32_t* __restrict__ order, DcDev dc,
                                                          const uint32_t* __restrict__ K, uint32_t after,
                                                          const uint32_t* __restrict__ isar, uint32_t rbits,
                                                          uint64_t* __restrict__ keys, uint32_t* __restrict__ vals) {
-    for (uint32_t a = blockIdx.x * 256 + threadIdx.x; a < A; a += gridDim.x * 256) {
+    for (uint64_t a = blockIdx.x * 256ull + threadIdx.x; a < A; a += gridDim.x * 256ull) {
         const uint32_t k = order[act_idx[a]];
         const uint32_t r2 = isar[sorted_index(K, after, class_major(dc, k))];
         keys[a] = ((uint64_t)gid[a] << rbits) | r2;
@@ -684,18 +684,18 @@ __global__ __launch_bounds__(256) void k_keys_rank2_kept(uint32_t A, const uint3
 }
 
 // ranks of the refined non-unique samples: 1 + their new position
-__global__ __launch_bounds__(256) void k_patch_ranks(uint32_t A, const uint32_t* __restrict__ act_idx,
+__global__ __launch_bounds__(256) void k_patch_ranks(uint64_t A, const uint32_t* __restrict__ act_idx,
                                                      const uint32_t* __restrict__ order, uint32_t* __restrict__ ranks) {
-    for (uint32_t a = blockIdx.x * 256 + threadIdx.x; a < A; a += gridDim.x * 256) {
+    for (uint64_t a = blockIdx.x * 256ull + threadIdx.x; a < A; a += gridDim.x * 256ull) {
         const uint32_t j = act_idx[a];
-        ranks[order[j]] = j + 1;
+        ranks[order[j]] = j + 1;  // sample ranks (m < 2^32)
     }
 }
 
 // recursive_call_direct (pdcx.hpp:182-212): the rank of map_back(SA_R[j]) is j + 1; dest[j] = map_back(SA_R[j])
 __global__ __launch_bounds__(256) void k_sa_dest(const uint32_t* __restrict__ sar, uint32_t m, DcDev dc,
                                                  uint32_t* __restrict__ dest) {
-    for (uint32_t j = blockIdx.x * 256 + threadIdx.x; j < m; j += gridDim.x * 256) dest[j] = class_major_inv(dc, sar[j]);
+    for (uint64_t j = blockIdx.x * 256ull + threadIdx.x; j < m; j += gridDim.x * 256ull) dest[j] = class_major_inv(dc, sar[j]);
 }
 
 // ------------------------------------------------------------------------------------------------
@@ -730,7 +730,7 @@ __global__ __launch_bounds__(256) void k_refine_small(const uint32_t* __restrict
                                                       const uint64_t* __restrict__ PT, uint32_t bits, uint32_t C,
                                                       uint32_t cbits, uint8_t* __restrict__ large) {
     const StreamKeys<false> chars{PT, bits, top_mask(cbits)};
-    for (uint32_t g = blockIdx.x * 256 + threadIdx.x; g < G; g += gridDim.x * 256) {
+    for (uint64_t g = blockIdx.x * 256ull + threadIdx.x; g < G; g += gridDim.x * 256ull) {
         const uint32_t a0 = gstart[g], s = gstart[g + 1] - a0;
         if (s > RF_SMALL) {
             large[g] = 1;
@@ -782,7 +782,7 @@ __global__ __launch_bounds__(256) void k_refine_small(const uint32_t* __restrict
 __global__ __launch_bounds__(256) void k_count_large(const uint32_t* __restrict__ gstart, uint32_t G,
                                                      uint32_t* __restrict__ count) {
     uint32_t c = 0;
-    for (uint32_t g = blockIdx.x * 256 + threadIdx.x; g < G; g += gridDim.x * 256)
+    for (uint64_t g = blockIdx.x * 256ull + threadIdx.x; g < G; g += gridDim.x * 256ull)
         c += gstart[g + 1] - gstart[g] > RF_SMALL;
     for (int d = 32; d >= 1; d >>= 1) c += __shfl_xor(c, d, 64);
     if (__lane_id() == 0 && c) atomicAdd(count, c);
@@ -817,7 +817,7 @@ struct LargeMembersOp {
 // radix keys of the large-group members: (large-group rank << (64 - lbits)) | next cbits bits of characters
 // (lbits + cbits <= 64); values are the suffixes.  cls (optional): the member's key against its group's
 // first member's, 0 smaller, 1 equal, 2 larger (that key recomputed: one stream window per group, cached)
-__global__ __launch_bounds__(256) void k_keys_large(const uint64_t* __restrict__ PT, uint32_t bits, uint32_t B,
+__global__ __launch_bounds__(256) void k_keys_large(const uint64_t* __restrict__ PT, uint32_t bits, uint64_t B,
                                                     const uint32_t* __restrict__ slots,
                                                     const uint32_t* __restrict__ act_idx,
                                                     const uint32_t* __restrict__ lrank,
@@ -826,7 +826,7 @@ __global__ __launch_bounds__(256) void k_keys_large(const uint64_t* __restrict__
                                                     uint32_t* __restrict__ vals, const uint32_t* __restrict__ lfirst,
                                                     uint8_t* __restrict__ cls) {
     const StreamKeys<false> chars{PT, bits, top_mask(cbits)};
-    for (uint32_t b = blockIdx.x * 256 + threadIdx.x; b < B; b += gridDim.x * 256) {
+    for (uint64_t b = blockIdx.x * 256ull + threadIdx.x; b < B; b += gridDim.x * 256ull) {
         const uint32_t p = order[act_idx[slots[b]]];
         const uint64_t ch = chars((uint64_t)p + C);
         keys[b] = lbits ? (((uint64_t)lrank[b] << (64 - lbits)) | (ch >> lbits)) : ch;
@@ -844,11 +844,11 @@ __global__ __launch_bounds__(256) void k_keys_large(const uint64_t* __restrict__
 // into H and the compact heads Ha
 __global__ __launch_bounds__(256) void k_writeback_large(const uint64_t* __restrict__ keys,
                                                          const uint32_t* __restrict__ vals,
-                                                         const uint32_t* __restrict__ slots, uint32_t B,
+                                                         const uint32_t* __restrict__ slots, uint64_t B,
                                                          const uint32_t* __restrict__ act_idx,
                                                          uint32_t* __restrict__ order, uint8_t* __restrict__ H,
                                                          uint8_t* __restrict__ Ha) {
-    for (uint32_t b = blockIdx.x * 256 + threadIdx.x; b < B; b += gridDim.x * 256) {
+    for (uint64_t b = blockIdx.x * 256ull + threadIdx.x; b < B; b += gridDim.x * 256ull) {
         const uint32_t a = slots[b];
         const uint32_t j = act_idx[a];
         order[j] = vals[b];
@@ -910,7 +910,7 @@ __device__ __forceinline__ void large_put(uint32_t dest, uint32_t pos, bool head
 }
 
 // the members equal to their group's first key: after the group's smaller minority, in list order
-__global__ __launch_bounds__(256) void k_place_equal(uint32_t B, const uint32_t* __restrict__ vals,
+__global__ __launch_bounds__(256) void k_place_equal(uint64_t B, const uint32_t* __restrict__ vals,
                                                      const uint32_t* __restrict__ slots,
                                                      const uint32_t* __restrict__ lrank,
                                                      const uint32_t* __restrict__ lfirst,
@@ -918,17 +918,18 @@ __global__ __launch_bounds__(256) void k_place_equal(uint32_t B, const uint32_t*
                                                      const uint32_t* __restrict__ c0pref,
                                                      const uint32_t* __restrict__ act_idx, uint32_t* __restrict__ order,
                                                      uint8_t* __restrict__ H, uint8_t* __restrict__ Ha) {
-    for (uint32_t b = blockIdx.x * 256 + threadIdx.x; b < B; b += gridDim.x * 256) {
+    for (uint64_t b = blockIdx.x * 256ull + threadIdx.x; b < B; b += gridDim.x * 256ull) {
         if (minpref[b + 1] != minpref[b]) continue;  // minority
         const LargeGroupSpan sp = large_span(b, lrank, lfirst);
         const uint32_t n0 = c0pref[sp.fe] - c0pref[sp.fb];
-        const uint32_t r1 = (b - sp.fb) - (minpref[b] - minpref[sp.fb]);  // rank among the equal members
+        // rank among the equal members (list indices below 2^32; differences of the prefix arrays modulo 2^32)
+        const uint32_t r1 = ((uint32_t)b - sp.fb) - (minpref[b] - minpref[sp.fb]);
         large_put(sp.fb + n0 + r1, vals[b], r1 == 0, slots, act_idx, order, H, Ha);
     }
 }
 
 // the sorted minority: the smaller members first in their group, the larger ones after the equal ones
-__global__ __launch_bounds__(256) void k_place_minority(uint32_t Bm, const uint64_t* __restrict__ mkeys,
+__global__ __launch_bounds__(256) void k_place_minority(uint64_t Bm, const uint64_t* __restrict__ mkeys,
                                                         const uint32_t* __restrict__ midx,
                                                         const uint32_t* __restrict__ vals,
                                                         const uint32_t* __restrict__ slots,
@@ -938,12 +939,12 @@ __global__ __launch_bounds__(256) void k_place_minority(uint32_t Bm, const uint6
                                                         const uint32_t* __restrict__ c0pref,
                                                         const uint32_t* __restrict__ act_idx, uint32_t* __restrict__ order,
                                                         uint8_t* __restrict__ H, uint8_t* __restrict__ Ha) {
-    for (uint32_t k = blockIdx.x * 256 + threadIdx.x; k < Bm; k += gridDim.x * 256) {
+    for (uint64_t k = blockIdx.x * 256ull + threadIdx.x; k < Bm; k += gridDim.x * 256ull) {
         const uint32_t b = midx[k];
         const LargeGroupSpan sp = large_span(b, lrank, lfirst);
         const uint32_t n0 = c0pref[sp.fe] - c0pref[sp.fb];
         const uint32_t nmin = minpref[sp.fe] - minpref[sp.fb];
-        const uint32_t local = k - minpref[sp.fb];  // the group's minority starts there in the sorted list
+        const uint32_t local = (uint32_t)k - minpref[sp.fb];  // the group's minority starts there in the sorted list
         const uint32_t dest = local < n0 ? sp.fb + local : sp.fb + (sp.fe - sp.fb - nmin) + local;
         const bool head = local == 0 || local == n0 || mkeys[k] != mkeys[k - 1];
         large_put(dest, vals[b], head, slots, act_idx, order, H, Ha);
@@ -951,43 +952,43 @@ __global__ __launch_bounds__(256) void k_place_minority(uint32_t Bm, const uint6
 }
 
 // groups whose members all share the first member's key: only the compact heads change
-__global__ __launch_bounds__(256) void k_uniform_heads(uint32_t B, const uint32_t* __restrict__ slots,
+__global__ __launch_bounds__(256) void k_uniform_heads(uint64_t B, const uint32_t* __restrict__ slots,
                                                        const uint32_t* __restrict__ lrank,
                                                        const uint32_t* __restrict__ lfirst, uint8_t* __restrict__ Ha) {
-    for (uint32_t b = blockIdx.x * 256 + threadIdx.x; b < B; b += gridDim.x * 256)
-        Ha[slots[b]] = b == lfirst[lrank[b]] ? 1 : 0;
+    for (uint64_t b = blockIdx.x * 256ull + threadIdx.x; b < B; b += gridDim.x * 256ull)
+        Ha[slots[b]] = (uint32_t)b == lfirst[lrank[b]] ? 1 : 0;
 }
 
 // The same when rank and characters do not fit one word: an LSD sort by the characters (values: member
 // indices b, suffixes kept in P), then a stable sort by the rank.
-__global__ __launch_bounds__(256) void k_keys_large_chars(const uint64_t* __restrict__ PT, uint32_t bits, uint32_t B,
+__global__ __launch_bounds__(256) void k_keys_large_chars(const uint64_t* __restrict__ PT, uint32_t bits, uint64_t B,
                                                           const uint32_t* __restrict__ slots,
                                                           const uint32_t* __restrict__ act_idx,
                                                           const uint32_t* __restrict__ order, uint32_t C,
                                                           uint32_t cbits, uint64_t* __restrict__ keys,
                                                           uint32_t* __restrict__ P) {
     const StreamKeys<false> chars{PT, bits, top_mask(cbits)};
-    for (uint32_t b = blockIdx.x * 256 + threadIdx.x; b < B; b += gridDim.x * 256) {
+    for (uint64_t b = blockIdx.x * 256ull + threadIdx.x; b < B; b += gridDim.x * 256ull) {
         const uint32_t p = order[act_idx[slots[b]]];
         keys[b] = chars((uint64_t)p + C);
         P[b] = p;
     }
 }
 __global__ __launch_bounds__(256) void k_keys_large_rank(const uint32_t* __restrict__ members,
-                                                         const uint32_t* __restrict__ lrank, uint32_t B,
+                                                         const uint32_t* __restrict__ lrank, uint64_t B,
                                                          uint64_t* __restrict__ keys) {
-    for (uint32_t i = blockIdx.x * 256 + threadIdx.x; i < B; i += gridDim.x * 256) keys[i] = lrank[members[i]];
+    for (uint64_t i = blockIdx.x * 256ull + threadIdx.x; i < B; i += gridDim.x * 256ull) keys[i] = lrank[members[i]];
 }
 __global__ __launch_bounds__(256) void k_writeback_large2(const uint64_t* __restrict__ rkeys,
                                                           const uint32_t* __restrict__ members,
                                                           const uint32_t* __restrict__ P,
                                                           const uint64_t* __restrict__ PT, uint32_t bits, uint32_t C,
-                                                          uint32_t cbits, const uint32_t* __restrict__ slots, uint32_t B,
+                                                          uint32_t cbits, const uint32_t* __restrict__ slots, uint64_t B,
                                                           const uint32_t* __restrict__ act_idx,
                                                           uint32_t* __restrict__ order, uint8_t* __restrict__ H,
                                                           uint8_t* __restrict__ Ha) {
     const StreamKeys<false> chars{PT, bits, top_mask(cbits)};
-    for (uint32_t i = blockIdx.x * 256 + threadIdx.x; i < B; i += gridDim.x * 256) {
+    for (uint64_t i = blockIdx.x * 256ull + threadIdx.x; i < B; i += gridDim.x * 256ull) {
         const uint32_t a = slots[i];
         const uint32_t j = act_idx[a];
         const uint32_t p = P[members[i]];
@@ -1007,7 +1008,7 @@ __global__ __launch_bounds__(256) void k_tiebreak_small(const uint32_t* __restri
                                                         const uint32_t* __restrict__ gfirst, uint32_t* __restrict__ order,
                                                         DcDev dc, const uint32_t* __restrict__ cmp,
                                                         const uint32_t* __restrict__ ranks, uint8_t* __restrict__ large) {
-    for (uint32_t g = blockIdx.x * 256 + threadIdx.x; g < G; g += gridDim.x * 256) {
+    for (uint64_t g = blockIdx.x * 256ull + threadIdx.x; g < G; g += gridDim.x * 256ull) {
         uint32_t a0 = gstart[g], s = gstart[g + 1] - a0;
         if (s > TB_SMALL) {
             large[g] = 1;
@@ -1061,7 +1062,7 @@ __device__ __forceinline__ bool item_less(uint64_t x, uint64_t y, const DcDev& d
 constexpr uint32_t TB_TILE = 1024;
 
 // sorts each tile of TB_TILE items in LDS (bitonic network, DC comparator)
-__global__ __launch_bounds__(256) void k_tb_tile_sort(uint64_t* __restrict__ items, uint32_t B, DcDev dc,
+__global__ __launch_bounds__(256) void k_tb_tile_sort(uint64_t* __restrict__ items, uint64_t B, DcDev dc,
                                                       const uint32_t* __restrict__ cmp, const uint32_t* __restrict__ ranks) {
     __shared__ uint64_t s[TB_TILE];
     const uint64_t base = (uint64_t)blockIdx.x * TB_TILE;
@@ -1086,8 +1087,8 @@ __global__ __launch_bounds__(256) void k_tb_tile_sort(uint64_t* __restrict__ ite
 }
 
 // one merge pass: runs of width w are merged pairwise; every item finds its rank in the partner run
-__global__ __launch_bounds__(256) void k_tb_merge(const uint64_t* __restrict__ in, uint64_t* __restrict__ out, uint32_t B,
-                                                  uint32_t w, DcDev dc, const uint32_t* __restrict__ cmp,
+__global__ __launch_bounds__(256) void k_tb_merge(const uint64_t* __restrict__ in, uint64_t* __restrict__ out, uint64_t B,
+                                                  uint64_t w, DcDev dc, const uint32_t* __restrict__ cmp,
                                                   const uint32_t* __restrict__ ranks) {
     for (uint64_t i = blockIdx.x * 256ull + threadIdx.x; i < B; i += (uint64_t)gridDim.x * 256) {
         uint64_t s = (i / (2ull * w)) * 2ull * w;
@@ -1117,8 +1118,8 @@ __global__ __launch_bounds__(256) void k_tb_merge(const uint64_t* __restrict__ i
 }
 
 __global__ __launch_bounds__(256) void k_tb_scatter(const uint64_t* __restrict__ items, const uint32_t* __restrict__ slots,
-                                                    uint32_t B, uint32_t* __restrict__ order) {
-    for (uint32_t b = blockIdx.x * 256 + threadIdx.x; b < B; b += gridDim.x * 256) order[slots[b]] = (uint32_t)items[b];
+                                                    uint64_t B, uint32_t* __restrict__ order) {
+    for (uint64_t b = blockIdx.x * 256ull + threadIdx.x; b < B; b += gridDim.x * 256ull) order[slots[b]] = (uint32_t)items[b];
 }
 
 // ------------------------------------------------------------------------------------------------
@@ -1253,16 +1254,20 @@ __device__ __forceinline__ uint64_t load_sa(const uint8_t* sa, uint32_t width, u
            ((uint64_t)p[4] << 32);
 }
 
+// permutation test: every entry below n and no position twice (one bit per position in `seen`); the
+// inverse rank[p] = k is 0-based (n may be 2^32)
 __global__ __launch_bounds__(256) void k_check_perm(const uint8_t* __restrict__ sa, uint32_t width, uint64_t n,
-                                                    uint32_t* __restrict__ rank, uint32_t* __restrict__ bad) {
+                                                    uint32_t* __restrict__ rank, uint32_t* __restrict__ seen,
+                                                    uint32_t* __restrict__ bad) {
     for (uint64_t k = blockIdx.x * 256ull + threadIdx.x; k < n; k += (uint64_t)gridDim.x * 256) {
         uint64_t p = load_sa(sa, width, k);
         if (p >= n) {
             atomicOr(bad, 1u);
             continue;
         }
-        uint32_t old = atomicExch(&rank[p], (uint32_t)(k + 1));
-        if (old != 0) atomicOr(bad, 1u);
+        const uint32_t bit = 1u << (uint32_t)(p & 31);
+        if (atomicOr(&seen[p >> 5], bit) & bit) atomicOr(bad, 1u);
+        rank[p] = (uint32_t)k;
     }
 }
 
@@ -1273,7 +1278,8 @@ __global__ __launch_bounds__(256) void k_check_order(const uint8_t* __restrict__
         uint64_t p = load_sa(sa, width, k), q = load_sa(sa, width, k + 1);
         if (p >= n || q >= n) continue;  // reported by the permutation check
         uint8_t cp = text[p], cq = text[q];
-        uint32_t rp = rank[p + 1], rq = rank[q + 1];
+        // rank of the next suffix; the empty suffix (p + 1 = n) is the smallest
+        const int64_t rp = p + 1 < n ? (int64_t)rank[p + 1] : -1, rq = q + 1 < n ? (int64_t)rank[q + 1] : -1;
         bool ok = cp != cq ? (cp < cq) : (rp <= rq);
         if (!ok) atomicOr(bad, 2u);
     }
@@ -1333,15 +1339,15 @@ __global__ __launch_bounds__(256) void k_batch_text(const uint8_t* __restrict__ 
 template <class OutT>
 __global__ __launch_bounds__(256) void k_batch_seps(const uint64_t* __restrict__ co, uint32_t count,
                                                     OutT* __restrict__ out) {
-    for (uint32_t k = blockIdx.x * 256 + threadIdx.x; k < count; k += gridDim.x * 256) out[co[k + 1] - 1] = (OutT)1;
+    for (uint64_t k = blockIdx.x * 256ull + threadIdx.x; k < count; k += gridDim.x * 256ull) out[co[k + 1] - 1] = (OutT)1;
 }
 
 // (text, local position) of every suffix of the joined text in SA order; separators go to bucket `count`
-__global__ __launch_bounds__(256) void k_batch_keys(const uint32_t* __restrict__ sa, uint32_t N,
+__global__ __launch_bounds__(256) void k_batch_keys(const uint32_t* __restrict__ sa, uint64_t N,
                                                     const uint64_t* __restrict__ co, uint32_t count,
                                                     const uint32_t* __restrict__ table, uint64_t* __restrict__ keys,
                                                     uint32_t* __restrict__ vals) {
-    for (uint32_t j = blockIdx.x * 256 + threadIdx.x; j < N; j += gridDim.x * 256) {
+    for (uint64_t j = blockIdx.x * 256ull + threadIdx.x; j < N; j += gridDim.x * 256ull) {
         uint32_t p = sa[j];
         uint32_t k = find_owner_tab(co, count, table, 0, p);
         uint32_t local = (uint32_t)(p - co[k]);

@@ -101,8 +101,9 @@ struct StreamKeys {
 
 // keys[i] = src(i) for i < N (sorts of a single element)
 template <class Src>
-__global__ void k_materialize(Src src, uint32_t N, uint64_t* __restrict__ keys) {
-    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < N; i += gridDim.x * blockDim.x) keys[i] = src(i);
+__global__ void k_materialize(Src src, uint64_t N, uint64_t* __restrict__ keys) {
+    for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < N; i += (uint64_t)gridDim.x * blockDim.x)
+        keys[i] = src(i);
 }
 
 // ------------------------------------------------------------------------------------------------
@@ -131,11 +132,13 @@ __device__ __forceinline__ uint32_t xcc_id() { return __builtin_amdgcn_s_getreg(
 
 // Joint histograms of every pass from one read of the first-pass keys.  Workgroup b covers a
 // contiguous share of pass-0 segment b / wps; part[b][p][s][d] counts its keys whose pass p digit is d
-// and that lie in segment s of pass p.  Workgroup 0 also clears h2 for k_hist_seg_reduce.
+// and that lie in segment s of pass p.  Workgroup 0 also clears h2 for k_hist_seg_reduce.  The counts of
+// a workgroup fit 32 bits; the joint counts h2 of all keys are 64-bit (a sort may hold 2^32 keys, all of
+// them in one bin).
 template <class Src>
-__global__ __launch_bounds__(RS_HIST_THREADS) void k_radix_hist_seg(Src src, uint32_t N, uint32_t lo_bit, uint32_t np,
+__global__ __launch_bounds__(RS_HIST_THREADS) void k_radix_hist_seg(Src src, uint64_t N, uint32_t lo_bit, uint32_t np,
                                                         uint32_t seg0_keys, uint32_t wps, uint32_t* __restrict__ part,
-                                                        uint32_t* __restrict__ h2) {
+                                                        uint64_t* __restrict__ h2) {
     __shared__ uint32_t s_h[8 * RS_SEG_BINS];
     const uint32_t bins = np * RS_SEG_BINS;
     for (uint32_t i = threadIdx.x; i < bins; i += RS_HIST_THREADS) s_h[i] = 0;
@@ -168,7 +171,7 @@ __global__ __launch_bounds__(RS_HIST_THREADS) void k_radix_hist_seg(Src src, uin
 // histogram of the top 11 bits of the windows at [0, N), kept per pass-0 segment, gives all passes
 // after corrections at the ends.  part[b][w] counts the windows at the characters of histogram
 // workgroup b (a share of pass-0 segment b / wps) whose top 11 bits are w.
-__global__ __launch_bounds__(RS_HIST_THREADS) void k_window_hist_seg(const uint64_t* __restrict__ s, uint32_t bits, uint32_t N,
+__global__ __launch_bounds__(RS_HIST_THREADS) void k_window_hist_seg(const uint64_t* __restrict__ s, uint32_t bits, uint64_t N,
                                                          uint32_t seg0_keys, uint32_t wps,
                                                          uint32_t* __restrict__ part) {
     __shared__ uint32_t s_h[RS_SEG_BINS];
@@ -211,30 +214,30 @@ __global__ __launch_bounds__(256) void k_window_seg_reduce(const uint32_t* __res
 // Workgroup p: the joint counts h2[p][s][d] of pass p from the window histograms.  Pass p >= 1:
 // windows at [s_p, N + s_p) = all windows - [0, s_p) + [N, N + s_p).  Pass 0, segment [a, b):
 // windows at [a + s_0, b + s_0) = the segment's windows - [a, a + s_0) + [b, b + s_0).
-__global__ __launch_bounds__(256) void k_window_seg_joint(const uint64_t* __restrict__ s, uint32_t bits, uint32_t N,
+__global__ __launch_bounds__(256) void k_window_seg_joint(const uint64_t* __restrict__ s, uint32_t bits, uint64_t N,
                                                           uint32_t lo_bit, uint32_t seg0_keys,
                                                           const uint32_t* __restrict__ h11,
-                                                          uint32_t* __restrict__ h2) {
+                                                          uint64_t* __restrict__ h2) {
     __shared__ int s_c[RS_SEG_BINS];
     const uint32_t p = blockIdx.x, d = threadIdx.x;
     const uint32_t sp = (56 - lo_bit - 8 * p) / bits;
     const StreamKeys<false> win{s, bits, ~0ull};
     for (uint32_t i = d; i < RS_SEG_BINS; i += 256) s_c[i] = 0;
     __syncthreads();
-    uint32_t* out = h2 + (uint64_t)p * RS_SEG_BINS;
+    uint64_t* out = h2 + (uint64_t)p * RS_SEG_BINS;
     if (p > 0) {
         for (uint32_t t = d; t < sp; t += 256) {
             atomicSub(&s_c[(uint32_t)(win(t) >> 53)], 1);
-            atomicAdd(&s_c[(uint32_t)(win((uint64_t)N + t) >> 53)], 1);
+            atomicAdd(&s_c[(uint32_t)(win(N + t) >> 53)], 1);
         }
         __syncthreads();
 #pragma unroll
         for (int j = 0; j < RS_SEGS; j++) {
             const uint32_t w = (d << 3) | j;
-            int c = s_c[w];
+            int64_t c = s_c[w];
 #pragma unroll
-            for (int sg = 0; sg < RS_SEGS; sg++) c += (int)h11[sg * RS_SEG_BINS + w];
-            out[j * RS_RADIX + d] = (uint32_t)c;
+            for (int sg = 0; sg < RS_SEGS; sg++) c += (int64_t)h11[sg * RS_SEG_BINS + w];
+            out[j * RS_RADIX + d] = (uint64_t)c;
         }
     } else {
         // s_c[sg][d]: boundary corrections of segment sg
@@ -248,41 +251,41 @@ __global__ __launch_bounds__(256) void k_window_seg_joint(const uint64_t* __rest
         __syncthreads();
 #pragma unroll
         for (int sg = 0; sg < RS_SEGS; sg++) {
-            int c = s_c[sg * RS_RADIX + d];
+            int64_t c = s_c[sg * RS_RADIX + d];
 #pragma unroll
-            for (int j = 0; j < RS_SEGS; j++) c += (int)h11[sg * RS_SEG_BINS + ((d << 3) | j)];
-            out[sg * RS_RADIX + d] = (uint32_t)c;
+            for (int j = 0; j < RS_SEGS; j++) c += (int64_t)h11[sg * RS_SEG_BINS + ((d << 3) | j)];
+            out[sg * RS_RADIX + d] = (uint64_t)c;
         }
     }
 }
 
 // h2[i] += the sum of part[b][i] over this workgroup's share of the histogram workgroups b
 __global__ __launch_bounds__(256) void k_hist_seg_reduce(const uint32_t* __restrict__ part, uint32_t blocks,
-                                                         uint32_t bins, uint32_t* __restrict__ h2) {
+                                                         uint32_t bins, uint64_t* __restrict__ h2) {
     const uint32_t i = blockIdx.x * 256 + threadIdx.x;
     if (i >= bins) return;
-    uint32_t c = 0;
+    uint64_t c = 0;
     for (uint32_t b = blockIdx.y; b < blocks; b += gridDim.y) c += part[(uint64_t)b * bins + i];
-    if (c) atomicAdd(&h2[i], c);
+    if (c) atomicAdd((unsigned long long*)&h2[i], (unsigned long long)c);
 }
 
 // Workgroup p plans pass p: hist[p][d] = global offset of digit d, segoff[p][s][d] = offset of the keys
 // of digit d in segment s, info[p] = segment starts and tile prefixes (tiles of `tile` keys); it also
-// clears the pass's tile counters.
-__global__ __launch_bounds__(256) void k_seg_setup(const uint32_t* __restrict__ h2, uint32_t N, uint32_t tile,
-                                                   uint32_t seg0_keys, uint32_t* __restrict__ hist,
-                                                   uint32_t* __restrict__ segoff, uint32_t* __restrict__ info,
+// clears the pass's tile counters.  Counts and offsets are 64-bit: a sort may hold 2^32 keys.
+__global__ __launch_bounds__(256) void k_seg_setup(const uint64_t* __restrict__ h2, uint64_t N, uint32_t tile,
+                                                   uint32_t seg0_keys, uint64_t* __restrict__ hist,
+                                                   uint64_t* __restrict__ segoff, uint64_t* __restrict__ info,
                                                    uint32_t* __restrict__ tick) {
-    __shared__ uint32_t s_tmp[4];
-    __shared__ uint32_t s_prev[RS_RADIX];
+    __shared__ uint64_t s_tmp[4];
+    __shared__ uint64_t s_prev[RS_RADIX];
     const uint32_t p = blockIdx.x, d = threadIdx.x;
-    const uint32_t* hp = h2 + (uint64_t)p * RS_SEG_BINS;
-    uint32_t t = 0;
+    const uint64_t* hp = h2 + (uint64_t)p * RS_SEG_BINS;
+    uint64_t t = 0;
 #pragma unroll
     for (int sg = 0; sg < RS_SEGS; sg++) t += hp[sg * RS_RADIX + d];
-    const uint32_t db = block_excl_scan<256>(t, s_tmp, nullptr);
+    const uint64_t db = block_excl_scan64<256>(t, s_tmp, nullptr);
     hist[p * RS_RADIX + d] = db;
-    uint32_t run = db;
+    uint64_t run = db;
 #pragma unroll
     for (int sg = 0; sg < RS_SEGS; sg++) {
         segoff[p * RS_SEG_BINS + sg * RS_RADIX + d] = run;
@@ -292,23 +295,23 @@ __global__ __launch_bounds__(256) void k_seg_setup(const uint32_t* __restrict__ 
     if (p > 0) {
         // segment starts: the digit offsets of the previous pass at multiples of 32
         __syncthreads();
-        const uint32_t* hq = hp - RS_SEG_BINS;
-        uint32_t tq = 0;
+        const uint64_t* hq = hp - RS_SEG_BINS;
+        uint64_t tq = 0;
 #pragma unroll
         for (int sg = 0; sg < RS_SEGS; sg++) tq += hq[sg * RS_RADIX + d];
-        s_prev[d] = block_excl_scan<256>(tq, s_tmp, nullptr);
+        s_prev[d] = block_excl_scan64<256>(tq, s_tmp, nullptr);
     }
     __syncthreads();
     if (d == 0) {
-        uint32_t* in = info + p * RS_SEG_INFO;
+        uint64_t* in = info + p * RS_SEG_INFO;
         for (int sg = 0; sg <= RS_SEGS; sg++) {
-            uint32_t a;
+            uint64_t a;
             if (sg == RS_SEGS) a = N;
-            else if (p == 0) a = (uint64_t)sg * seg0_keys < N ? sg * seg0_keys : N;
+            else if (p == 0) a = (uint64_t)sg * seg0_keys < N ? (uint64_t)sg * seg0_keys : N;
             else a = s_prev[sg << RS_SEG_SHIFT];
             in[sg] = a;
         }
-        uint32_t T = 0;
+        uint64_t T = 0;
         for (int sg = 0; sg <= RS_SEGS; sg++) {
             in[RS_SEGS + 1 + sg] = T;
             if (sg < RS_SEGS) T += (in[sg + 1] - in[sg] + tile - 1) / tile;
@@ -328,8 +331,8 @@ template <bool VALS, bool IDENT, class Src, class KeyT = uint64_t, int ITEMS = R
           class VinT = uint32_t, class VoutT = VinT>
 __global__ __launch_bounds__(RS_THREADS) void k_onesweep(Src src, OutT* __restrict__ kout,
                                                          const VinT* __restrict__ vin, VoutT* __restrict__ vout,
-                                                         uint32_t shift, const uint32_t* __restrict__ segoff,
-                                                         const uint32_t* __restrict__ info, uint32_t* __restrict__ tick,
+                                                         uint32_t shift, const uint64_t* __restrict__ segoff,
+                                                         const uint64_t* __restrict__ info, uint32_t* __restrict__ tick,
                                                          uint64_t* __restrict__ status, uint32_t tag,
                                                          uint32_t out_shift = 0, uint32_t kpos = 0) {
     constexpr int TILE = RS_THREADS * ITEMS;
@@ -343,6 +346,7 @@ __global__ __launch_bounds__(RS_THREADS) void k_onesweep(Src src, OutT* __restri
     __shared__ uint32_t s_gofs[RS_RADIX];
     __shared__ uint32_t s_scan[RS_WAVES];
     __shared__ uint32_t s_tile[4];
+    __shared__ uint64_t s_base;
 
     const uint32_t tid = threadIdx.x;
     const uint32_t lane = __lane_id();
@@ -350,25 +354,27 @@ __global__ __launch_bounds__(RS_THREADS) void k_onesweep(Src src, OutT* __restri
     if (tid == 0) {
         // the next tile of this XCD's segment, else of the next segment with tiles left (the grid has a
         // workgroup per tile, so every tile is taken)
-        const uint32_t* tp = info + RS_SEGS + 1;
+        // segment starts are 64-bit (a sort may hold 2^32 keys); tile prefixes fit 32 bits
+        const uint64_t* tp = info + RS_SEGS + 1;
         const uint32_t x = xcc_id();
         uint32_t sg = 0, i = 0, found = 0;
         for (uint32_t v = 0; v < (uint32_t)RS_SEGS && !found; v++) {
             sg = (x + v) & (RS_SEGS - 1);
-            const uint32_t T = tp[sg + 1] - tp[sg];
+            const uint32_t T = (uint32_t)(tp[sg + 1] - tp[sg]);
             if (T) {
                 i = atomicAdd(&tick[sg], 1u);
                 found = i < T;
             }
         }
-        uint32_t n = 0, b = 0;
+        uint32_t n = 0;
+        uint64_t b = 0;
         if (found) {
-            b = info[sg] + i * (uint32_t)TILE;
-            n = info[sg + 1] - b < (uint32_t)TILE ? info[sg + 1] - b : (uint32_t)TILE;
+            b = info[sg] + (uint64_t)i * TILE;
+            n = info[sg + 1] - b < (uint64_t)TILE ? (uint32_t)(info[sg + 1] - b) : (uint32_t)TILE;
         }
-        s_tile[0] = i;       // tile within the segment
-        s_tile[1] = tp[sg];  // status slot of the segment's first tile
-        s_tile[2] = b;       // first key
+        s_tile[0] = i;                 // tile within the segment
+        s_tile[1] = (uint32_t)tp[sg];  // status slot of the segment's first tile
+        s_base = b;                    // first key
         s_tile[3] = n | (sg << 29);
     }
     for (int i = tid; i < RS_WAVES * RS_RADIX; i += RS_THREADS) (&s_wcnt[0][0])[i] = 0;
@@ -378,7 +384,7 @@ __global__ __launch_bounds__(RS_THREADS) void k_onesweep(Src src, OutT* __restri
     const uint32_t tile = s_tile[0];
     const uint32_t slot0 = s_tile[1];
     const uint32_t sg = s_tile[3] >> 29;
-    const uint64_t tile_base = s_tile[2];
+    const uint64_t tile_base = s_base;
     const uint64_t tile_end = tile_base + tile_n;
     const uint64_t wbase = tile_base + (uint64_t)wave * (64 * ITEMS);
 
@@ -544,7 +550,9 @@ __global__ __launch_bounds__(RS_THREADS) void k_onesweep(Src src, OutT* __restri
             }
             st_agent(st, lb_pack(tag, LB_INC, excl + total));
         }
-        s_gofs[d] = segoff[sg * RS_RADIX + d] + (uint32_t)excl - start;
+        // output index of the tile's position i with this digit: gofs + i (modulo 2^32: every index
+        // written is below the key count, at most 2^32)
+        s_gofs[d] = (uint32_t)segoff[sg * RS_RADIX + d] + (uint32_t)excl - start;
     }
     __syncthreads();
     // write out: position i of the reordered tile goes to gofs[digit] + i; the digits of a thread's
@@ -676,7 +684,10 @@ __global__ __launch_bounds__(SC_THREADS) void k_scan(Op op, uint64_t N, uint64_t
             }
             const uint64_t excl = ex_lo | (ex_hi << 32);
             s_excl = excl;
-            if (tile == gridDim.x - 1 && total_out) *total_out = excl + tile_total;
+            if (tile == gridDim.x - 1 && total_out) {  // both lane totals in full (a lane may total 2^32)
+                total_out[0] = ex_lo + tot_lo;
+                total_out[1] = ex_hi + tot_hi;
+            }
         }
     }
     __syncthreads();
@@ -809,7 +820,8 @@ __global__ __launch_bounds__(SC_THREADS) void k_scan_reduce(Op op, uint64_t N, u
     }
 }
 
-// exclusive prefixes of nb <= 4 * 1024 block sums, packed as lo | hi << 32; grand total to *total_out
+// exclusive prefixes of nb <= 4 * 1024 block sums, packed as lo | hi << 32 (an exclusive prefix of either
+// lane is below 2^32); the grand totals of both lanes in full to total_out[0] and total_out[1]
 __global__ __launch_bounds__(1024) void k_scan_spine(const uint64_t* __restrict__ part, uint32_t nb,
                                                      uint64_t* __restrict__ prefix, uint64_t* __restrict__ total_out) {
     __shared__ uint64_t s_tmp[1024 / 64];
@@ -833,7 +845,10 @@ __global__ __launch_bounds__(1024) void k_scan_spine(const uint64_t* __restrict_
         elo += lo[k];
         ehi += hi[k];
     }
-    if (tid == 0 && total_out) *total_out = tlo | (thi << 32);
+    if (tid == 0 && total_out) {
+        total_out[0] = tlo;
+        total_out[1] = thi;
+    }
 }
 
 template <class Op>
@@ -947,17 +962,17 @@ struct Primitives {
     uint32_t scan_tag = 0;
     unsigned long long* ticket = nullptr;  // device counter of scan tile tickets
     unsigned long long ticket_host = 0;
-    uint64_t* scalars = nullptr;           // device scalars (scan totals)
+    uint64_t* scalars = nullptr;           // device scalars: scan totals, two per slot (low and high lane)
     uint64_t* scan_part = nullptr;         // two-pass scan: block sums (2 per block) and block prefixes
     uint64_t* scan_prefix = nullptr;
-    uint32_t* hist = nullptr;              // 8 passes x 256 global digit offsets of the last sort
+    uint64_t* hist = nullptr;              // 8 passes x 256 global digit offsets of the last sort
     // segmented radix passes (up to 8 passes): per-histogram-workgroup joint counts, their sums, the
     // (segment, digit) offsets, segment starts / tile prefixes and the per-segment tile counters
     uint32_t* seg_part = nullptr;
-    uint32_t* seg_h2 = nullptr;
+    uint64_t* seg_h2 = nullptr;
     uint32_t* seg_h11 = nullptr;  // window histograms per pass-0 segment
-    uint32_t* seg_off = nullptr;
-    uint32_t* seg_info = nullptr;
+    uint64_t* seg_off = nullptr;
+    uint64_t* seg_info = nullptr;
     uint32_t* seg_tick = nullptr;
     uint64_t* host_scalars = nullptr;      // pinned mirror
     uint32_t cus = 1;                      // compute units of the device
@@ -1008,12 +1023,12 @@ struct Primitives {
         DCX_HIP(hipMemsetAsync(ticket, 0, sizeof(unsigned long long), stream));
         DCX_HIP(hipMalloc(&scalars, 64 * sizeof(uint64_t)));
         DCX_HIP(hipMemsetAsync(scalars, 0, 64 * sizeof(uint64_t), stream));
-        DCX_HIP(hipMalloc(&hist, 8 * RS_RADIX * sizeof(uint32_t)));
+        DCX_HIP(hipMalloc(&hist, 8 * RS_RADIX * sizeof(uint64_t)));
         DCX_HIP(hipMalloc(&seg_part, (size_t)RS_SEGS * RS_HIST_WPS * 8 * RS_SEG_BINS * sizeof(uint32_t)));
-        DCX_HIP(hipMalloc(&seg_h2, 8 * RS_SEG_BINS * sizeof(uint32_t)));
+        DCX_HIP(hipMalloc(&seg_h2, 8 * RS_SEG_BINS * sizeof(uint64_t)));
         DCX_HIP(hipMalloc(&seg_h11, RS_SEGS * RS_SEG_BINS * sizeof(uint32_t)));
-        DCX_HIP(hipMalloc(&seg_off, 8 * RS_SEG_BINS * sizeof(uint32_t)));
-        DCX_HIP(hipMalloc(&seg_info, 8 * RS_SEG_INFO * sizeof(uint32_t)));
+        DCX_HIP(hipMalloc(&seg_off, 8 * RS_SEG_BINS * sizeof(uint64_t)));
+        DCX_HIP(hipMalloc(&seg_info, 8 * RS_SEG_INFO * sizeof(uint64_t)));
         DCX_HIP(hipMalloc(&seg_tick, 8 * RS_SEGS * sizeof(uint32_t)));
         DCX_HIP(hipMalloc(&scan_part, 2 * SC_BLOCKS * sizeof(uint64_t)));
         DCX_HIP(hipMalloc(&scan_prefix, SC_BLOCKS * sizeof(uint64_t)));
@@ -1027,9 +1042,11 @@ struct Primitives {
         if (ticket) (void)hipFree(ticket);
         if (scalars) (void)hipFree(scalars);
         if (hist) (void)hipFree(hist);
-        for (uint32_t* q : {seg_part, seg_h2, seg_h11, seg_off, seg_info, seg_tick})
+        for (void* q : {(void*)seg_part, (void*)seg_h2, (void*)seg_h11, (void*)seg_off, (void*)seg_info,
+                        (void*)seg_tick})
             if (q) (void)hipFree(q);
-        seg_part = seg_h2 = seg_h11 = seg_off = seg_info = seg_tick = nullptr;
+        seg_part = seg_h11 = seg_tick = nullptr;
+        seg_h2 = seg_off = seg_info = nullptr;
         if (scan_part) (void)hipFree(scan_part);
         if (scan_prefix) (void)hipFree(scan_prefix);
         scan_part = scan_prefix = nullptr;
@@ -1089,16 +1106,16 @@ struct Primitives {
 
     // Key sources whose digits are byte windows of a character stream (histograms from windows)
     template <class Src>
-    static bool window_aligned(const Src&, uint32_t, uint32_t, uint32_t) {
+    static bool window_aligned(const Src&, uint64_t, uint32_t, uint32_t) {
         return false;
     }
-    static bool window_aligned(const StreamKeys<false>& src, uint32_t N, uint32_t lo_bit, uint32_t np) {
+    static bool window_aligned(const StreamKeys<false>& src, uint64_t N, uint32_t lo_bit, uint32_t np) {
         return (8 % src.bits == 0) && lo_bit % 8 == 0 && lo_bit + 8 * np <= 64 &&
                (src.mask & (~0ull << lo_bit)) == (~0ull << lo_bit) && N > 64;
     }
     template <class Src>
-    void hist_windows(const Src&, uint32_t, uint32_t, uint32_t, uint32_t, uint32_t) {}
-    void hist_windows(const StreamKeys<false>& src, uint32_t N, uint32_t lo_bit, uint32_t np, uint32_t seg0_keys,
+    void hist_windows(const Src&, uint64_t, uint32_t, uint32_t, uint32_t, uint32_t) {}
+    void hist_windows(const StreamKeys<false>& src, uint64_t N, uint32_t lo_bit, uint32_t np, uint32_t seg0_keys,
                       uint32_t wps) {
         hipLaunchKernelGGL(k_window_hist_seg, dim3(RS_SEGS * wps), dim3(RS_HIST_THREADS), 0, stream, src.s, src.bits, N, seg0_keys,
                            wps, seg_part);
@@ -1112,7 +1129,7 @@ struct Primitives {
     // `tile` keys from one read of the first-pass keys: hist, seg_off, seg_info, cleared seg_tick.
     // Returns the grid of every pass: one workgroup per tile of all segments (at most N / tile + 8).
     template <class Src>
-    uint64_t seg_plan(const Src& first, uint32_t N, uint32_t lo_bit, uint32_t np, uint32_t tile) {
+    uint64_t seg_plan(const Src& first, uint64_t N, uint32_t lo_bit, uint32_t np, uint32_t tile) {
         const uint64_t tiles = ceil_div(N, tile);
         const uint64_t seg0_keys = ceil_div(tiles, (uint64_t)RS_SEGS) * tile;
         const uint64_t grid = tiles + RS_SEGS;
@@ -1128,7 +1145,7 @@ struct Primitives {
             hipLaunchKernelGGL(k_hist_seg_reduce, dim3(bins / 256, 16), dim3(256), 0, stream,
                                (const uint32_t*)seg_part, (uint32_t)(RS_SEGS * wps), bins, seg_h2);
         }
-        hipLaunchKernelGGL(k_seg_setup, dim3(np), dim3(256), 0, stream, (const uint32_t*)seg_h2, N, tile,
+        hipLaunchKernelGGL(k_seg_setup, dim3(np), dim3(256), 0, stream, (const uint64_t*)seg_h2, N, tile,
                            (uint32_t)seg0_keys, hist, seg_off, seg_info, seg_tick);
         return grid;
     }
@@ -1152,7 +1169,7 @@ struct Primitives {
     // up to N rounded up to 16 (DeviceCache blocks are).
     template <class Src>
     uint32_t sort_pairs_from(const Src& first, uint64_t*& keys, uint64_t*& keys_alt, uint32_t*& vals,
-                         uint32_t*& vals_alt, uint32_t N, uint32_t lo_bit, uint32_t hi_bit, bool ident_vals,
+                         uint32_t*& vals_alt, uint64_t N, uint32_t lo_bit, uint32_t hi_bit, bool ident_vals,
                          uint32_t* low32 = nullptr, uint8_t* idx_bytes = nullptr, uint8_t* idx_bytes_alt = nullptr) {
         if (hi_bit <= lo_bit) hi_bit = lo_bit + 1;
         if (N <= 1) {
@@ -1189,8 +1206,8 @@ struct Primitives {
             }
             uint32_t tag = ++sort_tag;
             const uint32_t shift = lo_bit + 8 * p;
-            const uint32_t* so = seg_off + p * RS_SEG_BINS;
-            const uint32_t* si = seg_info + p * RS_SEG_INFO;
+            const uint64_t* so = seg_off + p * RS_SEG_BINS;
+            const uint64_t* si = seg_info + p * RS_SEG_INFO;
             uint32_t* tk = seg_tick + p * RS_SEGS;
             const dim3 g((unsigned)tiles), b(RS_THREADS);
             if (split) {
@@ -1259,7 +1276,7 @@ struct Primitives {
     }
 
     // the same with the keys array as the first pass's key source
-    void sort_pairs(uint64_t*& keys, uint64_t*& keys_alt, uint32_t*& vals, uint32_t*& vals_alt, uint32_t N,
+    void sort_pairs(uint64_t*& keys, uint64_t*& keys_alt, uint32_t*& vals, uint32_t*& vals_alt, uint64_t N,
                     uint32_t lo_bit, uint32_t hi_bit) {
         sort_pairs_from(ArrayKeys<uint64_t>{keys}, keys, keys_alt, vals, vals_alt, N, lo_bit, hi_bit, false);
     }
@@ -1268,7 +1285,7 @@ struct Primitives {
     // The input arrays are left alone; the result ends in *keys_out / *vals_out, which point to either
     // (k1, v1) or (k2, v2).  vals == nullptr: the values are the input indices.
     void sort_u32(const uint32_t* keys, const uint32_t* vals, uint32_t* k1, uint32_t* v1, uint32_t* k2, uint32_t* v2,
-                  uint32_t N, uint32_t lo_bit, uint32_t np, uint32_t** keys_out, uint32_t** vals_out) {
+                  uint64_t N, uint32_t lo_bit, uint32_t np, uint32_t** keys_out, uint32_t** vals_out) {
         const uint32_t* kin = keys;
         const uint32_t* vin = vals;
         uint32_t* ko = k1;
@@ -1303,7 +1320,8 @@ struct Primitives {
         }
     }
 
-    // Launches a scan; the grand total lands in device scalar slot `slot` (read with read_scalars).
+    // Launches a scan; the grand totals of both 32-bit lanes land in device scalar slot `slot` (read with
+    // totals(slot)).
     // Blocks of the two-pass scan of Op. Every block scans the same number of tiles, so a round of
     // resident blocks that is only partly filled takes as long as a full one: the count is a whole number
     // of rounds of both kernels (the reduce kernel usually holds more blocks per compute unit than the
@@ -1325,7 +1343,7 @@ struct Primitives {
     template <class Op>
     void scan(const Op& op, uint64_t N, int slot) {
         if (N == 0) {
-            fill_u64(scalars + slot, 0, 1);
+            fill_u64(scalars + 2 * slot, 0, 2);
             return;
         }
         uint64_t tiles = ceil_div(N, SC_TILE);
@@ -1334,7 +1352,7 @@ struct Primitives {
             const uint32_t nb = (uint32_t)ceil_div(tiles, (uint64_t)tpb);
             hipLaunchKernelGGL(k_scan_reduce<Op>, dim3(nb), dim3(SC_THREADS), 0, stream, op, N, tpb, scan_part);
             hipLaunchKernelGGL(k_scan_spine, dim3(1), dim3(1024), 0, stream, (const uint64_t*)scan_part, nb,
-                               scan_prefix, scalars + slot);
+                               scan_prefix, scalars + 2 * slot);
             hipLaunchKernelGGL(k_scan_down<Op>, dim3(nb), dim3(SC_THREADS), 0, stream, op, N, tpb,
                                (const uint64_t*)scan_prefix);
             return;
@@ -1347,7 +1365,7 @@ struct Primitives {
         }
         uint32_t tag = ++scan_tag;
         hipLaunchKernelGGL(k_scan<Op>, dim3((unsigned)tiles), dim3(SC_THREADS), 0, stream, op, N, scan_lo, scan_hi,
-                           tag, ticket, ticket_host, scalars + slot);
+                           tag, ticket, ticket_host, scalars + 2 * slot);
         ticket_host += tiles;
     }
 
@@ -1363,7 +1381,7 @@ struct Primitives {
         const uint32_t nb = (uint32_t)ceil_div(tiles, (uint64_t)tpb);
         hipLaunchKernelGGL(k_scan_reduce<Op>, dim3(nb), dim3(SC_THREADS), 0, stream, op, N, tpb, scan_part);
         hipLaunchKernelGGL(k_scan_spine, dim3(1), dim3(1024), 0, stream, (const uint64_t*)scan_part, nb, scan_prefix,
-                           scalars + slot);
+                           scalars + 2 * slot);
     }
 
     // Copies `count` device scalars to the host (synchronizes the stream).
@@ -1371,6 +1389,15 @@ struct Primitives {
         DCX_HIP(hipMemcpyAsync(host_scalars, scalars, count * sizeof(uint64_t), hipMemcpyDeviceToHost, stream));
         DCX_HIP(hipStreamSynchronize(stream));
         return host_scalars;
+    }
+    // The lane totals of the last scan into slot `slot`: lo = sum of the low 32-bit halves of its values,
+    // hi = of the high halves (synchronizes the stream).
+    struct Totals {
+        uint64_t lo, hi;
+    };
+    Totals totals(int slot) {
+        const uint64_t* h = read_scalars(2 * slot + 2);
+        return {h[2 * slot], h[2 * slot + 1]};
     }
 };
 

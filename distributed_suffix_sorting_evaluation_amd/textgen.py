@@ -15,8 +15,8 @@ _LIB = None
 
 KINDS = {"uniform_acgt": 0, "genome": 1, "zipf256": 2, "acgtn": 3, "uniform_sigma": 4}
 
-# Longest text the engine takes (32-bit positions): 4 GiB - 4 MiB
-MAX_TEXT = (1 << 32) - (1 << 22)
+# Longest text the engine takes (32-bit positions 0 .. 2^32 - 1, 64-bit counts): 4 GiB
+MAX_TEXT = 1 << 32
 
 # The benchmark workloads (sizes and seeds) -- plain DCX on i.i.d. DNA, the genome-like text and the
 # byte-alphabet text (Zipf(1.1) over 256 values plus a 64 MiB period-17 block) at the longest supported

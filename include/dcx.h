@@ -107,14 +107,14 @@ int dcx_destroy(dcx_handle* h);
  * index type, cli.cpp:107, uint_types.hpp:262).  Replaces PDCX::compute_sa (pdcx.hpp:529) together
  * with compress_alphabet (cli.cpp:417-463) and the variant dispatch (cli.cpp:475-653).  The text is
  * not modified.  `stats` may be NULL.  A handle is not reentrant; different handles may be used
- * from different threads.  n may be at most 2^32 - 2^22 (32-bit positions inside the engine); longer
- * texts return DCX_E_UNSUPPORTED. */
+ * from different threads.  n may be at most 2^32 (32-bit positions inside the engine, 64-bit counts);
+ * longer texts return DCX_E_UNSUPPORTED. */
 int dcx_build_sa(dcx_handle* h, const uint8_t* text, uint64_t n, const dcx_options* opt, void* sa_out,
                  uint32_t sa_width_bytes, dcx_stats* stats);
 
 /* Suffix arrays of `count` independent texts in one call (batched mode).  Text k is
  * texts[offsets[k] .. offsets[k+1]) (offsets has count + 1 entries, host memory, non-decreasing, each text
- * at most 2^32 - 2^22 characters); its SA is written to sa_out[offsets[k] - offsets[0] .. offsets[k+1] - offsets[0])
+ * at most 2^32 - 1 characters: with its separator it must fit 32-bit positions); its SA is written to sa_out[offsets[k] - offsets[0] .. offsets[k+1] - offsets[0])
  * as 32-bit positions local to the text (sa_out holds offsets[count] - offsets[0] entries).  The texts are joined with a separator symbol smaller than every character and run through one
  * DCX pipeline per sub-batch (at most batch_max_chars characters); a stable radix partition by text then
  * yields the per-text suffix arrays.  No counterpart in the reference: it builds one SA per cli run

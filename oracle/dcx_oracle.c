@@ -549,7 +549,7 @@ void oracle_write_uint40(const uint64_t *SA, uint64_t n, uint8_t *out) {
 #include <time.h>
 #include "dcx.h"
 
-#define HOST_MAX_TEXT ((1ull << 32) - (1ull << 22)) /* the engine's 32-bit position limit */
+#define HOST_MAX_TEXT (1ull << 32) /* the engine's limit: 32-bit positions 0 .. 2^32 - 1      */
 #define HOST_BASE_CASE_MAX 16384ull                 /* the engine's base-case limit       */
 
 struct dcx_handle {
@@ -666,7 +666,7 @@ int dcx_host_build_sa_batch(dcx_handle *h, const uint8_t *texts, const uint64_t 
     const uint64_t base = offsets[0], total = offsets[count] - offsets[0];
     for (uint32_t k = 0; k < count; k++) {
         if (offsets[k + 1] < offsets[k]) return host_fail(h, DCX_E_ARG, "offsets must be non-decreasing");
-        if (offsets[k + 1] - offsets[k] > HOST_MAX_TEXT)
+        if (offsets[k + 1] - offsets[k] + 1 > HOST_MAX_TEXT) /* with its separator */
             return host_fail(h, DCX_E_UNSUPPORTED, "a text of the batch is too long for 32-bit positions");
     }
     if (total && (!texts || !sa_out)) return host_fail(h, DCX_E_ARG, "null argument");

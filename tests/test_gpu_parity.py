@@ -212,8 +212,8 @@ def test_errors(engine):
         assert e.value.code == code
     with pytest.raises(KeyError):
         engine.build_sa(t, sa_width=3)
-    # texts beyond the 32-bit build's longest (4 GiB - 4 MiB) are refused before anything is read: the
-    # zero pages of these arrays stay unmapped
+    # texts beyond the longest the build takes (2^32 characters: 32-bit positions) are refused before
+    # anything is read: the zero pages of these arrays stay unmapped
     from distributed_suffix_sorting_evaluation_amd import textgen
     with pytest.raises(DCXError) as e:
         engine.build_sa(np.zeros(textgen.MAX_TEXT + 1, dtype=np.uint8), sa_width=8)
@@ -283,9 +283,11 @@ def test_genome_1gib_digest(engine):
 
 
 def test_genome_max_text_device_check(engine):
-    """The longest text the build takes (4 GiB - 4 MiB) over ACGT: round 0 then sorts 40-bit keys whose
-    positions need all 32 bits (24 in the free key bits, 8 in the byte beside).  The 40-bit SA stays in
-    HBM and is checked there (permutation, suffix order by first character and the rank of the next)."""
+    """The longest text the build takes, 2^32 characters over ACGT: every count of level 0 (positions,
+    tied positions, radix bins, scan totals) reaches 2^32, past 32 bits, while the positions 0 .. 2^32 - 1
+    still fit them; round 0 sorts 40-bit keys whose positions need all 32 bits (24 in the free key bits, 8
+    in the byte beside).  The 40-bit SA stays in HBM and is checked there (permutation, suffix order by
+    first character and the rank of the next)."""
     import torch
     from distributed_suffix_sorting_evaluation_amd import textgen
     n = textgen.MAX_TEXT
