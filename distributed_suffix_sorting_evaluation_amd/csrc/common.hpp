@@ -4,6 +4,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
+#include <iterator>
 #include <map>
 #include <stdexcept>
 #include <string>
@@ -28,12 +29,40 @@ struct Error : std::runtime_error {
 // Stream-ordered caching allocator.  Every engine kernel runs on the handle's single stream, so a
 // block returned to the cache can be handed out again at once: later users are ordered after the
 // earlier kernels on that stream.
+//
+// Arena mode (set_arena): every allocation is carved from a fixed device range given by the caller
+// (first fit over the free ranges, frees coalesce with their neighbours) and nothing reaches hipMalloc or
+// hipFree; a request that does not fit throws DCX_E_NOMEM.  Blocks handed out before the arena was set go
+// back to the cache as usual.
 class DeviceCache {
   public:
     ~DeviceCache() { release_all(); }
+    void set_arena(void* base, size_t bytes) {
+        arena_ = static_cast<char*>(base);
+        arena_bytes_ = bytes & ~size_t(255);
+        arena_free_.clear();
+        arena_used_.clear();
+        arena_free_[0] = arena_bytes_;
+    }
+    // leaves arena mode; returns the number of arena blocks still held (0 unless a buffer outlived its call)
+    size_t clear_arena() {
+        const size_t held = arena_used_.size();
+        in_use_ -= arena_in_use_;
+        arena_in_use_ = 0;
+        arena_ = nullptr;
+        arena_bytes_ = 0;
+        arena_free_.clear();
+        arena_used_.clear();
+        return held;
+    }
+    bool in_arena(const void* p) const {
+        const char* c = static_cast<const char*>(p);
+        return arena_ && c >= arena_ && c < arena_ + arena_bytes_;
+    }
     void* alloc(size_t bytes) {
         bytes = (bytes + 255) & ~size_t(255);
         if (bytes == 0) bytes = 256;
+        if (arena_) return arena_alloc(bytes);
         auto it = free_.lower_bound(bytes);
         if (it != free_.end() && it->first <= bytes * 2 + (1u << 20)) {
             void* p = it->second;
@@ -64,6 +93,10 @@ class DeviceCache {
     }
     void free(void* p) {
         if (!p) return;
+        if (in_arena(p)) {
+            arena_release(p);
+            return;
+        }
         auto it = used_.find(p);
         if (it == used_.end()) return;
         in_use_ -= it->second;
@@ -89,9 +122,52 @@ class DeviceCache {
     size_t in_use() const { return in_use_; }
 
   private:
+    void* arena_alloc(size_t bytes) {
+        for (auto it = arena_free_.begin(); it != arena_free_.end(); ++it) {
+            if (it->second < bytes) continue;
+            const size_t off = it->first, len = it->second;
+            arena_free_.erase(it);
+            if (len > bytes) arena_free_[off + bytes] = len - bytes;
+            arena_used_[off] = bytes;
+            arena_in_use_ += bytes;
+            in_use_ += bytes;
+            peak_ = in_use_ > peak_ ? in_use_ : peak_;
+            return arena_ + off;
+        }
+        throw Error(-3, "scratch arena of " + std::to_string(arena_bytes_) + " bytes cannot hold another " +
+                            std::to_string(bytes) + " bytes");
+    }
+    void arena_release(void* p) {
+        const size_t off = static_cast<size_t>(static_cast<char*>(p) - arena_);
+        auto u = arena_used_.find(off);
+        if (u == arena_used_.end()) return;
+        size_t start = off, len = u->second;
+        arena_used_.erase(u);
+        arena_in_use_ -= len;
+        in_use_ -= len;
+        auto next = arena_free_.lower_bound(start);
+        if (next != arena_free_.end() && next->first == start + len) {  // merge with the following range
+            len += next->second;
+            next = arena_free_.erase(next);
+        }
+        if (next != arena_free_.begin()) {  // and with the preceding one
+            auto prev = std::prev(next);
+            if (prev->first + prev->second == start) {
+                start = prev->first;
+                len += prev->second;
+                arena_free_.erase(prev);
+            }
+        }
+        arena_free_[start] = len;
+    }
+
     std::multimap<size_t, void*> free_;
     std::map<void*, size_t> used_;
     size_t in_use_ = 0, peak_ = 0, total_ = 0;
+    char* arena_ = nullptr;
+    size_t arena_bytes_ = 0, arena_in_use_ = 0;
+    std::map<size_t, size_t> arena_free_;  // offset -> length of the free ranges
+    std::map<size_t, size_t> arena_used_;  // offset -> length of the blocks handed out
 };
 
 // RAII device buffer drawn from a DeviceCache.

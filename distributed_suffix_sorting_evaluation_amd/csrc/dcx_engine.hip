@@ -16,13 +16,49 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <thread>
 
 #include "../../include/dcx.h"
 #include "dcx_kernels.hpp"
 
+// Device memory of dcx_build_sa_noalloc: every buffer of such a build is carved from this array, which
+// each GPU the library runs on holds from the code object's load until the process ends.  Host code uses
+// only the address hipGetSymbolAddress returns for the current device (scratch_address), never the name.
+static __device__ __align__(256) uint8_t scratch[DCX_NOALLOC_SCRATCH_BYTES];
+
 namespace dcx {
+
+constexpr int MAX_DEVICES = 64;
+
+// This GPU's copy of `scratch`, looked up once per device with that device current (dcx_create calls it);
+// nullptr if the lookup failed.
+static void* scratch_address(int device) {
+    static std::mutex mu;
+    static void* addr[MAX_DEVICES] = {};
+    static bool done[MAX_DEVICES] = {};
+    if (device < 0 || device >= MAX_DEVICES) return nullptr;
+    std::lock_guard<std::mutex> lock(mu);
+    if (!done[device]) {
+        void* p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess) {
+            (void)hipGetLastError();
+            return nullptr;  // tried again by the next handle
+        }
+        addr[device] = p;
+        done[device] = true;
+    }
+    return addr[device];
+}
+
+// dcx_build_sa_noalloc calls on one GPU share its scratch array: they are serialized by this lock, held
+// until the call's work on the device has finished
+static std::mutex& scratch_call_mutex(int device) {
+    static std::mutex mu[MAX_DEVICES];
+    return mu[device];
+}
 
 struct Options {
     uint32_t X = 39;
@@ -59,6 +95,8 @@ class Engine {
     Primitives prim;
     uint64_t budget = 0;
     std::string last_error;
+    void* scratch = nullptr;  // this device's dcx_build_sa_noalloc array (scratch_address)
+    hipEvent_t evt[6] = {};   // phase timing events of a call, created with the handle
 
     // per call
     Options opt;
@@ -245,6 +283,13 @@ class Engine {
         DCX_HIP(hipSetDevice(device));
         DCX_HIP(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
         prim.init(stream, &cache);
+        for (auto& e : evt) DCX_HIP(hipEventCreate(&e));
+        // lookback status for the largest build without allocation (radix tiles of 64-bit keys, and the
+        // single-pass scans), so that such a build never has to grow it
+        prim.ensure_sort_status(ceil_div(DCX_NOALLOC_MAX_CHARS, (uint64_t)RS_TILE) + RS_SEGS);
+        prim.ensure_scan_status(SC_SINGLE_PASS_TILES);
+        scratch = scratch_address(device);
+        DCX_HIP(hipStreamSynchronize(stream));
     }
     ~Engine() {
         (void)hipSetDevice(device);
@@ -253,6 +298,8 @@ class Engine {
             if (h_stage[k]) (void)hipHostFree(h_stage[k]);
             if (stage_ev[k]) (void)hipEventDestroy(stage_ev[k]);
         }
+        for (auto& e : evt)
+            if (e) (void)hipEventDestroy(e);
         d_cmp.reset();
         prim.destroy();
         cache.release_all();
@@ -758,7 +805,7 @@ class Engine {
         if (n < (1u << 24)) return bits;
         const uint64_t stride = 61;  // odd: no alignment with periodic structure
         DBuf<unsigned long long> hist(cache, 257);
-        DCX_HIP(hipMemsetAsync(hist.get(), 0, 257 * 8, stream));
+        prim.fill_u64(reinterpret_cast<uint64_t*>(hist.get()), 0, 257);
         k_code_hist<<<grid(ceil_div(n, stride * 256) * 256, 4096), 256, 0, stream>>>(text, n, lut, stride, hist.get());
         unsigned long long h[257];
         DCX_HIP(hipMemcpyAsync(h, hist.get(), sizeof(h), hipMemcpyDeviceToHost, stream));
@@ -920,7 +967,7 @@ class Engine {
     // compress_alphabet (cli.cpp:417-463): used bytes -> 1..sigma in byte order; returns sigma and the map
     uint32_t alphabet(const uint8_t* d_text, uint64_t n, DBuf<uint16_t>& dlut) {
         DBuf<unsigned long long> mask(cache, 4);
-        DCX_HIP(hipMemsetAsync(mask.get(), 0, 4 * 8, stream));
+        prim.fill_u64(reinterpret_cast<uint64_t*>(mask.get()), 0, 4);
         if (n) k_byte_mask<<<grid(ceil_div(n, 16), 2048), 256, 0, stream>>>(d_text, n, mask.get());
         unsigned long long hm[4];
         DCX_HIP(hipMemcpyAsync(hm, mask.get(), sizeof(hm), hipMemcpyDeviceToHost, stream));
@@ -971,16 +1018,7 @@ class Engine {
                 p->drop_profile();
             }
         } profg{&prim};
-        hipEvent_t e0, e1;
-        DCX_HIP(hipEventCreate(&e0));
-        DCX_HIP(hipEventCreate(&e1));
-        struct EvGuard {
-            hipEvent_t a, b;
-            ~EvGuard() {
-                (void)hipEventDestroy(a);
-                (void)hipEventDestroy(b);
-            }
-        } evg{e0, e1};
+        hipEvent_t e0 = evt[0], e1 = evt[1];
         // texts into HBM
         DBuf<uint8_t> dtext;
         const uint8_t* d_texts = texts ? texts + base : nullptr;
@@ -1101,14 +1139,7 @@ class Engine {
         sa_width = width;
         cache.reset_peak();
 
-        hipEvent_t ev[6];
-        for (auto& e : ev) DCX_HIP(hipEventCreate(&e));
-        struct EvGuard {
-            hipEvent_t* e;
-            ~EvGuard() {
-                for (int i = 0; i < 6; i++) (void)hipEventDestroy(e[i]);
-            }
-        } evg{ev};
+        hipEvent_t* ev = evt;
 
         DCX_HIP(hipEventRecord(ev[0], stream));
         if (n == 0) {
@@ -1178,6 +1209,41 @@ class Engine {
                 std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
         }
         return DCX_OK;
+    }
+
+    // dcx_build_sa_noalloc: the same build of a device-resident text into a device SA, with every device
+    // buffer carved from this GPU's scratch array (the cache in arena mode) and the lookback status sized
+    // at handle creation: nothing is allocated.  Calls on one GPU are serialized (they share the array).
+    int build_sa_noalloc(const uint8_t* d_text, uint64_t n, const dcx_options* o, void* d_sa, uint32_t width,
+                         dcx_stats* stats) {
+        if (!o) throw Error(DCX_E_ARG, "null argument");
+        if (n > DCX_NOALLOC_MAX_CHARS)
+            throw Error(DCX_E_UNSUPPORTED, "dcx_build_sa_noalloc takes texts of at most " +
+                                               std::to_string(DCX_NOALLOC_MAX_CHARS) + " characters, not " +
+                                               std::to_string(n));
+        if (!scratch) throw Error(DCX_E_HIP, "the scratch array of this device could not be looked up");
+        dcx_options oo = *o;
+        oo.text_on_device = 1;
+        oo.sa_on_device = 1;
+        oo.profile_kernels = 0;
+        std::lock_guard<std::mutex> lock(scratch_call_mutex(device));
+        cache.set_arena(scratch, DCX_NOALLOC_SCRATCH_BYTES);
+        prim.no_alloc = true;
+        struct ArenaGuard {
+            Engine* e;
+            ~ArenaGuard() {
+                // the device is done with the array before the next call may use it; a comparison table
+                // carved from it is dropped (the next call builds it again)
+                (void)hipStreamSynchronize(e->stream);
+                if (e->d_cmp.get() && e->cache.in_arena(e->d_cmp.get())) {
+                    e->d_cmp.reset();
+                    e->cmp_X = 0;
+                }
+                e->prim.no_alloc = false;
+                e->cache.clear_arena();
+            }
+        } guard{this};
+        return build_sa(d_text, n, &oo, d_sa, width, stats);
     }
 
     // conservative device footprint of one build (level 0 dominates; deeper levels shrink by D/X)
@@ -1294,6 +1360,22 @@ int dcx_build_sa_batch(dcx_handle* h, const uint8_t* texts, const uint64_t* offs
     } catch (const Error& e) {
         h->eng->last_error = e.what();
         (void)hipStreamSynchronize(h->eng->stream);
+        (void)hipGetLastError();
+        return e.code;
+    } catch (const std::exception& e) {
+        h->eng->last_error = e.what();
+        return DCX_E_HIP;
+    }
+}
+
+int dcx_build_sa_noalloc(dcx_handle* h, const uint8_t* text, uint64_t n, const dcx_options* opt, void* sa_out,
+                         uint32_t sa_width_bytes, dcx_stats* stats) {
+    if (!h || !h->eng) return DCX_E_ARG;
+    try {
+        h->eng->last_error.clear();
+        return h->eng->build_sa_noalloc(text, n, opt, sa_out, sa_width_bytes, stats);
+    } catch (const Error& e) {
+        h->eng->last_error = e.what();
         (void)hipGetLastError();
         return e.code;
     } catch (const std::exception& e) {

@@ -976,6 +976,9 @@ struct Primitives {
     uint32_t* seg_tick = nullptr;
     uint64_t* host_scalars = nullptr;      // pinned mirror
     uint32_t cus = 1;                      // compute units of the device
+    // set while a call must not allocate: lookback status arrays that would have to grow throw instead
+    // (they are sized at handle creation for the largest such call)
+    bool no_alloc = false;
     // optional timing of the 64-bit radix passes: one event pair around the passes of each sort (they run
     // back to back; an event per pass would put a gap of a few microseconds after every pass)
     bool profile = false;
@@ -1082,6 +1085,7 @@ struct Primitives {
     void ensure_sort_status(uint64_t tiles) {
         size_t words = tiles * RS_RADIX;
         if (words <= sort_status_words) return;
+        if (no_alloc) throw Error(-3, "radix lookback status would have to grow in a call that must not allocate");
         if (sort_status) DCX_HIP(hipFree(sort_status));
         sort_status = nullptr;
         size_t cap = words + words / 4 + RS_RADIX;
@@ -1092,6 +1096,7 @@ struct Primitives {
     }
     void ensure_scan_status(uint64_t tiles) {
         if (tiles <= scan_tiles_cap) return;
+        if (no_alloc) throw Error(-3, "scan lookback status would have to grow in a call that must not allocate");
         if (scan_lo) DCX_HIP(hipFree(scan_lo));
         if (scan_hi) DCX_HIP(hipFree(scan_hi));
         scan_lo = scan_hi = nullptr;
@@ -1186,7 +1191,7 @@ struct Primitives {
         const bool split = low32 && ident_vals && idx_bytes && idx_bytes_alt && bit_width(N - 1) <= kpos + 8;
         const uint64_t tiles = seg_plan(first, N, lo_bit, np, RS_TILE);
         uint64_t span_bytes = 0;
-        if (profile) {  // the pair is listed at once, so drop_profile releases it whatever happens next
+        if (profile && !no_alloc) {  // the pair is listed at once, so drop_profile releases it whatever happens next
             hipEvent_t e0 = nullptr, e1 = nullptr;
             DCX_HIP(hipEventCreate(&e0));
             if (hipEventCreate(&e1) != hipSuccess) {
@@ -1266,7 +1271,7 @@ struct Primitives {
                 std::swap(vals, vals_alt);
             }
         }
-        if (profile) {
+        if (profile && !no_alloc) {
             DCX_HIP(hipEventRecord(pass_events.back().second, stream));
             pass_launches.back() = np;
             pass_items.back() = (uint64_t)N * np;

@@ -109,8 +109,12 @@ class Stats(ctypes.Structure):
 _lib = None
 _lib_lock = threading.Lock()
 
-EXPORTED = ["dcx_default_options", "dcx_create", "dcx_destroy", "dcx_build_sa", "dcx_build_sa_batch", "dcx_check_sa",
-            "dcx_strerror", "dcx_last_error"]
+EXPORTED = ["dcx_default_options", "dcx_create", "dcx_destroy", "dcx_build_sa", "dcx_build_sa_batch",
+            "dcx_build_sa_noalloc", "dcx_check_sa", "dcx_strerror", "dcx_last_error"]
+
+# dcx_build_sa_noalloc: longest text, and the device array per GPU it builds in (include/dcx.h)
+NOALLOC_MAX_CHARS = 1 << 20
+NOALLOC_SCRATCH_BYTES = 128 << 20
 
 
 def load_library(path: str = LIB_PATH):
@@ -142,6 +146,9 @@ def load_library(path: str = LIB_PATH):
         L.dcx_build_sa_batch.argtypes = [vp, vp, ctypes.POINTER(ctypes.c_uint64), ctypes.c_uint32,
                                          ctypes.POINTER(Options), vp, ctypes.POINTER(Stats)]
         L.dcx_build_sa_batch.restype = ctypes.c_int
+        L.dcx_build_sa_noalloc.argtypes = [vp, vp, ctypes.c_uint64, ctypes.POINTER(Options), vp, ctypes.c_uint32,
+                                           ctypes.POINTER(Stats)]
+        L.dcx_build_sa_noalloc.restype = ctypes.c_int
         L.dcx_check_sa.argtypes = [vp, vp, ctypes.c_uint64, vp, ctypes.c_uint32, ctypes.c_int,
                                    ctypes.POINTER(ctypes.c_int)]
         L.dcx_check_sa.restype = ctypes.c_int
@@ -239,6 +246,17 @@ class Engine:
         st = Stats()
         rc = self._L.dcx_build_sa(self._h, ctypes.c_void_p(text_ptr), n, ctypes.byref(o), ctypes.c_void_p(sa_ptr),
                                   sa_width, ctypes.byref(st))
+        if rc != DCX_OK:
+            self._raise(rc)
+        return st.to_dict()
+
+    def build_sa_noalloc(self, text_ptr: int, n: int, sa_ptr: int, sa_width: int = 8, **kw) -> dict:
+        """dcx_build_sa_noalloc: SA of a device-resident text (at most NOALLOC_MAX_CHARS characters) into a
+        device buffer, with every device buffer of the build carved from the library's per-GPU array."""
+        o = self.make_options(**kw)
+        st = Stats()
+        rc = self._L.dcx_build_sa_noalloc(self._h, ctypes.c_void_p(text_ptr), n, ctypes.byref(o),
+                                          ctypes.c_void_p(sa_ptr), sa_width, ctypes.byref(st))
         if rc != DCX_OK:
             self._raise(rc)
         return st.to_dict()

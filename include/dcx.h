@@ -122,6 +122,23 @@ int dcx_build_sa(dcx_handle* h, const uint8_t* text, uint64_t n, const dcx_optio
 int dcx_build_sa_batch(dcx_handle* h, const uint8_t* texts, const uint64_t* offsets, uint32_t count,
                        const dcx_options* opt, uint32_t* sa_out, dcx_stats* stats);
 
+/* Limits of dcx_build_sa_noalloc: the longest text, and the device array per GPU it builds in. */
+#define DCX_NOALLOC_MAX_CHARS (1ull << 20)
+#define DCX_NOALLOC_SCRATCH_BYTES (128ull << 20)
+
+/* dcx_build_sa for a device-resident text and a device output buffer (opt's pointer flags are ignored:
+ * both pointers are taken as device pointers) that allocates nothing: every device buffer of the build is
+ * carved from one array of DCX_NOALLOC_SCRATCH_BYTES that the library holds per GPU (from its first use on
+ * that GPU until the process ends), the lookback state was sized by dcx_create, and no event, stream or
+ * pinned buffer is created.  Texts of at most DCX_NOALLOC_MAX_CHARS characters; longer ones return
+ * DCX_E_UNSUPPORTED.  opt->profile_kernels is ignored.  Calls on one GPU share that array, so they are
+ * serialized: a call waits until the one before it on that GPU has finished on the device.  Output and
+ * status codes are those of dcx_build_sa; a build that would not fit the array returns DCX_E_NOMEM.
+ * No counterpart in the reference: it is for callers that cannot allocate device memory (tight serving
+ * loops, harnesses that forbid hipMalloc). */
+int dcx_build_sa_noalloc(dcx_handle* h, const uint8_t* text, uint64_t n, const dcx_options* opt, void* sa_out,
+                         uint32_t sa_width_bytes, dcx_stats* stats);
+
 /* Validity check of a suffix array on the device, after check_suffixarray
  * (lightweight_dcx/src/sa_check.hpp:56-184): permutation test plus non-decreasing
  * (T[SA[k]], rank[SA[k]+1]).  *ok = 1 if sa is the SA of text.  `on_device` flags both pointers as

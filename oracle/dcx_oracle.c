@@ -650,6 +650,20 @@ int dcx_host_build_sa(dcx_handle *h, const uint8_t *text, uint64_t n, const dcx_
     return rc == 0 ? DCX_OK : host_fail(h, DCX_E_NOMEM, "host allocation failed");
 }
 
+/* dcx_build_sa_noalloc on the CPU: the engine's argument checks in its order (the text limit first), then
+ * dcx_host_build_sa.  Here host memory stands in for the device memory the engine's call takes, so the
+ * pointers are host pointers and the pointer flags of the options are ignored, as the engine ignores them. */
+int dcx_host_build_sa_noalloc(dcx_handle *h, const uint8_t *text, uint64_t n, const dcx_options *o, void *sa_out,
+                              uint32_t width, dcx_stats *stats) {
+    if (!h) return DCX_E_ARG;
+    if (!o) return host_fail(h, DCX_E_ARG, "null argument");
+    if (n > DCX_NOALLOC_MAX_CHARS) return host_fail(h, DCX_E_UNSUPPORTED, "text too long for the call without allocation");
+    dcx_options oo = *o;
+    oo.text_on_device = 0;
+    oo.sa_on_device = 0;
+    return dcx_host_build_sa(h, text, n, &oo, sa_out, width, stats);
+}
+
 /* dcx_build_sa_batch on the CPU: each text's SA on its own (the joined-string pipeline of the engine
  * must give exactly these), written as 32-bit positions local to the text at sa_out + offsets[k] - offsets[0]. */
 int dcx_host_build_sa_batch(dcx_handle *h, const uint8_t *texts, const uint64_t *offsets, uint32_t count,

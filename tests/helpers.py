@@ -92,6 +92,8 @@ def host_abi():
     L.dcx_host_destroy.argtypes = [vp]
     L.dcx_host_build_sa.argtypes = [vp, vp, ctypes.c_uint64, ctypes.POINTER(eng.Options), vp, ctypes.c_uint32,
                                     ctypes.POINTER(eng.Stats)]
+    L.dcx_host_build_sa_noalloc.argtypes = [vp, vp, ctypes.c_uint64, ctypes.POINTER(eng.Options), vp, ctypes.c_uint32,
+                                            ctypes.POINTER(eng.Stats)]
     L.dcx_host_build_sa_batch.argtypes = [vp, vp, ctypes.POINTER(ctypes.c_uint64), ctypes.c_uint32,
                                           ctypes.POINTER(eng.Options), vp, ctypes.POINTER(eng.Stats)]
     L.dcx_host_check_sa.argtypes = [vp, vp, ctypes.c_uint64, vp, ctypes.c_uint32, ctypes.c_int,
