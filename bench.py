@@ -1,24 +1,35 @@
 #!/usr/bin/env python3
 """Benchmark: MB of text per second to a bit-exact suffix array on one MI355X.
 
-Workload (default): the 1 GiB genome-like text -- order-3 Markov over {A,C,G,T} with 5% of the length
-overwritten by copies of earlier segments (1-100 KiB, 1% point mutations) -- DC39 with character packing,
-40-bit (uint40) SA output.  A step is one full SA construction of that text, with the text already resident in
-HBM and the SA left in HBM.  After the timed steps the SA is validated on the device (sa_check.hpp restated:
-permutation + ordering); a suffix array is unique, so a valid SA is bit-exact.
+Workload (default): the 4 GiB byte-alphabet text -- 2^32 characters, Zipf(s = 1.1) over 256 byte values with a
+64 MiB period-17 block at a seeded offset -- DC39 with character packing (9-bit characters), 64-bit SA.
+A step is one full SA construction of that text, with the text already resident in HBM and the SA left in
+HBM.  After the timed steps the SA is validated on the device (sa_check.hpp restated: permutation + order);
+a suffix array is unique, so a valid SA is bit-exact.  The same line carries:
 
-The roofline object covers the dominant kernel, the LSD radix pass (k_onesweep with 64-bit keys): a launch
-moves N (u64 key, u32 value) pairs, 24 B per pair of algorithmic traffic, except that the first pass of a sort
-keyed by the character stream reads the stream (bits/8 B per key) and no identity values, and a last pass that
-keeps only the low key words writes 4 B keys; the engine sums these bytes per launch.  The launches are timed
-with hipEvents on the engine's stream during the timed steps.  cpu_baseline: the scalar C restatement of the same algorithm
-(oracle/, kind "port") on the first 16 MiB of the same text, on the host cores of this machine.
+* the same text at DC21 and DC133, the 1 GiB genome-like text at DC39 with 40-bit SA and the 8 MiB
+  i.i.d. DNA text at DC39 with 64-bit SA (``other_configs``, each timed and validated the same way);
+* the host-to-host rate of the default configuration (text in pageable host memory, SA back into
+  pageable host memory, PCIe included), its SA compared with the device SA byte for byte;
+* ``roofline``: the dominant kernel, the LSD radix pass k_onesweep over 64-bit keys.  A launch moves N
+  (u64 key, u32 value) pairs, 24 B per pair of algorithmic traffic (a first pass keyed by the character
+  stream reads the stream and no identity values; a last pass that keeps only the low key words writes 4 B
+  keys; passes with split positions move a byte instead of a 4-byte value); the engine sums these bytes per
+  launch, and the launches are timed with hipEvents on the engine's stream during the timed steps.
+  ``traffic`` is the HBM bytes per launch measured by rocprofv3 (FETCH_SIZE, WRITE_SIZE) in a separate
+  profiled run, committed under profiles/ (named in ``traffic_source``);
+* ``cpu_baseline``: the multi-threaded host restatement of the same algorithm (oracle/host_dcx.cpp, every
+  host core) on a prefix of the same text, median of 5 runs after 1 warm-up.
 
-    python bench.py [--gpus 1] [--steps 3] [--warmup 1] [--workload genome_1gib_dc39]
+    python bench.py [--gpus 1] [--steps 3] [--warmup 1] [--workload zipf_4gib_dc39]
+    python bench.py --workload batch_acgtn | serve_poisson
+    python bench.py --phase-marks ...          (profiling: phase-boundary marker kernels, a 1 GiB copy first)
+    python bench.py --phase-table READ_DIR WRITE_DIR TRACE_DIR     (per-phase table from rocprofv3 csv output)
 """
 import argparse
 import json
 import os
+import statistics
 import sys
 import time
 
@@ -27,6 +38,13 @@ sys.path.insert(0, ROOT)
 
 METRIC = "MB text/s to bit-exact SA vs host DCX; batched SAs/s + p99 latency vs plain"
 HBM_PEAK_GBS = 8000.0
+TRAFFIC_FILE = os.path.join("profiles", "onesweep_traffic.json")
+
+# level-0 phases between the engine's marker kernels (k_phase_mark<P>), named after the reference's timers
+# (lightweight_dcx/src/pdcx/*.hpp: phase_01_02_sort_samples, phase_02_ranks, phase_03_recursion,
+# phase_04_merge); the single global sort of this engine orders the samples and the merge tuples at once
+PHASES = ["compress_alphabet", "phase_01_02_sort_samples+phase_04_sort_merge_samples", "phase_02_ranks",
+          "phase_03_recursion", "phase_04_merge (DC comparator on tied groups)", "output (SA width)"]
 
 
 def log(msg):
@@ -35,7 +53,7 @@ def log(msg):
 
 def load_traffic():
     """HBM bytes per onesweep launch from the committed rocprofv3 --pmc summary, if present."""
-    path = os.path.join(ROOT, "profiles", "onesweep_traffic.json")
+    path = os.path.join(ROOT, TRAFFIC_FILE)
     if not os.path.exists(path):
         return None
     try:
@@ -45,15 +63,62 @@ def load_traffic():
         return None
 
 
-def cpu_baseline(text, sample_bytes, X):
+def cpu_model():
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def cpu_baseline(text, sample_bytes, X, reps=5):
+    """The host DCX (oracle/host_dcx.cpp) with one thread per host core on a prefix of the text: median of
+    `reps` runs after one warm-up."""
     import oracle
     s = text[:sample_bytes].copy()
+    T = len(os.sched_getaffinity(0))
+    oracle.host_dcx_sa(s, X=X, threads=T)
+    times = []
+    for _ in range(reps):
+        t0 = time.perf_counter()
+        oracle.host_dcx_sa(s, X=X, threads=T)
+        times.append(time.perf_counter() - t0)
+    med = statistics.median(times)
+    return {"value": round(s.size / 1e6 / med, 3), "unit": "MB/s", "cores": T, "kind": "port",
+            "sample": f"first {sample_bytes >> 20} MiB of the same text, multi-threaded C++ restatement of PDCX "
+                      f"(oracle/host_dcx.cpp: packed sample records, sample sort over {T} threads, names, "
+                      f"recursion with discarding, merge records sorted by the DC comparator), DC{X} packed, "
+                      f"median of {reps} runs after 1 warm-up",
+            "runs_s": [round(t, 3) for t in times], "cpu_model": cpu_model(), "nproc": os.cpu_count()}
+
+
+def build_steps(eng, dev_text, n, dev_sa, width, X, steps, warmup, profile):
+    import torch
+    kw = dict(X=X, pack_samples=True, pack_merging=True)
+    for i in range(warmup):
+        st = eng.build_sa_device(dev_text.data_ptr(), n, dev_sa.data_ptr(), width, **kw)
+        log(f"[bench] dc{X} n={n} warmup {i} device_ms={st['ms_device']:.2f}")
+    torch.cuda.synchronize()
+    stats = []
     t0 = time.perf_counter()
-    oracle.dcx_sa(s, X=X, pack_samples=True, pack_merging=True)
-    dt = time.perf_counter() - t0
-    return {"value": round(s.size / 1e6 / dt, 4), "unit": "MB/s", "cores": 1, "kind": "port",
-            "sample": f"first {sample_bytes >> 20} MiB of the same text, scalar C restatement of PDCX "
-                      f"(oracle/dcx_oracle.c), DC{X} packed, {dt:.1f} s, host nproc={os.cpu_count()}"}
+    for _ in range(steps):
+        stats.append(eng.build_sa_device(dev_text.data_ptr(), n, dev_sa.data_ptr(), width, profile_kernels=profile, **kw))
+    torch.cuda.synchronize()
+    ms = (time.perf_counter() - t0) * 1000.0 / steps
+    return ms, stats
+
+
+def config_summary(name, n, X, width, ms, stats, valid):
+    last = stats[-1]
+    return {"workload": name, "n": n, "dcx": f"dc{X}", "sa_width_bytes": width, "ms_per_step": round(ms, 3),
+            "mb_s": round(n / 1e6 / (ms / 1000.0), 2), "device_ms": round(last["ms_device"], 3),
+            "phases_ms": {k: round(v, 3) for k, v in last["ms_phase"].items()}, "sa_valid": valid,
+            "alg_bytes": last["alg_bytes"],
+            "pipeline_alg_frac": round(last["alg_bytes"] / (ms / 1000.0) / 1e9 / HBM_PEAK_GBS, 4),
+            "peak_device_bytes": last["peak_device_bytes"],
+            "levels": [{k: l[k] for k in ("n", "samples", "names", "reduced", "discarding")} for l in last["levels"]]}
 
 
 def bench_batch(args):
@@ -117,7 +182,8 @@ def bench_batch(args):
 
 def bench_serve(args):
     """Serving: Poisson arrivals, sizes log-uniform 64 KiB - 256 MiB, alphabet 4 or 256 (50/50), 80 GB
-    admission budget; continuous batching (2 workers) against the plain one-at-a-time path."""
+    admission budget, 64-bit SAs; continuous batching (2 workers, a batch window) against the plain
+    one-at-a-time path."""
     import numpy as np
     import torch
     from distributed_suffix_sorting_evaluation_amd import textgen
@@ -135,31 +201,32 @@ def bench_serve(args):
     arrivals = poisson_arrivals(count, rate, 5)
     classes = np.where(sig256, 3, 0)  # the client knows its alphabet: 256 letters or ACGT
     log(f"[bench] serve: {count} requests at {rate}/s, {int(sizes.sum()) / 2 ** 30:.2f} GiB offered")
-    # an untimed pass of the whole request set first (all at once): process-wide first-use costs (allocator
-    # growth, code loading) are paid here and not by whichever timed run comes first; with only the first
-    # 100 requests, the first timed run still met one 40 ms build of a size class not seen before
-    srv = SAServer(hbm_budget=80 * 10 ** 9, X=39, warm_chars=256 << 20, reserve_bytes=64 << 30)
+    common = dict(hbm_budget=80 * 10 ** 9, X=39, warm_chars=256 << 20, reserve_bytes=96 << 30, sa_width=8)
+    # an untimed pass of the whole request set first: process-wide first-use costs (allocator growth, code
+    # loading) are paid here and not by whichever timed run comes first
+    srv = SAServer(**common)
     try:
         run_open_loop(srv, texts, np.zeros(count), keep_results=False, sigma_classes=classes)
     finally:
         srv.close()
     out = {}
-    runs = (("continuous_batching", dict(workers=2, batching=True), arrivals),
+    runs = (("continuous_batching", dict(workers=2, batching=True, batch_window_ms=args.batch_window_ms), arrivals),
             ("plain", dict(workers=1, batching=False), arrivals),
             ("continuous_batching_burst", dict(workers=2, batching=True), np.zeros(count)),
             ("plain_burst", dict(workers=1, batching=False), np.zeros(count)))
     for name, kwargs, arr in runs:
-        # every server first builds throw-away SAs of every size class and reserves one block in torch's
-        # allocator (outside the timed run), so no request waits for hipMalloc / hipFree
-        srv = SAServer(hbm_budget=80 * 10 ** 9, X=39, warm_chars=256 << 20, reserve_bytes=64 << 30, **kwargs)
+        srv = SAServer(**common, **kwargs)
         try:
             res = run_open_loop(srv, texts, arr, keep_results=True, sigma_classes=classes)
             reqs = res.pop("requests_list")
             from distributed_suffix_sorting_evaluation_amd import Engine
             with Engine(0) as chk:
-                picks = [r for r in reqs if r.error == 0][:: max(1, count // 6)]
-                res["sa_valid_sample"] = all(chk.check_sa_device(r.text.data_ptr(), r.n, r.sa.data_ptr(), 4)
+                picks = [r for r in reqs if r.error == 0][:: max(1, count // 8)]
+                res["sa_valid_sample"] = all(chk.check_sa_device(r.text.data_ptr(), r.n, r.sa.data_ptr(), 8)
                                              for r in picks)
+            res["batched_requests"] = sum(1 for r in reqs if not r.large and r.error == 0 and
+                                          sum(1 for q in reqs if q.batch == r.batch) > 1)
+            res["large_requests"] = sum(1 for r in reqs if r.large)
             for r in reqs:
                 r.sa = None
         finally:
@@ -173,7 +240,7 @@ def bench_serve(args):
             "vs_baseline": None, "dtype": "u8", "data": "synthetic",
             "config": {"workload": "serve_poisson", "requests": count, "rate_per_s": rate,
                        "sizes": "log-uniform 64 KiB-256 MiB", "alphabet": "4 or 256 (50/50)", "hbm_budget_bytes": 80 * 10 ** 9,
-                       "dcx": "dc39", "sa_width_bytes": 4},
+                       "dcx": "dc39", "sa_width_bytes": 8, "batch_window_ms": args.batch_window_ms},
             "p50_ms": cb["p50_ms"], "p99_ms": cb["p99_ms"], "continuous_batching": cb, "plain": pl,
             "burst": {"continuous_batching": out["continuous_batching_burst"], "plain": out["plain_burst"]},
             "texts_per_s_over_plain": round(cb["texts_per_s"] / max(pl["texts_per_s"], 1e-9), 2),
@@ -182,24 +249,154 @@ def bench_serve(args):
             "p99_over_plain": round(cb["p99_ms"] / max(pl["p99_ms"], 1e-9), 3)}
 
 
+def _rocprof_rows(d, suffix):
+    import csv
+    import glob
+    files = sorted(glob.glob(os.path.join(d, "**", f"*{suffix}"), recursive=True))
+    if not files:
+        raise SystemExit(f"no *{suffix} under {d}")
+    with open(files[0]) as f:
+        return list(csv.DictReader(f))
+
+
+READ_COUNTERS = "TCC_EA0_RDREQ_sum TCC_EA0_RDREQ_32B_sum TCC_EA0_RDREQ_128B_sum TCC_BUBBLE_sum"
+WRITE_COUNTERS = "TCC_EA0_WRREQ_sum TCC_EA0_WRREQ_64B_sum"
+
+
+def _dispatch_counters(d):
+    """dispatch id -> (kernel name, {counter: value}) from a rocprofv3 --pmc csv run"""
+    out = {}
+    for r in _rocprof_rows(d, "counter_collection.csv"):
+        did = int(r["Dispatch_Id"])
+        name, vals = out.setdefault(did, (r["Kernel_Name"], {}))
+        vals[r["Counter_Name"]] = vals.get(r["Counter_Name"], 0.0) + float(r["Counter_Value"])
+    return out
+
+
+def _read_bytes(c):
+    """HBM bytes read by a dispatch from the L2's memory requests: 32-byte and 128-byte requests counted
+    apart, the others 64 bytes"""
+    r, r32, r128 = c["TCC_EA0_RDREQ_sum"], c["TCC_EA0_RDREQ_32B_sum"], c["TCC_EA0_RDREQ_128B_sum"]
+    return 32.0 * r32 + 128.0 * r128 + 64.0 * (r - r32 - r128)
+
+
+def _write_bytes(c):
+    w, w64 = c["TCC_EA0_WRREQ_sum"], c["TCC_EA0_WRREQ_64B_sum"]
+    return 32.0 * (w - w64) + 64.0 * w64
+
+
+def phase_table(read_dir, write_dir, trace_dir):
+    """Per-phase HBM traffic and kernel time of one build from three rocprofv3 runs of `bench.py --phase-marks`
+    (csv output): --pmc READ_COUNTERS, --pmc WRITE_COUNTERS and --kernel-trace.  A dispatch's bytes come
+    from the L2's memory request counts by request size (the derived FETCH_SIZE of this ROCm prices the
+    128-byte reads of gfx950 at 64 bytes); the 1 GiB copy that --phase-marks runs first, which reads and
+    writes exactly 2^30 bytes, checks them.  Dispatches are assigned to phases by the marker kernels.
+    Prints a markdown table and a JSON line."""
+    rd = _dispatch_counters(read_dir)
+    wr = _dispatch_counters(write_dir)
+    trace = _rocprof_rows(trace_dir, "kernel_trace.csv")
+    tnames = {int(r["Dispatch_Id"]): r["Kernel_Name"] for r in trace}
+    tdur = {int(r["Dispatch_Id"]): (int(r["End_Timestamp"]) - int(r["Start_Timestamp"])) / 1e6 for r in trace}
+
+    def phases_of(names):
+        """dispatch id -> phase index from the marker kernels in dispatch order: after mark P, phase P (0: from
+        the engine's first kernel to the first mark; the calibration copy before it is left out)"""
+        ph, cur, started = {}, 0, False
+        for did in sorted(names):
+            name = names[did]
+            if "k_phase_mark<" in name:
+                cur = int(name.split("k_phase_mark<")[1].split(">")[0])
+                continue
+            started = started or "dcx::" in name
+            if started:
+                ph[did] = cur
+        return ph
+
+    def copy_check(items, fn):
+        # the largest dispatch before the engine's first kernel: the 1 GiB fill (writes) and copy (reads and writes)
+        best = (None, None, -1.0)
+        for did in sorted(items):
+            name, c = items[did]
+            if "dcx::" in name:
+                break
+            v = fn(c)
+            if v > best[2]:
+                best = (v / float(1 << 30), name.split("(")[0], v)
+        return best[0], best[1]
+
+    rcheck, rname = copy_check(rd, _read_bytes)
+    wcheck, wname = copy_check(wr, _write_bytes)
+    pr = phases_of({k: v[0] for k, v in rd.items()})
+    pw = phases_of({k: v[0] for k, v in wr.items()})
+    pt = phases_of(tnames)
+    agg = {p: {"read": 0.0, "write": 0.0, "ms": 0.0, "launches": 0, "kernels": {}} for p in range(len(PHASES))}
+    for did, p in pr.items():
+        if p in agg:
+            agg[p]["read"] += _read_bytes(rd[did][1])
+    for did, p in pw.items():
+        if p in agg:
+            agg[p]["write"] += _write_bytes(wr[did][1])
+    for did, p in pt.items():
+        if p in agg:
+            agg[p]["ms"] += tdur[did]
+            agg[p]["launches"] += 1
+            k = tnames[did].split("(")[0]
+            agg[p]["kernels"][k] = agg[p]["kernels"].get(k, 0.0) + tdur[did]
+    print(f"check on the 1 GiB calibration dispatches: read by `{rname}` / 2^30 = {rcheck:.6f}, "
+          f"written by `{wname}` / 2^30 = {wcheck:.6f}")
+    print()
+    print("| phase (reference timer) | launches | kernel ms | read GB | written GB | measured GB/s | of 8 TB/s | top kernels (ms) |")
+    print("|---|---:|---:|---:|---:|---:|---:|---|")
+    tot = {"read": 0.0, "write": 0.0, "ms": 0.0, "launches": 0}
+    res = {}
+    for p in range(len(PHASES)):
+        a = agg[p]
+        gbs = (a["read"] + a["write"]) / (a["ms"] / 1e3) / 1e9 if a["ms"] else 0.0
+        top = sorted(a["kernels"].items(), key=lambda kv: -kv[1])[:3]
+        tops = "; ".join(f"`{k.replace('dcx::', '')[:60]}` {v:.2f}" for k, v in top)
+        print(f"| {PHASES[p]} | {a['launches']} | {a['ms']:.3f} | {a['read'] / 1e9:.3f} | {a['write'] / 1e9:.3f} | "
+              f"{gbs:.0f} | {gbs / HBM_PEAK_GBS:.3f} | {tops} |")
+        for k in tot:
+            tot[k] += a[k]
+        res[PHASES[p]] = {"launches": a["launches"], "kernel_ms": round(a["ms"], 3),
+                              "read_bytes": int(a["read"]), "write_bytes": int(a["write"]),
+                              "measured_gb_s": round(gbs, 1)}
+    gbs = (tot["read"] + tot["write"]) / (tot["ms"] / 1e3) / 1e9 if tot["ms"] else 0.0
+    print(f"| all phases | {tot['launches']} | {tot['ms']:.3f} | {tot['read'] / 1e9:.3f} | {tot['write'] / 1e9:.3f} | "
+          f"{gbs:.0f} | {gbs / HBM_PEAK_GBS:.3f} | |")
+    print()
+    print(json.dumps({"phases": res, "copy_read_check": rcheck, "copy_write_check": wcheck}))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--workload", default="genome_1gib_dc39")
+    ap.add_argument("--workload", default="zipf_4gib_dc39")
     ap.add_argument("--cpu-sample-mib", type=int, default=16)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-validate", action="store_true")
-    ap.add_argument("--host-to-host", action="store_true", help="also time one PCIe-inclusive run")
+    ap.add_argument("--no-host-to-host", action="store_true")
+    ap.add_argument("--no-other-configs", action="store_true")
     ap.add_argument("--batch-count", type=int, default=10000)
     ap.add_argument("--plain-sample", type=int, default=500)
-    ap.add_argument("--serve-count", type=int, default=400)
+    ap.add_argument("--serve-count", type=int, default=2000)
     ap.add_argument("--serve-rate", type=float, default=200.0)
+    ap.add_argument("--batch-window-ms", type=float, default=2.0)
+    ap.add_argument("--phase-marks", action="store_true",
+                    help="profiling: the engine launches marker kernels at its level-0 phase boundaries, and a "
+                         "1 GiB device copy runs first (the reference dispatch for the counter scale)")
+    ap.add_argument("--phase-table", nargs=3, metavar=("READ_DIR", "WRITE_DIR", "TRACE_DIR"),
+                    help="print the per-phase table of rocprofv3 csv output of --phase-marks runs")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a 1 GiB device-to-device copy three times (reference kernel for the "
                          "FETCH_SIZE / WRITE_SIZE counter scale)")
     args = ap.parse_args()
+
+    if args.phase_table:
+        phase_table(*args.phase_table)
+        return
 
     if args.copy_calibration:
         import torch
@@ -217,6 +414,9 @@ def main():
                                    "(each GPU box here has one MI355X); run with --gpus 1"}))
         sys.exit(2)
 
+    if args.phase_marks:
+        os.environ["DCX_PHASE_MARKS"] = "1"
+
     import numpy as np
     import torch
     from distributed_suffix_sorting_evaluation_amd import Engine, textgen
@@ -232,6 +432,13 @@ def main():
         print(json.dumps(bench_serve(args)), flush=True)
         return
 
+    if args.phase_marks:
+        a = torch.ones(1 << 30, dtype=torch.uint8, device="cuda")
+        b = torch.empty_like(a)
+        b.copy_(a)
+        torch.cuda.synchronize()
+        del a, b
+
     wl = textgen.WORKLOADS[args.workload]
     n, X, width = wl["n"], wl["X"], wl["sa_width"]
     log(f"[bench] generating {args.workload} n={n}")
@@ -239,26 +446,12 @@ def main():
     dev_text = torch.from_numpy(text).to("cuda")
     dev_sa = torch.empty(n * width, dtype=torch.uint8, device="cuda")
     torch.cuda.synchronize()
-    kw = dict(X=X, pack_samples=True, pack_merging=True)
 
     eng = Engine(0)
-    for i in range(args.warmup):
-        st = eng.build_sa_device(dev_text.data_ptr(), n, dev_sa.data_ptr(), width, **kw)
-        log(f"[bench] warmup {i} device_ms={st['ms_device']:.2f}")
-    torch.cuda.synchronize()
-
-    steps = []
-    t0 = time.perf_counter()
-    for i in range(args.steps):
-        steps.append(eng.build_sa_device(dev_text.data_ptr(), n, dev_sa.data_ptr(), width, profile_kernels=True,
-                                         **kw))
-    torch.cuda.synchronize()
-    t1 = time.perf_counter()
-    ms_per_step = (t1 - t0) * 1000.0 / args.steps
+    ms_per_step, steps = build_steps(eng, dev_text, n, dev_sa, width, X, args.steps, args.warmup, True)
     value = n / 1e6 / (ms_per_step / 1000.0)
     for i, st in enumerate(steps):
         log(f"[bench] step {i} device_ms={st['ms_device']:.2f} phases={st['ms_phase']}")
-
     valid = None
     if not args.no_validate:
         valid = eng.check_sa_device(dev_text.data_ptr(), n, dev_sa.data_ptr(), width)
@@ -276,12 +469,12 @@ def main():
     roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": round(achieved / HBM_PEAK_GBS, 4),
                 "traffic": traffic.get("bytes_per_launch") if traffic else None,
+                "traffic_source": (f"{TRAFFIC_FILE}: rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE per launch on "
+                                   f"{traffic.get('workload')} (committed, not measured in this run)") if traffic else None,
                 "kernel": "k_onesweep (LSD radix pass over 64-bit keys: u32 values, or split indices with byte values)",
                 "launches": passes, "avg_launch_ms": round(avg_ms, 4), "avg_pairs_per_launch": int(avg_items),
                 "alg_bytes_per_launch": int(avg_bytes)}
-
-    last = steps[-1]
-    alg_bytes = last["alg_bytes"]
+    summary = config_summary(args.workload, n, X, width, ms_per_step, steps, valid)
     result = {
         "metric": METRIC,
         "value": round(value, 2),
@@ -299,25 +492,69 @@ def main():
                    "packing": True, "sa_width_bytes": width, "input": "text resident in HBM, SA left in HBM"},
         "roofline": roofline,
         "sa_valid": valid,
-        "device_ms": round(last["ms_device"], 3),
-        "phases_ms": {k: round(v, 3) for k, v in last["ms_phase"].items()},
-        "levels": [{k: l[k] for k in ("n", "samples", "names", "reduced", "discarding")} for l in last["levels"]],
-        "alg_bytes": alg_bytes,
-        "pipeline_alg_frac": round(alg_bytes / (ms_per_step / 1000.0) / 1e9 / HBM_PEAK_GBS, 4),
-        "peak_device_bytes": last["peak_device_bytes"],
+        "device_ms": summary["device_ms"],
+        "phases_ms": summary["phases_ms"],
+        "levels": summary["levels"],
+        "alg_bytes": summary["alg_bytes"],
+        "pipeline_alg_frac": summary["pipeline_alg_frac"],
+        "peak_device_bytes": summary["peak_device_bytes"],
     }
 
-    if args.host_to_host:
+    if not args.no_host_to_host:
+        # text from pageable host memory, SA back into pageable host memory; compared with the device SA
+        torch.cuda.synchronize()
         t2 = time.perf_counter()
-        sa_host = eng.build_sa(text, sa_width=width, **kw)
+        sa_host = eng.build_sa(text, sa_width=width, X=X)
         t3 = time.perf_counter()
+        host_bytes = sa_host.reshape(-1).view(np.uint8)
+        chunk = 1 << 30
+        same = host_bytes.size == dev_sa.numel() and all(
+            torch.equal(torch.from_numpy(host_bytes[i:i + chunk]).cuda(), dev_sa[i:i + chunk])
+            for i in range(0, host_bytes.size, chunk))
         result["host_to_host_mb_s"] = round(n / 1e6 / (t3 - t2), 2)
+        result["host_to_host_sa_equals_device_sa"] = same
+        log(f"[bench] host to host {result['host_to_host_mb_s']} MB/s, same SA: {same}")
         del sa_host
+        torch.cuda.empty_cache()
 
+    gpu_sample = None
+    if not args.no_cpu_baseline:
+        # the GPU on the same prefix the CPU baseline builds
+        sb = args.cpu_sample_mib << 20
+        out = torch.empty(sb * width, dtype=torch.uint8, device="cuda")
+        ms_s, _ = build_steps(eng, dev_text, sb, out, width, X, 3, 1, False)
+        gpu_sample = round(sb / 1e6 / (ms_s / 1000.0), 2)
+        del out
+
+    others = {}
+    if not args.no_other_configs:
+        # the other difference covers on the same text, then the genome and DNA configurations
+        for Xo in (21, 133):
+            ms, sts = build_steps(eng, dev_text, n, dev_sa, width, Xo, 2, 1, False)
+            ok = None if args.no_validate else eng.check_sa_device(dev_text.data_ptr(), n, dev_sa.data_ptr(), width)
+            others[f"{args.workload.rsplit('_dc', 1)[0]}_dc{Xo}"] = config_summary(args.workload, n, Xo, width, ms, sts, ok)
+            log(f"[bench] dc{Xo}: {ms:.1f} ms")
+        del dev_sa
+        torch.cuda.empty_cache()
+        for name in ("genome_1gib_dc39", "dna_8mib_dc39"):
+            w2 = textgen.WORKLOADS[name]
+            t2 = torch.from_numpy(textgen.generate(w2["kind"], w2["n"], w2["seed"])).cuda()
+            s2 = torch.empty(w2["n"] * w2["sa_width"], dtype=torch.uint8, device="cuda")
+            ms, sts = build_steps(eng, t2, w2["n"], s2, w2["sa_width"], w2["X"], 3, 1, False)
+            ok = None if args.no_validate else eng.check_sa_device(t2.data_ptr(), w2["n"], s2.data_ptr(), w2["sa_width"])
+            others[name] = config_summary(name, w2["n"], w2["X"], w2["sa_width"], ms, sts, ok)
+            log(f"[bench] {name}: {ms:.2f} ms")
+            del t2, s2
+        result["other_configs"] = others
     eng.close()
+
     if not args.no_cpu_baseline:
         log(f"[bench] cpu baseline on {args.cpu_sample_mib} MiB")
-        result["cpu_baseline"] = cpu_baseline(text, args.cpu_sample_mib << 20, X)
+        cb = cpu_baseline(text, args.cpu_sample_mib << 20, X)
+        cb["gpu_same_sample_mb_s"] = gpu_sample
+        result["cpu_baseline"] = cb
+        result["gpu_over_cpu_baseline"] = round(value / cb["value"], 1)
+        result["gpu_over_cpu_same_sample"] = round(gpu_sample / cb["value"], 1) if gpu_sample else None
     else:
         result["cpu_baseline"] = None
     print(json.dumps(result), flush=True)

@@ -96,6 +96,21 @@ class Engine {
     uint64_t budget = 0;
     std::string last_error;
     void* scratch = nullptr;  // this device's dcx_build_sa_noalloc array (scratch_address)
+    // DCX_PHASE_MARKS=1 at handle creation: level 0 launches k_phase_mark<P> at its phase boundaries
+    // (1 after the alphabet, 2 after the suffix sort, 3 after the names, 4 after the recursion, 5 after the
+    // merge, 6 after the output), so that per-dispatch profiles can be summed per phase
+    bool phase_marks = false;
+    void mark(int P) {
+        if (!phase_marks) return;
+        switch (P) {
+            case 1: k_phase_mark<1><<<1, 64, 0, stream>>>(); break;
+            case 2: k_phase_mark<2><<<1, 64, 0, stream>>>(); break;
+            case 3: k_phase_mark<3><<<1, 64, 0, stream>>>(); break;
+            case 4: k_phase_mark<4><<<1, 64, 0, stream>>>(); break;
+            case 5: k_phase_mark<5><<<1, 64, 0, stream>>>(); break;
+            default: k_phase_mark<6><<<1, 64, 0, stream>>>(); break;
+        }
+    }
     hipEvent_t evt[6] = {};   // phase timing events of a call, created with the handle
 
     // per call
@@ -276,6 +291,7 @@ class Engine {
     }
 
     explicit Engine(int dev) : device(dev) {
+        if (const char* e = getenv("DCX_PHASE_MARKS")) phase_marks = e[0] == '1';
         if (const char* e = getenv("DCX_STAGE_BYTES")) {
             const unsigned long long v = strtoull(e, nullptr, 10);
             if (v >= 4096 && v <= (1ull << 30)) stage_bytes = (size_t)(v & ~4095ull);
@@ -902,6 +918,7 @@ class Engine {
         DBuf<uint32_t> order;
         DBuf<uint8_t> H;
         ActiveSet tied;
+        if (depth == 0) mark(1);
         {
             DBuf<uint64_t> PT;
             make_stream(text, lut, n, mbits, dense ? 1 : 0, n + Ls + 2, PT);
@@ -909,6 +926,7 @@ class Engine {
             if (dense && tied.A > 0) boundary_fixup(PT.get(), mbits, Ls, n, dc.bnd, order, H, tied);
         }
 
+        if (depth == 0) mark(2);
         DBuf<uint32_t> ranks(cache, (size_t)m + D);
         prim.fill_u32(ranks.get() + m, 0, D);
         {
@@ -928,6 +946,7 @@ class Engine {
             const uint32_t M = (uint32_t)prim.totals(0).lo;
             if (ls) ls->names = M;
             if (ev && depth == 0) DCX_HIP(hipEventRecord(ev[2], stream));
+            if (depth == 0) mark(3);
 
             // ---------------- phase 3: recursion ----------------
             if (M >= m) {
@@ -939,11 +958,13 @@ class Engine {
             }
         }
         if (ev && depth == 0) DCX_HIP(hipEventRecord(ev[3], stream));
+        if (depth == 0) mark(4);
 
         // ---------------- phase 4: merge ----------------
         // suffixes with different L_s-character keys are already in order; groups of equal keys are ordered
         // by the DC comparator on the sample ranks
         tie_break(dc, order, H, tied, ranks.get());
+        if (depth == 0) mark(5);
         if (sa_take)
             *sa_take = std::move(order);
         else
@@ -1185,6 +1206,7 @@ class Engine {
             widen(sa.get(), n, width, wide.get());
             download(sa_out, wide.get(), n * width);
         }
+        mark(6);
         DCX_HIP(hipEventRecord(ev[5], stream));
         DCX_HIP(hipStreamSynchronize(stream));
         if (st) {

@@ -1186,6 +1186,11 @@ __global__ __launch_bounds__(1024) void k_base_case(TV tv, uint32_t n, uint32_t*
     for (uint32_t j = threadIdx.x; j < n; j += 1024) sa[j] = (uint32_t)(s[j] & 0x3FFF);
 }
 
+// Empty kernel launched at the phase boundaries of level 0 when phase marks are on (profiling): in a
+// per-dispatch counter list, the dispatches between marks P - 1 and P belong to phase P.
+template <int P>
+__global__ void k_phase_mark() {}
+
 // ------------------------------------------------------------------------------------------------
 // output width conversion: uint32 SA -> 4 / 5 (uint40, uint_types.hpp:262) / 8 bytes little endian
 // ------------------------------------------------------------------------------------------------

@@ -35,6 +35,7 @@ DIGEST_WORKLOADS = {
     "genome_64mib": {"kind": "genome", "n": 64 << 20, "seed": 2},
     "genome_1gib": {"kind": "genome", "n": 1 << 30, "seed": 2},
     "zipf_16mib": {"kind": "zipf256", "n": 16 << 20, "seed": 3},
+    "zipf_4gib": {"kind": "zipf256", "n": 1 << 32, "seed": 3},
 }
 
 

@@ -5,6 +5,8 @@ this package.  It binds
 
 * ``liboracle.so`` -- ``dcx_oracle.c``, the C restatement of lightweight_dcx's PDCX path
   (pdcx.hpp, sample_string.hpp, compute_ranks.hpp, merge_samples.hpp, cli.cpp:417-539);
+* ``libhostdcx.so`` -- ``host_dcx.cpp``, the multi-threaded C++ restatement of the same path (the
+  host-DCX baseline bench.py times; checked here against libsais like the scalar restatement);
 * ``_ref/libsais.so`` -- libsais 2.10.0 built by ``oracle/Makefile`` from the reference's own
   sources (libsais/src/libsais.c, libsais64.c), used as an independent SACA.
 """
@@ -19,6 +21,7 @@ import numpy as np
 HERE = os.path.dirname(os.path.abspath(__file__))
 _LIB = None
 _SAIS = None
+_HOST = None
 
 MAX_LEVELS = 64
 
@@ -202,3 +205,33 @@ def libsais_sa(text, threads=1):
         if rc != 0:
             raise RuntimeError(f"libsais64 rc={rc}")
     return sa[:n].astype(np.uint64)
+
+
+# ---------------------------------------------------------------------------------------------
+# host DCX: multi-threaded restatement (host_dcx.cpp)
+# ---------------------------------------------------------------------------------------------
+def host_lib():
+    global _HOST
+    if _HOST is None:
+        path = os.path.join(HERE, "libhostdcx.so")
+        if not os.path.exists(path):
+            build()
+        L = ctypes.CDLL(path)
+        L.host_dcx_sa.argtypes = [ctypes.c_void_p, ctypes.c_uint64, ctypes.c_uint32, ctypes.c_int, ctypes.c_int,
+                                  ctypes.c_double, ctypes.c_uint64, ctypes.c_uint32, ctypes.c_void_p]
+        L.host_dcx_sa.restype = ctypes.c_int
+        _HOST = L
+    return _HOST
+
+
+def host_dcx_sa(text, X=39, threads=1, pack_samples=True, pack_merging=True, discarding_threshold=0.7,
+                base_case_chars=10000):
+    """SA (uint64) by the multi-threaded host restatement of PDCX with `threads` threads."""
+    t = np.ascontiguousarray(np.frombuffer(bytes(text), dtype=np.uint8) if not isinstance(text, np.ndarray) else text,
+                             dtype=np.uint8)
+    sa = np.zeros(max(t.size, 1), dtype=np.uint64)
+    rc = host_lib().host_dcx_sa(t.ctypes.data, t.size, X, int(pack_samples), int(pack_merging),
+                                float(discarding_threshold), int(base_case_chars), int(threads), sa.ctypes.data)
+    if rc != 0:
+        raise RuntimeError(f"host_dcx_sa rc={rc}")
+    return sa[:t.size]

@@ -14,8 +14,10 @@ Fixtures written (all data -- inputs and expected outputs):
   dPD/tests/test_data/.
 * ``dc_tables.json``: per X, the DC set, IN_DC, NEXT_RANK and the SHA-256 of cmpDepthRanks (u32 LE,
   row-major X*X*3) parsed from lightweight_dcx/src/pdcx/difference_cover.hpp.
-* with ``--digests``: ``sa_digests.json``: SHA-256 of the uint64 LE SA computed by libsais (oracle/_ref,
-  built from the reference's libsais sources) for the synthetic workloads made by the project generator.
+* with ``--digests [name,...]``: ``sa_digests.json``: SHA-256 of the SA computed by libsais (oracle/_ref, built
+  from the reference's libsais sources) for the synthetic workloads made by the project generator, as uint64
+  LE and as the uint40 LE stream of the reference's output files (lightweight_dcx/src/mpi/io.hpp:80-98).
+  Named workloads are (re)computed and merged into the existing file; without names, all of them.
 """
 from __future__ import annotations
 
@@ -68,26 +70,48 @@ def parse_dc_tables():
     return out
 
 
-def sa_digests():
-    sys.path.insert(0, ROOT)
+def _libsais_digests(text, threads):
+    """SHA-256 of libsais64's SA of `text` as uint64 LE and as uint40 LE, hashed in chunks so that a 4 GiB
+    text needs its SA once (32 GiB) and no converted copy of it."""
     import oracle  # noqa: E402
+    n = int(text.size)
+    sa = np.empty(max(n, 1), dtype=np.int64)
+    rc = oracle.sais().libsais64_omp(text.ctypes.data, sa.ctypes.data, n, 0, None, threads)
+    if rc != 0:
+        raise RuntimeError(f"libsais64_omp rc={rc}")
+    h8, h5 = hashlib.sha256(), hashlib.sha256()
+    chunk = 1 << 26
+    for a in range(0, n, chunk):
+        c = sa[a:a + chunk].view("<u8")
+        h8.update(c.tobytes())
+        h5.update(np.ascontiguousarray(c.view(np.uint8).reshape(-1, 8)[:, :5]).tobytes())
+    del sa
+    return h8.hexdigest(), h5.hexdigest()
+
+
+def sa_digests(names=None):
+    sys.path.insert(0, ROOT)
     from distributed_suffix_sorting_evaluation_amd import textgen  # noqa: E402
 
-    res = {}
+    path = os.path.join(HERE, "sa_digests.json")
+    res = json.load(open(path)) if os.path.exists(path) else {}
     for name, kw in textgen.DIGEST_WORKLOADS.items():
+        if names and name not in names:
+            continue
         text = textgen.generate(**kw)
-        sa = oracle.libsais_sa(text, threads=os.cpu_count() or 1)
+        d8, d5 = _libsais_digests(text, os.cpu_count() or 1)
         res[name] = {"params": kw, "n": int(text.size),
                      "text_sha256": hashlib.sha256(text.tobytes()).hexdigest(),
-                     "sa_u64_sha256": hashlib.sha256(sa.astype("<u8").tobytes()).hexdigest()}
+                     "sa_u64_sha256": d8, "sa_u40_sha256": d5}
         print(name, res[name], flush=True)
-        del sa, text
-    with open(os.path.join(HERE, "sa_digests.json"), "w") as f:
-        json.dump(res, f, indent=1, sort_keys=True)
+        del text
+        with open(path, "w") as f:
+            json.dump(res, f, indent=1, sort_keys=True)
 
 
 if __name__ == "__main__":
     copy_fixtures()
     parse_dc_tables()
     if "--digests" in sys.argv:
-        sa_digests()
+        i = sys.argv.index("--digests")
+        sa_digests(sys.argv[i + 1].split(",") if i + 1 < len(sys.argv) else None)

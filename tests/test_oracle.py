@@ -301,3 +301,31 @@ def test_host_noalloc_twin():
     rc = L.dcx_host_build_sa_noalloc(h, big.ctypes.data, NOALLOC_MAX_CHARS + 1, ctypes.byref(o), out.ctypes.data, 3,
                                      None)
     assert rc == -9  # the limit before the width check, as in the engine
+
+
+@pytest.mark.parametrize("X", ALL_X)
+def test_host_dcx_matches_libsais(X):
+    """The multi-threaded host restatement (oracle/host_dcx.cpp, the CPU baseline bench.py times) gives
+    libsais's SA for every X, packed and unpacked, at the discarding thresholds 0, 0.7 and 1, on random,
+    periodic, copy-heavy and single-letter texts, with one and with four threads."""
+    if not oracle.sais_available():
+        pytest.skip("libsais not built")
+    texts = [random_text(40000, 4, X), random_text(30000, 256, X + 1), periodic_text(50000, 17, 3, X),
+             copies_text(60000, X, sigma=4), np.full(20000, 97, dtype=np.uint8)]
+    for t in texts:
+        ref = oracle.libsais_sa(t)
+        for threads, ps, thr in ((4, True, 0.7), (1, False, 0.0), (4, True, 1.0)):
+            got = oracle.host_dcx_sa(t, X=X, threads=threads, pack_samples=ps, pack_merging=ps,
+                                     discarding_threshold=thr, base_case_chars=0)
+            assert np.array_equal(got, ref), (X, t.size, threads, ps, thr)
+
+
+def test_host_dcx_golden_and_large():
+    """The host restatement on the reference's fixtures and on 8 MiB of the genome-like generator."""
+    from distributed_suffix_sorting_evaluation_amd import textgen
+    t = read("the_three_brothers.txt")
+    expect = np.frombuffer(read("the_three_brothers_size_t_sa"), dtype="<u8")
+    assert np.array_equal(oracle.host_dcx_sa(t, X=39, threads=4), expect)
+    g = textgen.generate("genome", 8 << 20, 2)
+    if oracle.sais_available():
+        assert np.array_equal(oracle.host_dcx_sa(g, X=39, threads=8), oracle.libsais_sa(g, threads=8))
