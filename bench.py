@@ -201,7 +201,7 @@ def bench_serve(args):
     arrivals = poisson_arrivals(count, rate, 5)
     classes = np.where(sig256, 3, 0)  # the client knows its alphabet: 256 letters or ACGT
     log(f"[bench] serve: {count} requests at {rate}/s, {int(sizes.sum()) / 2 ** 30:.2f} GiB offered")
-    common = dict(hbm_budget=80 * 10 ** 9, X=39, warm_chars=256 << 20, reserve_bytes=96 << 30, sa_width=8)
+    common = dict(hbm_budget=80 * 10 ** 9, X=39, warm_chars=256 << 20, reserve_bytes=40 << 30, sa_width=8)
     # an untimed pass of the whole request set first: process-wide first-use costs (allocator growth, code
     # loading) are paid here and not by whichever timed run comes first
     srv = SAServer(**common)
@@ -214,14 +214,19 @@ def bench_serve(args):
             ("plain", dict(workers=1, batching=False), arrivals),
             ("continuous_batching_burst", dict(workers=2, batching=True), np.zeros(count)),
             ("plain_burst", dict(workers=1, batching=False), np.zeros(count)))
+    if args.serve_runs == "cb":  # profiling: the continuous-batching Poisson run alone
+        runs = runs[:1]
     for name, kwargs, arr in runs:
         srv = SAServer(**common, **kwargs)
         try:
-            res = run_open_loop(srv, texts, arr, keep_results=True, sigma_classes=classes)
+            # results are released as requests complete (2,000 64-bit SAs would not fit HBM together),
+            # except every keep-th, which is validated after the run
+            keep = max(1, count // 16)
+            res = run_open_loop(srv, texts, arr, keep_results=False, sigma_classes=classes, keep_every=keep)
             reqs = res.pop("requests_list")
             from distributed_suffix_sorting_evaluation_amd import Engine
             with Engine(0) as chk:
-                picks = [r for r in reqs if r.error == 0][:: max(1, count // 8)]
+                picks = [r for r in reqs if r.error == 0 and r.keep]
                 res["sa_valid_sample"] = all(chk.check_sa_device(r.text.data_ptr(), r.n, r.sa.data_ptr(), 8)
                                              for r in picks)
             res["batched_requests"] = sum(1 for r in reqs if not r.large and r.error == 0 and
@@ -234,6 +239,9 @@ def bench_serve(args):
         torch.cuda.synchronize()
         out[name] = {k: (round(v, 3) if isinstance(v, float) else v) for k, v in res.items()}
         log(f"[bench] {name}: {out[name]}")
+    if args.serve_runs == "cb":
+        print(json.dumps({"continuous_batching": out["continuous_batching"]}), flush=True)
+        sys.exit(0)
     cb, pl = out["continuous_batching"], out["plain"]
     return {"metric": METRIC, "value": cb["texts_per_s"], "unit": "texts/s", "n_gpus": 1, "steps": 1, "warmup": 0,
             "ms_per_step": round(cb["wall_s"] * 1e3, 1), "higher_is_better": True, "scaling": "weak",
@@ -368,13 +376,46 @@ def phase_table(read_dir, write_dir, trace_dir):
     print(json.dumps({"phases": res, "copy_read_check": rcheck, "copy_write_check": wcheck}))
 
 
+def overlap_table(trace_dir):
+    """Device overlap of the serving workers from a rocprofv3 --kernel-trace csv run of `--workload
+    serve_poisson`: per stream the kernel count and busy time, and the time during which kernels of two or
+    more streams were in flight together (a sweep over the dispatch intervals)."""
+    rows = _rocprof_rows(trace_dir, "kernel_trace.csv")
+    key = "Stream_Id" if rows and "Stream_Id" in rows[0] else "Queue_Id"
+    ev, per = [], {}
+    for r in rows:
+        if "dcx::" not in r["Kernel_Name"]:
+            continue
+        a, b, q = int(r["Start_Timestamp"]), int(r["End_Timestamp"]), r[key]
+        ev += [(a, 1, q), (b, -1, q)]
+        st = per.setdefault(q, [0, 0])
+        st[0] += 1
+        st[1] += b - a
+    ev.sort()
+    active, both, busy, last = {}, 0, 0, None
+    for t, d, q in ev:
+        if last is not None:
+            n_streams = sum(1 for v in active.values() if v > 0)
+            if n_streams >= 2:
+                both += t - last
+            if n_streams >= 1:
+                busy += t - last
+        active[q] = active.get(q, 0) + d
+        last = t
+    print(f"streams ({key}): " + ", ".join(f"{q}: {c} kernels, {ns / 1e6:.1f} ms busy" for q, (c, ns) in sorted(per.items())))
+    print(f"device busy with engine kernels: {busy / 1e6:.1f} ms; kernels of two or more streams in flight together: "
+          f"{both / 1e6:.1f} ms ({both / max(busy, 1):.3f} of the busy time)")
+    print(json.dumps({"streams": {q: {"kernels": c, "busy_ms": round(ns / 1e6, 3)} for q, (c, ns) in per.items()},
+                      "busy_ms": round(busy / 1e6, 3), "overlap_ms": round(both / 1e6, 3)}))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--workload", default="zipf_4gib_dc39")
-    ap.add_argument("--cpu-sample-mib", type=int, default=16)
+    ap.add_argument("--cpu-sample-mib", type=int, default=64)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-validate", action="store_true")
     ap.add_argument("--no-host-to-host", action="store_true")
@@ -384,11 +425,15 @@ def main():
     ap.add_argument("--serve-count", type=int, default=2000)
     ap.add_argument("--serve-rate", type=float, default=200.0)
     ap.add_argument("--batch-window-ms", type=float, default=2.0)
+    ap.add_argument("--serve-runs", choices=("all", "cb"), default="all",
+                    help="serve_poisson: all four runs (default) or only the continuous-batching Poisson run")
     ap.add_argument("--phase-marks", action="store_true",
                     help="profiling: the engine launches marker kernels at its level-0 phase boundaries, and a "
                          "1 GiB device copy runs first (the reference dispatch for the counter scale)")
     ap.add_argument("--phase-table", nargs=3, metavar=("READ_DIR", "WRITE_DIR", "TRACE_DIR"),
                     help="print the per-phase table of rocprofv3 csv output of --phase-marks runs")
+    ap.add_argument("--overlap-table", metavar="TRACE_DIR",
+                    help="print the stream overlap of a rocprofv3 --kernel-trace csv run of serve_poisson")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a 1 GiB device-to-device copy three times (reference kernel for the "
                          "FETCH_SIZE / WRITE_SIZE counter scale)")
@@ -396,6 +441,9 @@ def main():
 
     if args.phase_table:
         phase_table(*args.phase_table)
+        return
+    if args.overlap_table:
+        overlap_table(args.overlap_table)
         return
 
     if args.copy_calibration:
@@ -532,7 +580,8 @@ def main():
         for Xo in (21, 133):
             ms, sts = build_steps(eng, dev_text, n, dev_sa, width, Xo, 2, 1, False)
             ok = None if args.no_validate else eng.check_sa_device(dev_text.data_ptr(), n, dev_sa.data_ptr(), width)
-            others[f"{args.workload.rsplit('_dc', 1)[0]}_dc{Xo}"] = config_summary(args.workload, n, Xo, width, ms, sts, ok)
+            name = f"{args.workload.rsplit('_dc', 1)[0]}_dc{Xo}"
+            others[name] = config_summary(name, n, Xo, width, ms, sts, ok)
             log(f"[bench] dc{Xo}: {ms:.1f} ms")
         del dev_sa
         torch.cuda.empty_cache()

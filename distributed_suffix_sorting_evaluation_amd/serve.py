@@ -6,8 +6,10 @@ as their estimated HBM footprint fits the budget that is still free, joins their
 all their suffix arrays with one ``dcx_build_sa_batch`` call.  Requests that arrive while batches run are
 admitted by the next free worker; with two workers the batches of both overlap on the device.  A large request
 (it fills the GPU by itself) is built alone, and at most one large build runs at a time, so it is as fast as
-on an idle GPU while the other worker serves the small requests.  Results are the per-text 32-bit suffix
-arrays, bit-identical to building each text alone (the batched call is exact).
+on an idle GPU while the other worker serves the small requests.  With a batch window, a small request that
+finds the GPU busy waits up to that long for others to join its batch.  Results are the per-text suffix arrays
+(32- or 64-bit positions), bit-identical to building each text alone (the batched call is exact); a request
+whose result the client does not keep releases it as soon as it completes.
 
 The reference has no serving layer (it builds one SA per ``mpiexec`` run, lightweight_dcx/src/executables/cli.cpp);
 this module serves callers with a stream of requests.
@@ -27,13 +29,13 @@ from .engine import DCX_E_ADMISSION, DCX_E_CANCELLED, DCX_E_HIP, DCXError, Engin
 _STREAMS: dict = {}
 
 # device bytes per text character while a request is in flight: engine working set (see the engine's
-# estimate) plus the 4-byte SA and the joined copy of the text
-BYTES_PER_CHAR = 64 + 4 + 1
+# estimate) plus the joined copy of the text, and the SA (4 or 8 bytes per character)
+BYTES_PER_CHAR = 64 + 1
 FIXED_BYTES = 64 << 20
 
 
-def footprint(n: int) -> int:
-    return FIXED_BYTES + BYTES_PER_CHAR * int(n)
+def footprint(n: int, sa_width: int = 4) -> int:
+    return FIXED_BYTES + (BYTES_PER_CHAR + sa_width) * int(n)
 
 
 @dataclass(eq=False)  # identity semantics: requests are compared and removed by object
@@ -48,6 +50,7 @@ class Request:
     sigma_class: int = -1           # packing width class of the alphabet (3/5/8/9-bit characters)
     batch: int = -1
     large: bool = False             # built alone as a large request
+    keep: bool = True               # False: the result is released as soon as the request completes
     event: threading.Event = field(default_factory=threading.Event)
 
     @property
@@ -167,7 +170,7 @@ class SAServer:
                 r.arrival = time.perf_counter()
             if self.batching and r.sigma_class < 0 and r.n < self.solo_chars:
                 r.sigma_class = self._sigma_class(r.text)
-            if footprint(r.n) > self.budget:
+            if footprint(r.n, self.sa_width) > self.budget:
                 r.error = DCX_E_ADMISSION
                 r.done = time.perf_counter()
                 r.event.set()
@@ -221,7 +224,7 @@ class SAServer:
         head = self.queue[0]
         if head.n >= self.solo_chars:
             if not (self.batching and self.large_running):
-                fp = footprint(head.n)
+                fp = footprint(head.n, self.sa_width)
                 if fp > self.free:
                     return [], 0.05
                 self.queue.remove(head)
@@ -233,7 +236,7 @@ class SAServer:
             if not smalls:
                 return [], 0.05
             head = smalls[0]
-        fp = footprint(head.n)
+        fp = footprint(head.n, self.sa_width)
         if fp > self.free:
             return [], 0.05
         batch, chars, need = [head], head.n + 1, fp
@@ -243,7 +246,7 @@ class SAServer:
                     break
                 if r.n >= self.solo_chars or r.sigma_class != head.sigma_class:
                     continue
-                fr = footprint(r.n)
+                fr = footprint(r.n, self.sa_width)
                 if chars + r.n + 1 > self.max_chars or need + fr > self.free:
                     continue
                 batch.append(r)
@@ -290,13 +293,13 @@ class SAServer:
                 # the budget, the large-build slot and the waiting clients are released whatever happened
                 done = time.perf_counter()
                 with self.cv:
-                    self.free += sum(footprint(r.n) for r in batch)
+                    self.free += sum(footprint(r.n, self.sa_width) for r in batch)
                     self.running -= 1
                     if batch[0].large:
                         self.large_running -= 1
                     self.cv.notify_all()
                 for r in batch:
-                    if err:
+                    if err or not r.keep:
                         r.sa = None
                     r.error = err
                     r.done = done
@@ -343,7 +346,7 @@ def log_uniform_sizes(count: int, lo: int, hi: int, seed: int) -> np.ndarray:
 
 
 def run_open_loop(server: SAServer, texts: list, arrivals: np.ndarray, keep_results: bool = True,
-                  sigma_classes=None) -> dict:
+                  sigma_classes=None, keep_every: int = 0) -> dict:
     """Submits texts at the given arrival offsets (seconds) and waits for all; returns latency statistics.
     sigma_classes (optional): the alphabet class of every text as the client knows it (see
     SAServer._sigma_class), so that submitting does not have to count the alphabet on the GPU."""
@@ -353,7 +356,8 @@ def run_open_loop(server: SAServer, texts: list, arrivals: np.ndarray, keep_resu
         delay = t0 + at - time.perf_counter()
         if delay > 0:
             time.sleep(delay)
-        r = Request(text=text, arrival=t0 + at)
+        r = Request(text=text, arrival=t0 + at,
+                    keep=keep_results or (keep_every > 0 and k % keep_every == 0))
         if sigma_classes is not None:
             r.sigma_class = int(sigma_classes[k])
         reqs.append(server.submit(r))
@@ -369,8 +373,8 @@ def run_open_loop(server: SAServer, texts: list, arrivals: np.ndarray, keep_resu
         "p50_ms": float(np.percentile(lat, 50) * 1e3), "p99_ms": float(np.percentile(lat, 99) * 1e3),
         "mean_ms": float(lat.mean() * 1e3), "batches": server.batches,
     }
-    if not keep_results:
-        for r in reqs:
+    for r in reqs:
+        if not r.keep:
             r.sa = None
     res["requests_list"] = reqs
     return res

@@ -124,7 +124,7 @@ def test_serving_admission_order():
         s = object.__new__(SAServer)
         s.batching, s.solo_chars, s.max_texts, s.max_chars = batching, 4 << 20, 1024 if batching else 1, 64 << 20
         s.free, s.budget, s.large_running, s.queue, s.cv = free, free, 0, [], threading.Condition()
-        s.window, s.running, s.stop = 0.0, 0, False
+        s.window, s.running, s.stop, s.sa_width = 0.0, 0, False, 4
         return s
 
     def take(s):
@@ -172,7 +172,7 @@ def test_serving_batch_window():
     s = object.__new__(SAServer)
     s.batching, s.solo_chars, s.max_texts, s.max_chars = True, 4 << 20, 1024, 64 << 20
     s.free, s.budget, s.large_running, s.queue, s.cv = 10 ** 12, 10 ** 12, 0, [], threading.Condition()
-    s.window, s.running, s.stop = 0.010, 1, False
+    s.window, s.running, s.stop, s.sa_width = 0.010, 1, False, 4
     now = time.perf_counter()
     a = Request(text=Text(1 << 16), sigma_class=0, arrival=now)
     s.queue = [a]
@@ -205,6 +205,7 @@ def test_serving_worker_survives_unexpected_errors():
     s.batching, s.solo_chars, s.max_texts, s.max_chars = True, 4 << 20, 1024, 64 << 20
     s.free, s.budget, s.large_running, s.queue, s.cv = 10 ** 12, 10 ** 12, 0, [], threading.Condition()
     s.window, s.running, s.stop, s.batches, s.worker_errors = 0.0, 0, False, 0, []
+    s.sa_width = 4
     s.torch, s.engines, s.streams = None, [None], [None]
     calls = []
 
