@@ -376,6 +376,9 @@ class Engine {
         DBuf<uint32_t> act, gid, gstart, gfirst;  // gstart: G + 1 entries, the last one A; gfirst: G entries
         uint64_t A = 0;
         uint32_t G = 0;
+        // groups of more than RF_SMALL members and their members, members of groups of more than TB_SMALL
+        uint32_t nlarge = 0;
+        uint64_t B8 = 0, B16 = 0;
     };
 
     // restricted to the previous active list `src` (its superset) or to all N positions (src == nullptr).
@@ -391,10 +394,15 @@ class Engine {
         else
             prim.scan(ActiveScanOp<false>{H, src, out.act.get(), out.gid.get(), out.gstart.get(), out.gfirst.get()},
                       Nsrc, 0);
-        const auto tot = prim.totals(0);
-        out.A = tot.lo;
-        out.G = (uint32_t)tot.hi;
-        prim.fill_u32(out.gstart.get() + out.G, (uint32_t)out.A, 1);
+        // the end sentinel and the group-size counts on the device, then one readback for all of them
+        k_active_finish<<<1, 64, 0, stream>>>(prim.scalars, out.gstart.get());
+        k_group_counts<<<grid(Nsrc / 2 + 1, 2048), 256, 0, stream>>>(out.gstart.get(), prim.scalars);
+        const uint64_t* t = prim.read_scalars(11);
+        out.A = t[0];
+        out.G = (uint32_t)t[1];
+        out.nlarge = (uint32_t)t[8];
+        out.B8 = t[9];
+        out.B16 = t[10];
     }
 
     // -------------------------------------------------------------------------------------------
@@ -508,12 +516,7 @@ class Engine {
             // compare the characters alone
             c = std::min<uint32_t>(cpw, L - C);
             {
-                DBuf<uint32_t> cnt(cache, 1);
-                prim.fill_u32(cnt.get(), 0, 1);
-                k_count_large<<<grid(cur.G, 2048), 256, 0, stream>>>(cur.gstart.get(), cur.G, cnt.get());
-                uint32_t nlarge = 0;
-                DCX_HIP(hipMemcpyAsync(&nlarge, cnt.get(), 4, hipMemcpyDeviceToHost, stream));
-                DCX_HIP(hipStreamSynchronize(stream));
+                const uint32_t nlarge = cur.nlarge;  // counted with the active set
                 const uint32_t lbits = nlarge > 1 ? bit_width(nlarge - 1) : 0;
                 if (lbits + c * bits > 64 && 64 - lbits >= bits) c = (64 - lbits) / bits;
             }
@@ -565,12 +568,12 @@ class Engine {
         k_refine_small<<<grid(G), 256, 0, stream>>>(cur.gstart.get(), G, cur.gfirst.get(), order.get(), H.get(),
                                                     Ha.get(), PT, bits, C, c * bits, large.get());
         prim.fill_u8(Ha.get() + A, 1, 1);
-        DBuf<uint32_t> slots(cache, A), lrank(cache, A), lfirst(cache, (size_t)G + 1);
-        prim.scan(LargeMembersOp{large.get(), cur.gid.get(), slots.get(), lrank.get(), lfirst.get()}, A, 1);
-        const auto tot = prim.totals(1);
-        const uint64_t B = tot.lo;
-        const uint32_t nlarge = (uint32_t)tot.hi;
+        // members of the large groups and their count, known from the active set's group counts
+        const uint64_t B = cur.B8;
+        const uint32_t nlarge = cur.nlarge;
         if (B == 0) return;
+        DBuf<uint32_t> slots(cache, B), lrank(cache, B), lfirst(cache, (size_t)nlarge + 1);
+        prim.scan(LargeMembersOp{large.get(), cur.gid.get(), slots.get(), lrank.get(), lfirst.get()}, A, 1);
         prim.fill_u32(lfirst.get() + nlarge, (uint32_t)B, 1);  // B = 2^32 reads as 0: spans are differences
         const uint32_t lbits = bit_width(nlarge - 1);
         const uint32_t cbits = c * bits;
@@ -648,14 +651,11 @@ class Engine {
         DBuf<uint32_t> found(cache, 2 * (size_t)nb), cnt(cache, 1);
         prim.fill_u32(cnt.get(), 0, 1);
         k_boundary_find<<<(nb + 63) / 64, 64, 0, stream>>>(PT, bits, L, order.get(), n, bnd, found.get(), cnt.get(), nb);
-        uint32_t hc = 0;
-        DCX_HIP(hipMemcpyAsync(&hc, cnt.get(), 4, hipMemcpyDeviceToHost, stream));
-        DCX_HIP(hipStreamSynchronize(stream));
+        const uint32_t hc = (uint32_t)prim.read_device(cnt.get(), 4)[0];
         if (hc == 0) return;
         if (hc > nb) throw Error(DCX_E_HIP, "boundary group count exceeds the boundary suffixes");
         std::vector<uint32_t> rg(2 * (size_t)hc);
-        DCX_HIP(hipMemcpyAsync(rg.data(), found.get(), 8 * (size_t)hc, hipMemcpyDeviceToHost, stream));
-        DCX_HIP(hipStreamSynchronize(stream));
+        memcpy(rg.data(), prim.read_device(found.get(), 8 * (size_t)hc), 8 * (size_t)hc);
         std::vector<std::pair<uint32_t, uint32_t>> g;  // (first slot, size)
         for (uint32_t i = 0; i < hc; i++) g.emplace_back(rg[2 * i], rg[2 * i + 1]);
         std::sort(g.begin(), g.end());
@@ -680,13 +680,14 @@ class Engine {
         DBuf<uint8_t> large(cache, G);
         k_tiebreak_small<<<grid(G), 256, 0, stream>>>(as.gstart.get(), G, as.gfirst.get(), order.get(), dc, d_cmp.get(),
                                                       ranks, large.get());
-        // large groups: merge sort with the composite (group, DC comparator) order
-        DBuf<uint64_t> items(cache, A), items2(cache, A);
-        DBuf<uint32_t> slots(cache, A);
+        // large groups: merge sort with the composite (group, DC comparator) order; their member count is
+        // known from the active set's group counts
+        const uint64_t B = as.B16;
+        if (B == 0) return;
+        DBuf<uint64_t> items(cache, B), items2(cache, B);
+        DBuf<uint32_t> slots(cache, B);
         LargeScanOp lop{large.get(), as.gid.get(), as.act.get(), order.get(), items.get(), slots.get()};
         prim.scan(lop, A, 1);
-        const uint64_t B = prim.totals(1).lo;
-        if (B == 0) return;
         k_tb_tile_sort<<<(unsigned)ceil_div(B, TB_TILE), 256, 0, stream>>>(items.get(), B, dc, d_cmp.get(), ranks);
         uint64_t* in = items.get();
         uint64_t* out = items2.get();
@@ -824,8 +825,7 @@ class Engine {
         prim.fill_u64(reinterpret_cast<uint64_t*>(hist.get()), 0, 257);
         k_code_hist<<<grid(ceil_div(n, stride * 256) * 256, 4096), 256, 0, stream>>>(text, n, lut, stride, hist.get());
         unsigned long long h[257];
-        DCX_HIP(hipMemcpyAsync(h, hist.get(), sizeof(h), hipMemcpyDeviceToHost, stream));
-        DCX_HIP(hipStreamSynchronize(stream));
+        memcpy(h, prim.read_device(hist.get(), sizeof(h)), sizeof(h));
         double total = 0, e = 0;
         for (int i = 0; i < 257; i++) total += (double)h[i];
         for (int i = 0; i < 257; i++)
@@ -991,8 +991,7 @@ class Engine {
         prim.fill_u64(reinterpret_cast<uint64_t*>(mask.get()), 0, 4);
         if (n) k_byte_mask<<<grid(ceil_div(n, 16), 2048), 256, 0, stream>>>(d_text, n, mask.get());
         unsigned long long hm[4];
-        DCX_HIP(hipMemcpyAsync(hm, mask.get(), sizeof(hm), hipMemcpyDeviceToHost, stream));
-        DCX_HIP(hipStreamSynchronize(stream));
+        memcpy(hm, prim.read_device(mask.get(), sizeof(hm)), sizeof(hm));
         uint16_t lut[256];
         uint32_t next = 1;
         for (int c = 0; c < 256; c++) lut[c] = ((hm[c >> 6] >> (c & 63)) & 1) ? (uint16_t)next++ : 0;
@@ -1299,9 +1298,7 @@ class Engine {
         prim.fill_u32(bad.get(), 0, 1);
         k_check_perm<<<grid(n), 256, 0, stream>>>(d_sa, width, n, rank.get(), seen.get(), bad.get());
         k_check_order<<<grid(n), 256, 0, stream>>>(d_sa, width, n, d_text, rank.get(), bad.get());
-        uint32_t hb = 0;
-        DCX_HIP(hipMemcpyAsync(&hb, bad.get(), 4, hipMemcpyDeviceToHost, stream));
-        DCX_HIP(hipStreamSynchronize(stream));
+        const uint32_t hb = (uint32_t)prim.read_device(bad.get(), 4)[0];
         *ok = hb == 0;
         return DCX_OK;
     }

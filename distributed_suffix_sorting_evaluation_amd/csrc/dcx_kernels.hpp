@@ -777,17 +777,6 @@ __global__ __launch_bounds__(256) void k_refine_small(const uint32_t* __restrict
     }
 }
 
-// members of the large groups of the active list: their list indices, in list order
-// number of groups with more than RF_SMALL members (they take the radix path of a group round)
-__global__ __launch_bounds__(256) void k_count_large(const uint32_t* __restrict__ gstart, uint32_t G,
-                                                     uint32_t* __restrict__ count) {
-    uint32_t c = 0;
-    for (uint64_t g = blockIdx.x * 256ull + threadIdx.x; g < G; g += gridDim.x * 256ull)
-        c += gstart[g + 1] - gstart[g] > RF_SMALL;
-    for (int d = 32; d >= 1; d >>= 1) c += __shfl_xor(c, d, 64);
-    if (__lane_id() == 0 && c) atomicAdd(count, c);
-}
-
 // members of the large groups of the active list: their list indices, in list order, and the rank of
 // their group among the large groups (the high counter counts the first members of large groups; a
 // group's members are consecutive in the list)
@@ -1001,6 +990,41 @@ __global__ __launch_bounds__(256) void k_writeback_large2(const uint64_t* __rest
 }
 
 constexpr uint32_t TB_SMALL = 16;
+
+// After an active-set scan whose totals are sc[0] = members A and sc[1] = groups G: gstart[G] = A (the end of
+// the last group; 0 when A = 2^32, sizes are differences modulo 2^32), and the group-size counts cleared
+__global__ void k_active_finish(uint64_t* sc, uint32_t* __restrict__ gstart) {
+    if (threadIdx.x == 0 && blockIdx.x == 0) {
+        gstart[sc[1]] = (uint32_t)sc[0];
+        sc[8] = 0;
+        sc[9] = 0;
+        sc[10] = 0;
+    }
+}
+
+// Counts of the active set's groups by size, for the host to size the next steps without another
+// readback: sc[8] = groups with more than RF_SMALL members (the radix path of a group round), sc[9] = their
+// members, sc[10] = members of groups with more than TB_SMALL members (the merge-sort path of the tie-break)
+__global__ __launch_bounds__(256) void k_group_counts(const uint32_t* __restrict__ gstart, uint64_t* sc) {
+    const uint64_t G = sc[1];
+    unsigned long long nl = 0, b8 = 0, b16 = 0;
+    for (uint64_t g = blockIdx.x * 256ull + threadIdx.x; g < G; g += gridDim.x * 256ull) {
+        const uint32_t sz = gstart[g + 1] - gstart[g];
+        nl += sz > RF_SMALL;
+        b8 += sz > RF_SMALL ? sz : 0;
+        b16 += sz > TB_SMALL ? sz : 0;
+    }
+    for (int d = 32; d >= 1; d >>= 1) {
+        nl += __shfl_xor(nl, d, 64);
+        b8 += __shfl_xor(b8, d, 64);
+        b16 += __shfl_xor(b16, d, 64);
+    }
+    if (__lane_id() == 0 && (nl | b16)) {
+        atomicAdd(reinterpret_cast<unsigned long long*>(&sc[8]), nl);
+        atomicAdd(reinterpret_cast<unsigned long long*>(&sc[9]), b8);
+        atomicAdd(reinterpret_cast<unsigned long long*>(&sc[10]), b16);
+    }
+}
 
 // groups of size <= TB_SMALL: one thread sorts its group by insertion sort with the DC comparator;
 // larger groups are flagged for the merge-sort path

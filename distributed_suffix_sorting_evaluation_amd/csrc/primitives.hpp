@@ -14,6 +14,8 @@
 // (lightweight_dcx/src/sorters/sample_sort.hpp:37-171, lightweight_dcx/src/mpi/reduce.hpp ex_prefix_sum).
 #pragma once
 
+#include <cstring>
+
 #include "common.hpp"
 
 namespace dcx {
@@ -946,6 +948,18 @@ __global__ __launch_bounds__(SC_THREADS) void k_scan_down(Op op, uint64_t N, uin
     }
 }
 
+// Small results for the host (scan totals, counts, a few ranges): one thread copies them into a pinned,
+// device-mapped, coherent host buffer with system-scope stores and then stores the call's sequence number
+// with release ordering; the host spins on that number.  Neither a copy-engine transfer nor a stream
+// synchronization is involved.
+constexpr uint32_t MAIL_WORDS = 512;  // payload capacity in 64-bit words (4 KiB)
+__global__ void k_publish(const uint64_t* __restrict__ src, uint32_t words, uint64_t* box, uint64_t seq) {
+    if (threadIdx.x != 0) return;
+    for (uint32_t i = 0; i < words; i++)
+        __hip_atomic_store(&box[1 + i], src[i], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+    __hip_atomic_store(&box[0], seq, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
+}
+
 // ------------------------------------------------------------------------------------------------
 // Host side: scratch for lookback state and the launch wrappers
 // ------------------------------------------------------------------------------------------------
@@ -974,7 +988,9 @@ struct Primitives {
     uint64_t* seg_off = nullptr;
     uint64_t* seg_info = nullptr;
     uint32_t* seg_tick = nullptr;
-    uint64_t* host_scalars = nullptr;      // pinned mirror
+    uint64_t* mailbox = nullptr;           // pinned, device-mapped, coherent: [0] sequence, [1..] payload
+    uint64_t* dev_mailbox = nullptr;       // the device's address of the same buffer
+    uint64_t mail_seq = 0;
     uint32_t cus = 1;                      // compute units of the device
     // set while a call must not allocate: lookback status arrays that would have to grow throw instead
     // (they are sized at handle creation for the largest such call)
@@ -1035,7 +1051,10 @@ struct Primitives {
         DCX_HIP(hipMalloc(&seg_tick, 8 * RS_SEGS * sizeof(uint32_t)));
         DCX_HIP(hipMalloc(&scan_part, 2 * SC_BLOCKS * sizeof(uint64_t)));
         DCX_HIP(hipMalloc(&scan_prefix, SC_BLOCKS * sizeof(uint64_t)));
-        DCX_HIP(hipHostMalloc(&host_scalars, 64 * sizeof(uint64_t), hipHostMallocDefault));
+        DCX_HIP(hipHostMalloc((void**)&mailbox, (MAIL_WORDS + 1) * sizeof(uint64_t), hipHostMallocMapped | hipHostMallocCoherent));
+        memset(mailbox, 0, (MAIL_WORDS + 1) * sizeof(uint64_t));
+        DCX_HIP(hipHostGetDevicePointer((void**)&dev_mailbox, mailbox, 0));
+        mail_seq = 0;
         ticket_host = 0;
     }
     void destroy() {
@@ -1053,13 +1072,16 @@ struct Primitives {
         if (scan_part) (void)hipFree(scan_part);
         if (scan_prefix) (void)hipFree(scan_prefix);
         scan_part = scan_prefix = nullptr;
-        if (host_scalars) (void)hipHostFree(host_scalars);
+        if (mailbox) {
+            (void)hipStreamSynchronize(stream);
+            (void)hipHostFree(mailbox);
+        }
         sort_status = nullptr;
         scan_lo = scan_hi = nullptr;
         ticket = nullptr;
         scalars = nullptr;
         hist = nullptr;
-        host_scalars = nullptr;
+        mailbox = dev_mailbox = nullptr;
     }
 
     static unsigned grid_for(uint64_t n, unsigned threads, unsigned max_blocks = 8192) {
@@ -1389,12 +1411,33 @@ struct Primitives {
                            scalars + 2 * slot);
     }
 
-    // Copies `count` device scalars to the host (synchronizes the stream).
-    const uint64_t* read_scalars(int count) {
-        DCX_HIP(hipMemcpyAsync(host_scalars, scalars, count * sizeof(uint64_t), hipMemcpyDeviceToHost, stream));
-        DCX_HIP(hipStreamSynchronize(stream));
-        return host_scalars;
+    // `bytes` (at most 8 * MAIL_WORDS) of device memory at `src` (8-byte aligned, readable up to the next
+    // multiple of 8), ordered after the work on the stream: published by k_publish into the mailbox and
+    // awaited by spinning on its sequence number.  The stream is queried now and then, so that a failed
+    // stream raises an error instead of a wait that never ends.  The result stays valid until the next call.
+    const uint64_t* read_device(const void* src, size_t bytes) {
+        const uint32_t words = (uint32_t)((bytes + 7) / 8);
+        if (words > MAIL_WORDS) throw Error(-6, "read_device: at most 4 KiB per read");
+        const uint64_t seq = ++mail_seq;
+        hipLaunchKernelGGL(k_publish, dim3(1), dim3(64), 0, stream, static_cast<const uint64_t*>(src), words,
+                           dev_mailbox, seq);
+        for (uint64_t spin = 1;; spin++) {
+            if (__atomic_load_n(&mailbox[0], __ATOMIC_ACQUIRE) == seq) return mailbox + 1;
+            if ((spin & 1023) == 0) {
+                const hipError_t e = hipStreamQuery(stream);
+                if (e == hipSuccess) {  // drained: the kernel has run and its stores are visible
+                    if (__atomic_load_n(&mailbox[0], __ATOMIC_ACQUIRE) == seq) return mailbox + 1;
+                    throw Error(-4, "read_device: the stream drained without publishing");
+                }
+                if (e != hipErrorNotReady) {
+                    (void)hipGetLastError();
+                    throw Error(-4, std::string("read_device: ") + hipGetErrorString(e));
+                }
+            }
+        }
     }
+    // Copies `count` device scalars to the host (ordered after the work on the stream).
+    const uint64_t* read_scalars(int count) { return read_device(scalars, count * sizeof(uint64_t)); }
     // The lane totals of the last scan into slot `slot`: lo = sum of the low 32-bit halves of its values,
     // hi = of the high halves (synchronizes the stream).
     struct Totals {
