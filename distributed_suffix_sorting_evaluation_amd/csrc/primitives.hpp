@@ -469,14 +469,20 @@ __global__ __launch_bounds__(RS_THREADS) void k_onesweep(Src src, OutT* __restri
             uint64_t idx = wbase + k * 64 + lane;
             bool valid = idx < tile_end;
             uint32_t d = (uint32_t)(key[k] >> shift) & 0xFFu;
-            uint64_t peers = __ballot(valid);
+            // peers: the lanes with this lane's digit, as two 32-bit halves; bit b of the digit as a word of
+            // all ones or zeros (m), the wave's ballot of it (bb), and the lanes that agree: ~(bb ^ m)
+            const uint64_t vb = __ballot(valid);
+            uint32_t plo = (uint32_t)vb, phi = (uint32_t)(vb >> 32);
 #pragma unroll
             for (int b = 0; b < 8; b++) {
-                uint32_t bit = (d >> b) & 1u;
-                uint64_t bb = __ballot(bit);
-                peers &= bit ? bb : ~bb;
+                const uint32_t m = (uint32_t)((int32_t)(d << (31 - b)) >> 31);
+                const uint64_t bb = __ballot(m != 0);
+                plo &= ~((uint32_t)bb ^ m);
+                phi &= ~((uint32_t)(bb >> 32) ^ m);
             }
-            uint32_t below = (uint32_t)__popcll(peers & lanemask_lt());
+            const uint64_t peers = ((uint64_t)phi << 32) | plo;
+            // peers in lower lanes (a masked population count in two instructions)
+            uint32_t below = __builtin_amdgcn_mbcnt_hi(phi, __builtin_amdgcn_mbcnt_lo(plo, 0u));
             uint32_t lead = peers ? (uint32_t)__builtin_ctzll(peers) : lane;
             old[k - k0] = 0;
             if (valid && below == 0) old[k - k0] = atomicAdd(&s_wcnt[wave][d], (uint32_t)__popcll(peers));
