@@ -552,7 +552,9 @@ class Engine {
         DBuf<uint32_t> k1(cache, N), v1(cache, N), k2(cache, N), v2(cache, N);
         uint32_t *ks = nullptr, *vs = nullptr;
         prim.sort_u32(dest, val, k1.get(), v1.get(), k2.get(), v2.get(), N, wbits, np, &ks, &vs);
-        k_scatter_window<<<(unsigned)ceil_div((uint64_t)N, 1ull << wbits), 256, 0, stream>>>(ks, vs, N, add, wbits, out);
+        const unsigned threads = wbits >= SCATTER_WIDE_BITS ? SCATTER_WIDE_THREADS : 256u;
+        k_scatter_window<<<(unsigned)ceil_div((uint64_t)N, 1ull << wbits), threads, (size_t)4 << wbits, stream>>>(
+            ks, vs, N, add, wbits, out);
     }
 
     // One character round of the refinement: orders the members of every tied group of `cur` by their

@@ -586,21 +586,26 @@ __global__ __launch_bounds__(256) void k_iota(uint32_t* __restrict__ out, uint64
 
 // out[dest[i]] = val[i] + add for a permutation dest of [0, N) whose pairs are ordered by dest >> wbits:
 // pairs [b << wbits, (b + 1) << wbits) are exactly the destinations of window b, which workgroup b
-// assembles in LDS and writes out whole (coalesced stores instead of one cache line per word).
-constexpr uint32_t SCATTER_WINDOW_BITS = 13;  // at most 8192 words (32 KB of LDS) per window
-__global__ __launch_bounds__(256) void k_scatter_window(const uint32_t* __restrict__ dest,
-                                                        const uint32_t* __restrict__ val, uint64_t N, uint32_t add,
-                                                        uint32_t wbits, uint32_t* __restrict__ out) {
-    __shared__ uint32_t s_w[1u << SCATTER_WINDOW_BITS];
+// assembles in LDS (4 << wbits bytes of dynamic LDS) and writes out whole (coalesced stores instead of one
+// cache line per word).  Launched with SCATTER_WIDE_THREADS threads for windows of at least
+// 2^SCATTER_WIDE_BITS words, else 256.
+constexpr uint32_t SCATTER_WINDOW_BITS = 14;  // at most 16384 words (64 KB of LDS) per window
+constexpr uint32_t SCATTER_WIDE_BITS = 12;
+constexpr uint32_t SCATTER_WIDE_THREADS = 1024;
+__global__ __launch_bounds__(SCATTER_WIDE_THREADS) void k_scatter_window(const uint32_t* __restrict__ dest,
+                                                                         const uint32_t* __restrict__ val, uint64_t N,
+                                                                         uint32_t add, uint32_t wbits,
+                                                                         uint32_t* __restrict__ out) {
+    extern __shared__ uint32_t s_w[];
     const uint64_t beg = (uint64_t)blockIdx.x << wbits;
     const uint64_t end = beg + (1ull << wbits) < N ? beg + (1ull << wbits) : N;
     const uint32_t len = (uint32_t)(end - beg);
-    for (uint32_t t = threadIdx.x; t < len; t += 256) {
+    for (uint32_t t = threadIdx.x; t < len; t += blockDim.x) {
         const uint32_t o = dest[beg + t] - (uint32_t)beg;
         if (o < len) s_w[o] = val[beg + t] + add;  // always true for a permutation
     }
     __syncthreads();
-    for (uint32_t t = threadIdx.x; t < len; t += 256) out[beg + t] = s_w[t];
+    for (uint32_t t = threadIdx.x; t < len; t += blockDim.x) out[beg + t] = s_w[t];
 }
 
 // out[dest[i]] = val[i] + add
