@@ -682,22 +682,39 @@ class Engine {
         DBuf<uint8_t> large(cache, G);
         k_tiebreak_small<<<grid(G), 256, 0, stream>>>(as.gstart.get(), G, as.gfirst.get(), order.get(), dc, d_cmp.get(),
                                                       ranks, large.get());
-        // large groups: merge sort with the composite (group, DC comparator) order; their member count is
-        // known from the active set's group counts
+        // large groups (their member count is known from the active set's group counts): those whose members
+        // already follow the comparator, or its reverse, pair by pair are written back at once; the others
+        // go through a merge sort with the composite (group, DC comparator) order
         const uint64_t B = as.B16;
         if (B == 0) return;
-        DBuf<uint64_t> items(cache, B), items2(cache, B);
+        DBuf<uint64_t> items(cache, B);
         DBuf<uint32_t> slots(cache, B);
         LargeScanOp lop{large.get(), as.gid.get(), as.act.get(), order.get(), items.get(), slots.get()};
         prim.scan(lop, A, 1);
-        k_tb_tile_sort<<<(unsigned)ceil_div(B, TB_TILE), 256, 0, stream>>>(items.get(), B, dc, d_cmp.get(), ranks);
-        uint64_t* in = items.get();
-        uint64_t* out = items2.get();
-        for (uint64_t w = TB_TILE; w < B; w *= 2) {
-            k_tb_merge<<<grid(B), 256, 0, stream>>>(in, out, B, w, dc, d_cmp.get(), ranks);
+        DBuf<uint32_t> lidx(cache, G);
+        prim.scan(LargeIndexOp{large.get(), lidx.get()}, G, 2);
+        const uint64_t GLmax = B / (TB_SMALL + 1) + 1;  // a large group has more than TB_SMALL members
+        DBuf<uint32_t> mono(cache, GLmax);
+        prim.fill_u32(mono.get(), 0, GLmax);
+        const uint64_t chunks = ceil_div(B, (uint64_t)TB_MONO_CHUNK);
+        k_tb_monotone<<<(unsigned)std::min<uint64_t>(ceil_div(chunks, 4ull), 65536), 256, 0, stream>>>(
+            items.get(), B, lidx.get(), dc, d_cmp.get(), ranks, mono.get());
+        k_tb_reverse<<<grid(B), 256, 0, stream>>>(items.get(), slots.get(), B, lidx.get(), mono.get(), as.gstart.get(),
+                                                  as.gfirst.get(), order.get());
+        DBuf<uint64_t> items_m(cache, B);
+        DBuf<uint32_t> slots_m(cache, B);
+        prim.scan(MixedScanOp{items.get(), slots.get(), lidx.get(), mono.get(), items_m.get(), slots_m.get()}, B, 1);
+        const uint64_t Bm = prim.totals(1).lo;
+        if (Bm == 0) return;
+        if (Bm > B) throw Error(DCX_E_HIP, "tie-break member count exceeds the large groups");
+        k_tb_tile_sort<<<(unsigned)ceil_div(Bm, TB_TILE), 256, 0, stream>>>(items_m.get(), Bm, dc, d_cmp.get(), ranks);
+        uint64_t* in = items_m.get();
+        uint64_t* out = items.get();
+        for (uint64_t w = TB_TILE; w < Bm; w *= 2) {
+            k_tb_merge<<<grid(Bm), 256, 0, stream>>>(in, out, Bm, w, dc, d_cmp.get(), ranks);
             std::swap(in, out);
         }
-        k_tb_scatter<<<grid(B), 256, 0, stream>>>(in, slots.get(), B, order.get());
+        k_tb_scatter<<<grid(Bm), 256, 0, stream>>>(in, slots_m.get(), Bm, order.get());
     }
 
     void base_case(const uint8_t* t, const uint16_t* lut, uint32_t n, uint32_t* sa_out) {

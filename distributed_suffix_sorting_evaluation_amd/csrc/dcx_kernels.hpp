@@ -1152,6 +1152,99 @@ __global__ __launch_bounds__(256) void k_tb_scatter(const uint64_t* __restrict__
 }
 
 // ------------------------------------------------------------------------------------------------
+// Large groups already in order.  The DC comparator is a strict total order, so a group whose consecutive
+// members (in their current order) all compare ascending is sorted, and one whose pairs all compare
+// descending is sorted once reversed.  The groups of a periodic run are of this kind: two of its suffixes
+// first differ where the run ends, and all members meet the same pair of characters there.  Only the
+// groups with pairs of both directions go through the merge sort.
+// ------------------------------------------------------------------------------------------------
+
+// index of each large group among the large groups (in group order)
+struct LargeIndexOp {
+    static constexpr bool kBlocked = true;
+    static constexpr bool kCompact = true;  // writes only the large groups
+    const uint8_t* large;
+    uint32_t* lidx;
+    __device__ __forceinline__ uint64_t value(uint64_t g) const { return large[g]; }
+    __device__ __forceinline__ void write(uint64_t g, uint64_t excl, uint64_t v) const {
+        if (v) lidx[g] = (uint32_t)excl;
+    }
+};
+
+// mono[large group] |= 1 for a pair of consecutive members in comparator order, 2 for a reversed pair
+// (items: group << 32 | suffix, in group order).  A wave takes TB_MONO_CHUNK consecutive members, each lane
+// keeps the flags of its current group and ORs them into mono only when its group changes and at the end
+// (one OR per group and wave): the groups are few and large, so per-pair atomics would queue on a few words.
+constexpr uint32_t TB_MONO_CHUNK = 4096;  // a multiple of 64
+__global__ __launch_bounds__(256) void k_tb_monotone(const uint64_t* __restrict__ items, uint64_t B,
+                                                     const uint32_t* __restrict__ lidx, DcDev dc,
+                                                     const uint32_t* __restrict__ cmp, const uint32_t* __restrict__ ranks,
+                                                     uint32_t* __restrict__ mono) {
+    const uint32_t lane = __lane_id();
+    const uint64_t waves = (uint64_t)gridDim.x * 4;
+    for (uint64_t c = (uint64_t)blockIdx.x * 4 + threadIdx.x / 64; c * TB_MONO_CHUNK < B; c += waves) {
+        const uint64_t b0 = c * TB_MONO_CHUNK;
+        const uint64_t b1 = b0 + TB_MONO_CHUNK < B ? b0 + TB_MONO_CHUNK : B;
+        uint32_t gcur = 0, fcur = 0;
+        for (uint64_t b = b0 + lane; b < b1; b += 64) {
+            if (b + 1 >= B) continue;
+            const uint64_t x = items[b], y = items[b + 1];
+            if ((x >> 32) != (y >> 32)) continue;
+            const uint32_t g = lidx[x >> 32];
+            const uint32_t f = dc_rank_less((uint32_t)x, (uint32_t)y, dc, cmp, ranks) ? 1u : 2u;
+            if (fcur && g != gcur) {
+                atomicOr(&mono[gcur], fcur);
+                fcur = 0;
+            }
+            gcur = g;
+            fcur |= f;
+        }
+        // the lanes still holding flags, one group at a time: one OR per group
+        for (uint64_t pend = __ballot(fcur != 0); pend; pend = __ballot(fcur != 0)) {
+            const int l0 = __builtin_ctzll(pend);
+            const uint32_t g0 = (uint32_t)__shfl((int)gcur, l0, 64);
+            const bool mine = fcur != 0 && gcur == g0;
+            const uint32_t fw = (__ballot(mine && (fcur & 1u)) ? 1u : 0u) | (__ballot(mine && (fcur & 2u)) ? 2u : 0u);
+            if ((int)lane == l0) atomicOr(&mono[g0], fw);
+            if (mine) fcur = 0;
+        }
+    }
+}
+
+// members of the groups found in reverse order go to the mirrored slot of their group
+__global__ __launch_bounds__(256) void k_tb_reverse(const uint64_t* __restrict__ items, const uint32_t* __restrict__ slots,
+                                                    uint64_t B, const uint32_t* __restrict__ lidx,
+                                                    const uint32_t* __restrict__ mono, const uint32_t* __restrict__ gstart,
+                                                    const uint32_t* __restrict__ gfirst, uint32_t* __restrict__ order) {
+    for (uint64_t b = blockIdx.x * 256ull + threadIdx.x; b < B; b += gridDim.x * 256ull) {
+        const uint64_t x = items[b];
+        const uint32_t g = (uint32_t)(x >> 32);
+        if (mono[lidx[g]] != 2u) continue;
+        const uint32_t last = gfirst[g] + (gstart[g + 1] - gstart[g]) - 1;  // the group's last slot
+        order[gfirst[g] + (last - slots[b])] = (uint32_t)x;
+    }
+}
+
+// the members of the groups with pairs of both directions, compacted (group order kept) for the merge sort
+struct MixedScanOp {
+    static constexpr bool kBlocked = true;
+    static constexpr bool kCompact = true;  // writes only members of mixed groups
+    const uint64_t* items;
+    const uint32_t* slots;
+    const uint32_t* lidx;
+    const uint32_t* mono;
+    uint64_t* items_m;
+    uint32_t* slots_m;
+    __device__ __forceinline__ uint64_t value(uint64_t b) const { return mono[lidx[items[b] >> 32]] == 3u ? 1ull : 0ull; }
+    __device__ __forceinline__ void write(uint64_t b, uint64_t excl, uint64_t v) const {
+        if (v) {
+            items_m[excl] = items[b];
+            slots_m[excl] = slots[b];
+        }
+    }
+};
+
+// ------------------------------------------------------------------------------------------------
 // Base case (slow_suffixarray, string_util.hpp:87-106; sequential_sa.hpp:17-28): one workgroup,
 // prefix doubling with a bitonic sort of (rank, rank+h, index) keys held in LDS.  n <= 16384.
 // ------------------------------------------------------------------------------------------------

@@ -331,6 +331,37 @@ def test_large_groups_mixed_continuations(engine):
     assert np.array_equal(sa, oracle.dcx_sa(t, X=39))
 
 
+def _periodic_block(length, period, follower, seed):
+    """A periodic run over {0..3} whose period continues with character 1 after `length` characters, then
+    `follower`: the run's tied groups compare at its end, all members the same way (ascending positions for
+    a follower above 1, descending below)."""
+    rng = np.random.default_rng(seed)
+    p = rng.integers(0, 4, period, dtype=np.uint8)
+    p[length % period] = 1
+    return np.concatenate([np.resize(p, length), np.array([follower], dtype=np.uint8)])
+
+
+@pytest.mark.parametrize("X", [21, 39, 133])
+def test_tie_break_monotone_groups(engine, X):
+    """Large tied groups already in comparator order, in reverse order, and mixed, in one text: periodic runs
+    whose groups all ascend (follower above the period's next character) or all descend (follower below),
+    groups just above and below the 16-member threshold, repeated words (groups whose members compare both
+    ways, which take the merge sort), and a periodic run reaching the text end (boundary suffixes in its
+    groups).  Against libsais."""
+    rng = np.random.default_rng(90 + X)
+    words = [rng.integers(0, 4, 60, dtype=np.uint8).tobytes() for _ in range(3)]
+    parts = [random_text(1 << 18, 4, 91), _periodic_block(40000, 11, 3, 92), random_text(1 << 16, 4, 93),
+             _periodic_block(30000, 7, 0, 94), random_text(1 << 16, 4, 95),
+             _periodic_block(17 * 23 + 150, 23, 2, 96), random_text(4096, 4, 97),
+             _periodic_block(15 * 29 + 150, 29, 0, 98), random_text(4096, 4, 99),
+             repeat_words_text(words, 3000, 100), random_text(1 << 16, 4, 101),
+             periodic_text(20000, 5, 4, 102)]
+    t = np.concatenate(parts)
+    say(f"tie-break monotone groups X={X} n={len(t)}")
+    sa = engine.build_sa(t, sa_width=8, X=X)
+    assert np.array_equal(sa, oracle.libsais_sa(t, threads=8))
+
+
 def test_dense_codes_text_end(engine):
     # alphabets of 2, 4 and 256 symbols make the merge use dense character codes, where the padding
     # reads like the smallest character; texts ending in long runs of that character exercise the
