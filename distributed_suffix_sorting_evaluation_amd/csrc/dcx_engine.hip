@@ -1098,11 +1098,10 @@ class Engine {
             DCX_HIP(hipMemcpyAsync(d_co.get(), co.data(), (cnt + 1) * 8, hipMemcpyHostToDevice, stream));
             const uint64_t N = joined;
             const uint64_t chars = offsets[k1] - offsets[k0];
-            // owner tables of the input and of the joined positions
-            const uint64_t in_chunks = (chars >> OWNER_SHIFT) + 1, co_chunks = ((uint64_t)N >> OWNER_SHIFT) + 1;
-            DBuf<uint32_t> t_in(cache, in_chunks + 1), t_co(cache, co_chunks + 1);
+            // owner table of the input positions
+            const uint64_t in_chunks = (chars >> OWNER_SHIFT) + 1;
+            DBuf<uint32_t> t_in(cache, in_chunks + 1);
             k_owner_table<<<grid(in_chunks + 1), 256, 0, stream>>>(d_in.get(), cnt, in_off[0], in_chunks, t_in.get());
-            k_owner_table<<<grid(co_chunks + 1), 256, 0, stream>>>(d_co.get(), cnt, 0, co_chunks, t_co.get());
             DBuf<uint32_t> sa;
             sigma = maxchar;
             if (maxchar <= 255) {
@@ -1122,22 +1121,29 @@ class Engine {
                 k_batch_seps<uint32_t><<<grid(cnt), 256, 0, stream>>>(d_co.get(), cnt, jt.get());
                 level<uint32_t>(jt.get(), nullptr, N, maxchar, 0, nullptr, nullptr, &sa);
             }
-            // stable partition of the joined SA by text; separators sort last
-            DBuf<uint64_t> ka(cache, N), kb(cache, N);
-            DBuf<uint32_t> va(cache, N), vb(cache, N);
-            k_batch_keys<<<grid(N), 256, 0, stream>>>(sa.get(), N, d_co.get(), cnt, t_co.get(), ka.get(), va.get());
-            sa.reset();
-            uint64_t* x = ka.get();
-            uint64_t* x2 = kb.get();
-            uint32_t* y = va.get();
-            uint32_t* y2 = vb.get();
-            prim.sort_pairs(x, x2, y, y2, N, 0, bit_width(cnt));
-            uint32_t* dst = sa_out + (offsets[k0] - base);
+            // stable partition of the joined SA by text: keys (text << 32 | position in the text), the
+            // separators keyed past the last text; the last pass keeps the low words of the first `chars`
+            // keys, written straight into a device SA
             if (chars) {
-                if (o->sa_on_device)
-                    DCX_HIP(hipMemcpyAsync(dst, y, chars * 4, hipMemcpyDeviceToDevice, stream));
-                else
-                    download(dst, y, chars * 4);
+                DBuf<uint64_t> ka(cache, N), kb(cache, N);
+                if (cnt <= BK_MAX_TEXTS) {
+                    uint32_t cshift = 0;
+                    while (((N - 1) >> cshift) + 1 > BK_TABLE) cshift++;
+                    k_batch_keys_lds<<<2 * prim.cus, BK_THREADS, 0, stream>>>(sa.get(), N, d_co.get(), cnt, cshift,
+                                                                              ka.get());
+                } else {
+                    const uint64_t co_chunks = ((uint64_t)N >> OWNER_SHIFT) + 1;
+                    DBuf<uint32_t> t_co(cache, co_chunks + 1);
+                    k_owner_table<<<grid(co_chunks + 1), 256, 0, stream>>>(d_co.get(), cnt, 0, co_chunks, t_co.get());
+                    k_batch_keys64<<<grid(N), 256, 0, stream>>>(sa.get(), N, d_co.get(), cnt, t_co.get(), ka.get());
+                }
+                sa.reset();
+                uint32_t* dst = sa_out + (offsets[k0] - base);
+                DBuf<uint32_t> staged;
+                if (!o->sa_on_device) staged = DBuf<uint32_t>(cache, chars);
+                uint32_t* out = o->sa_on_device ? dst : staged.get();
+                prim.sort_keys_low32(ka.get(), kb.get(), N, 32, 32 + bit_width(cnt), out, chars);
+                if (!o->sa_on_device) download(dst, out, chars * 4);
             }
             DCX_HIP(hipStreamSynchronize(stream));
             k0 = k1;

@@ -1470,17 +1470,79 @@ __global__ __launch_bounds__(256) void k_batch_seps(const uint64_t* __restrict__
 }
 
 // (text, local position) of every suffix of the joined text in SA order; separators go to bucket `count`
-__global__ __launch_bounds__(256) void k_batch_keys(const uint32_t* __restrict__ sa, uint64_t N,
-                                                    const uint64_t* __restrict__ co, uint32_t count,
-                                                    const uint32_t* __restrict__ table, uint64_t* __restrict__ keys,
-                                                    uint32_t* __restrict__ vals) {
+// Partition keys of the joined SA for at most BK_MAX_TEXTS texts: (text << 32) | position within the
+// text, a separator keyed (count << 32) so that it sorts after every text.  Every workgroup holds the
+// texts' separator positions in LDS (32-bit: a joined string has at most 2^32 positions) and a table of
+// the owner of every 2^cshift-th joined position, so a lookup is a short binary search in LDS instead of
+// random reads of global tables.  Four SA entries per thread and step: one 16-byte load, two 16-byte
+// stores.
+constexpr uint32_t BK_THREADS = 1024;
+constexpr uint32_t BK_MAX_TEXTS = 12288;  // u32 separator positions: 48 KB of LDS
+constexpr uint32_t BK_TABLE = 8192;       // u16 owners: 16 KB of LDS
+__device__ __forceinline__ uint64_t bk_key(const uint32_t* s_end, const uint16_t* s_tab, uint32_t count,
+                                           uint32_t cshift, uint32_t tab_n, uint32_t p) {
+    const uint32_t c = p >> cshift;
+    uint32_t lo = s_tab[c], hi = c + 1 < tab_n ? s_tab[c + 1] : count - 1;  // owner(p) lies in [lo, hi]
+    while (lo < hi) {
+        const uint32_t mid = (lo + hi) >> 1;
+        if (s_end[mid] >= p)
+            hi = mid;
+        else
+            lo = mid + 1;
+    }
+    const uint32_t start = lo ? s_end[lo - 1] + 1 : 0u;
+    return p == s_end[lo] ? (uint64_t)count << 32 : ((uint64_t)lo << 32) | (p - start);
+}
+__global__ __launch_bounds__(BK_THREADS) void k_batch_keys_lds(const uint32_t* __restrict__ sa, uint64_t N,
+                                                               const uint64_t* __restrict__ co, uint32_t count,
+                                                               uint32_t cshift, uint64_t* __restrict__ keys) {
+    __shared__ uint32_t s_end[BK_MAX_TEXTS];
+    __shared__ uint16_t s_tab[BK_TABLE];
+    const uint32_t tid = threadIdx.x;
+    for (uint32_t k = tid; k < count; k += BK_THREADS) s_end[k] = (uint32_t)(co[k + 1] - 1);
+    __syncthreads();
+    // owner of the joined position c << cshift: the first text whose separator lies at or after it
+    const uint32_t tab_n = (uint32_t)((N - 1) >> cshift) + 1;
+    for (uint32_t c = tid; c < tab_n; c += BK_THREADS) {
+        const uint64_t p = (uint64_t)c << cshift;
+        uint32_t lo = 0, hi = count - 1;
+        while (lo < hi) {
+            const uint32_t mid = (lo + hi) >> 1;
+            if ((uint64_t)s_end[mid] >= p)
+                hi = mid;
+            else
+                lo = mid + 1;
+        }
+        s_tab[c] = (uint16_t)lo;
+    }
+    __syncthreads();
+    const uint64_t N4 = N >> 2;
+    const uint64_t stride = (uint64_t)gridDim.x * BK_THREADS;
+    for (uint64_t q = blockIdx.x * (uint64_t)BK_THREADS + tid; q < N4; q += stride) {
+        const uint4 p = reinterpret_cast<const uint4*>(sa)[q];
+        ulonglong2 a, b;
+        a.x = bk_key(s_end, s_tab, count, cshift, tab_n, p.x);
+        a.y = bk_key(s_end, s_tab, count, cshift, tab_n, p.y);
+        b.x = bk_key(s_end, s_tab, count, cshift, tab_n, p.z);
+        b.y = bk_key(s_end, s_tab, count, cshift, tab_n, p.w);
+        reinterpret_cast<ulonglong2*>(keys)[2 * q] = a;
+        reinterpret_cast<ulonglong2*>(keys)[2 * q + 1] = b;
+    }
+    if (blockIdx.x == 0 && tid < (uint32_t)(N & 3)) {
+        const uint64_t j = (N4 << 2) + tid;
+        keys[j] = bk_key(s_end, s_tab, count, cshift, tab_n, sa[j]);
+    }
+}
+
+// the same for any number of texts, through the global owner table (keys as above)
+__global__ __launch_bounds__(256) void k_batch_keys64(const uint32_t* __restrict__ sa, uint64_t N,
+                                                      const uint64_t* __restrict__ co, uint32_t count,
+                                                      const uint32_t* __restrict__ table, uint64_t* __restrict__ keys) {
     for (uint64_t j = blockIdx.x * 256ull + threadIdx.x; j < N; j += gridDim.x * 256ull) {
-        uint32_t p = sa[j];
-        uint32_t k = find_owner_tab(co, count, table, 0, p);
-        uint32_t local = (uint32_t)(p - co[k]);
-        bool sep = (uint64_t)p == co[k + 1] - 1;
-        keys[j] = sep ? count : k;
-        vals[j] = local;
+        const uint32_t p = sa[j];
+        const uint32_t k = find_owner_tab(co, count, table, 0, p);
+        const bool sep = (uint64_t)p == co[k + 1] - 1;
+        keys[j] = sep ? (uint64_t)count << 32 : ((uint64_t)k << 32) | (uint32_t)(p - co[k]);
     }
 }
 

@@ -329,14 +329,18 @@ __global__ __launch_bounds__(256) void k_seg_setup(const uint64_t* __restrict__ 
 // Byte values (VinT or VoutT uint8_t, 64-bit keys whose digits lie at or above bit kpos): an index is
 // carried as its low kpos bits in the key's free low bits and its high bits in a byte, 9 bytes per
 // element instead of 12.  An IDENT pass splits the index; a pass with VoutT = uint32_t joins it again.
+// LIMIT (keys only): output indices at or above out_limit are not written (a partition whose last
+// bucket is dropped writes into a buffer that holds only the buckets before it).
 template <bool VALS, bool IDENT, class Src, class KeyT = uint64_t, int ITEMS = RS_ITEMS, class OutT = KeyT,
-          class VinT = uint32_t, class VoutT = VinT>
+          class VinT = uint32_t, class VoutT = VinT, bool LIMIT = false>
 __global__ __launch_bounds__(RS_THREADS) void k_onesweep(Src src, OutT* __restrict__ kout,
                                                          const VinT* __restrict__ vin, VoutT* __restrict__ vout,
                                                          uint32_t shift, const uint64_t* __restrict__ segoff,
                                                          const uint64_t* __restrict__ info, uint32_t* __restrict__ tick,
                                                          uint64_t* __restrict__ status, uint32_t tag,
-                                                         uint32_t out_shift = 0, uint32_t kpos = 0) {
+                                                         uint32_t out_shift = 0, uint32_t kpos = 0,
+                                                         uint64_t out_limit = 0) {
+    static_assert(!LIMIT || !VALS, "an output limit applies to key-only passes");
     constexpr int TILE = RS_THREADS * ITEMS;
     constexpr bool SPLIT = VALS && (sizeof(VinT) == 1 || sizeof(VoutT) == 1);
     static_assert(!SPLIT || sizeof(KeyT) == 8, "split indices need 64-bit keys");
@@ -574,7 +578,7 @@ __global__ __launch_bounds__(RS_THREADS) void k_onesweep(Src src, OutT* __restri
             const KeyT kk = s_keys[i];
             dg[r] = (uint32_t)((uint64_t)kk >> shift) & 0xFFu;
             const uint32_t o = s_gofs[dg[r]] + i;
-            kout[o] = (OutT)((uint64_t)kk >> out_shift);
+            if (!LIMIT || (uint64_t)o < out_limit) kout[o] = (OutT)((uint64_t)kk >> out_shift);
             if constexpr (SPLIT) {
                 if constexpr (sizeof(VoutT) == 1)
                     vout[o] = s_vb[i];
@@ -1312,6 +1316,65 @@ struct Primitives {
     void sort_pairs(uint64_t*& keys, uint64_t*& keys_alt, uint32_t*& vals, uint32_t*& vals_alt, uint64_t N,
                     uint32_t lo_bit, uint32_t hi_bit) {
         sort_pairs_from(ArrayKeys<uint64_t>{keys}, keys, keys_alt, vals, vals_alt, N, lo_bit, hi_bit, false);
+    }
+
+    // Stable partition of N >= 2 u64 keys by bits [lo_bit, hi_bit) (lo_bit >= 32, higher bits equal): LSD
+    // passes over the keys alone, the last of which writes only each key's low word, to out32[0, out_limit)
+    // (the keys whose sorted index is out_limit or more are dropped).  keys and keys_alt are both
+    // overwritten.  The bucket id rides in the high word, the payload in the low word: 16 bytes per
+    // element in a middle pass, 12 in the last, instead of 24 for (key, value) pairs.
+    void sort_keys_low32(uint64_t* keys, uint64_t* keys_alt, uint64_t N, uint32_t lo_bit, uint32_t hi_bit,
+                         uint32_t* out32, uint64_t out_limit) {
+        if (N < 2 || lo_bit < 32 || hi_bit > 64) throw Error(-6, "sort_keys_low32: bad arguments");
+        if (hi_bit <= lo_bit) hi_bit = lo_bit + 1;
+        uint32_t np = (hi_bit - lo_bit + 7) / 8;
+        if (lo_bit + 8 * np > 64) lo_bit = 64 - 8 * np;
+        const uint64_t tiles = seg_plan(ArrayKeys<uint64_t>{keys}, N, lo_bit, np, RS_TILE);
+        uint64_t span_bytes = 0;
+        if (profile && !no_alloc) {
+            hipEvent_t e0 = nullptr, e1 = nullptr;
+            DCX_HIP(hipEventCreate(&e0));
+            if (hipEventCreate(&e1) != hipSuccess) {
+                (void)hipEventDestroy(e0);
+                throw Error(-4, "hipEventCreate failed");
+            }
+            pass_events.emplace_back(e0, e1);
+            pass_launches.push_back(0);
+            pass_items.push_back(0);
+            pass_bytes.push_back(0);
+            DCX_HIP(hipEventRecord(e0, stream));
+        }
+        for (uint32_t p = 0; p < np; p++) {
+            if (sort_tag >= LB_TAG_LIMIT) {
+                fill_u64(sort_status, 0, sort_status_words);
+                sort_tag = 0;
+            }
+            const uint32_t tag = ++sort_tag;
+            const uint32_t shift = lo_bit + 8 * p;
+            const uint64_t* so = seg_off + p * RS_SEG_BINS;
+            const uint64_t* si = seg_info + p * RS_SEG_INFO;
+            uint32_t* tk = seg_tick + p * RS_SEGS;
+            const dim3 g((unsigned)tiles), b(RS_THREADS);
+            const ArrayKeys<uint64_t> in{keys};
+            if (p + 1 < np) {
+                hipLaunchKernelGGL((k_onesweep<false, false, ArrayKeys<uint64_t>>), g, b, 0, stream, in, keys_alt,
+                                   (const uint32_t*)nullptr, (uint32_t*)nullptr, shift, so, si, tk, sort_status, tag, 0u);
+                std::swap(keys, keys_alt);
+                span_bytes += 16ull * N;
+            } else {
+                hipLaunchKernelGGL((k_onesweep<false, false, ArrayKeys<uint64_t>, uint64_t, RS_ITEMS, uint32_t, uint32_t,
+                                               uint32_t, true>),
+                                   g, b, 0, stream, in, out32, (const uint32_t*)nullptr, (uint32_t*)nullptr, shift, so,
+                                   si, tk, sort_status, tag, 0u, 0u, out_limit);
+                span_bytes += 8ull * N + 4ull * std::min<uint64_t>(N, out_limit);
+            }
+        }
+        if (profile && !no_alloc) {
+            DCX_HIP(hipEventRecord(pass_events.back().second, stream));
+            pass_launches.back() = np;
+            pass_items.back() = (uint64_t)N * np;
+            pass_bytes.back() = span_bytes;
+        }
     }
 
     // Stable LSD radix sort of (u32 key, u32 value) pairs by key bits [lo_bit, lo_bit + 8 * np) (np <= 4).
