@@ -16,6 +16,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <functional>
 #include <mutex>
 #include <string>
 #include <thread>
@@ -837,8 +838,14 @@ class Engine {
     // -------------------------------------------------------------------------------------------
     // order-0 entropy (bits per character, within [1, bits]) of a byte text from a strided sample of its
     // codes; names of deeper levels are taken at their full width
-    double info_bits(const uint8_t* text, const uint16_t* lut, uint64_t n, uint32_t bits) {
+    double info_bits(const uint8_t* text, const uint16_t* lut, uint64_t n, uint32_t bits, double entropy = -1.0) {
         if (n < (1u << 24)) return bits;
+        const double e = entropy >= 0 ? entropy : code_entropy(text, lut, n);
+        return std::min<double>(bits, std::max(1.0, e));
+    }
+    double info_bits(const uint32_t*, const uint16_t*, uint64_t, uint32_t bits, double = -1.0) { return bits; }
+    // order-0 entropy of the codes lut[text[i]] over a strided sample of i
+    double code_entropy(const uint8_t* text, const uint16_t* lut, uint64_t n) {
         const uint64_t stride = 61;  // odd: no alignment with periodic structure
         DBuf<unsigned long long> hist(cache, 257);
         prim.fill_u64(reinterpret_cast<uint64_t*>(hist.get()), 0, 257);
@@ -852,9 +859,8 @@ class Engine {
                 const double p = (double)h[i] / total;
                 e -= p * std::log2(p);
             }
-        return std::min<double>(bits, std::max(1.0, e));
+        return e;
     }
-    double info_bits(const uint32_t*, const uint16_t*, uint64_t, uint32_t bits) { return bits; }
 
     // the level's characters as a packed code stream (k_pack_text) of `chars` characters
     template <class CharT>
@@ -876,10 +882,14 @@ class Engine {
     }
 
     // characters of the level are 1..maxchar (0 is the padding)
+    using StreamFn = std::function<void(uint32_t bits, uint32_t dense, uint64_t chars, DBuf<uint64_t>& PT)>;
     template <class CharT>
     // The SA goes to sa_out, or with sa_take != nullptr into *sa_take (the level's own buffer, no copy).
+    // A caller that has the level's text only as pieces (a batch) passes `stream_fn`, which packs the
+    // level's character stream of the given width, dense flag and length, and `entropy` (of the codes).
     void level(const CharT* text, const uint16_t* lut, uint64_t n, uint32_t maxchar, int depth, uint32_t* sa_out,
-               hipEvent_t* ev, DBuf<uint32_t>* sa_take = nullptr) {
+               hipEvent_t* ev, DBuf<uint32_t>* sa_take = nullptr, const StreamFn* stream_fn = nullptr,
+               double entropy = -1.0) {
         const uint32_t X = dcs->X, D = dcs->D;
         const uint32_t bits = bit_width(maxchar);
         // dense merge words (c -> c - 1) when they save a bit per character
@@ -940,8 +950,11 @@ class Engine {
         if (depth == 0) mark(1);
         {
             DBuf<uint64_t> PT;
-            make_stream(text, lut, n, mbits, dense ? 1 : 0, n + Ls + 2, PT);
-            refine_sort(PT.get(), n, Ls, mbits, dense, info_bits(text, lut, n, mbits), order, H, tied);
+            if (stream_fn)
+                (*stream_fn)(mbits, dense ? 1 : 0, n + Ls + 2, PT);
+            else
+                make_stream(text, lut, n, mbits, dense ? 1 : 0, n + Ls + 2, PT);
+            refine_sort(PT.get(), n, Ls, mbits, dense, info_bits(text, lut, n, mbits, entropy), order, H, tied);
             if (dense && tied.A > 0) boundary_fixup(PT.get(), mbits, Ls, n, dc.bnd, order, H, tied);
         }
 
@@ -1042,7 +1055,10 @@ class Engine {
                 throw Error(DCX_E_UNSUPPORTED, "a text of the batch is too long for 32-bit positions");
         }
         if (total && (!texts || !sa_out)) throw Error(DCX_E_ARG, "null argument");
-        uint64_t limit = o->batch_max_chars ? o->batch_max_chars : (1ull << 31);
+        // sub-batches of 2^26 characters by default: a shorter joined string needs fewer key bits in
+        // round 0 and its texts fewer partition passes (C4 at 2^31: 381 ms, 2^27: 348, 2^26: 343, 2^24: 365,
+        // where fixed costs per sub-batch start to dominate)
+        uint64_t limit = o->batch_max_chars ? o->batch_max_chars : (1ull << 26);
         if (limit > MAX_TEXT) limit = MAX_TEXT;  // the joined string, its separators included
         if (budget) {
             const uint64_t by_budget = budget > (64ull << 20) ? (budget - (64ull << 20)) / 64 : 0;
@@ -1078,6 +1094,10 @@ class Engine {
             ident = DBuf<uint16_t>(cache, 256);
             DCX_HIP(hipMemcpyAsync(ident.get(), id, sizeof(id), hipMemcpyHostToDevice, stream));
         }
+        // entropy of the codes, once for the batch (it sizes round 0 of the sub-batches of 2^24 characters
+        // and more; shorter ones key at full width)
+        const double entropy = total >= (1ull << 24) ? code_entropy(d_texts, dlut.get(), total) : 8.0;
+        const uint64_t base_threshold = opt.base_case_chars > 2ull * opt.X ? opt.base_case_chars : 2ull * opt.X;
         uint32_t k0 = 0;
         while (k0 < count) {
             // greedy sub-batch: at least one text, at most `limit` joined characters
@@ -1098,13 +1118,27 @@ class Engine {
             DCX_HIP(hipMemcpyAsync(d_co.get(), co.data(), (cnt + 1) * 8, hipMemcpyHostToDevice, stream));
             const uint64_t N = joined;
             const uint64_t chars = offsets[k1] - offsets[k0];
-            // owner table of the input positions
-            const uint64_t in_chunks = (chars >> OWNER_SHIFT) + 1;
-            DBuf<uint32_t> t_in(cache, in_chunks + 1);
-            k_owner_table<<<grid(in_chunks + 1), 256, 0, stream>>>(d_in.get(), cnt, in_off[0], in_chunks, t_in.get());
+            // joined text (short sub-batches and 32-bit characters): owner table of the input positions
+            const auto owner_table_in = [&]() {
+                const uint64_t in_chunks = (chars >> OWNER_SHIFT) + 1;
+                DBuf<uint32_t> t(cache, in_chunks + 1);
+                k_owner_table<<<grid(in_chunks + 1), 256, 0, stream>>>(d_in.get(), cnt, in_off[0], in_chunks, t.get());
+                return t;
+            };
             DBuf<uint32_t> sa;
             sigma = maxchar;
-            if (maxchar <= 255) {
+            if (maxchar <= 255 && N > base_threshold) {
+                // the level's character stream straight from the texts (no joined text)
+                const StreamFn fn = [&](uint32_t bits, uint32_t dense, uint64_t stream_chars, DBuf<uint64_t>& PT) {
+                    const uint64_t nwords = stream_words(stream_chars, bits);
+                    PT = DBuf<uint64_t>(cache, nwords);
+                    const uint64_t chunks = ceil_div(nwords, (uint64_t)PK_CHARS * bits / 64);
+                    k_batch_pack<<<(unsigned)std::min<uint64_t>(chunks, 16384), 256, 0, stream>>>(
+                        d_texts, d_in.get(), d_co.get(), cnt, N, dlut.get(), bits, dense, nwords, PT.get());
+                };
+                level<uint8_t>(nullptr, ident.get(), N, maxchar, 0, nullptr, nullptr, &sa, &fn, entropy);
+            } else if (maxchar <= 255) {
+                DBuf<uint32_t> t_in = owner_table_in();
                 DBuf<uint8_t> jt(cache, (size_t)N + 16);
                 if (chars)
                     k_batch_text<uint8_t><<<grid(chars), 256, 0, stream>>>(d_texts, in_off[0], in_off[cnt], d_in.get(),
@@ -1113,6 +1147,7 @@ class Engine {
                 k_batch_seps<uint8_t><<<grid(cnt), 256, 0, stream>>>(d_co.get(), cnt, jt.get());
                 level<uint8_t>(jt.get(), ident.get(), N, maxchar, 0, nullptr, nullptr, &sa);
             } else {
+                DBuf<uint32_t> t_in = owner_table_in();
                 DBuf<uint32_t> jt(cache, (size_t)N + 16);
                 if (chars)
                     k_batch_text<uint32_t><<<grid(chars), 256, 0, stream>>>(d_texts, in_off[0], in_off[cnt],

@@ -1469,7 +1469,137 @@ __global__ __launch_bounds__(256) void k_batch_seps(const uint64_t* __restrict__
     for (uint64_t k = blockIdx.x * 256ull + threadIdx.x; k < count; k += gridDim.x * 256ull) out[co[k + 1] - 1] = (OutT)1;
 }
 
-// (text, local position) of every suffix of the joined text in SA order; separators go to bucket `count`
+// The character stream of a joined sub-batch straight from the input texts (no joined text in HBM): a
+// workgroup packs PK_CHARS joined positions as k_pack_text does.  Joined characters: a text byte b is
+// lut[b] + 1, the separator after each text 1, positions from N on are padding 0; the stream code is the
+// character minus `dense` (padding stays 0).  Codes go through LDS one byte each, so the largest
+// character must be at most 255.  A thread fills 16 positions at a time; when they lie inside one text
+// (the common case) their bytes come from two aligned 16-byte loads (an aligned chunk that holds a text
+// byte lies inside the input's pages) instead of 16 byte loads.  The texts of the chunk are found in LDS
+// (at most BP_TEXTS of them; a chunk that meets more searches the global starts).
+constexpr uint32_t BP_TEXTS = 256;
+__global__ __launch_bounds__(256) void k_batch_pack(const uint8_t* __restrict__ texts,
+                                                    const uint64_t* __restrict__ in_off,
+                                                    const uint64_t* __restrict__ co, uint32_t count, uint64_t N,
+                                                    const uint16_t* __restrict__ lut, uint32_t bits, uint32_t dense,
+                                                    uint64_t nwords, uint64_t* __restrict__ PT) {
+    __shared__ __align__(16) uint8_t s_c[PK_CHARS];
+    __shared__ uint16_t s_lut[256];
+    __shared__ uint64_t s_co[BP_TEXTS + 1];  // joined starts of the chunk's texts
+    __shared__ uint64_t s_in[BP_TEXTS];      // their input offsets
+    __shared__ uint32_t s_k0, s_nk;
+    const uint32_t tid = threadIdx.x;
+    s_lut[tid] = lut[tid];
+    const uint32_t wpb = PK_CHARS * bits / 64;
+    for (uint64_t chunk = blockIdx.x; chunk * wpb < nwords; chunk += gridDim.x) {
+        const uint64_t q0 = chunk * PK_CHARS;
+        if (tid == 0) {
+            uint32_t k0 = 0, nk = 0;
+            if (q0 < N) {
+                k0 = find_owner(co, count, q0);
+                const uint64_t qe = q0 + PK_CHARS < N ? q0 + PK_CHARS : N;
+                nk = find_owner(co, count, qe - 1) - k0 + 1;  // > BP_TEXTS: search the global starts
+            }
+            s_k0 = k0;
+            s_nk = nk;
+        }
+        __syncthreads();
+        const uint32_t k0 = s_k0, nk = s_nk;
+        const bool local = nk <= BP_TEXTS;
+        if (local)
+            for (uint32_t i = tid; i <= nk; i += 256) {
+                s_co[i] = co[k0 + i];
+                if (i < nk) s_in[i] = in_off[k0 + i];
+            }
+        __syncthreads();
+        for (uint32_t g = tid; g < PK_CHARS / 16; g += 256) {
+            const uint64_t q = q0 + 16ull * g;
+            uint32_t c[16];
+            // owner of q: the last text that starts at or before it
+            uint32_t k = 0;
+            uint64_t cs = 0, ce = 0, io = 0;  // the owner's joined start, separator position, input offset
+            if (q < N) {
+                if (local) {
+                    uint32_t lo = 0, hi = nk;
+                    while (hi - lo > 1) {
+                        const uint32_t mid = (lo + hi) >> 1;
+                        if (s_co[mid] <= q)
+                            lo = mid;
+                        else
+                            hi = mid;
+                    }
+                    k = lo;
+                    cs = s_co[k];
+                    ce = s_co[k + 1] - 1;
+                    io = s_in[k];
+                } else {
+                    k = find_owner(co, count, q);
+                    cs = co[k];
+                    ce = co[k + 1] - 1;
+                    io = in_off[k];
+                }
+            }
+            if (q + 16 <= ce) {
+                // 16 text bytes at input address a: two aligned 16-byte chunks, then byte alignment
+                // (v_alignbyte) of the 4 words that start at the right one
+                const uintptr_t a = reinterpret_cast<uintptr_t>(texts + io + (q - cs));
+                const uint4* p = reinterpret_cast<const uint4*>(a & ~(uintptr_t)15);
+                const uint32_t sh = (uint32_t)(a & 15), ws = sh >> 2, bs = sh & 3;
+                const uint4 x = p[0];
+                const uint4 y = sh ? p[1] : make_uint4(0, 0, 0, 0);
+                const uint32_t t[8] = {x.x, x.y, x.z, x.w, y.x, y.y, y.z, y.w};
+                uint32_t o[5];
+#pragma unroll
+                for (int j = 0; j < 5; j++)
+                    o[j] = ws == 0 ? t[j] : ws == 1 ? t[j + 1] : ws == 2 ? t[j + 2] : (j + 3 < 8 ? t[j + 3] : 0u);
+#pragma unroll
+                for (int j = 0; j < 4; j++) {
+                    const uint32_t wd = __builtin_amdgcn_alignbyte(o[j + 1], o[j], bs);
+#pragma unroll
+                    for (int b = 0; b < 4; b++) c[4 * j + b] = (uint32_t)s_lut[(wd >> (8 * b)) & 0xFFu] + 1u - dense;
+                }
+            } else {
+                uint32_t kk = k;
+                uint64_t ks = cs, ke = ce, ki = io;
+                for (int i = 0; i < 16; i++) {
+                    const uint64_t pos = q + i;
+                    uint32_t ch = 0;
+                    if (pos < N) {
+                        while (pos > ke) {  // the next text (still one of the chunk's texts when local)
+                            kk++;
+                            ks = local ? s_co[kk] : co[kk];
+                            ke = (local ? s_co[kk + 1] : co[kk + 1]) - 1;
+                            ki = local ? s_in[kk] : in_off[kk];
+                        }
+                        ch = (pos == ke ? 1u : (uint32_t)s_lut[texts[ki + (pos - ks)]] + 1u) - dense;
+                    }
+                    c[i] = ch;
+                }
+            }
+            uint4 v;
+            v.x = c[0] | (c[1] << 8) | (c[2] << 16) | (c[3] << 24);
+            v.y = c[4] | (c[5] << 8) | (c[6] << 16) | (c[7] << 24);
+            v.z = c[8] | (c[9] << 8) | (c[10] << 16) | (c[11] << 24);
+            v.w = c[12] | (c[13] << 8) | (c[14] << 16) | (c[15] << 24);
+            reinterpret_cast<uint4*>(s_c)[g] = v;
+        }
+        __syncthreads();
+        for (uint32_t wl = tid; wl < wpb; wl += 256) {
+            const uint64_t w = chunk * wpb + wl;
+            if (w >= nwords) break;
+            uint64_t word = 0;
+            const uint32_t qa = (64 * wl) / bits, qb = (64 * wl + 63) / bits;  // characters overlapping the word
+            for (uint32_t qq = qa; qq <= qb; qq++) {
+                const int32_t pos = (int32_t)(64 * (wl + 1)) - (int32_t)((qq + 1) * bits);
+                const uint64_t cc = s_c[qq];
+                word |= pos >= 0 ? (cc << pos) : (cc >> (-pos));
+            }
+            PT[w] = word;
+        }
+        __syncthreads();
+    }
+}
+
 // Partition keys of the joined SA for at most BK_MAX_TEXTS texts: (text << 32) | position within the
 // text, a separator keyed (count << 32) so that it sorts after every text.  Every workgroup holds the
 // texts' separator positions in LDS (32-bit: a joined string has at most 2^32 positions) and a table of
