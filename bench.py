@@ -137,18 +137,19 @@ def bench_batch(args):
     dt = torch.from_numpy(text).cuda()
     dsa = torch.empty(total, dtype=torch.int32, device="cuda")
     eng = Engine(0)
-    # X by measurement on the first 10% of the texts (every X gives the same SAs)
-    from distributed_suffix_sorting_evaluation_amd.autotune import choose_x_batch
+    # X and the sub-batch size by measurement on the first 10% of the texts (the SAs depend on neither)
+    from distributed_suffix_sorting_evaluation_amd.autotune import choose_batch_params
     k = max(1, count // 10)
-    x, x_ms = choose_x_batch(eng, dt.data_ptr(), off[:k + 1], dsa.data_ptr(), pack_samples=True, pack_merging=True)
-    log(f"[bench] autotuned X={x} on {k} texts: {x_ms}")
+    x, size, x_ms, size_ms = choose_batch_params(eng, dt.data_ptr(), off[:k + 1], dsa.data_ptr(),
+                                                 pack_samples=True, pack_merging=True)
+    log(f"[bench] autotuned X={x} sub-batch={size} on {k} texts: {x_ms} {size_ms}")
     kw = dict(X=x, pack_samples=True, pack_merging=True)
     for _ in range(args.warmup):
-        eng.build_sa_batch_device(dt.data_ptr(), off, dsa.data_ptr(), **kw)
+        eng.build_sa_batch_device(dt.data_ptr(), off, dsa.data_ptr(), batch_max_chars=size, **kw)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     for _ in range(args.steps):
-        eng.build_sa_batch_device(dt.data_ptr(), off, dsa.data_ptr(), **kw)
+        eng.build_sa_batch_device(dt.data_ptr(), off, dsa.data_ptr(), batch_max_chars=size, **kw)
     torch.cuda.synchronize()
     batched_s = (time.perf_counter() - t0) / args.steps
     # plain path on an evenly spaced sample of the texts
@@ -172,8 +173,9 @@ def bench_batch(args):
             "warmup": args.warmup, "ms_per_step": round(batched_s * 1e3, 2), "higher_is_better": True,
             "scaling": "weak", "vs_baseline": None, "dtype": "u8", "data": "synthetic",
             "config": {"workload": "batch_acgtn", "texts": count, "total_bytes": total, "lengths": "log-uniform 1 KiB-4 MiB",
-                       "alphabet": "ACGT + N at 0.1%", "dcx": f"dc{x} (autotuned)", "sa_width_bytes": 4},
-            "autotune_ms_per_x": x_ms,
+                       "alphabet": "ACGT + N at 0.1%", "dcx": f"dc{x} (autotuned)", "sa_width_bytes": 4,
+                       "streams": 2},
+            "autotune_ms_per_x": x_ms, "autotune_ms_per_sub_batch_chars": size_ms, "sub_batch_chars": size,
             "batched_mb_s": round(total / 1e6 / batched_s, 1),
             "plain": {"texts_per_s": round(plain_rate, 1), "sample_texts": int(len(sample)),
                       "mb_s": round(sample_chars / 1e6 / plain_s, 1)},

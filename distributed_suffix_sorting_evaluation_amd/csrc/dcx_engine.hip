@@ -17,6 +17,7 @@
 #include <cstdlib>
 #include <cstring>
 #include <functional>
+#include <memory>
 #include <mutex>
 #include <string>
 #include <thread>
@@ -123,6 +124,9 @@ class Engine {
     dcx_stats* st = nullptr;
     uint64_t alg_bytes = 0;
     uint32_t sa_width = 8;
+    // the second stream of batched calls (its own engine: stream, device cache, sort state), made on first
+    // use by a batch of several sub-batches
+    std::unique_ptr<Engine> twin;
 
     // pinned staging ring for copies between device memory and pageable host memory (allocated on the
     // first large copy).  The stage size is 64 MiB; the environment variable DCX_STAGE_BYTES (read when
@@ -1055,10 +1059,11 @@ class Engine {
                 throw Error(DCX_E_UNSUPPORTED, "a text of the batch is too long for 32-bit positions");
         }
         if (total && (!texts || !sa_out)) throw Error(DCX_E_ARG, "null argument");
-        // sub-batches of 2^26 characters by default: a shorter joined string needs fewer key bits in
-        // round 0 and its texts fewer partition passes (C4 at 2^31: 381 ms, 2^27: 348, 2^26: 343, 2^24: 365,
-        // where fixed costs per sub-batch start to dominate)
-        uint64_t limit = o->batch_max_chars ? o->batch_max_chars : (1ull << 26);
+        // sub-batches of 2^24 characters by default: a shorter joined string needs fewer key bits in
+        // round 0 and its texts fewer partition passes, and with two sub-batches in flight the fixed costs
+        // of each overlap with the other's kernels (10,000 ACGTN texts, 5 GB, X = 39: 2^31 381 ms, 2^27
+        // 299, 2^26 288, 2^25 294, 2^24 273, 2^23 299, 2^22 335)
+        uint64_t limit = o->batch_max_chars ? o->batch_max_chars : (1ull << 24);
         if (limit > MAX_TEXT) limit = MAX_TEXT;  // the joined string, its separators included
         if (budget) {
             const uint64_t by_budget = budget > (64ull << 20) ? (budget - (64ull << 20)) / 64 : 0;
@@ -1097,91 +1102,70 @@ class Engine {
         // entropy of the codes, once for the batch (it sizes round 0 of the sub-batches of 2^24 characters
         // and more; shorter ones key at full width)
         const double entropy = total >= (1ull << 24) ? code_entropy(d_texts, dlut.get(), total) : 8.0;
-        const uint64_t base_threshold = opt.base_case_chars > 2ull * opt.X ? opt.base_case_chars : 2ull * opt.X;
-        uint32_t k0 = 0;
-        while (k0 < count) {
-            // greedy sub-batch: at least one text, at most `limit` joined characters
-            uint32_t k1 = k0 + 1;
-            uint64_t joined = offsets[k1] - offsets[k0] + 1;
+        // sub-batches: greedy, at least one text, at most `limit` joined characters each
+        std::vector<uint32_t> cuts{0};
+        while (cuts.back() < count) {
+            uint32_t k1 = cuts.back() + 1;
+            uint64_t joined = offsets[k1] - offsets[k1 - 1] + 1;
             while (k1 < count && joined + (offsets[k1 + 1] - offsets[k1]) + 1 <= limit) {
                 joined += offsets[k1 + 1] - offsets[k1] + 1;
                 k1++;
             }
-            const uint32_t cnt = k1 - k0;
-            std::vector<uint64_t> in_off(cnt + 1), co(cnt + 1);
-            for (uint32_t k = 0; k <= cnt; k++) {
-                in_off[k] = offsets[k0 + k] - base;
-                co[k] = (offsets[k0 + k] - offsets[k0]) + k;
-            }
-            DBuf<uint64_t> d_in(cache, cnt + 1), d_co(cache, cnt + 1);
-            DCX_HIP(hipMemcpyAsync(d_in.get(), in_off.data(), (cnt + 1) * 8, hipMemcpyHostToDevice, stream));
-            DCX_HIP(hipMemcpyAsync(d_co.get(), co.data(), (cnt + 1) * 8, hipMemcpyHostToDevice, stream));
-            const uint64_t N = joined;
-            const uint64_t chars = offsets[k1] - offsets[k0];
-            // joined text (short sub-batches and 32-bit characters): owner table of the input positions
-            const auto owner_table_in = [&]() {
-                const uint64_t in_chunks = (chars >> OWNER_SHIFT) + 1;
-                DBuf<uint32_t> t(cache, in_chunks + 1);
-                k_owner_table<<<grid(in_chunks + 1), 256, 0, stream>>>(d_in.get(), cnt, in_off[0], in_chunks, t.get());
-                return t;
+            cuts.push_back(k1);
+        }
+        const size_t nsub = cuts.size() - 1;
+        const BatchInput in{d_texts, offsets, base, maxchar, dlut.get(), ident.get(), entropy, o->sa_on_device != 0,
+                            sa_out};
+        // Two sub-batches in flight when there are several: a second engine on its own stream (created on
+        // first use and kept with the handle) takes every other one from a shared counter, in a host thread
+        // of its own, so one sub-batch's kernels run while the other waits for a readback.  Both write
+        // disjoint parts of sa_out.  Not with an HBM budget that cannot hold two sub-batches.
+        const bool two = nsub >= 2 && (!budget || budget >= 2 * (64 * limit + (64ull << 20)));
+        if (!two) {
+            for (size_t i = 0; i < nsub; i++) sub_batch(in, cuts[i], cuts[i + 1]);
+        } else {
+            if (!twin) twin.reset(new Engine(device));
+            twin->set_options(o);
+            twin->st = nullptr;
+            twin->alg_bytes = 0;
+            twin->sa_width = 4;
+            twin->cache.reset_peak();
+            DCX_HIP(hipStreamSynchronize(stream));  // the texts, the map and the identity map are in place
+            std::atomic<size_t> next{0};
+            std::atomic<bool> failed{false};
+            int twin_code = DCX_OK;
+            std::string twin_msg;
+            const auto take = [&](Engine& e) {
+                for (size_t i; !failed.load() && (i = next.fetch_add(1)) < nsub;) e.sub_batch(in, cuts[i], cuts[i + 1]);
             };
-            DBuf<uint32_t> sa;
-            sigma = maxchar;
-            if (maxchar <= 255 && N > base_threshold) {
-                // the level's character stream straight from the texts (no joined text)
-                const StreamFn fn = [&](uint32_t bits, uint32_t dense, uint64_t stream_chars, DBuf<uint64_t>& PT) {
-                    const uint64_t nwords = stream_words(stream_chars, bits);
-                    PT = DBuf<uint64_t>(cache, nwords);
-                    const uint64_t chunks = ceil_div(nwords, (uint64_t)PK_CHARS * bits / 64);
-                    k_batch_pack<<<(unsigned)std::min<uint64_t>(chunks, 16384), 256, 0, stream>>>(
-                        d_texts, d_in.get(), d_co.get(), cnt, N, dlut.get(), bits, dense, nwords, PT.get());
-                };
-                level<uint8_t>(nullptr, ident.get(), N, maxchar, 0, nullptr, nullptr, &sa, &fn, entropy);
-            } else if (maxchar <= 255) {
-                DBuf<uint32_t> t_in = owner_table_in();
-                DBuf<uint8_t> jt(cache, (size_t)N + 16);
-                if (chars)
-                    k_batch_text<uint8_t><<<grid(chars), 256, 0, stream>>>(d_texts, in_off[0], in_off[cnt], d_in.get(),
-                                                                            d_co.get(), cnt, dlut.get(), t_in.get(),
-                                                                            jt.get());
-                k_batch_seps<uint8_t><<<grid(cnt), 256, 0, stream>>>(d_co.get(), cnt, jt.get());
-                level<uint8_t>(jt.get(), ident.get(), N, maxchar, 0, nullptr, nullptr, &sa);
-            } else {
-                DBuf<uint32_t> t_in = owner_table_in();
-                DBuf<uint32_t> jt(cache, (size_t)N + 16);
-                if (chars)
-                    k_batch_text<uint32_t><<<grid(chars), 256, 0, stream>>>(d_texts, in_off[0], in_off[cnt],
-                                                                             d_in.get(), d_co.get(), cnt, dlut.get(),
-                                                                             t_in.get(), jt.get());
-                k_batch_seps<uint32_t><<<grid(cnt), 256, 0, stream>>>(d_co.get(), cnt, jt.get());
-                level<uint32_t>(jt.get(), nullptr, N, maxchar, 0, nullptr, nullptr, &sa);
-            }
-            // stable partition of the joined SA by text: keys (text << 32 | position in the text), the
-            // separators keyed past the last text; the last pass keeps the low words of the first `chars`
-            // keys, written straight into a device SA
-            if (chars) {
-                DBuf<uint64_t> ka(cache, N), kb(cache, N);
-                if (cnt <= BK_MAX_TEXTS) {
-                    uint32_t cshift = 0;
-                    while (((N - 1) >> cshift) + 1 > BK_TABLE) cshift++;
-                    k_batch_keys_lds<<<2 * prim.cus, BK_THREADS, 0, stream>>>(sa.get(), N, d_co.get(), cnt, cshift,
-                                                                              ka.get());
-                } else {
-                    const uint64_t co_chunks = ((uint64_t)N >> OWNER_SHIFT) + 1;
-                    DBuf<uint32_t> t_co(cache, co_chunks + 1);
-                    k_owner_table<<<grid(co_chunks + 1), 256, 0, stream>>>(d_co.get(), cnt, 0, co_chunks, t_co.get());
-                    k_batch_keys64<<<grid(N), 256, 0, stream>>>(sa.get(), N, d_co.get(), cnt, t_co.get(), ka.get());
+            std::thread worker([&] {
+                try {
+                    DCX_HIP(hipSetDevice(device));
+                    take(*twin);
+                } catch (const Error& e) {
+                    twin_code = e.code;
+                    twin_msg = e.what();
+                    failed = true;
+                } catch (const std::exception& e) {
+                    twin_code = DCX_E_HIP;
+                    twin_msg = e.what();
+                    failed = true;
                 }
-                sa.reset();
-                uint32_t* dst = sa_out + (offsets[k0] - base);
-                DBuf<uint32_t> staged;
-                if (!o->sa_on_device) staged = DBuf<uint32_t>(cache, chars);
-                uint32_t* out = o->sa_on_device ? dst : staged.get();
-                prim.sort_keys_low32(ka.get(), kb.get(), N, 32, 32 + bit_width(cnt), out, chars);
-                if (!o->sa_on_device) download(dst, out, chars * 4);
-            }
-            DCX_HIP(hipStreamSynchronize(stream));
-            k0 = k1;
+            });
+            struct Join {
+                std::thread& t;
+                std::atomic<bool>& f;
+                ~Join() {
+                    if (t.joinable()) {
+                        f = true;
+                        t.join();
+                    }
+                }
+            } join{worker, failed};
+            take(*this);
+            worker.join();
+            if (twin_code != DCX_OK) throw Error(twin_code, "second sub-batch stream: " + twin_msg);
+            alg_bytes += twin->alg_bytes;
         }
         DCX_HIP(hipEventRecord(e1, stream));
         DCX_HIP(hipStreamSynchronize(stream));
@@ -1191,10 +1175,109 @@ class Engine {
             st->ms_device = a;
             st->alg_bytes = alg_bytes;
             if (prim.profile) prim.collect_profile(&st->sort_passes, &st->sort_pass_items, &st->sort_pass_bytes, &st->sort_pass_ms);
-            st->peak_device_bytes = cache.peak();
+            st->peak_device_bytes = cache.peak() + (twin ? twin->cache.peak() : 0);
             st->ms_total = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
         }
         return DCX_OK;
+    }
+
+    // the inputs every sub-batch of one batched call shares (device pointers of the calling engine, read only)
+    struct BatchInput {
+        const uint8_t* d_texts;   // the batch's bytes, text k at d_texts + offsets[k] - base
+        const uint64_t* offsets;  // host
+        uint64_t base;
+        uint32_t maxchar;         // joined characters: separator 1, text bytes lut[b] + 1 <= maxchar
+        const uint16_t* lut;
+        const uint16_t* ident;    // identity map (maxchar <= 255)
+        double entropy;
+        bool sa_on_device;
+        uint32_t* sa_out;
+    };
+
+    // One sub-batch, texts [k0, k1): joined with separators, one DCX pipeline, partition by text into
+    // sa_out.  Returns once its part of sa_out is written.
+    void sub_batch(const BatchInput& in, uint32_t k0, uint32_t k1) {
+        const uint8_t* d_texts = in.d_texts;
+        const uint64_t* offsets = in.offsets;
+        const uint64_t base = in.base;
+        const uint32_t maxchar = in.maxchar;
+        const uint64_t base_threshold = opt.base_case_chars > 2ull * opt.X ? opt.base_case_chars : 2ull * opt.X;
+        uint64_t joined = 0;
+        for (uint32_t k = k0; k < k1; k++) joined += offsets[k + 1] - offsets[k] + 1;
+        const uint32_t cnt = k1 - k0;
+        std::vector<uint64_t> in_off(cnt + 1), co(cnt + 1);
+        for (uint32_t k = 0; k <= cnt; k++) {
+            in_off[k] = offsets[k0 + k] - base;
+            co[k] = (offsets[k0 + k] - offsets[k0]) + k;
+        }
+        DBuf<uint64_t> d_in(cache, cnt + 1), d_co(cache, cnt + 1);
+        DCX_HIP(hipMemcpyAsync(d_in.get(), in_off.data(), (cnt + 1) * 8, hipMemcpyHostToDevice, stream));
+        DCX_HIP(hipMemcpyAsync(d_co.get(), co.data(), (cnt + 1) * 8, hipMemcpyHostToDevice, stream));
+        const uint64_t N = joined;
+        const uint64_t chars = offsets[k1] - offsets[k0];
+        // joined text (short sub-batches and 32-bit characters): owner table of the input positions
+        const auto owner_table_in = [&]() {
+            const uint64_t in_chunks = (chars >> OWNER_SHIFT) + 1;
+            DBuf<uint32_t> t(cache, in_chunks + 1);
+            k_owner_table<<<grid(in_chunks + 1), 256, 0, stream>>>(d_in.get(), cnt, in_off[0], in_chunks, t.get());
+            return t;
+        };
+        DBuf<uint32_t> sa;
+        sigma = maxchar;
+        if (maxchar <= 255 && N > base_threshold) {
+            // the level's character stream straight from the texts (no joined text)
+            const StreamFn fn = [&](uint32_t bits, uint32_t dense, uint64_t stream_chars, DBuf<uint64_t>& PT) {
+                const uint64_t nwords = stream_words(stream_chars, bits);
+                PT = DBuf<uint64_t>(cache, nwords);
+                const uint64_t chunks = ceil_div(nwords, (uint64_t)PK_CHARS * bits / 64);
+                k_batch_pack<<<(unsigned)std::min<uint64_t>(chunks, 16384), 256, 0, stream>>>(
+                    d_texts, d_in.get(), d_co.get(), cnt, N, in.lut, bits, dense, nwords, PT.get());
+            };
+            level<uint8_t>(nullptr, in.ident, N, maxchar, 0, nullptr, nullptr, &sa, &fn, in.entropy);
+        } else if (maxchar <= 255) {
+            DBuf<uint32_t> t_in = owner_table_in();
+            DBuf<uint8_t> jt(cache, (size_t)N + 16);
+            if (chars)
+                k_batch_text<uint8_t><<<grid(chars), 256, 0, stream>>>(d_texts, in_off[0], in_off[cnt], d_in.get(),
+                                                                        d_co.get(), cnt, in.lut, t_in.get(),
+                                                                        jt.get());
+            k_batch_seps<uint8_t><<<grid(cnt), 256, 0, stream>>>(d_co.get(), cnt, jt.get());
+            level<uint8_t>(jt.get(), in.ident, N, maxchar, 0, nullptr, nullptr, &sa);
+        } else {
+            DBuf<uint32_t> t_in = owner_table_in();
+            DBuf<uint32_t> jt(cache, (size_t)N + 16);
+            if (chars)
+                k_batch_text<uint32_t><<<grid(chars), 256, 0, stream>>>(d_texts, in_off[0], in_off[cnt],
+                                                                         d_in.get(), d_co.get(), cnt, in.lut,
+                                                                         t_in.get(), jt.get());
+            k_batch_seps<uint32_t><<<grid(cnt), 256, 0, stream>>>(d_co.get(), cnt, jt.get());
+            level<uint32_t>(jt.get(), nullptr, N, maxchar, 0, nullptr, nullptr, &sa);
+        }
+        // stable partition of the joined SA by text: keys (text << 32 | position in the text), the
+        // separators keyed past the last text; the last pass keeps the low words of the first `chars`
+        // keys, written straight into a device SA
+        if (chars) {
+            DBuf<uint64_t> ka(cache, N), kb(cache, N);
+            if (cnt <= BK_MAX_TEXTS) {
+                uint32_t cshift = 0;
+                while (((N - 1) >> cshift) + 1 > BK_TABLE) cshift++;
+                k_batch_keys_lds<<<2 * prim.cus, BK_THREADS, 0, stream>>>(sa.get(), N, d_co.get(), cnt, cshift,
+                                                                          ka.get());
+            } else {
+                const uint64_t co_chunks = ((uint64_t)N >> OWNER_SHIFT) + 1;
+                DBuf<uint32_t> t_co(cache, co_chunks + 1);
+                k_owner_table<<<grid(co_chunks + 1), 256, 0, stream>>>(d_co.get(), cnt, 0, co_chunks, t_co.get());
+                k_batch_keys64<<<grid(N), 256, 0, stream>>>(sa.get(), N, d_co.get(), cnt, t_co.get(), ka.get());
+            }
+            sa.reset();
+            uint32_t* dst = in.sa_out + (offsets[k0] - base);
+            DBuf<uint32_t> staged;
+            if (!in.sa_on_device) staged = DBuf<uint32_t>(cache, chars);
+            uint32_t* out = in.sa_on_device ? dst : staged.get();
+            prim.sort_keys_low32(ka.get(), kb.get(), N, 32, 32 + bit_width(cnt), out, chars);
+            if (!in.sa_on_device) download(dst, out, chars * 4);
+        }
+        DCX_HIP(hipStreamSynchronize(stream));
     }
 
     // -------------------------------------------------------------------------------------------
