@@ -137,12 +137,16 @@ def bench_batch(args):
     dt = torch.from_numpy(text).cuda()
     dsa = torch.empty(total, dtype=torch.int32, device="cuda")
     eng = Engine(0)
-    # X and the sub-batch size by measurement on the first 10% of the texts (the SAs depend on neither)
-    from distributed_suffix_sorting_evaluation_amd.autotune import choose_batch_params
-    k = max(1, count // 10)
-    x, size, x_ms, size_ms = choose_batch_params(eng, dt.data_ptr(), off[:k + 1], dsa.data_ptr(),
-                                                 pack_samples=True, pack_merging=True)
-    log(f"[bench] autotuned X={x} sub-batch={size} on {k} texts: {x_ms} {size_ms}")
+    if args.batch_x:
+        # fixed parameters (profiling runs): no tuning builds in the trace
+        x, size, x_ms, size_ms = args.batch_x, args.batch_chars, None, None
+    else:
+        # X and the sub-batch size by measurement on the first 10% of the texts (the SAs depend on neither)
+        from distributed_suffix_sorting_evaluation_amd.autotune import choose_batch_params
+        k = max(1, count // 10)
+        x, size, x_ms, size_ms = choose_batch_params(eng, dt.data_ptr(), off[:k + 1], dsa.data_ptr(),
+                                                     pack_samples=True, pack_merging=True)
+        log(f"[bench] autotuned X={x} sub-batch={size} on {k} texts: {x_ms} {size_ms}")
     kw = dict(X=x, pack_samples=True, pack_merging=True)
     for _ in range(args.warmup):
         eng.build_sa_batch_device(dt.data_ptr(), off, dsa.data_ptr(), batch_max_chars=size, **kw)
@@ -152,7 +156,7 @@ def bench_batch(args):
         eng.build_sa_batch_device(dt.data_ptr(), off, dsa.data_ptr(), batch_max_chars=size, **kw)
     torch.cuda.synchronize()
     batched_s = (time.perf_counter() - t0) / args.steps
-    # plain path on an evenly spaced sample of the texts
+    # plain path on an evenly spaced sample of the texts (--plain-sample 0: not measured)
     sample = np.linspace(0, count - 1, min(count, args.plain_sample)).astype(np.int64)
     out8 = torch.empty(int(lens.max()) * 4, dtype=torch.uint8, device="cuda")
     torch.cuda.synchronize()
@@ -162,24 +166,25 @@ def bench_batch(args):
         eng.build_sa_device(dt.data_ptr() + a, b - a, out8.data_ptr(), 4, **kw)
     torch.cuda.synchronize()
     plain_s = time.perf_counter() - t0
-    plain_rate = len(sample) / plain_s
+    plain_rate = len(sample) / plain_s if len(sample) else None
     sample_chars = int(sum(int(lens[k]) for k in sample))
     # validate a few texts of the batched output on the device
+    check = sample[:8] if len(sample) else np.linspace(0, count - 1, 8).astype(np.int64)
     valid = all(eng.check_sa_device(dt.data_ptr() + int(off[k]), int(lens[k]), dsa.data_ptr() + 4 * int(off[k]), 4)
-                for k in sample[:8])
+                for k in check)
     eng.close()
     rate = count / batched_s
     return {"metric": METRIC, "value": round(rate, 1), "unit": "SAs/s", "n_gpus": 1, "steps": args.steps,
             "warmup": args.warmup, "ms_per_step": round(batched_s * 1e3, 2), "higher_is_better": True,
             "scaling": "weak", "vs_baseline": None, "dtype": "u8", "data": "synthetic",
             "config": {"workload": "batch_acgtn", "texts": count, "total_bytes": total, "lengths": "log-uniform 1 KiB-4 MiB",
-                       "alphabet": "ACGT + N at 0.1%", "dcx": f"dc{x} (autotuned)", "sa_width_bytes": 4,
+                       "alphabet": "ACGT + N at 0.1%", "dcx": f"dc{x}" + ("" if args.batch_x else " (autotuned)"), "sa_width_bytes": 4,
                        "streams": 2},
             "autotune_ms_per_x": x_ms, "autotune_ms_per_sub_batch_chars": size_ms, "sub_batch_chars": size,
             "batched_mb_s": round(total / 1e6 / batched_s, 1),
             "plain": {"texts_per_s": round(plain_rate, 1), "sample_texts": int(len(sample)),
-                      "mb_s": round(sample_chars / 1e6 / plain_s, 1)},
-            "batched_over_plain": round(rate / plain_rate, 2), "sa_valid_sample": valid}
+                      "mb_s": round(sample_chars / 1e6 / plain_s, 1)} if plain_rate else None,
+            "batched_over_plain": round(rate / plain_rate, 2) if plain_rate else None, "sa_valid_sample": valid}
 
 
 def bench_serve(args):
@@ -424,6 +429,9 @@ def main():
     ap.add_argument("--no-other-configs", action="store_true")
     ap.add_argument("--batch-count", type=int, default=10000)
     ap.add_argument("--plain-sample", type=int, default=500)
+    ap.add_argument("--batch-x", type=int, default=0, help="batch_acgtn: this X instead of the autotuned one")
+    ap.add_argument("--batch-chars", type=int, default=0,
+                    help="batch_acgtn with --batch-x: characters per sub-batch (0: the engine's default)")
     ap.add_argument("--serve-count", type=int, default=2000)
     ap.add_argument("--serve-rate", type=float, default=200.0)
     ap.add_argument("--batch-window-ms", type=float, default=2.0)

@@ -48,7 +48,7 @@ typedef struct dcx_options {
     double discarding_threshold;  /* --discarding-threshold (0.7, config.hpp:42)             */
     uint64_t base_case_chars;     /* sequential base case: n <= max(2X, this) (pdcx.hpp:623)  */
     uint64_t seed;                /* --seed (accepted; the SA does not depend on it)         */
-    uint64_t batch_max_chars;     /* dcx_build_sa_batch: characters per sub-batch (0 = auto)  */
+    uint64_t batch_max_chars;     /* dcx_build_sa_batch: characters per sub-batch (0 = 2^24)  */
     uint64_t reserved[5];
 } dcx_options;
 
@@ -117,8 +117,10 @@ int dcx_build_sa(dcx_handle* h, const uint8_t* text, uint64_t n, const dcx_optio
  * at most 2^32 - 1 characters: with its separator it must fit 32-bit positions); its SA is written to sa_out[offsets[k] - offsets[0] .. offsets[k+1] - offsets[0])
  * as 32-bit positions local to the text (sa_out holds offsets[count] - offsets[0] entries).  The texts are joined with a separator symbol smaller than every character and run through one
  * DCX pipeline per sub-batch (at most batch_max_chars characters); a stable radix partition by text then
- * yields the per-text suffix arrays.  No counterpart in the reference: it builds one SA per cli run
- * (cli.cpp:747-820); this entry point serves callers with many texts at once. */
+ * yields the per-text suffix arrays.  With several sub-batches, two are built at once: a second stream of
+ * the handle (created on first use, kept until dcx_destroy) takes every other one in a host thread of the
+ * library's own; the call returns when both are done.  No counterpart in the reference: it builds one SA
+ * per cli run (cli.cpp:747-820); this entry point serves callers with many texts at once. */
 int dcx_build_sa_batch(dcx_handle* h, const uint8_t* texts, const uint64_t* offsets, uint32_t count,
                        const dcx_options* opt, uint32_t* sa_out, dcx_stats* stats);
 
