@@ -234,3 +234,44 @@ def test_serving_worker_survives_unexpected_errors():
     assert big.error == DCX_E_HIP and big.sa is None and "out of memory" in s.worker_errors[0]
     assert small.error == 0 and small.sa == "ok"
     assert s.free == s.budget and s.large_running == 0 and s.running == 0
+
+
+def test_autotune_searches_x_then_sub_batch_size(monkeypatch):
+    """choose_batch_params times every X at the engine's default sub-batch size, then every size at the
+    fastest X, and keeps the fastest; XTuner caches both per key.  A fake engine whose call costs follow a
+    known model (fastest at X = 21 and 2^24 characters) and a fake clock stand in for the GPU."""
+    from distributed_suffix_sorting_evaluation_amd import autotune
+
+    clock = [0.0]
+
+    class FakeTime:
+        @staticmethod
+        def perf_counter():
+            return clock[0]
+
+    class FakeEngine:
+        def __init__(self):
+            self.calls = []
+
+        def build_sa_batch_device(self, texts_ptr, offsets, sa_ptr, batch_max_chars=0, X=39, **kw):
+            self.calls.append((X, batch_max_chars))
+            size_cost = 0.0 if batch_max_chars in (0, 1 << 24) else 0.05 * abs(batch_max_chars.bit_length() - 25)
+            clock[0] += 1.0 + 0.01 * abs(X - 21) + size_cost
+
+    monkeypatch.setattr(autotune, "time", FakeTime)
+    eng = FakeEngine()
+    off = np.array([0, 10, 30], dtype=np.uint64)
+    x, size, x_ms, size_ms = autotune.choose_batch_params(eng, 0, off, 0, candidates=(7, 21, 39),
+                                                          sizes=(1 << 22, 1 << 24, 1 << 26), reps=2,
+                                                          batch_max_chars=123)
+    assert (x, size) == (21, 1 << 24)
+    assert set(x_ms) == {7, 21, 39} and set(size_ms) == {1 << 22, 1 << 24, 1 << 26}
+    assert x_ms[21] == 1000.0 and size_ms[1 << 22] == 1100.0
+    # three calls per candidate (one untimed); X at the default size, then the sizes at X = 21
+    assert eng.calls[:9] == [(7, 0)] * 3 + [(21, 0)] * 3 + [(39, 0)] * 3
+    assert eng.calls[9:] == [(21, 1 << 22)] * 3 + [(21, 1 << 24)] * 3 + [(21, 1 << 26)] * 3
+    tuner = autotune.XTuner(eng, candidates=(7, 21), reps=1, sizes=(1 << 22, 1 << 24))
+    assert tuner.x_for("acgt", 0, off, 0) == 21 and tuner.size_for("acgt") == 1 << 24
+    n = len(eng.calls)
+    assert tuner.x_for("acgt", 0, off, 0) == 21 and len(eng.calls) == n  # cached
+    assert autotune.XTuner(eng, candidates=(21,), reps=1).size_for("acgt") == 0
