@@ -19,7 +19,10 @@ a suffix array is unique, so a valid SA is bit-exact.  The same line carries:
   ``traffic`` is the HBM bytes per launch measured by rocprofv3 (FETCH_SIZE, WRITE_SIZE) in a separate
   profiled run, committed under profiles/ (named in ``traffic_source``);
 * ``cpu_baseline``: the multi-threaded host restatement of the same algorithm (oracle/host_dcx.cpp, every
-  host core) on a prefix of the same text, median of 5 runs after 1 warm-up.
+  host core) on a prefix of the same text, median of 5 runs after 1 warm-up;
+* ``batched_acgtn``: the batched workload (10,000 ACGTN texts, log-uniform 1 KiB - 4 MiB, autotuned X and
+  sub-batch size) in SAs/s against the plain per-text path, and ``serving_poisson``: the serving workload
+  (2,000 requests, Poisson 200/s) with continuous batching against plain one-at-a-time serving, p50/p99.
 
     python bench.py [--gpus 1] [--steps 3] [--warmup 1] [--workload zipf_4gib_dc39]
     python bench.py --workload batch_acgtn | serve_poisson
@@ -223,6 +226,8 @@ def bench_serve(args):
             ("plain_burst", dict(workers=1, batching=False), np.zeros(count)))
     if args.serve_runs == "cb":  # profiling: the continuous-batching Poisson run alone
         runs = runs[:1]
+    elif args.serve_runs == "poisson":  # the Poisson runs, without the bursts
+        runs = runs[:2]
     for name, kwargs, arr in runs:
         srv = SAServer(**common, **kwargs)
         try:
@@ -257,10 +262,12 @@ def bench_serve(args):
                        "sizes": "log-uniform 64 KiB-256 MiB", "alphabet": "4 or 256 (50/50)", "hbm_budget_bytes": 80 * 10 ** 9,
                        "dcx": "dc39", "sa_width_bytes": 8, "batch_window_ms": args.batch_window_ms},
             "p50_ms": cb["p50_ms"], "p99_ms": cb["p99_ms"], "continuous_batching": cb, "plain": pl,
-            "burst": {"continuous_batching": out["continuous_batching_burst"], "plain": out["plain_burst"]},
+            "burst": ({"continuous_batching": out["continuous_batching_burst"], "plain": out["plain_burst"]}
+                      if "plain_burst" in out else None),
             "texts_per_s_over_plain": round(cb["texts_per_s"] / max(pl["texts_per_s"], 1e-9), 2),
-            "burst_texts_per_s_over_plain": round(out["continuous_batching_burst"]["texts_per_s"] /
-                                                  max(out["plain_burst"]["texts_per_s"], 1e-9), 2),
+            "burst_texts_per_s_over_plain": (round(out["continuous_batching_burst"]["texts_per_s"] /
+                                                   max(out["plain_burst"]["texts_per_s"], 1e-9), 2)
+                                             if "plain_burst" in out else None),
             "p99_over_plain": round(cb["p99_ms"] / max(pl["p99_ms"], 1e-9), 3)}
 
 
@@ -435,8 +442,12 @@ def main():
     ap.add_argument("--serve-count", type=int, default=2000)
     ap.add_argument("--serve-rate", type=float, default=200.0)
     ap.add_argument("--batch-window-ms", type=float, default=2.0)
-    ap.add_argument("--serve-runs", choices=("all", "cb"), default="all",
-                    help="serve_poisson: all four runs (default) or only the continuous-batching Poisson run")
+    ap.add_argument("--no-batch-serve", action="store_true",
+                    help="default workload: leave out the batched and serving measurements "
+                         "(--no-other-configs leaves them out too)")
+    ap.add_argument("--serve-runs", choices=("all", "cb", "poisson"), default="all",
+                    help="serve_poisson: all four runs (default), only the continuous-batching Poisson run, "
+                         "or the two Poisson runs")
     ap.add_argument("--phase-marks", action="store_true",
                     help="profiling: the engine launches marker kernels at its level-0 phase boundaries, and a "
                          "1 GiB device copy runs first (the reference dispatch for the counter scale)")
@@ -616,6 +627,21 @@ def main():
         result["gpu_over_cpu_same_sample"] = round(gpu_sample / cb["value"], 1) if gpu_sample else None
     else:
         result["cpu_baseline"] = None
+
+    if not (args.no_batch_serve or args.no_other_configs):
+        # the batched and the serving workloads in the same run (each also has a line of its own:
+        # --workload batch_acgtn, serve_poisson), after the 4 GiB text's buffers are released
+        dev_text = dev_sa = text = None
+        torch.cuda.empty_cache()
+        b = bench_batch(args)
+        result["batched_acgtn"] = {k: b[k] for k in ("value", "unit", "ms_per_step", "batched_mb_s", "plain",
+                                                     "batched_over_plain", "sa_valid_sample", "sub_batch_chars")}
+        result["batched_acgtn"]["config"] = b["config"]
+        torch.cuda.empty_cache()
+        sv = bench_serve(argparse.Namespace(**{**vars(args), "serve_runs": "poisson"}))
+        result["serving_poisson"] = {k: sv[k] for k in ("value", "unit", "p50_ms", "p99_ms", "texts_per_s_over_plain",
+                                                        "p99_over_plain", "continuous_batching", "plain")}
+        result["serving_poisson"]["config"] = sv["config"]
     print(json.dumps(result), flush=True)
 
 
