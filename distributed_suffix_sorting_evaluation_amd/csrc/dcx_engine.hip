@@ -381,9 +381,11 @@ class Engine {
         DBuf<uint32_t> act, gid, gstart, gfirst;  // gstart: G + 1 entries, the last one A; gfirst: G entries
         uint64_t A = 0;
         uint32_t G = 0;
-        // groups of more than RF_SMALL members and their members, members of groups of more than TB_SMALL
+        // groups of more than RF_WAVE members and their members, members of groups of more than TB_SMALL,
+        // groups of RF_SMALL + 1 .. RF_WAVE members
         uint32_t nlarge = 0;
         uint64_t B8 = 0, B16 = 0;
+        uint64_t nwave = 0;
     };
 
     // restricted to the previous active list `src` (its superset) or to all N positions (src == nullptr).
@@ -402,12 +404,13 @@ class Engine {
         // the end sentinel and the group-size counts on the device, then one readback for all of them
         k_active_finish<<<1, 64, 0, stream>>>(prim.scalars, out.gstart.get());
         k_group_counts<<<grid(Nsrc / 2 + 1, 2048), 256, 0, stream>>>(out.gstart.get(), prim.scalars);
-        const uint64_t* t = prim.read_scalars(11);
+        const uint64_t* t = prim.read_scalars(12);
         out.A = t[0];
         out.G = (uint32_t)t[1];
         out.nlarge = (uint32_t)t[8];
         out.B8 = t[9];
         out.B16 = t[10];
+        out.nwave = t[11];
     }
 
     // -------------------------------------------------------------------------------------------
@@ -565,8 +568,9 @@ class Engine {
     // One character round of the refinement: orders the members of every tied group of `cur` by their
     // next c characters (after the first C, c * bits <= 64), writes the new group starts into H and the
     // compact heads of the list into Ha (A + 1 entries).  Small groups are sorted in place by one thread
-    // each; the members of large groups go through a radix sort keyed by (rank among the large groups,
-    // characters), or two stable sorts when those do not fit 64 bits.
+    // each, middle ones (up to RF_WAVE members) by one wave each; the members of large groups go through a
+    // radix sort keyed by (rank among the large groups, characters), or two stable sorts when those do not
+    // fit 64 bits.
     void refine_round(const uint64_t* PT, uint32_t bits, uint32_t C, uint32_t c, const ActiveSet& cur,
                       DBuf<uint32_t>& order, DBuf<uint8_t>& H, DBuf<uint8_t>& Ha) {
         const uint64_t A = cur.A;
@@ -574,6 +578,12 @@ class Engine {
         DBuf<uint8_t> large(cache, G);
         k_refine_small<<<grid(G), 256, 0, stream>>>(cur.gstart.get(), G, cur.gfirst.get(), order.get(), H.get(),
                                                     Ha.get(), PT, bits, C, c * bits, large.get());
+        if (cur.nwave) {
+            const uint64_t chunks = ceil_div((uint64_t)G, 64ull);
+            k_refine_wave<<<(unsigned)std::min<uint64_t>(ceil_div(chunks, (uint64_t)RFW_WAVES), 8192), 64 * RFW_WAVES, 0,
+                            stream>>>(cur.gstart.get(), G, cur.gfirst.get(), order.get(), H.get(), Ha.get(), PT, bits,
+                                      C, c * bits);
+        }
         prim.fill_u8(Ha.get() + A, 1, 1);
         // members of the large groups and their count, known from the active set's group counts
         const uint64_t B = cur.B8;

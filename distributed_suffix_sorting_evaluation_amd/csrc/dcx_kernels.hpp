@@ -724,10 +724,12 @@ __device__ __forceinline__ bool dc_rank_less(uint32_t a, uint32_t b, const DcDev
 
 // ------------------------------------------------------------------------------------------------
 // Refinement rounds by group size: a tied group of at most RF_SMALL suffixes is ordered by one thread
-// (insertion sort of the members' next characters); the members of larger groups go through the radix
-// sort.  Both write the new group heads into H and the compact heads Ha of the active list.
+// (insertion sort of the members' next characters), a group of at most RF_WAVE by one wave (a bitonic
+// sort in the wave's LDS); the members of larger groups go through the radix sort.  All write the new
+// group heads into H and the compact heads Ha of the active list.
 // ------------------------------------------------------------------------------------------------
 constexpr uint32_t RF_SMALL = 8;
+constexpr uint32_t RF_WAVE = 512;
 
 __global__ __launch_bounds__(256) void k_refine_small(const uint32_t* __restrict__ gstart, uint32_t G,
                                                       const uint32_t* __restrict__ gfirst, uint32_t* __restrict__ order,
@@ -738,7 +740,7 @@ __global__ __launch_bounds__(256) void k_refine_small(const uint32_t* __restrict
     for (uint64_t g = blockIdx.x * 256ull + threadIdx.x; g < G; g += gridDim.x * 256ull) {
         const uint32_t a0 = gstart[g], s = gstart[g + 1] - a0;
         if (s > RF_SMALL) {
-            large[g] = 1;
+            large[g] = s > RF_WAVE ? 1 : 0;  // the wave path (k_refine_wave) takes the others
             continue;
         }
         large[g] = 0;
@@ -778,6 +780,179 @@ __global__ __launch_bounds__(256) void k_refine_small(const uint32_t* __restrict
             const bool head = t == 0 || k[t] != k[t - 1];
             Ha[a0 + t] = head ? 1 : 0;
             if (head && t > 0) H[j0 + t] = 1;
+        }
+    }
+}
+
+// Groups of RF_SMALL + 1 .. RF_WAVE members, by waves: every wave takes 64 consecutive groups at a time
+// and sorts those of this size in batches of at most RF_WAVE elements in its LDS.  A batch is the longest
+// run (in group order) of such groups whose sizes, padded to powers of two P >= 16, sum to at most
+// RF_WAVE; its groups are laid out by decreasing P, so each starts at a multiple of its P.  All members of
+// the batch are loaded together (their order slots, then their next characters: a random stream window
+// each, as in k_refine_small), so a batch costs two memory round trips instead of two per group.  One
+// bitonic network over the batch sorts every group: an element takes part in the stages of block sizes up
+// to its group's P (the pairs of those stages stay inside the group), and a group's last merge is
+// ascending.  Elements compare by (key, member index), so the order is that of a stable sort; padding
+// elements (the largest key, indices from the group size on) sort last.  The LDS accesses of one wave run
+// in program order, so the steps only need the compiler barrier between them.
+constexpr uint32_t RFW_WAVES = 4;   // waves per workgroup
+constexpr uint32_t RFW_MAXG = 32;   // groups in a batch (each pads to at least 16 elements)
+__device__ __forceinline__ uint32_t rfw_class_p(uint32_t i, const uint32_t (&cend)[6]) {
+    // elements [0, cend[5]) belong to groups of P = 512, then [cend[5], cend[4]) to P = 256, ...
+#pragma unroll
+    for (int c = 5; c >= 0; c--)
+        if (i < cend[c]) return 16u << c;
+    return 0u;
+}
+__global__ __launch_bounds__(64 * RFW_WAVES) void k_refine_wave(const uint32_t* __restrict__ gstart, uint32_t G,
+                                                              const uint32_t* __restrict__ gfirst,
+                                                              uint32_t* __restrict__ order, uint8_t* __restrict__ H,
+                                                              uint8_t* __restrict__ Ha,
+                                                              const uint64_t* __restrict__ PT, uint32_t bits,
+                                                              uint32_t C, uint32_t cbits) {
+    __shared__ uint64_t s_key[RFW_WAVES][RF_WAVE];
+    __shared__ uint32_t s_pos[RFW_WAVES][RF_WAVE];
+    __shared__ uint16_t s_idx[RFW_WAVES][RF_WAVE];
+    __shared__ uint32_t s_boff[RFW_WAVES][RFW_MAXG];  // the batch's groups by offset: offset, size, slot, list index
+    __shared__ uint32_t s_bs[RFW_WAVES][RFW_MAXG];
+    __shared__ uint32_t s_bj0[RFW_WAVES][RFW_MAXG];
+    __shared__ uint32_t s_ba0[RFW_WAVES][RFW_MAXG];
+    const StreamKeys<false> chars{PT, bits, top_mask(cbits)};
+    const uint32_t lane = __lane_id(), w = threadIdx.x / 64;
+    uint64_t* key = s_key[w];
+    uint32_t* pos = s_pos[w];
+    uint16_t* idx = s_idx[w];
+    uint32_t* boff = s_boff[w];
+    uint32_t* bsz = s_bs[w];
+    uint32_t* bj0 = s_bj0[w];
+    uint32_t* ba0 = s_ba0[w];
+    constexpr int R = RF_WAVE / 64;  // elements per lane
+    const uint64_t below = lanemask_lt();
+    const uint64_t chunks = ((uint64_t)G + 63) / 64;
+    for (uint64_t ch = (uint64_t)blockIdx.x * RFW_WAVES + w; ch < chunks; ch += (uint64_t)gridDim.x * RFW_WAVES) {
+        const uint64_t gl = ch * 64 + lane;
+        uint32_t sz = 0, a0 = 0, j0 = 0;
+        if (gl < G) {
+            a0 = gstart[gl];
+            sz = gstart[gl + 1] - a0;
+        }
+        const bool med = sz > RF_SMALL && sz <= RF_WAVE;
+        uint32_t lp = 4;  // P = 2^lp >= sz, for the groups of this path only (sizes up to RF_WAVE)
+        if (med) {
+            j0 = gfirst[gl];
+            while ((1u << lp) < sz) lp++;
+        }
+        uint64_t pending = __ballot(med);
+        while (pending) {
+            // the batch: pending groups in lane order while their padded sizes sum to at most RF_WAVE
+            const bool pend = (pending >> lane) & 1;
+            const uint32_t pre = (uint32_t)wave_incl_scan64(pend ? (1ull << lp) : 0ull);
+            const bool inb = pend && pre <= RF_WAVE;
+            const uint64_t bmask = __ballot(inb);
+            // layout by decreasing P; cend[c] = end of the elements of groups with P >= 16 << c
+            uint32_t cend[6], base = 0, nb = 0, myoff = 0, myrank = 0;
+#pragma unroll
+            for (int c = 5; c >= 0; c--) {
+                const uint64_t cm = __ballot(inb && lp == (uint32_t)(c + 4));
+                if (inb && lp == (uint32_t)(c + 4)) {
+                    const uint32_t k = (uint32_t)__popcll(cm & below);
+                    myoff = base + (k << (c + 4));
+                    myrank = nb + k;
+                }
+                base += (uint32_t)__popcll(cm) << (c + 4);
+                nb += (uint32_t)__popcll(cm);
+                cend[c] = base;
+            }
+            const uint32_t T = base;
+            if (inb) {
+                boff[myrank] = myoff;
+                bsz[myrank] = sz;
+                bj0[myrank] = j0;
+                ba0[myrank] = a0;
+            }
+            __builtin_amdgcn_wave_barrier();
+            // flattened load: element e = lane + 64 r of the batch -> its group (by offset) and member t
+            uint32_t ek[R], et[R], ep[R];
+            uint64_t kv[R];
+#pragma unroll
+            for (int r = 0; r < R; r++) {
+                const uint32_t e = lane + 64u * r;
+                ek[r] = 0;
+                et[r] = 0xFFFFu;
+                ep[r] = 0;
+                if (e < T) {
+                    uint32_t lo = 0, hi = nb;
+                    while (hi - lo > 1) {
+                        const uint32_t mid = (lo + hi) >> 1;
+                        if (boff[mid] <= e)
+                            lo = mid;
+                        else
+                            hi = mid;
+                    }
+                    ek[r] = lo;
+                    et[r] = e - boff[lo];
+                    if (et[r] < bsz[lo]) ep[r] = order[bj0[lo] + et[r]];
+                }
+            }
+#pragma unroll
+            for (int r = 0; r < R; r++) {
+                const uint32_t e = lane + 64u * r;
+                kv[r] = ~0ull;
+                if (e < T && et[r] < bsz[ek[r]]) kv[r] = chars((uint64_t)ep[r] + C);
+            }
+#pragma unroll
+            for (int r = 0; r < R; r++) {
+                const uint32_t e = lane + 64u * r;
+                if (e < T) {
+                    key[e] = kv[r];
+                    idx[e] = (uint16_t)et[r];
+                    pos[e] = ep[r];
+                }
+            }
+            __builtin_amdgcn_wave_barrier();
+            // one network for every group of the batch
+            const uint32_t pmax = rfw_class_p(0, cend);
+            for (uint32_t lk = 1; (1u << lk) <= pmax; lk++) {
+                const uint32_t kk = 1u << lk;
+                const uint32_t S = cend[lk > 4 ? lk - 4 : 0];  // the elements of groups with P >= kk
+                for (int lj = (int)lk - 1; lj >= 0; lj--) {
+                    const uint32_t j = 1u << lj;
+                    for (uint32_t t = lane; t < S / 2; t += 64) {
+                        const uint32_t i = ((t >> lj) << (lj + 1)) | (t & (j - 1)), l = i + j;
+                        const uint32_t Pi = rfw_class_p(i, cend);
+                        if (kk > Pi) continue;
+                        const uint64_t ki = key[i], kl = key[l];
+                        const uint32_t ii = idx[i], il = idx[l];
+                        const bool gt = ki > kl || (ki == kl && ii > il);
+                        const bool asc = kk == Pi || (i & kk) == 0;
+                        if (gt == asc) {
+                            key[i] = kl;
+                            key[l] = ki;
+                            idx[i] = (uint16_t)il;
+                            idx[l] = (uint16_t)ii;
+                        }
+                    }
+                    __builtin_amdgcn_wave_barrier();
+                }
+            }
+            // write back: new order, group heads of the order and of the list
+#pragma unroll
+            for (int r = 0; r < R; r++) {
+                const uint32_t e = lane + 64u * r;
+                if (e < T) {
+                    const uint32_t k = ek[r], t = et[r];
+                    if (t < bsz[k]) {
+                        const uint64_t kk2 = key[e];
+                        const uint32_t jj = bj0[k];
+                        order[jj + t] = pos[boff[k] + idx[e]];
+                        const bool head = t == 0 || kk2 != key[e - 1];
+                        Ha[ba0[k] + t] = head ? 1 : 0;
+                        if (head && t > 0) H[jj + t] = 1;
+                    }
+                }
+            }
+            __builtin_amdgcn_wave_barrier();
+            pending &= ~bmask;
         }
     }
 }
@@ -1004,30 +1179,35 @@ __global__ void k_active_finish(uint64_t* sc, uint32_t* __restrict__ gstart) {
         sc[8] = 0;
         sc[9] = 0;
         sc[10] = 0;
+        sc[11] = 0;
     }
 }
 
 // Counts of the active set's groups by size, for the host to size the next steps without another
-// readback: sc[8] = groups with more than RF_SMALL members (the radix path of a group round), sc[9] = their
-// members, sc[10] = members of groups with more than TB_SMALL members (the merge-sort path of the tie-break)
+// readback: sc[8] = groups with more than RF_WAVE members (the radix path of a group round), sc[9] = their
+// members, sc[10] = members of groups with more than TB_SMALL members (the merge-sort path of the
+// tie-break), sc[11] = groups of RF_SMALL + 1 .. RF_WAVE members (the wave path of a group round)
 __global__ __launch_bounds__(256) void k_group_counts(const uint32_t* __restrict__ gstart, uint64_t* sc) {
     const uint64_t G = sc[1];
-    unsigned long long nl = 0, b8 = 0, b16 = 0;
+    unsigned long long nl = 0, b8 = 0, b16 = 0, nw = 0;
     for (uint64_t g = blockIdx.x * 256ull + threadIdx.x; g < G; g += gridDim.x * 256ull) {
         const uint32_t sz = gstart[g + 1] - gstart[g];
-        nl += sz > RF_SMALL;
-        b8 += sz > RF_SMALL ? sz : 0;
+        nl += sz > RF_WAVE;
+        b8 += sz > RF_WAVE ? sz : 0;
         b16 += sz > TB_SMALL ? sz : 0;
+        nw += sz > RF_SMALL && sz <= RF_WAVE;
     }
     for (int d = 32; d >= 1; d >>= 1) {
         nl += __shfl_xor(nl, d, 64);
         b8 += __shfl_xor(b8, d, 64);
         b16 += __shfl_xor(b16, d, 64);
+        nw += __shfl_xor(nw, d, 64);
     }
-    if (__lane_id() == 0 && (nl | b16)) {
+    if (__lane_id() == 0 && (nl | b16 | nw)) {
         atomicAdd(reinterpret_cast<unsigned long long*>(&sc[8]), nl);
         atomicAdd(reinterpret_cast<unsigned long long*>(&sc[9]), b8);
         atomicAdd(reinterpret_cast<unsigned long long*>(&sc[10]), b16);
+        atomicAdd(reinterpret_cast<unsigned long long*>(&sc[11]), nw);
     }
 }
 
