@@ -28,6 +28,7 @@ a suffix array is unique, so a valid SA is bit-exact.  The same line carries:
     python bench.py --workload batch_acgtn | serve_poisson
     python bench.py --phase-marks ...          (profiling: phase-boundary marker kernels, a 1 GiB copy first)
     python bench.py --phase-table READ_DIR WRITE_DIR TRACE_DIR     (per-phase table from rocprofv3 csv output)
+    python bench.py --radix-traffic READ_DIR WRITE_DIR TRACE_DIR   (HBM bytes per launch of the radix pass)
 """
 import argparse
 import json
@@ -390,6 +391,24 @@ def phase_table(read_dir, write_dir, trace_dir):
     print(json.dumps({"phases": res, "copy_read_check": rcheck, "copy_write_check": wcheck}))
 
 
+def radix_traffic(read_dir, write_dir, trace_dir):
+    """HBM bytes per launch of the 64-bit radix pass (k_onesweep with 64-bit keys: the key sources
+    ArrayKeys<unsigned long> and StreamKeys) from the same three rocprofv3 runs as --phase-table, with the
+    launch count and the mean launch time of the trace.  Prints a JSON line."""
+    def radix(name):
+        return "k_onesweep<" in name and ("ArrayKeys<unsigned long>" in name or "StreamKeys<" in name)
+    rd = {d: v for d, v in _dispatch_counters(read_dir).items() if radix(v[0])}
+    wr = {d: v for d, v in _dispatch_counters(write_dir).items() if radix(v[0])}
+    trace = [r for r in _rocprof_rows(trace_dir, "kernel_trace.csv") if radix(r["Kernel_Name"])]
+    ms = [(int(r["End_Timestamp"]) - int(r["Start_Timestamp"])) / 1e6 for r in trace]
+    rb = sum(_read_bytes(c) for _, c in rd.values()) / max(len(rd), 1)
+    wb = sum(_write_bytes(c) for _, c in wr.values()) / max(len(wr), 1)
+    print(json.dumps({"launches": len(trace), "pmc_launches": [len(rd), len(wr)],
+                      "read_bytes_per_launch": int(rb), "write_bytes_per_launch": int(wb),
+                      "bytes_per_launch": int(rb + wb),
+                      "trace_avg_launch_us": round(1e3 * sum(ms) / max(len(ms), 1), 1)}))
+
+
 def overlap_table(trace_dir):
     """Device overlap of the serving workers from a rocprofv3 --kernel-trace csv run of `--workload
     serve_poisson`: per stream the kernel count and busy time, and the time during which kernels of two or
@@ -453,6 +472,8 @@ def main():
                          "1 GiB device copy runs first (the reference dispatch for the counter scale)")
     ap.add_argument("--phase-table", nargs=3, metavar=("READ_DIR", "WRITE_DIR", "TRACE_DIR"),
                     help="print the per-phase table of rocprofv3 csv output of --phase-marks runs")
+    ap.add_argument("--radix-traffic", nargs=3, metavar=("READ_DIR", "WRITE_DIR", "TRACE_DIR"),
+                    help="HBM bytes per launch of the 64-bit radix pass from the --phase-table runs")
     ap.add_argument("--overlap-table", metavar="TRACE_DIR",
                     help="print the stream overlap of a rocprofv3 --kernel-trace csv run of serve_poisson")
     ap.add_argument("--copy-calibration", action="store_true",
@@ -465,6 +486,9 @@ def main():
         return
     if args.overlap_table:
         overlap_table(args.overlap_table)
+        return
+    if args.radix_traffic:
+        radix_traffic(*args.radix_traffic)
         return
 
     if args.copy_calibration:
