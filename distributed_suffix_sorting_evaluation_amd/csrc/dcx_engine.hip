@@ -447,7 +447,7 @@ class Engine {
             prim.sort_pairs_from(StreamKeys<true>{PT, bits, hi}, k, k2, nv, nv2, N, 64 - kbits, 64, false);
             H = DBuf<uint8_t>(cache, (size_t)N + 1);
             order = DBuf<uint32_t>(cache, N);
-            k_heads_packed<<<grid(N + 1ull), 256, 0, stream>>>(k, N, kbits, (1ull << pbits) - 1, H.get(), order.get());
+            k_heads_packed<<<grid(N / 16 + 1ull), 256, 0, stream>>>(k, N, kbits, (1ull << pbits) - 1, H.get(), order.get());
         } else {
             if (N >= (1u << 24)) {
                 // large sorts: round 0 takes the characters that carry about log2(N) + ROUND0_SLACK bits of
@@ -478,7 +478,7 @@ class Engine {
                 k_heads_u32<<<grid(N / 16 + 1ull), 256, 0, stream>>>(low.get(), N, H.get());
                 k_mark_runs<<<1, RS_RADIX, 0, stream>>>(prim.hist + (np - 1) * RS_RADIX, N, H.get());
             } else {
-                k_heads<<<grid(N + 1ull), 256, 0, stream>>>(k, N, c0 * bits, H.get());
+                k_heads<<<grid(N / 16 + 1ull), 256, 0, stream>>>(k, N, c0 * bits, H.get());
             }
             if (v == va.get())
                 order = std::move(va);

@@ -271,10 +271,32 @@ __device__ __forceinline__ uint64_t top_bits(uint64_t w, uint32_t nbits) {
 }
 
 // group heads of sorted keys, comparing only the top kbits
+// A thread per 16 consecutive positions (a grid of N / 16 + 1 threads): eight 16-byte key loads, one
+// 16-byte store of flags (keys and H are 256-byte aligned buffers).
 __global__ __launch_bounds__(256) void k_heads(const uint64_t* __restrict__ keys, uint64_t N, uint32_t kbits,
                                                uint8_t* __restrict__ H) {
-    for (uint64_t j = blockIdx.x * 256ull + threadIdx.x; j <= N; j += gridDim.x * 256ull)
-        H[j] = (j == 0 || j == N) ? 1 : (top_bits(keys[j], kbits) != top_bits(keys[j - 1], kbits));
+    const uint64_t mask = top_mask(kbits);
+    const uint64_t groups = N / 16 + 1;  // covers positions 0..N
+    for (uint64_t g = blockIdx.x * 256ull + threadIdx.x; g < groups; g += gridDim.x * 256ull) {
+        const uint64_t j0 = g * 16;
+        if (j0 + 16 <= N) {
+            uint64_t w[17];
+            w[0] = j0 ? (keys[j0 - 1] & mask) : ~(keys[0] & mask);  // position 0 is always a head
+#pragma unroll
+            for (int v = 0; v < 8; v++) {
+                const uint4 x = reinterpret_cast<const uint4*>(keys + j0)[v];
+                w[1 + 2 * v] = ((uint64_t)x.y << 32 | x.x) & mask;
+                w[2 + 2 * v] = ((uint64_t)x.w << 32 | x.z) & mask;
+            }
+            uint32_t h[4] = {0, 0, 0, 0};
+#pragma unroll
+            for (int k = 0; k < 16; k++) h[k >> 2] |= (uint32_t)(w[k + 1] != w[k]) << (8 * (k & 3));
+            reinterpret_cast<uint4*>(H + j0)[0] = make_uint4(h[0], h[1], h[2], h[3]);
+        } else {
+            for (uint64_t j = j0; j < j0 + 16 && j <= N; j++)
+                H[j] = (j == 0 || j == N) ? 1 : (top_bits(keys[j], kbits) != top_bits(keys[j - 1], kbits));
+        }
+    }
 }
 
 // H[j] = 1 where a[j] != a[j - 1], and at 0 and N.  Used for the heads from the low words written by the
@@ -314,12 +336,40 @@ __global__ __launch_bounds__(256) void k_mark_runs(const uint64_t* __restrict__ 
 
 // packed round 0: keys are (characters << pbits) | position; heads compare the characters, order gets the
 // positions
+// positions, a thread per 16 consecutive ones (a grid of N / 16 + 1 threads) as in k_heads
 __global__ __launch_bounds__(256) void k_heads_packed(const uint64_t* __restrict__ keys, uint64_t N, uint32_t kbits,
                                                       uint64_t pmask, uint8_t* __restrict__ H,
                                                       uint32_t* __restrict__ order) {
-    for (uint64_t j = blockIdx.x * 256ull + threadIdx.x; j <= N; j += gridDim.x * 256ull) {
-        if (j < N) order[j] = (uint32_t)(keys[j] & pmask);
-        H[j] = (j == 0 || j == N) ? 1 : (top_bits(keys[j], kbits) != top_bits(keys[j - 1], kbits));
+    const uint64_t mask = top_mask(kbits);
+    const uint64_t groups = N / 16 + 1;
+    for (uint64_t g = blockIdx.x * 256ull + threadIdx.x; g < groups; g += gridDim.x * 256ull) {
+        const uint64_t j0 = g * 16;
+        if (j0 + 16 <= N) {
+            uint64_t w[17], kw[16];
+            w[0] = j0 ? (keys[j0 - 1] & mask) : ~(keys[0] & mask);
+#pragma unroll
+            for (int v = 0; v < 8; v++) {
+                const uint4 x = reinterpret_cast<const uint4*>(keys + j0)[v];
+                kw[2 * v] = (uint64_t)x.y << 32 | x.x;
+                kw[2 * v + 1] = (uint64_t)x.w << 32 | x.z;
+                w[1 + 2 * v] = kw[2 * v] & mask;
+                w[2 + 2 * v] = kw[2 * v + 1] & mask;
+            }
+            uint32_t h[4] = {0, 0, 0, 0};
+#pragma unroll
+            for (int k = 0; k < 16; k++) h[k >> 2] |= (uint32_t)(w[k + 1] != w[k]) << (8 * (k & 3));
+            reinterpret_cast<uint4*>(H + j0)[0] = make_uint4(h[0], h[1], h[2], h[3]);
+#pragma unroll
+            for (int v = 0; v < 4; v++)
+                reinterpret_cast<uint4*>(order + j0)[v] =
+                    make_uint4((uint32_t)(kw[4 * v] & pmask), (uint32_t)(kw[4 * v + 1] & pmask),
+                               (uint32_t)(kw[4 * v + 2] & pmask), (uint32_t)(kw[4 * v + 3] & pmask));
+        } else {
+            for (uint64_t j = j0; j < j0 + 16 && j <= N; j++) {
+                if (j < N) order[j] = (uint32_t)(keys[j] & pmask);
+                H[j] = (j == 0 || j == N) ? 1 : (top_bits(keys[j], kbits) != top_bits(keys[j - 1], kbits));
+            }
+        }
     }
 }
 
