@@ -471,12 +471,18 @@ class Engine {
                 ia = DBuf<uint8_t>(cache, N);
                 ib = DBuf<uint8_t>(cache, N);
             }
-            const uint32_t np = prim.sort_pairs_from(StreamKeys<false>{PT, bits, top_mask(c0 * bits)}, k, k2, v, v2, N,
-                                                     64 - c0 * bits, 64, true, low.get(), ia.get(), ib.get());
+            // the heads: from the low words (at most five passes), else written by the last pass itself
             H = DBuf<uint8_t>(cache, (size_t)N + 1);
+            bool heads_done = false;
+            const uint32_t np = prim.sort_pairs_from(StreamKeys<false>{PT, bits, top_mask(c0 * bits)}, k, k2, v, v2, N,
+                                                     64 - c0 * bits, 64, true, low.get(), ia.get(), ib.get(), H.get(),
+                                                     top_mask(c0 * bits), &heads_done);
             if (np) {
                 k_heads_u32<<<grid(N / 16 + 1ull), 256, 0, stream>>>(low.get(), N, H.get());
                 k_mark_runs<<<1, RS_RADIX, 0, stream>>>(prim.hist + (np - 1) * RS_RADIX, N, H.get());
+            } else if (heads_done) {
+                k_heads_fix<<<grid(N / 16 + 1ull), 256, 0, stream>>>(k, N, top_mask(c0 * bits), H.get());
+                prim.fill_u8(H.get() + N, 1, 1);
             } else {
                 k_heads<<<grid(N / 16 + 1ull), 256, 0, stream>>>(k, N, c0 * bits, H.get());
             }

@@ -328,6 +328,34 @@ __global__ __launch_bounds__(256) void k_heads_u32(const uint32_t* __restrict__ 
         }
     }
 }
+// Heads written by the last radix pass (k_onesweep HEADS): an entry 2 is the first element of a tile's run
+// of a digit, and its predecessor in the sorted order is the last element of another run; the pass wrote
+// the keys of both, so the entry becomes their comparison under hmask (1 at position 0).  A thread per 16
+// flags, one 16-byte load each; only entries 2 are rewritten.  H[N] is set by the caller.
+__global__ __launch_bounds__(256) void k_heads_fix(const uint64_t* __restrict__ keys, uint64_t N, uint64_t hmask,
+                                                   uint8_t* __restrict__ H) {
+    const uint64_t groups = (N + 15) / 16;
+    for (uint64_t g = blockIdx.x * 256ull + threadIdx.x; g < groups; g += gridDim.x * 256ull) {
+        const uint64_t j0 = g * 16;
+        uint32_t h[4] = {0x02020202u, 0x02020202u, 0x02020202u, 0x02020202u};  // a tail group checks each entry
+        if (j0 + 16 <= N) {
+            const uint4 x = reinterpret_cast<const uint4*>(H + j0)[0];
+            if (((x.x | x.y | x.z | x.w) & 0x02020202u) == 0) continue;
+            h[0] = x.x;
+            h[1] = x.y;
+            h[2] = x.z;
+            h[3] = x.w;
+        }
+        for (uint32_t b = 0; b < 16; b++) {
+            const uint64_t j = j0 + b;
+            if (j >= N) break;
+            if (((h[b >> 2] >> (8 * (b & 3))) & 0xFFu) != 2u) continue;
+            if (j0 + 16 > N && H[j] != 2) continue;
+            H[j] = j == 0 ? 1 : (((keys[j] ^ keys[j - 1]) & hmask) != 0);
+        }
+    }
+}
+
 __global__ __launch_bounds__(256) void k_mark_runs(const uint64_t* __restrict__ offsets, uint64_t N,
                                                    uint8_t* __restrict__ H) {
     const uint64_t o = offsets[threadIdx.x];  // exclusive digit offsets: every run starts at one

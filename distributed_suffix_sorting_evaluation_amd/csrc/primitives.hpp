@@ -331,16 +331,22 @@ __global__ __launch_bounds__(256) void k_seg_setup(const uint64_t* __restrict__ 
 // element instead of 12.  An IDENT pass splits the index; a pass with VoutT = uint32_t joins it again.
 // LIMIT (keys only): output indices at or above out_limit are not written (a partition whose last
 // bucket is dropped writes into a buffer that holds only the buckets before it).
+// HEADS (the last pass of a sort whose keys are needed only for their group heads): instead of every key
+// the pass writes hout[o] = 1 where the key (under hmask) differs from the previous element of the tile's
+// run of its digit, 0 where it equals it, and 2 at a run's first element, whose predecessor in the sorted
+// order lies in another run; only the runs' first and last keys are written, for k_heads_fix.
 template <bool VALS, bool IDENT, class Src, class KeyT = uint64_t, int ITEMS = RS_ITEMS, class OutT = KeyT,
-          class VinT = uint32_t, class VoutT = VinT, bool LIMIT = false>
+          class VinT = uint32_t, class VoutT = VinT, bool LIMIT = false, bool HEADS = false>
 __global__ __launch_bounds__(RS_THREADS) void k_onesweep(Src src, OutT* __restrict__ kout,
                                                          const VinT* __restrict__ vin, VoutT* __restrict__ vout,
                                                          uint32_t shift, const uint64_t* __restrict__ segoff,
                                                          const uint64_t* __restrict__ info, uint32_t* __restrict__ tick,
                                                          uint64_t* __restrict__ status, uint32_t tag,
                                                          uint32_t out_shift = 0, uint32_t kpos = 0,
-                                                         uint64_t out_limit = 0) {
+                                                         uint64_t out_limit = 0, uint8_t* __restrict__ hout = nullptr,
+                                                         uint64_t hmask = 0) {
     static_assert(!LIMIT || !VALS, "an output limit applies to key-only passes");
+    static_assert(!HEADS || (sizeof(KeyT) == 8 && sizeof(OutT) == 8), "heads come from 64-bit keys");
     constexpr int TILE = RS_THREADS * ITEMS;
     constexpr bool SPLIT = VALS && (sizeof(VinT) == 1 || sizeof(VoutT) == 1);
     static_assert(!SPLIT || sizeof(KeyT) == 8, "split indices need 64-bit keys");
@@ -578,7 +584,14 @@ __global__ __launch_bounds__(RS_THREADS) void k_onesweep(Src src, OutT* __restri
             const KeyT kk = s_keys[i];
             dg[r] = (uint32_t)((uint64_t)kk >> shift) & 0xFFu;
             const uint32_t o = s_gofs[dg[r]] + i;
-            if (!LIMIT || (uint64_t)o < out_limit) kout[o] = (OutT)((uint64_t)kk >> out_shift);
+            if constexpr (HEADS) {
+                const bool rs = i == s_start[dg[r]];
+                const bool re = i + 1 == tile_n || (((uint64_t)s_keys[i + 1] >> shift) & 0xFFu) != dg[r];
+                hout[o] = rs ? (uint8_t)2 : (uint8_t)((((uint64_t)kk ^ (uint64_t)s_keys[i - 1]) & hmask) != 0);
+                if (rs || re) kout[o] = (OutT)kk;
+            } else if (!LIMIT || (uint64_t)o < out_limit) {
+                kout[o] = (OutT)((uint64_t)kk >> out_shift);
+            }
             if constexpr (SPLIT) {
                 if constexpr (sizeof(VoutT) == 1)
                     vout[o] = s_vb[i];
@@ -1204,10 +1217,17 @@ struct Primitives {
     // byte in idx_bytes: 9 instead of 12 bytes per element); the last pass joins it into vals.  The
     // passes read idx_bytes in aligned 16-byte chunks: both arrays must be 16-byte aligned and readable
     // up to N rounded up to 16 (DeviceCache blocks are).
+    //
+    // heads (optional, with values, when neither low32 nor split indices are used and there are at least
+    // two passes): the last pass writes the group heads of the top key bits under hmask into heads[0, N)
+    // (k_onesweep HEADS; 2 marks entries for k_heads_fix) and only the keys that fix needs; *heads_done
+    // tells whether it did.
     template <class Src>
     uint32_t sort_pairs_from(const Src& first, uint64_t*& keys, uint64_t*& keys_alt, uint32_t*& vals,
                          uint32_t*& vals_alt, uint64_t N, uint32_t lo_bit, uint32_t hi_bit, bool ident_vals,
-                         uint32_t* low32 = nullptr, uint8_t* idx_bytes = nullptr, uint8_t* idx_bytes_alt = nullptr) {
+                         uint32_t* low32 = nullptr, uint8_t* idx_bytes = nullptr, uint8_t* idx_bytes_alt = nullptr,
+                         uint8_t* heads = nullptr, uint64_t hmask = 0, bool* heads_done = nullptr) {
+        if (heads_done) *heads_done = false;
         if (hi_bit <= lo_bit) hi_bit = lo_bit + 1;
         if (N <= 1) {
             // nothing to order: materialize the inputs in place
@@ -1221,6 +1241,7 @@ struct Primitives {
         if (np > 5 || np < 2 || !vals) low32 = nullptr;  // the last pass must not be the first
         const uint32_t kpos = std::min<uint32_t>(lo_bit, 24);
         const bool split = low32 && ident_vals && idx_bytes && idx_bytes_alt && bit_width(N - 1) <= kpos + 8;
+        if (low32 || split || np < 2 || !vals) heads = nullptr;
         const uint64_t tiles = seg_plan(first, N, lo_bit, np, RS_TILE);
         uint64_t span_bytes = 0;
         if (profile && !no_alloc) {  // the pair is listed at once, so drop_profile releases it whatever happens next
@@ -1275,6 +1296,13 @@ struct Primitives {
                 const ArrayKeys<uint64_t> in{keys};
                 hipLaunchKernelGGL((k_onesweep<true, false, ArrayKeys<uint64_t>, uint64_t, RS_ITEMS, uint32_t>), g, b, 0,
                                    stream, in, low32, (const uint32_t*)vals, vals_alt, shift, so, si, tk, sort_status, tag, lo_bit);
+            } else if (heads && p == np - 1) {
+                const ArrayKeys<uint64_t> in{keys};
+                hipLaunchKernelGGL((k_onesweep<true, false, ArrayKeys<uint64_t>, uint64_t, RS_ITEMS, uint64_t, uint32_t,
+                                               uint32_t, false, true>),
+                                   g, b, 0, stream, in, keys_alt, (const uint32_t*)vals, vals_alt, shift, so, si, tk,
+                                   sort_status, tag, 0u, 0u, 0ull, heads, hmask);
+                if (heads_done) *heads_done = true;
             } else {
                 const ArrayKeys<uint64_t> in{keys};
                 if (!vals)
@@ -1288,7 +1316,8 @@ struct Primitives {
                 // keys in (the first pass: its key source) and out (a low32 last pass: 4 B), values in
                 // (not for identity values) and out
                 const bool out32 = low32 && p == np - 1;
-                uint64_t b = (p == 0 ? first.read_bytes(N) : 8ull * N) + (out32 ? 4ull : 8ull) * N;
+                const bool outh = heads && p == np - 1;  // a byte of heads instead of the key (the few run-end keys aside)
+                uint64_t b = (p == 0 ? first.read_bytes(N) : 8ull * N) + (out32 ? 4ull : outh ? 1ull : 8ull) * N;
                 if (split)  // index bytes in (not in the first pass) and out (the last pass: 4-byte indices)
                     b += (p == 0 ? 0ull : 1ull) * N + (p == np - 1 ? 4ull : 1ull) * N;
                 else if (vals)
