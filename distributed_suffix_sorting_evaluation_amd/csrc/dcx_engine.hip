@@ -749,6 +749,21 @@ class Engine {
         k_base_case<TextU32><<<1, 1024, 0, stream>>>(TextU32{t, n}, n, sa_out, rank.get());
     }
 
+    // The base case of a level of more than BASE_CASE_MAX characters (a base_case_chars above it): every
+    // suffix sorted by the refinement over its whole length in sentinel codes (characters 1..maxchar,
+    // the padding 0 below them), whose rounds double the compared prefix once it is long: prefix doubling
+    // on the device.  Suffixes of different lengths differ within n characters, so no group stays tied.
+    template <class CharT>
+    void base_case_large(const CharT* text, const uint16_t* lut, uint64_t n, uint32_t maxchar, DBuf<uint32_t>& sa) {
+        const uint32_t bits = bit_width(maxchar);
+        DBuf<uint64_t> PT;
+        make_stream(text, lut, n, bits, 0, 2 * n + 2, PT);
+        DBuf<uint8_t> H;
+        ActiveSet fin;
+        refine_sort(PT.get(), n, (uint32_t)n, bits, false, info_bits(text, lut, n, bits), sa, H, fin);
+        if (fin.A != 0) throw Error(DCX_E_HIP, "base case: suffixes left tied after their whole length");
+    }
+
     dcx_level_stats* level_stat(int depth) {
         if (!st || depth >= DCX_MAX_LEVELS) return nullptr;
         if ((uint32_t)depth + 1 > st->levels) st->levels = depth + 1;
@@ -921,6 +936,15 @@ class Engine {
         const uint64_t threshold = opt.base_case_chars > 2ull * X ? opt.base_case_chars : 2ull * X;
         if (n <= threshold) {
             if (ls) ls->base_case = 1;
+            if (n > BASE_CASE_MAX) {
+                DBuf<uint32_t> sa;
+                base_case_large(text, lut, n, maxchar, sa);
+                if (sa_take)
+                    *sa_take = std::move(sa);
+                else
+                    DCX_HIP(hipMemcpyAsync(sa_out, sa.get(), n * 4, hipMemcpyDeviceToDevice, stream));
+                return;
+            }
             if (sa_take) {
                 *sa_take = DBuf<uint32_t>(cache, n > 0 ? n : 1);
                 sa_out = sa_take->get();
@@ -1033,8 +1057,8 @@ class Engine {
         if (!(opt.discarding_threshold >= 0.0 && opt.discarding_threshold <= 1.0))
             throw Error(DCX_E_ARG, "discarding_threshold must lie in [0, 1]");
         set_dc(opt.X);
-        if (opt.base_case_chars > BASE_CASE_MAX)
-            throw Error(DCX_E_ARG, "base_case_chars must be <= " + std::to_string(BASE_CASE_MAX));
+        if (opt.base_case_chars > BASE_CASE_CHARS_MAX)
+            throw Error(DCX_E_ARG, "base_case_chars must be <= " + std::to_string(BASE_CASE_CHARS_MAX));
     }
 
     // compress_alphabet (cli.cpp:417-463): used bytes -> 1..sigma in byte order; returns sigma and the map

@@ -9,6 +9,9 @@ namespace dcx {
 constexpr uint32_t MAX_X = 133;
 constexpr uint32_t MAX_D = 12;
 constexpr uint32_t BASE_CASE_MAX = 16384;
+// the largest base_case_chars: levels of more than BASE_CASE_MAX characters at or below the threshold are
+// sorted by the refinement over their whole length, whose prefix lengths are 32-bit
+constexpr uint64_t BASE_CASE_CHARS_MAX = 1ull << 31;
 
 // Difference covers of lightweight_dcx/src/pdcx/difference_cover.hpp (DC3Param .. DC133Param, DC[D]).
 struct DcSet {

@@ -46,7 +46,7 @@ typedef struct dcx_options {
     uint32_t sa_on_device;        /* `sa_out` is a device pointer                            */
     uint32_t profile_kernels;     /* time every radix pass with hipEvents (fills sort_* stats) */
     double discarding_threshold;  /* --discarding-threshold (0.7, config.hpp:42)             */
-    uint64_t base_case_chars;     /* sequential base case: n <= max(2X, this) (pdcx.hpp:623)  */
+    uint64_t base_case_chars;     /* base case: n <= max(2X, this), at most 2^31 (pdcx.hpp:623) */
     uint64_t seed;                /* --seed (accepted; the SA does not depend on it)         */
     uint64_t batch_max_chars;     /* dcx_build_sa_batch: characters per sub-batch (0 = 2^24)  */
     uint64_t reserved[5];

@@ -550,7 +550,7 @@ void oracle_write_uint40(const uint64_t *SA, uint64_t n, uint8_t *out) {
 #include "dcx.h"
 
 #define HOST_MAX_TEXT (1ull << 32) /* the engine's limit: 32-bit positions 0 .. 2^32 - 1      */
-#define HOST_BASE_CASE_MAX 16384ull                 /* the engine's base-case limit       */
+#define HOST_BASE_CASE_MAX (1ull << 31)             /* the engine's base-case limit       */
 
 struct dcx_handle {
     char err[256];
@@ -592,7 +592,7 @@ static int host_check_options(dcx_handle *h, const dcx_options *o) {
     int known = 0;
     for (size_t i = 0; i < sizeof(DC_SETS) / sizeof(DC_SETS[0]); i++) known |= DC_SETS[i][0] == o->X;
     if (!known) return host_fail(h, DCX_E_INVALID_X, "X is not a supported difference cover");
-    if (o->base_case_chars > HOST_BASE_CASE_MAX) return host_fail(h, DCX_E_ARG, "base_case_chars must be <= 16384");
+    if (o->base_case_chars > HOST_BASE_CASE_MAX) return host_fail(h, DCX_E_ARG, "base_case_chars must be <= 2^31");
     return DCX_OK;
 }
 

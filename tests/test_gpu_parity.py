@@ -82,6 +82,27 @@ def test_base_case_kernel(engine):
     assert np.array_equal(sa, np.arange(16383, -1, -1, dtype=np.uint64))
 
 
+def test_large_base_case(engine):
+    """base_case_chars above the single-workgroup kernel's 16,384 characters: such a level is sorted by the
+    refinement over its whole length (prefix doubling).  Random, periodic and copied texts solved whole by
+    the base case, and a text whose deeper levels (32-bit names) fall under a large threshold; SAs and
+    per-level statistics (which level took the base case) against the oracle."""
+    # the oracle's base case is the reference's comparison sort of whole suffixes: repetitive texts stay short
+    cases = [("random4", random_text(60000, 4, 31), 100000), ("random256", random_text(40000, 256, 32), 40000),
+             ("run", periodic_text(20000, 1, 4, 33), 1 << 20), ("period7", periodic_text(20000, 7, 3, 34), 20000),
+             ("copies", copies_text(60000, 35, copy_frac=0.5, max_len=3000), 200000),
+             ("copies_deep", copies_text(400000, 36, copy_frac=0.4, max_len=5000), 60000)]
+    for name, t, base in cases:
+        for X in (7, 39):
+            st = check_case(engine, name, t, X, base=base)
+            assert any(l["base_case"] for l in st["levels"]), name
+    t = random_text(1 << 20, 4, 37)
+    say("base case of 2^20 characters")
+    sa, st = gpu_sa(engine, t, X=39, base_case_chars=1 << 20)
+    assert st["levels"][0]["base_case"] == 1 and len(st["levels"]) == 1
+    assert np.array_equal(sa, oracle.libsais_sa(t))
+
+
 @pytest.mark.parametrize("X", ALL_X)
 def test_random_all_x(engine, X):
     for n, sigma in ((3 * X + 5, 2), (5000, 4), (40000, 4), (40000, 26), (30000, 255), (30000, 256)):

@@ -252,7 +252,9 @@ def test_host_abi_argument_errors():
     assert abi_build(L, "dcx_host_", h, t, _host_opts(L, X=40), 8)[0] == -1
     assert abi_build(L, "dcx_host_", h, t, _host_opts(L), 6)[0] == -6
     assert abi_build(L, "dcx_host_", h, t, _host_opts(L, discarding_threshold=1.5), 8)[0] == -6
-    assert abi_build(L, "dcx_host_", h, t, _host_opts(L, base_case_chars=16385), 8)[0] == -6
+    assert abi_build(L, "dcx_host_", h, t, _host_opts(L, base_case_chars=(1 << 31) + 1), 8)[0] == -6
+    rc, out, _ = abi_build(L, "dcx_host_", h, t, _host_opts(L, base_case_chars=1 << 31), 8)
+    assert rc == 0 and np.frombuffer(out, dtype="<u8").tolist() == brute_sa(t).tolist()
     assert abi_build(L, "dcx_host_", h, t, _host_opts(L, sa_on_device=1), 8)[0] == -6
     rc, out, _ = abi_build(L, "dcx_host_", h, b"", _host_opts(L), 8)
     assert rc == 0 and out == b""
