@@ -166,6 +166,38 @@ class Engine {
             DCX_HIP(hipStreamSynchronize(stream));
             return;
         }
+        download_staged(src, bytes, 1, [dst](size_t off, const uint8_t* p, size_t len) {
+            memcpy(static_cast<uint8_t*>(dst) + off, p, len);
+        });
+    }
+
+    // n 32-bit SA entries from device memory to pageable host memory as `width`-byte entries (5: the low
+    // four bytes and a zero byte, little endian; 8), widened by the host threads as they copy each chunk out
+    // of the pinned ring: the link carries 4 bytes per entry instead of `width` (the positions are below
+    // 2^32).  For outputs of at least a quarter stage.
+    void download_widen(void* dst, const uint32_t* src, uint64_t n, uint32_t width) {
+        const bool aligned8 = (reinterpret_cast<uintptr_t>(dst) & 7) == 0;
+        download_staged(src, n * 4, 4, [dst, width, aligned8](size_t off, const uint8_t* p, size_t len) {
+            const uint32_t* q = reinterpret_cast<const uint32_t*>(p);
+            const size_t e0 = off / 4, ne = len / 4;
+            uint8_t* d = static_cast<uint8_t*>(dst) + e0 * width;
+            if (width == 8 && aligned8) {
+                uint64_t* d8 = reinterpret_cast<uint64_t*>(d);
+                for (size_t i = 0; i < ne; i++) d8[i] = q[i];
+            } else {
+                for (size_t i = 0; i < ne; i++) {
+                    const uint64_t v = q[i];
+                    memcpy(d + i * width, &v, width);  // little endian: the low `width` bytes of v
+                }
+            }
+        });
+    }
+
+    // The staged ring of download(): chunk k of the device bytes goes through stage k mod kStages; the host
+    // threads split each chunk into slices of whole `align`-byte units and call copy_out(offset in the
+    // source, pinned bytes, length) for them.
+    template <class CopyOut>
+    void download_staged(const void* src, size_t bytes, size_t align, CopyOut&& copy_out) {
         alloc_stages();
         const size_t SB = stage_bytes;
         const size_t nch = (bytes + SB - 1) / SB;
@@ -191,9 +223,9 @@ class Engine {
                         failed = 1;
                         return;
                     }
-                    const size_t len = chunk_len(k);
-                    const size_t a = len * t / T, b = len * (t + 1) / T;
-                    memcpy(static_cast<uint8_t*>(dst) + k * SB + a, h_stage[s] + a, b - a);
+                    const size_t units = chunk_len(k) / align;
+                    const size_t a = units * t / T * align, b = units * (t + 1) / T * align;
+                    if (b > a) copy_out(k * SB + a, h_stage[s] + a, b - a);
                     done[s].fetch_add(1);
                 }
             });
@@ -1383,6 +1415,8 @@ class Engine {
             widen(sa.get(), n, width, static_cast<uint8_t*>(sa_out));
         } else if (width == 4) {
             download(sa_out, sa.get(), n * 4);
+        } else if (n * 4 >= stage_bytes / 4) {
+            download_widen(sa_out, sa.get(), n, width);
         } else {
             DBuf<uint8_t> wide(cache, n * width);
             widen(sa.get(), n, width, wide.get());
