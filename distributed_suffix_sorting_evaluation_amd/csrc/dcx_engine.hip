@@ -84,6 +84,78 @@ constexpr uint32_t PACKED_SLACK = DCX_PACKED_SLACK;
 constexpr uint32_t LARGE_PARTITION_MIN = 1u << 16;
 constexpr uint32_t LARGE_PARTITION_AVG = 64;
 
+// Optimal alphabetic prefix code of the weights w[0, m): the binary tree with the symbols as leaves in
+// order that minimizes the weighted depth (Hu-Tucker's problem), by the dynamic program over contiguous
+// symbol ranges with the monotone split bounds of Knuth and Yao (root(i, j - 1) <= root(i, j) <=
+// root(i + 1, j)), O(m^2).  Any split gives a valid alphabetic code; the bounds only make it fast.
+// len[s] and code[s] (the codeword's len[s] bits, MSB first) per symbol; returns the longest length, or 0
+// when a codeword would exceed max_len bits.
+static uint32_t alphabetic_code(const std::vector<double>& w, uint32_t max_len, std::vector<uint32_t>& len,
+                                std::vector<uint32_t>& code) {
+    const int m = (int)w.size();
+    len.assign(m, 0);
+    code.assign(m, 0);
+    if (m == 0) return 0;
+    if (m == 1) {
+        len[0] = 1;
+        return 1;
+    }
+    std::vector<double> pre(m + 1, 0.0);
+    for (int i = 0; i < m; i++) pre[i + 1] = pre[i] + w[i];
+    std::vector<double> cost((size_t)m * m, 0.0);
+    std::vector<int> root((size_t)m * m, 0);
+    auto at = [m](int i, int j) { return (size_t)i * m + j; };
+    for (int i = 0; i < m; i++) root[at(i, i)] = i;
+    for (int d = 1; d < m; d++)
+        for (int i = 0; i + d < m; i++) {
+            const int j = i + d;
+            // split after k: leaves [i, k] left, [k + 1, j] right
+            const int lo = std::max(i, root[at(i, j - 1)]);
+            const int hi = std::max(lo, std::min(j - 1, root[at(i + 1, j)]));
+            double best = 0;
+            int bk = -1;
+            for (int k = lo; k <= hi; k++) {
+                const double c = cost[at(i, k)] + cost[at(k + 1, j)];
+                if (bk < 0 || c < best) {
+                    best = c;
+                    bk = k;
+                }
+            }
+            cost[at(i, j)] = best + (pre[j + 1] - pre[i]);
+            root[at(i, j)] = bk;
+        }
+    struct Node {
+        int i, j;
+        uint32_t depth, prefix;
+    };
+    std::vector<Node> st{{0, m - 1, 0, 0}};
+    uint32_t longest = 0;
+    while (!st.empty()) {
+        const Node nd = st.back();
+        st.pop_back();
+        if (nd.i == nd.j) {
+            len[nd.i] = nd.depth;
+            code[nd.i] = nd.prefix;
+            longest = std::max(longest, nd.depth);
+            continue;
+        }
+        if (nd.depth >= max_len) return 0;
+        const int k = root[at(nd.i, nd.j)];
+        st.push_back({nd.i, k, nd.depth + 1, nd.prefix << 1});
+        st.push_back({k + 1, nd.j, nd.depth + 1, (nd.prefix << 1) | 1u});
+    }
+    return longest;
+}
+
+// The round-0 key code of a level (ctab == nullptr: raw characters).  ctab: device table of 256 entries
+// (length << 24 | codeword left-aligned in 24 bits) indexed by character code; avg_len: the mean code
+// length of the sampled characters, entropy: their order-0 entropy.
+struct KeyCode {
+    const uint32_t* ctab = nullptr;
+    uint32_t max_len = 0;
+    double avg_len = 0, entropy = 0;
+};
+
 // Longest text (and longest joined batch, separators included) of this build: positions and the values of
 // the sorts are 32-bit (a text of 2^32 characters has positions 0 .. 2^32 - 1), while counts, offsets and
 // the grid-stride loops of the kernels are 64-bit.
@@ -457,10 +529,14 @@ class Engine {
     // ahead (exact: the last step takes only the L - C characters still missing).  Doubling needs the
     // sentinel coding (dense = false): the empty suffix past the end must be the smallest key.
     // `info`: estimated bits of information per character (at most `bits`), which sizes round 0.
+    // `code`: an order-preserving prefix code of the characters (dense words of at most 8 bits), which
+    // keys round 0 of a large sort when its key then needs fewer radix passes than the raw characters.
     void refine_sort(const uint64_t* PT, uint64_t N, uint32_t L, uint32_t bits, bool dense, double info,
-                     DBuf<uint32_t>& order, DBuf<uint8_t>& H, ActiveSet& fin) {
+                     DBuf<uint32_t>& order, DBuf<uint8_t>& H, ActiveSet& fin, const KeyCode& code = KeyCode{}) {
         const uint32_t cpw = 64 / bits;
         uint32_t c0 = L < cpw ? L : cpw;
+        bool coded = false;  // round 0 keyed by `code`: kbits code bits per suffix
+        uint32_t kbits = 0;
         const uint32_t pbits = bit_width(N - 1);
         const uint32_t cpack = std::min<uint32_t>(c0, (64 - pbits) / bits);
         if (N >= (1u << 16) && cpack * info >= bit_width(N) + PACKED_SLACK) {
@@ -481,6 +557,7 @@ class Engine {
             order = DBuf<uint32_t>(cache, N);
             k_heads_packed<<<grid(N / 16 + 1ull), 256, 0, stream>>>(k, N, kbits, (1ull << pbits) - 1, H.get(), order.get());
         } else {
+            kbits = c0 * bits;
             if (N >= (1u << 24)) {
                 // large sorts: round 0 takes the characters that carry about log2(N) + ROUND0_SLACK bits of
                 // information, rounded up to whole 8-bit digits; the positions still tied after it go
@@ -488,6 +565,17 @@ class Engine {
                 const uint32_t want = (uint32_t)std::ceil((bit_width(N) + ROUND0_SLACK) / info);
                 const uint32_t digits = std::min<uint32_t>(8, (std::max<uint32_t>(1, want) * bits + 7) / 8);
                 c0 = std::max<uint32_t>(1, std::min<uint32_t>(c0, 8 * digits / bits));
+                kbits = c0 * bits;
+                if (code.ctab && dense && bits <= 8 && code.entropy > 0 && L > c0) {
+                    // the same information from code bits (about entropy / avg_len of it per bit), at
+                    // most CODE_KEY_BITS_MAX of them: taken when that saves a radix pass
+                    const double need = (bit_width(N) + ROUND0_SLACK) * code.avg_len / code.entropy;
+                    const uint32_t cb = std::min<uint32_t>(CODE_KEY_BITS_MAX, 8 * (uint32_t)std::ceil(need / 8));
+                    if (cb / 8 < (kbits + 7) / 8) {
+                        coded = true;
+                        kbits = cb;
+                    }
+                }
             }
             DBuf<uint64_t> ka(cache, N), kb(cache, N);
             DBuf<uint32_t> va(cache, N), vb(cache, N);
@@ -499,41 +587,52 @@ class Engine {
             // pass writes only the 32 key bits below its digit, enough for the heads
             DBuf<uint32_t> low(cache, N);
             DBuf<uint8_t> ia, ib;  // keys of at most 40 bits leave room for the low 24 bits of a position
-            if (c0 * bits <= 40) {
+            if (kbits <= 40) {
                 ia = DBuf<uint8_t>(cache, N);
                 ib = DBuf<uint8_t>(cache, N);
             }
             // the heads: from the low words (at most five passes), else written by the last pass itself
             H = DBuf<uint8_t>(cache, (size_t)N + 1);
             bool heads_done = false;
-            const uint32_t np = prim.sort_pairs_from(StreamKeys<false>{PT, bits, top_mask(c0 * bits)}, k, k2, v, v2, N,
-                                                     64 - c0 * bits, 64, true, low.get(), ia.get(), ib.get(), H.get(),
-                                                     top_mask(c0 * bits), &heads_done);
+            uint32_t np;
+            if (coded) {
+                // the code keys in text order (the first pass's input; its output overwrites them later)
+                k_code_keys<<<grid(N, 16384), 256, 0, stream>>>(PT, bits, N, code.ctab, kbits, L, k);
+                np = prim.sort_pairs_from(ArrayKeys<uint64_t>{k}, k, k2, v, v2, N, 64 - kbits, 64, true, low.get(),
+                                          ia.get(), ib.get(), H.get(), top_mask(kbits), &heads_done);
+            } else {
+                np = prim.sort_pairs_from(StreamKeys<false>{PT, bits, top_mask(kbits)}, k, k2, v, v2, N, 64 - kbits,
+                                          64, true, low.get(), ia.get(), ib.get(), H.get(), top_mask(kbits),
+                                          &heads_done);
+            }
             if (np) {
                 k_heads_u32<<<grid(N / 16 + 1ull), 256, 0, stream>>>(low.get(), N, H.get());
                 k_mark_runs<<<1, RS_RADIX, 0, stream>>>(prim.hist + (np - 1) * RS_RADIX, N, H.get());
             } else if (heads_done) {
-                k_heads_fix<<<grid(N / 16 + 1ull), 256, 0, stream>>>(k, N, top_mask(c0 * bits), H.get());
+                k_heads_fix<<<grid(N / 16 + 1ull), 256, 0, stream>>>(k, N, top_mask(kbits), H.get());
                 prim.fill_u8(H.get() + N, 1, 1);
             } else {
-                k_heads<<<grid(N / 16 + 1ull), 256, 0, stream>>>(k, N, c0 * bits, H.get());
+                k_heads<<<grid(N / 16 + 1ull), 256, 0, stream>>>(k, N, kbits, H.get());
             }
             if (v == va.get())
                 order = std::move(va);
             else
                 order = std::move(vb);
         }
-        uint32_t C = c0;
+        // After a coded round 0 a group's members share k >= kmin whole characters (k <= L); the rounds
+        // then go on from character k + C of each member, and every group is resolved once kmin + C >= L.
+        uint32_t C = coded ? 0 : c0;
+        const uint32_t kmin = coded ? std::min<uint32_t>(L, kbits / code.max_len) : 0;
         ActiveSet cur;
         active_set(H.get(), nullptr, N, cur);
         DBuf<uint32_t> G;
-        while (C < L && cur.A > 0) {
+        while (kmin + C < L && cur.A > 0) {
             const uint64_t A = cur.A;
             const uint32_t gbits = bit_width(cur.G - 1);
             uint32_t c = (64 - gbits) / bits;
-            if (c > L - C) c = L - C;
+            if (c > L - kmin - C) c = L - kmin - C;
             if (c > cpw) c = cpw;
-            if (!dense && (L - C) > 3 * std::max<uint32_t>(c, 1)) {
+            if (!coded && !dense && (L - C) > 3 * std::max<uint32_t>(c, 1)) {
                 // doubling step
                 if (!G.get()) G = DBuf<uint32_t>(cache, N);
                 prim.scan(GroupScatterOp{H.get(), order.get(), G.get()}, N, 1);
@@ -560,14 +659,14 @@ class Engine {
             // a character round takes as many characters as one stream window holds, as far as the large
             // groups' keys (rank among the large groups, characters) still fit one word; small groups
             // compare the characters alone
-            c = std::min<uint32_t>(cpw, L - C);
+            c = std::min<uint32_t>(cpw, L - kmin - C);
             {
                 const uint32_t nlarge = cur.nlarge;  // counted with the active set
                 const uint32_t lbits = nlarge > 1 ? bit_width(nlarge - 1) : 0;
                 if (lbits + c * bits > 64 && 64 - lbits >= bits) c = (64 - lbits) / bits;
             }
             DBuf<uint8_t> Ha(cache, (size_t)A + 1);
-            refine_round(PT, bits, C, c, cur, order, H, Ha);
+            refine_round(RoundChars{PT, bits, C, c * bits, coded ? code.ctab : nullptr, kbits, L}, cur, order, H, Ha);
             C += c;
             ActiveSet next;
             active_set(Ha.get(), cur.act.get(), A, next, true);
@@ -609,18 +708,17 @@ class Engine {
     // each, middle ones (up to RF_WAVE members) by one wave each; the members of large groups go through a
     // radix sort keyed by (rank among the large groups, characters), or two stable sorts when those do not
     // fit 64 bits.
-    void refine_round(const uint64_t* PT, uint32_t bits, uint32_t C, uint32_t c, const ActiveSet& cur,
-                      DBuf<uint32_t>& order, DBuf<uint8_t>& H, DBuf<uint8_t>& Ha) {
+    void refine_round(const RoundChars& rc, const ActiveSet& cur, DBuf<uint32_t>& order, DBuf<uint8_t>& H,
+                      DBuf<uint8_t>& Ha) {
         const uint64_t A = cur.A;
         const uint32_t G = cur.G;
         DBuf<uint8_t> large(cache, G);
         k_refine_small<<<grid(G), 256, 0, stream>>>(cur.gstart.get(), G, cur.gfirst.get(), order.get(), H.get(),
-                                                    Ha.get(), PT, bits, C, c * bits, large.get());
+                                                    Ha.get(), rc, large.get());
         if (cur.nwave) {
             const uint64_t chunks = ceil_div((uint64_t)G, 64ull);
             k_refine_wave<<<(unsigned)std::min<uint64_t>(ceil_div(chunks, (uint64_t)RFW_WAVES), 8192), 64 * RFW_WAVES, 0,
-                            stream>>>(cur.gstart.get(), G, cur.gfirst.get(), order.get(), H.get(), Ha.get(), PT, bits,
-                                      C, c * bits);
+                            stream>>>(cur.gstart.get(), G, cur.gfirst.get(), order.get(), H.get(), Ha.get(), rc);
         }
         prim.fill_u8(Ha.get() + A, 1, 1);
         // members of the large groups and their count, known from the active set's group counts
@@ -631,7 +729,7 @@ class Engine {
         prim.scan(LargeMembersOp{large.get(), cur.gid.get(), slots.get(), lrank.get(), lfirst.get()}, A, 1);
         prim.fill_u32(lfirst.get() + nlarge, (uint32_t)B, 1);  // B = 2^32 reads as 0: spans are differences
         const uint32_t lbits = bit_width(nlarge - 1);
-        const uint32_t cbits = c * bits;
+        const uint32_t cbits = rc.cbits;
         DBuf<uint64_t> rka(cache, B), rkb(cache, B);
         DBuf<uint32_t> rva(cache, B), rvb(cache, B);
         uint64_t* x = rka.get();
@@ -641,9 +739,8 @@ class Engine {
         if (lbits + cbits <= 64) {
             const bool part = B >= LARGE_PARTITION_MIN && B >= (uint64_t)LARGE_PARTITION_AVG * nlarge;
             DBuf<uint8_t> cls(cache, part ? B : 0);
-            k_keys_large<<<grid(B), 256, 0, stream>>>(PT, bits, B, slots.get(), cur.act.get(), lrank.get(),
-                                                      order.get(), C, cbits, lbits, x, y, lfirst.get(),
-                                                      part ? cls.get() : nullptr);
+            k_keys_large<<<grid(B), 256, 0, stream>>>(rc, B, slots.get(), cur.act.get(), lrank.get(), order.get(),
+                                                      lbits, x, y, lfirst.get(), part ? cls.get() : nullptr);
             if (part && partition_large(cur, slots.get(), lrank.get(), lfirst.get(), cls.get(), B, 64 - lbits - cbits,
                                         x, y, x2, y2, order, H, Ha))
                 return;
@@ -654,14 +751,13 @@ class Engine {
         }
         // characters first, then a stable sort by the group rank (the members are identified by index)
         DBuf<uint32_t> P(cache, B);
-        k_keys_large_chars<<<grid(B), 256, 0, stream>>>(PT, bits, B, slots.get(), cur.act.get(), order.get(), C,
-                                                        cbits, x, P.get());
+        k_keys_large_chars<<<grid(B), 256, 0, stream>>>(rc, B, slots.get(), cur.act.get(), order.get(), x, P.get());
         prim.sort_pairs_from(ArrayKeys<uint64_t>{x}, x, x2, y, y2, B, 64 - cbits, 64, true);
         k_keys_large_rank<<<grid(B), 256, 0, stream>>>(y, lrank.get(), B, x2);
         std::swap(x, x2);
         prim.sort_pairs(x, x2, y, y2, B, 0, lbits);
-        k_writeback_large2<<<grid(B), 256, 0, stream>>>(x, y, P.get(), PT, bits, C, cbits, slots.get(), B,
-                                                        cur.act.get(), order.get(), H.get(), Ha.get());
+        k_writeback_large2<<<grid(B), 256, 0, stream>>>(x, y, P.get(), rc, slots.get(), B, cur.act.get(), order.get(),
+                                                        H.get(), Ha.get());
     }
 
     // Large groups by a stable three-way partition around each group's first key (PivotScanOp), when at
@@ -906,19 +1002,27 @@ class Engine {
     // order-0 entropy (bits per character, within [1, bits]) of a byte text from a strided sample of its
     // codes; names of deeper levels are taken at their full width
     double info_bits(const uint8_t* text, const uint16_t* lut, uint64_t n, uint32_t bits, double entropy = -1.0) {
+        have_counts = false;
         if (n < (1u << 24)) return bits;
         const double e = entropy >= 0 ? entropy : code_entropy(text, lut, n);
         return std::min<double>(bits, std::max(1.0, e));
     }
-    double info_bits(const uint32_t*, const uint16_t*, uint64_t, uint32_t bits, double = -1.0) { return bits; }
-    // order-0 entropy of the codes lut[text[i]] over a strided sample of i
+    double info_bits(const uint32_t*, const uint16_t*, uint64_t, uint32_t bits, double = -1.0) {
+        have_counts = false;
+        return bits;
+    }
+    // order-0 entropy of the codes lut[text[i]] over a strided sample of i; the sample's counts stay in
+    // code_counts (have_counts) for the round-0 key code
+    unsigned long long code_counts[257];
+    bool have_counts = false;
     double code_entropy(const uint8_t* text, const uint16_t* lut, uint64_t n) {
         const uint64_t stride = 61;  // odd: no alignment with periodic structure
         DBuf<unsigned long long> hist(cache, 257);
         prim.fill_u64(reinterpret_cast<uint64_t*>(hist.get()), 0, 257);
         k_code_hist<<<grid(ceil_div(n, stride * 256) * 256, 4096), 256, 0, stream>>>(text, n, lut, stride, hist.get());
-        unsigned long long h[257];
-        memcpy(h, prim.read_device(hist.get(), sizeof(h)), sizeof(h));
+        unsigned long long* h = code_counts;
+        memcpy(h, prim.read_device(hist.get(), sizeof(code_counts)), sizeof(code_counts));
+        have_counts = true;
         double total = 0, e = 0;
         for (int i = 0; i < 257; i++) total += (double)h[i];
         for (int i = 0; i < 257; i++)
@@ -927,6 +1031,44 @@ class Engine {
                 e -= p * std::log2(p);
             }
         return e;
+    }
+
+    // The round-0 key code of a level-0 text (alphabetic_code) from the sampled counts code_counts of its
+    // characters 1..maxchar, which the dense stream codes as 0..maxchar - 1.  Characters the sample missed
+    // weigh as one in 2^18 sampled characters, which keeps the codewords short; a code whose longest
+    // codeword would exceed CODE_MAX_LEN bits is not used.  The device table goes to d_ctab.
+    uint32_t h_ctab[256];
+    KeyCode key_code(uint32_t maxchar, DBuf<uint32_t>& d_ctab) {
+        KeyCode kc;
+        if (!have_counts || maxchar < 2 || maxchar > 256) return kc;
+        double total = 0;
+        for (uint32_t c = 1; c <= maxchar; c++) total += (double)code_counts[c];
+        if (total <= 0) return kc;
+        const double floor_w = std::max(1.0, total / (double)(1u << 18));
+        std::vector<double> w(maxchar);
+        double e = 0;
+        for (uint32_t c = 0; c < maxchar; c++) {
+            const double h = (double)code_counts[c + 1];
+            w[c] = std::max(h, floor_w);
+            if (h > 0) e -= h / total * std::log2(h / total);
+        }
+        std::vector<uint32_t> len, cw;
+        const uint32_t longest = alphabetic_code(w, CODE_MAX_LEN, len, cw);
+        if (longest == 0) return kc;
+        double avg = 0;
+        for (uint32_t c = 0; c < maxchar; c++) avg += (double)code_counts[c + 1] * len[c];
+        avg /= total;
+        for (uint32_t c = 0; c < 256; c++) {
+            const uint32_t d = c < maxchar ? c : maxchar - 1;  // codes past the alphabet do not occur
+            h_ctab[c] = (len[d] << 24) | (cw[d] << (24 - len[d]));
+        }
+        d_ctab = DBuf<uint32_t>(cache, 256);
+        DCX_HIP(hipMemcpyAsync(d_ctab.get(), h_ctab, sizeof(h_ctab), hipMemcpyHostToDevice, stream));
+        kc.ctab = d_ctab.get();
+        kc.max_len = longest;
+        kc.avg_len = avg;
+        kc.entropy = e;
+        return kc;
     }
 
     // the level's characters as a packed code stream (k_pack_text) of `chars` characters
@@ -1030,7 +1172,10 @@ class Engine {
                 (*stream_fn)(mbits, dense ? 1 : 0, n + Ls + 2, PT);
             else
                 make_stream(text, lut, n, mbits, dense ? 1 : 0, n + Ls + 2, PT);
-            refine_sort(PT.get(), n, Ls, mbits, dense, info_bits(text, lut, n, mbits, entropy), order, H, tied);
+            const double info = info_bits(text, lut, n, mbits, entropy);
+            DBuf<uint32_t> d_ctab;
+            const KeyCode kc = depth == 0 && !stream_fn && dense && mbits <= 8 ? key_code(maxchar, d_ctab) : KeyCode{};
+            refine_sort(PT.get(), n, Ls, mbits, dense, info, order, H, tied, kc);
             if (dense && tied.A > 0) boundary_fixup(PT.get(), mbits, Ls, n, dc.bnd, order, H, tied);
         }
 

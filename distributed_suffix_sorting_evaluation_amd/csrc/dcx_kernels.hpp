@@ -273,6 +273,87 @@ __device__ __forceinline__ uint64_t top_bits(uint64_t w, uint32_t nbits) {
     return nbits >= 64 ? w : (nbits == 0 ? 0ull : (w & (~0ull << (64 - nbits))));
 }
 
+// ------------------------------------------------------------------------------------------------
+// Round 0 keyed by an order-preserving prefix code.  A byte text of order-0 entropy H carries H bits per
+// 8-bit character, so a key of raw characters spends 8 bits for H bits of ordering.  An alphabetic code
+// (codewords in character order, none a prefix of another: CODE_MAX_LEN bits at most) orders the bit
+// strings of coded suffixes as the suffixes themselves, because two suffixes first differ inside the
+// codewords of their first differing characters.  A round-0 key is the first kbits bits of the codes of
+// the suffix's first L characters (zeros after the L-th), so a key of kbits bits resolves about kbits / H
+// characters instead of kbits / 8.  Suffixes with equal keys share the same whole characters inside the
+// kbits bits (a prefix code decodes one way), k of them, and the group rounds go on from character k,
+// which every member finds from its own characters (code_chars).  ctab[c] = (length << 24) | the codeword
+// of character code c, left-aligned in the low 24 bits.
+// ------------------------------------------------------------------------------------------------
+constexpr uint32_t CODE_MAX_LEN = 24;
+constexpr uint32_t CODE_KEY_BITS_MAX = 40;  // codewords are placed at bit 40 - (bits filled) and above
+
+// whole characters covered by the first kbits code bits of suffix p (at most L)
+__device__ __forceinline__ uint32_t code_chars(const uint64_t* __restrict__ s, uint32_t bits, uint64_t p,
+                                               const uint32_t* __restrict__ ctab, uint32_t kbits, uint32_t L) {
+    const StreamKeys<false> win{s, bits, ~0ull};
+    const uint32_t cpw = 64 / bits, cm = (1u << bits) - 1;
+    uint32_t k = 0, sum = 0;
+    while (k < L) {
+        const uint64_t x = win(p + k);
+        for (uint32_t i = 0; i < cpw; i++) {
+            if (k == L) return L;
+            sum += ctab[(uint32_t)(x >> (64 - bits * (i + 1))) & cm] >> 24;
+            if (sum > kbits) return k;
+            k++;
+        }
+    }
+    return L;
+}
+
+// keys[p] = the first kbits bits (top-aligned) of the codes of characters p, p + 1, ..., p + L - 1 of the
+// stream s (`bits` <= 8 bits per character), for p < N.  kbits <= CODE_KEY_BITS_MAX.
+__global__ __launch_bounds__(256) void k_code_keys(const uint64_t* __restrict__ s, uint32_t bits, uint64_t N,
+                                                   const uint32_t* __restrict__ ctab, uint32_t kbits, uint32_t L,
+                                                   uint64_t* __restrict__ keys) {
+    __shared__ uint32_t s_tab[256];
+    s_tab[threadIdx.x] = ctab[threadIdx.x];
+    __syncthreads();
+    const StreamKeys<false> win{s, bits, ~0ull};
+    const uint32_t cpw = 64 / bits, cm = (1u << bits) - 1;
+    const uint64_t km = top_mask(kbits);
+    for (uint64_t p = blockIdx.x * 256ull + threadIdx.x; p < N; p += gridDim.x * 256ull) {
+        uint64_t acc = 0;
+        uint32_t filled = 0, k = 0;
+        while (filled < kbits && k < L) {
+            const uint64_t x = win(p + k);
+            for (uint32_t i = 0; i < cpw && filled < kbits && k < L; i++, k++) {
+                const uint32_t e = s_tab[(uint32_t)(x >> (64 - bits * (i + 1))) & cm];
+                acc |= (uint64_t)(e & 0xFFFFFFu) << (CODE_KEY_BITS_MAX - filled);
+                filled += e >> 24;
+            }
+        }
+        keys[p] = acc & km;
+    }
+}
+
+// The next characters of a suffix in a group round: `cbits` bits of its characters from C on.  After a
+// coded round 0 (ctab != nullptr) a member's characters start at k + C instead, k = code_chars of the
+// suffix (the same for every member of a group), and stop at character L.
+struct RoundChars {
+    const uint64_t* s;
+    uint32_t bits, C, cbits;
+    const uint32_t* ctab;
+    uint32_t kbits, L;
+    __device__ __forceinline__ uint64_t operator()(uint64_t p) const {
+        uint64_t start = p + C;
+        uint32_t nb = cbits;
+        if (ctab) {
+            const uint32_t k = code_chars(s, bits, p, ctab, kbits, L);
+            const uint32_t c0 = k + C < L ? k + C : L;
+            start = p + c0;
+            nb = cbits < (L - c0) * bits ? cbits : (L - c0) * bits;
+        }
+        const StreamKeys<false> w{s, bits, top_mask(nb)};
+        return w(start);
+    }
+};
+
 // group heads of sorted keys, comparing only the top kbits
 // A thread per 16 consecutive positions (a grid of N / 16 + 1 threads): eight 16-byte key loads, one
 // 16-byte store of flags (keys and H are 256-byte aligned buffers).
@@ -439,19 +520,6 @@ struct ActiveScanOp {
         }
     }
 };
-
-__global__ __launch_bounds__(256) void k_keys_rr(const uint64_t* __restrict__ PT, uint32_t bits, uint64_t A,
-                                                 const uint32_t* __restrict__ act_idx, const uint32_t* __restrict__ gid,
-                                                 const uint32_t* __restrict__ order, uint32_t C, uint32_t cbits,
-                                                 uint32_t gbits, uint64_t* __restrict__ keys, uint32_t* __restrict__ vals) {
-    const StreamKeys<false> chars{PT, bits, top_mask(cbits)};
-    for (uint64_t a = blockIdx.x * 256ull + threadIdx.x; a < A; a += gridDim.x * 256ull) {
-        uint32_t p = order[act_idx[a]];
-        uint64_t ch = chars((uint64_t)p + C);
-        keys[a] = gbits ? (((uint64_t)gid[a] << (64 - gbits)) | (ch >> gbits)) : ch;
-        vals[a] = p;
-    }
-}
 
 // sorted active list back into `order`; with `mark`, new group starts into H and, if Ha is given, the
 // compact heads of the list (Ha[a] for entry a, Ha[A] = 1) for the next active set
@@ -815,9 +883,7 @@ constexpr uint32_t RF_WAVE = 512;
 __global__ __launch_bounds__(256) void k_refine_small(const uint32_t* __restrict__ gstart, uint32_t G,
                                                       const uint32_t* __restrict__ gfirst, uint32_t* __restrict__ order,
                                                       uint8_t* __restrict__ H, uint8_t* __restrict__ Ha,
-                                                      const uint64_t* __restrict__ PT, uint32_t bits, uint32_t C,
-                                                      uint32_t cbits, uint8_t* __restrict__ large) {
-    const StreamKeys<false> chars{PT, bits, top_mask(cbits)};
+                                                      RoundChars chars, uint8_t* __restrict__ large) {
     for (uint64_t g = blockIdx.x * 256ull + threadIdx.x; g < G; g += gridDim.x * 256ull) {
         const uint32_t a0 = gstart[g], s = gstart[g + 1] - a0;
         if (s > RF_SMALL) {
@@ -828,7 +894,7 @@ __global__ __launch_bounds__(256) void k_refine_small(const uint32_t* __restrict
         const uint32_t j0 = gfirst[g];  // a group occupies consecutive slots of the order
         if (s == 2) {  // the common case (a copied segment and its source): one comparison
             const uint32_t e0 = order[j0], e1 = order[j0 + 1];
-            const uint64_t k0 = chars((uint64_t)e0 + C), k1 = chars((uint64_t)e1 + C);
+            const uint64_t k0 = chars((uint64_t)e0), k1 = chars((uint64_t)e1);
             if (k1 < k0) {
                 order[j0] = e1;
                 order[j0 + 1] = e0;
@@ -842,7 +908,7 @@ __global__ __launch_bounds__(256) void k_refine_small(const uint32_t* __restrict
         uint32_t e[RF_SMALL];
         for (uint32_t t = 0; t < s; t++) {
             e[t] = order[j0 + t];
-            k[t] = chars((uint64_t)e[t] + C);
+            k[t] = chars((uint64_t)e[t]);
         }
         for (uint32_t t = 1; t < s; t++) {  // stable: equal keys keep their order
             const uint64_t kx = k[t];
@@ -888,9 +954,7 @@ __device__ __forceinline__ uint32_t rfw_class_p(uint32_t i, const uint32_t (&cen
 __global__ __launch_bounds__(64 * RFW_WAVES) void k_refine_wave(const uint32_t* __restrict__ gstart, uint32_t G,
                                                               const uint32_t* __restrict__ gfirst,
                                                               uint32_t* __restrict__ order, uint8_t* __restrict__ H,
-                                                              uint8_t* __restrict__ Ha,
-                                                              const uint64_t* __restrict__ PT, uint32_t bits,
-                                                              uint32_t C, uint32_t cbits) {
+                                                              uint8_t* __restrict__ Ha, RoundChars chars) {
     __shared__ uint64_t s_key[RFW_WAVES][RF_WAVE];
     __shared__ uint32_t s_pos[RFW_WAVES][RF_WAVE];
     __shared__ uint16_t s_idx[RFW_WAVES][RF_WAVE];
@@ -898,7 +962,6 @@ __global__ __launch_bounds__(64 * RFW_WAVES) void k_refine_wave(const uint32_t* 
     __shared__ uint32_t s_bs[RFW_WAVES][RFW_MAXG];
     __shared__ uint32_t s_bj0[RFW_WAVES][RFW_MAXG];
     __shared__ uint32_t s_ba0[RFW_WAVES][RFW_MAXG];
-    const StreamKeys<false> chars{PT, bits, top_mask(cbits)};
     const uint32_t lane = __lane_id(), w = threadIdx.x / 64;
     uint64_t* key = s_key[w];
     uint32_t* pos = s_pos[w];
@@ -979,7 +1042,7 @@ __global__ __launch_bounds__(64 * RFW_WAVES) void k_refine_wave(const uint32_t* 
             for (int r = 0; r < R; r++) {
                 const uint32_t e = lane + 64u * r;
                 kv[r] = ~0ull;
-                if (e < T && et[r] < bsz[ek[r]]) kv[r] = chars((uint64_t)ep[r] + C);
+                if (e < T && et[r] < bsz[ek[r]]) kv[r] = chars((uint64_t)ep[r]);
             }
 #pragma unroll
             for (int r = 0; r < R; r++) {
@@ -1067,23 +1130,20 @@ struct LargeMembersOp {
 // radix keys of the large-group members: (large-group rank << (64 - lbits)) | next cbits bits of characters
 // (lbits + cbits <= 64); values are the suffixes.  cls (optional): the member's key against its group's
 // first member's, 0 smaller, 1 equal, 2 larger (that key recomputed: one stream window per group, cached)
-__global__ __launch_bounds__(256) void k_keys_large(const uint64_t* __restrict__ PT, uint32_t bits, uint64_t B,
-                                                    const uint32_t* __restrict__ slots,
+__global__ __launch_bounds__(256) void k_keys_large(RoundChars chars, uint64_t B, const uint32_t* __restrict__ slots,
                                                     const uint32_t* __restrict__ act_idx,
                                                     const uint32_t* __restrict__ lrank,
-                                                    const uint32_t* __restrict__ order, uint32_t C, uint32_t cbits,
-                                                    uint32_t lbits, uint64_t* __restrict__ keys,
-                                                    uint32_t* __restrict__ vals, const uint32_t* __restrict__ lfirst,
-                                                    uint8_t* __restrict__ cls) {
-    const StreamKeys<false> chars{PT, bits, top_mask(cbits)};
+                                                    const uint32_t* __restrict__ order, uint32_t lbits,
+                                                    uint64_t* __restrict__ keys, uint32_t* __restrict__ vals,
+                                                    const uint32_t* __restrict__ lfirst, uint8_t* __restrict__ cls) {
     for (uint64_t b = blockIdx.x * 256ull + threadIdx.x; b < B; b += gridDim.x * 256ull) {
         const uint32_t p = order[act_idx[slots[b]]];
-        const uint64_t ch = chars((uint64_t)p + C);
+        const uint64_t ch = chars((uint64_t)p);
         keys[b] = lbits ? (((uint64_t)lrank[b] << (64 - lbits)) | (ch >> lbits)) : ch;
         vals[b] = p;
         if (cls) {
             const uint32_t fb = lfirst[lrank[b]];
-            const uint64_t cf = chars((uint64_t)order[act_idx[slots[fb]]] + C);
+            const uint64_t cf = chars((uint64_t)order[act_idx[slots[fb]]]);
             const uint64_t x = ch >> lbits, pv = cf >> lbits;  // the same group rank above both
             cls[b] = x < pv ? 0 : (x == pv ? 1 : 2);
         }
@@ -1211,16 +1271,14 @@ __global__ __launch_bounds__(256) void k_uniform_heads(uint64_t B, const uint32_
 
 // The same when rank and characters do not fit one word: an LSD sort by the characters (values: member
 // indices b, suffixes kept in P), then a stable sort by the rank.
-__global__ __launch_bounds__(256) void k_keys_large_chars(const uint64_t* __restrict__ PT, uint32_t bits, uint64_t B,
+__global__ __launch_bounds__(256) void k_keys_large_chars(RoundChars chars, uint64_t B,
                                                           const uint32_t* __restrict__ slots,
                                                           const uint32_t* __restrict__ act_idx,
-                                                          const uint32_t* __restrict__ order, uint32_t C,
-                                                          uint32_t cbits, uint64_t* __restrict__ keys,
-                                                          uint32_t* __restrict__ P) {
-    const StreamKeys<false> chars{PT, bits, top_mask(cbits)};
+                                                          const uint32_t* __restrict__ order,
+                                                          uint64_t* __restrict__ keys, uint32_t* __restrict__ P) {
     for (uint64_t b = blockIdx.x * 256ull + threadIdx.x; b < B; b += gridDim.x * 256ull) {
         const uint32_t p = order[act_idx[slots[b]]];
-        keys[b] = chars((uint64_t)p + C);
+        keys[b] = chars((uint64_t)p);
         P[b] = p;
     }
 }
@@ -1231,20 +1289,18 @@ __global__ __launch_bounds__(256) void k_keys_large_rank(const uint32_t* __restr
 }
 __global__ __launch_bounds__(256) void k_writeback_large2(const uint64_t* __restrict__ rkeys,
                                                           const uint32_t* __restrict__ members,
-                                                          const uint32_t* __restrict__ P,
-                                                          const uint64_t* __restrict__ PT, uint32_t bits, uint32_t C,
-                                                          uint32_t cbits, const uint32_t* __restrict__ slots, uint64_t B,
+                                                          const uint32_t* __restrict__ P, RoundChars chars,
+                                                          const uint32_t* __restrict__ slots, uint64_t B,
                                                           const uint32_t* __restrict__ act_idx,
                                                           uint32_t* __restrict__ order, uint8_t* __restrict__ H,
                                                           uint8_t* __restrict__ Ha) {
-    const StreamKeys<false> chars{PT, bits, top_mask(cbits)};
     for (uint64_t i = blockIdx.x * 256ull + threadIdx.x; i < B; i += gridDim.x * 256ull) {
         const uint32_t a = slots[i];
         const uint32_t j = act_idx[a];
         const uint32_t p = P[members[i]];
         order[j] = p;
         const bool head = i == 0 || rkeys[i] != rkeys[i - 1] ||
-                          chars((uint64_t)p + C) != chars((uint64_t)P[members[i - 1]] + C);
+                          chars((uint64_t)p) != chars((uint64_t)P[members[i - 1]]);
         Ha[a] = head ? 1 : 0;
         if (head) H[j] = 1;
     }

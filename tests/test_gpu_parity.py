@@ -272,6 +272,38 @@ def test_genome_64mib_digest(engine):
     assert engine.check_sa(t, sa, 8)
 
 
+def _coded_round0_texts():
+    """Texts of 2^24 + a few characters over power-of-two alphabets (dense character words) whose first sort
+    round is keyed by the alphabetic code of their characters: Zipf bytes with a periodic run, bytes that are
+    95% one value (codewords of one bit: 40 code bits reach past the key length), and a skewed 4-letter text
+    (the code resolves 30 characters in 4 radix passes instead of 8)."""
+    rng = np.random.default_rng(77)
+    n = (1 << 24) + 4097
+    r = np.arange(1, 257, dtype=np.float64) ** -1.1
+    z = rng.choice(256, size=n, p=r / r.sum()).astype(np.uint8)
+    z[5_000_000:5_000_000 + 170_000] = np.tile(z[:17], 10_000)
+    z[-300:] = z[1000:1300]  # the text's end repeats an earlier stretch: boundary suffixes in tied groups
+    skew = np.where(rng.random(n) < 0.95, 7, rng.integers(0, 256, size=n)).astype(np.uint8)
+    skew[:256] = np.arange(256, dtype=np.uint8)  # all 256 byte values occur
+    dna = rng.choice(4, size=n, p=[0.82, 0.1, 0.05, 0.03]).astype(np.uint8) + ord("A")
+    return [("zipf", z), ("one_value_95", skew), ("dna_skewed", dna)]
+
+
+@pytest.mark.parametrize("case", [0, 1, 2])
+def test_coded_round0_against_oracle(engine, case):
+    """Round 0 keyed by an alphabetic code (texts of at least 2^24 characters with dense character words):
+    the SA and every level's statistics (names, reduced, recursion) equal the oracle's, so the groups after
+    the code-keyed round and the group rounds that continue from each member's decoded character count are
+    exactly the groups of equal L-character keys."""
+    name, t = _coded_round0_texts()[case]
+    for X in (39, 7):
+        say(f"coded round 0 {name} n={t.size} X={X}")
+        sa, st = gpu_sa(engine, t, X=X)
+        ref, ost = oracle.dcx_sa(t, X=X, with_stats=True)
+        assert np.array_equal(sa, ref), (name, X)
+        assert same_stats(st["levels"], ost), (name, X, st["levels"], ost)
+
+
 def test_zipf_16mib_digest_dc21_dc39_dc133(engine):
     """The byte-alphabet workload (Zipf(1.1) over 256 values plus a period-17 block), sigma = 256 (9-bit
     packing), at 16 MiB: every X gives the SA whose libsais digest is committed."""
