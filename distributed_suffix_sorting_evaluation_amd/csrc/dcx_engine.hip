@@ -596,10 +596,10 @@ class Engine {
             bool heads_done = false;
             uint32_t np;
             if (coded) {
-                // the code keys in text order (the first pass's input; its output overwrites them later)
-                k_code_keys<<<grid(N, 16384), 256, 0, stream>>>(PT, bits, N, code.ctab, kbits, L, k);
-                np = prim.sort_pairs_from(ArrayKeys<uint64_t>{k}, k, k2, v, v2, N, 64 - kbits, 64, true, low.get(),
-                                          ia.get(), ib.get(), H.get(), top_mask(kbits), &heads_done);
+                // the histogram kernel writes the code keys into k, the first pass reads them from there
+                np = prim.sort_pairs_from(CodedKeys{PT, bits, code.ctab, kbits, L, k}, k, k2, v, v2, N, 64 - kbits,
+                                          64, true, low.get(), ia.get(), ib.get(), H.get(), top_mask(kbits),
+                                          &heads_done);
             } else {
                 np = prim.sort_pairs_from(StreamKeys<false>{PT, bits, top_mask(kbits)}, k, k2, v, v2, N, 64 - kbits,
                                           64, true, low.get(), ia.get(), ib.get(), H.get(), top_mask(kbits),

@@ -286,50 +286,29 @@ __device__ __forceinline__ uint64_t top_bits(uint64_t w, uint32_t nbits) {
 // of character code c, left-aligned in the low 24 bits.
 // ------------------------------------------------------------------------------------------------
 constexpr uint32_t CODE_MAX_LEN = 24;
-constexpr uint32_t CODE_KEY_BITS_MAX = 40;  // codewords are placed at bit 40 - (bits filled) and above
+constexpr uint32_t CODE_KEY_BITS_MAX = CK_BITS;  // round-0 keys are CodedKeys (primitives.hpp)
 
 // whole characters covered by the first kbits code bits of suffix p (at most L)
 __device__ __forceinline__ uint32_t code_chars(const uint64_t* __restrict__ s, uint32_t bits, uint64_t p,
                                                const uint32_t* __restrict__ ctab, uint32_t kbits, uint32_t L) {
+    // eight characters per step, their lengths loaded together (bits <= 8: a window holds at least eight)
     const StreamKeys<false> win{s, bits, ~0ull};
-    const uint32_t cpw = 64 / bits, cm = (1u << bits) - 1;
+    const uint32_t cm = (1u << bits) - 1;
     uint32_t k = 0, sum = 0;
     while (k < L) {
         const uint64_t x = win(p + k);
-        for (uint32_t i = 0; i < cpw; i++) {
+        uint32_t len[8];
+#pragma unroll
+        for (int i = 0; i < 8; i++) len[i] = ctab[(uint32_t)(x >> (64 - bits * (i + 1))) & cm] >> 24;
+#pragma unroll
+        for (int i = 0; i < 8; i++) {
             if (k == L) return L;
-            sum += ctab[(uint32_t)(x >> (64 - bits * (i + 1))) & cm] >> 24;
+            sum += len[i];
             if (sum > kbits) return k;
             k++;
         }
     }
     return L;
-}
-
-// keys[p] = the first kbits bits (top-aligned) of the codes of characters p, p + 1, ..., p + L - 1 of the
-// stream s (`bits` <= 8 bits per character), for p < N.  kbits <= CODE_KEY_BITS_MAX.
-__global__ __launch_bounds__(256) void k_code_keys(const uint64_t* __restrict__ s, uint32_t bits, uint64_t N,
-                                                   const uint32_t* __restrict__ ctab, uint32_t kbits, uint32_t L,
-                                                   uint64_t* __restrict__ keys) {
-    __shared__ uint32_t s_tab[256];
-    s_tab[threadIdx.x] = ctab[threadIdx.x];
-    __syncthreads();
-    const StreamKeys<false> win{s, bits, ~0ull};
-    const uint32_t cpw = 64 / bits, cm = (1u << bits) - 1;
-    const uint64_t km = top_mask(kbits);
-    for (uint64_t p = blockIdx.x * 256ull + threadIdx.x; p < N; p += gridDim.x * 256ull) {
-        uint64_t acc = 0;
-        uint32_t filled = 0, k = 0;
-        while (filled < kbits && k < L) {
-            const uint64_t x = win(p + k);
-            for (uint32_t i = 0; i < cpw && filled < kbits && k < L; i++, k++) {
-                const uint32_t e = s_tab[(uint32_t)(x >> (64 - bits * (i + 1))) & cm];
-                acc |= (uint64_t)(e & 0xFFFFFFu) << (CODE_KEY_BITS_MAX - filled);
-                filled += e >> 24;
-            }
-        }
-        keys[p] = acc & km;
-    }
 }
 
 // The next characters of a suffix in a group round: `cbits` bits of its characters from C on.  After a

@@ -101,6 +101,24 @@ struct StreamKeys {
     uint64_t read_bytes(uint64_t n) const { return (n * bits + 63) / 64 * 8; }  // the stream words of n keys
 };
 
+// Keys of an order-preserving prefix code of a character stream: key(i) = the first kbits bits
+// (top-aligned) of the codewords of characters i, i + 1, ..., i + L - 1 of s (`bits` <= 8 bits per
+// character), zeros after the L-th; ctab[c] = (length << 24) | the codeword of character code c,
+// left-aligned in the low 24 bits.  Every codeword has at least one bit, so a key reads fewer than
+// halo() characters from i on.  The sort's histogram kernel (k_coded_hist_seg) computes the keys and
+// writes them to mat (N words), which the first pass then reads.
+constexpr uint32_t CK_BITS = 40;  // longest key: codewords are placed at bit CK_BITS - (bits filled) and up
+struct CodedKeys {
+    const uint64_t* s;
+    uint32_t bits;
+    const uint32_t* ctab;
+    uint32_t kbits, L;
+    uint64_t* mat;
+    __device__ __forceinline__ uint64_t operator()(uint64_t i) const { return mat[i]; }
+    uint64_t read_bytes(uint64_t n) const { return 8 * n; }
+    __host__ __device__ uint32_t halo() const { return L < CK_BITS ? L : CK_BITS; }
+};
+
 // keys[i] = src(i) for i < N (sorts of a single element)
 template <class Src>
 __global__ void k_materialize(Src src, uint64_t N, uint64_t* __restrict__ keys) {
@@ -160,6 +178,79 @@ __global__ __launch_bounds__(RS_HIST_THREADS) void k_radix_hist_seg(Src src, uin
             const uint32_t dg = (uint32_t)(k >> (lo_bit + 8 * p)) & 0xFFu;
             atomicAdd(&s_h[p * RS_SEG_BINS + sg * RS_RADIX + dg], 1u);
             sg = dg >> RS_SEG_SHIFT;
+        }
+    }
+    __syncthreads();
+    uint32_t* out = part + (uint64_t)blockIdx.x * bins;
+    for (uint32_t i = threadIdx.x; i < bins; i += RS_HIST_THREADS) out[i] = s_h[i];
+}
+
+// The same for prefix-code keys, which it also computes and writes to src.mat.  Each workgroup stages
+// its share of the stream, CKH_CHARS positions (plus the characters their keys read beyond) at a time,
+// and the code table in LDS.  A thread takes CKH_RUN consecutive positions and slides one bit window along
+// them: acc holds the codewords of characters [i, j) from its top bit (filled bits), the key of i is its
+// top kbits bits once filled >= kbits or j = i + L, and moving to i + 1 drops character i's codeword and
+// appends the next ones: about one table lookup each way per key instead of one per key character.
+constexpr uint32_t CKH_RUN = 8;
+constexpr uint32_t CKH_CHARS = RS_HIST_THREADS * CKH_RUN;
+__global__ __launch_bounds__(RS_HIST_THREADS) void k_coded_hist_seg(CodedKeys src, uint64_t N, uint32_t lo_bit,
+                                                                    uint32_t np, uint32_t seg0_keys, uint32_t wps,
+                                                                    uint32_t* __restrict__ part,
+                                                                    uint64_t* __restrict__ h2) {
+    __shared__ uint32_t s_h[8 * RS_SEG_BINS];
+    __shared__ uint64_t s_w[(CKH_CHARS + CK_BITS) * 8 / 64 + 3];
+    __shared__ uint32_t s_tab[256];
+    const uint32_t bins = np * RS_SEG_BINS;
+    for (uint32_t i = threadIdx.x; i < bins; i += RS_HIST_THREADS) s_h[i] = 0;
+    for (uint32_t i = threadIdx.x; i < 256; i += RS_HIST_THREADS) s_tab[i] = src.ctab[i];
+    if (blockIdx.x == 0)
+        for (uint32_t i = threadIdx.x; i < bins; i += RS_HIST_THREADS) h2[i] = 0;
+    const uint32_t sg0 = blockIdx.x / wps, c = blockIdx.x % wps;
+    const uint64_t a = (uint64_t)sg0 * seg0_keys < N ? (uint64_t)sg0 * seg0_keys : N;
+    const uint64_t e = (uint64_t)(sg0 + 1) * seg0_keys < N ? (uint64_t)(sg0 + 1) * seg0_keys : N;
+    const uint64_t len = e - a, chunk = (len + wps - 1) / wps;
+    const uint64_t lo = a + (c * chunk < len ? c * chunk : len);
+    const uint64_t hi = a + ((c + 1) * chunk < len ? (c + 1) * chunk : len);
+    const uint32_t bits = src.bits, cm = (1u << bits) - 1, kbits = src.kbits, L = src.L;
+    const uint64_t km = ~0ull << (64 - kbits);
+    for (uint64_t base = lo; base < hi; base += CKH_CHARS) {
+        const uint64_t end = base + CKH_CHARS < hi ? base + CKH_CHARS : hi;
+        // the words of characters [base, end - 1 + halo], and the one after (a window's second word)
+        const uint64_t wfirst = (base * bits) >> 6;
+        const uint32_t nw = (uint32_t)((((end - 1 + src.halo()) * bits) >> 6) + 2 - wfirst);
+        __syncthreads();  // the previous chunk's words are read (and the table and counts are set)
+        for (uint32_t q = threadIdx.x; q < nw; q += RS_HIST_THREADS) s_w[q] = src.s[wfirst + q];
+        __syncthreads();
+        const uint64_t p0 = base + (uint64_t)threadIdx.x * CKH_RUN;
+        const uint64_t o0 = p0 * bits - (wfirst << 6);  // bit offset of character p0 in s_w
+        // code table entry of character p0 + t
+        auto entry = [&](uint32_t t) -> uint32_t {
+            const uint64_t q = o0 + (uint64_t)t * bits;
+            const uint32_t wi = (uint32_t)(q >> 6), sh = (uint32_t)(q & 63);
+            const uint64_t x = (s_w[wi] << sh) | ((s_w[wi + 1] >> 1) >> (63 - sh));
+            return s_tab[(uint32_t)(x >> (64 - bits)) & cm];
+        };
+        const uint32_t cnt = p0 < end ? (uint32_t)(end - p0 < CKH_RUN ? end - p0 : CKH_RUN) : 0u;
+        uint64_t acc = 0;
+        uint32_t filled = 0, j = 0;  // acc: the codewords of characters [p0 + r, p0 + j)
+        for (uint32_t r = 0; r < cnt; r++) {
+            while (filled < kbits && j < r + L) {
+                const uint32_t en = entry(j);
+                acc |= (uint64_t)(en & 0xFFFFFFu) << (CK_BITS - filled);
+                filled += en >> 24;
+                j++;
+            }
+            const uint64_t k = acc & km;
+            src.mat[p0 + r] = k;
+            uint32_t sg = sg0;
+            for (uint32_t p = 0; p < np; p++) {
+                const uint32_t dg = (uint32_t)(k >> (lo_bit + 8 * p)) & 0xFFu;
+                atomicAdd(&s_h[p * RS_SEG_BINS + sg * RS_RADIX + dg], 1u);
+                sg = dg >> RS_SEG_SHIFT;
+            }
+            const uint32_t l0 = entry(r) >> 24;  // character p0 + r leaves the window (j > r)
+            acc <<= l0;
+            filled -= l0;
         }
     }
     __syncthreads();
@@ -1190,8 +1281,12 @@ struct Primitives {
         if (window_aligned(first, N, lo_bit, np)) {
             hist_windows(first, N, lo_bit, np, (uint32_t)seg0_keys, (uint32_t)wps);
         } else {
-            hipLaunchKernelGGL(k_radix_hist_seg<Src>, dim3((unsigned)(RS_SEGS * wps)), dim3(RS_HIST_THREADS), 0, stream, first, N,
-                               lo_bit, np, (uint32_t)seg0_keys, (uint32_t)wps, seg_part, seg_h2);
+            if constexpr (std::is_same<Src, CodedKeys>::value)
+                hipLaunchKernelGGL(k_coded_hist_seg, dim3((unsigned)(RS_SEGS * wps)), dim3(RS_HIST_THREADS), 0, stream,
+                                   first, N, lo_bit, np, (uint32_t)seg0_keys, (uint32_t)wps, seg_part, seg_h2);
+            else
+                hipLaunchKernelGGL(k_radix_hist_seg<Src>, dim3((unsigned)(RS_SEGS * wps)), dim3(RS_HIST_THREADS), 0, stream,
+                                   first, N, lo_bit, np, (uint32_t)seg0_keys, (uint32_t)wps, seg_part, seg_h2);
             hipLaunchKernelGGL(k_hist_seg_reduce, dim3(bins / 256, 16), dim3(256), 0, stream,
                                (const uint32_t*)seg_part, (uint32_t)(RS_SEGS * wps), bins, seg_h2);
         }
