@@ -82,6 +82,8 @@ constexpr uint32_t PACKED_SLACK = DCX_PACKED_SLACK;
 // large-group lists from this many members, in groups of this many on average, try the partition around
 // each group's first key (long periodic runs make groups of millions; chance repeats, groups of tens)
 constexpr uint32_t LARGE_PARTITION_MIN = 1u << 16;
+// after a coded round 0, the exact least k of the active groups is found while at most this many are left
+constexpr uint32_t GROUP_KMIN_MAX = 1u << 22;
 constexpr uint32_t LARGE_PARTITION_AVG = 64;
 
 // Optimal alphabetic prefix code of the weights w[0, m): the binary tree with the symbols as leaves in
@@ -621,8 +623,10 @@ class Engine {
         }
         // After a coded round 0 a group's members share k >= kmin whole characters (k <= L); the rounds
         // then go on from character k + C of each member, and every group is resolved once kmin + C >= L.
+        // kmin: a lower bound of the active groups' k, their exact minimum once few groups are left
         uint32_t C = coded ? 0 : c0;
-        const uint32_t kmin = coded ? std::min<uint32_t>(L, kbits / code.max_len) : 0;
+        uint32_t kmin = coded ? std::min<uint32_t>(L, kbits / code.max_len) : 0;
+        const RoundChars rk{PT, bits, 0, 0, code.ctab, kbits, L};
         ActiveSet cur;
         active_set(H.get(), nullptr, N, cur);
         DBuf<uint32_t> G;
@@ -671,6 +675,13 @@ class Engine {
             ActiveSet next;
             active_set(Ha.get(), cur.act.get(), A, next, true);
             cur = std::move(next);
+            if (coded && cur.A > 0 && cur.G <= GROUP_KMIN_MAX) {
+                DBuf<uint32_t> km(cache, 1);
+                prim.fill_u32(km.get(), L, 1);
+                k_group_k<<<grid(cur.G), 256, 0, stream>>>(rk, cur.G, cur.gfirst.get(), nullptr, nullptr, order.get(),
+                                                           nullptr, km.get());
+                kmin = std::max(kmin, (uint32_t)prim.read_device(km.get(), 4)[0]);
+            }
         }
         fin = std::move(cur);
     }
@@ -739,8 +750,14 @@ class Engine {
         if (lbits + cbits <= 64) {
             const bool part = B >= LARGE_PARTITION_MIN && B >= (uint64_t)LARGE_PARTITION_AVG * nlarge;
             DBuf<uint8_t> cls(cache, part ? B : 0);
+            DBuf<uint32_t> lk;  // after a coded round 0: k of each large group, decoded once
+            if (rc.ctab) {
+                lk = DBuf<uint32_t>(cache, nlarge);
+                k_group_k<<<grid(nlarge), 256, 0, stream>>>(rc, nlarge, lfirst.get(), slots.get(), cur.act.get(),
+                                                            order.get(), lk.get(), nullptr);
+            }
             k_keys_large<<<grid(B), 256, 0, stream>>>(rc, B, slots.get(), cur.act.get(), lrank.get(), order.get(),
-                                                      lbits, x, y, lfirst.get(), part ? cls.get() : nullptr);
+                                                      lbits, x, y, lfirst.get(), part ? cls.get() : nullptr, lk.get());
             if (part && partition_large(cur, slots.get(), lrank.get(), lfirst.get(), cls.get(), B, 64 - lbits - cbits,
                                         x, y, x2, y2, order, H, Ha))
                 return;

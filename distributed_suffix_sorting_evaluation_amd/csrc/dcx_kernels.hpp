@@ -320,10 +320,13 @@ struct RoundChars {
     const uint32_t* ctab;
     uint32_t kbits, L;
     __device__ __forceinline__ uint64_t operator()(uint64_t p) const {
+        return with_k(p, ctab ? code_chars(s, bits, p, ctab, kbits, L) : 0u);
+    }
+    // the same when the member's k is known (its group's)
+    __device__ __forceinline__ uint64_t with_k(uint64_t p, uint32_t k) const {
         uint64_t start = p + C;
         uint32_t nb = cbits;
         if (ctab) {
-            const uint32_t k = code_chars(s, bits, p, ctab, kbits, L);
             const uint32_t c0 = k + C < L ? k + C : L;
             start = p + c0;
             nb = cbits < (L - c0) * bits ? cbits : (L - c0) * bits;
@@ -332,6 +335,31 @@ struct RoundChars {
         return w(start);
     }
 };
+
+// k of every group's first member after a coded round 0 (code_chars), lk[g] for g < G; with kmin, their
+// minimum into *kmin (set to L beforehand).  first(g) = order[act_idx[slot of g's first member]] with
+// the slot given directly (gfirst: order slots, act_idx == nullptr) or through the large list.
+__global__ __launch_bounds__(256) void k_group_k(RoundChars rc, uint32_t G, const uint32_t* __restrict__ gfirst,
+                                                 const uint32_t* __restrict__ slots,
+                                                 const uint32_t* __restrict__ act_idx,
+                                                 const uint32_t* __restrict__ order, uint32_t* __restrict__ lk,
+                                                 uint32_t* __restrict__ kmin) {
+    uint32_t m = rc.L;
+    for (uint64_t g = blockIdx.x * 256ull + threadIdx.x; g < G; g += gridDim.x * 256ull) {
+        const uint32_t j = act_idx ? act_idx[slots[gfirst[g]]] : gfirst[g];
+        const uint32_t k = code_chars(rc.s, rc.bits, order[j], rc.ctab, rc.kbits, rc.L);
+        if (lk) lk[g] = k;
+        m = k < m ? k : m;
+    }
+    if (kmin) {
+        // the workgroup's minimum, then one atomic
+        for (int o = 32; o > 0; o >>= 1) {
+            const uint32_t y = (uint32_t)__shfl_xor((int)m, o);
+            m = y < m ? y : m;
+        }
+        if (__lane_id() == 0) atomicMin(kmin, m);
+    }
+}
 
 // group heads of sorted keys, comparing only the top kbits
 // A thread per 16 consecutive positions (a grid of N / 16 + 1 threads): eight 16-byte key loads, one
@@ -1114,15 +1142,17 @@ __global__ __launch_bounds__(256) void k_keys_large(RoundChars chars, uint64_t B
                                                     const uint32_t* __restrict__ lrank,
                                                     const uint32_t* __restrict__ order, uint32_t lbits,
                                                     uint64_t* __restrict__ keys, uint32_t* __restrict__ vals,
-                                                    const uint32_t* __restrict__ lfirst, uint8_t* __restrict__ cls) {
+                                                    const uint32_t* __restrict__ lfirst, uint8_t* __restrict__ cls,
+                                                    const uint32_t* __restrict__ lk) {
     for (uint64_t b = blockIdx.x * 256ull + threadIdx.x; b < B; b += gridDim.x * 256ull) {
         const uint32_t p = order[act_idx[slots[b]]];
-        const uint64_t ch = chars((uint64_t)p);
+        const uint32_t k = lk ? lk[lrank[b]] : 0u;  // the group's k after a coded round 0
+        const uint64_t ch = chars.with_k((uint64_t)p, k);
         keys[b] = lbits ? (((uint64_t)lrank[b] << (64 - lbits)) | (ch >> lbits)) : ch;
         vals[b] = p;
         if (cls) {
             const uint32_t fb = lfirst[lrank[b]];
-            const uint64_t cf = chars((uint64_t)order[act_idx[slots[fb]]]);
+            const uint64_t cf = chars.with_k((uint64_t)order[act_idx[slots[fb]]], k);
             const uint64_t x = ch >> lbits, pv = cf >> lbits;  // the same group rank above both
             cls[b] = x < pv ? 0 : (x == pv ? 1 : 2);
         }

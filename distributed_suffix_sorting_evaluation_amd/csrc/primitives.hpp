@@ -191,15 +191,20 @@ __global__ __launch_bounds__(RS_HIST_THREADS) void k_radix_hist_seg(Src src, uin
 // them: acc holds the codewords of characters [i, j) from its top bit (filled bits), the key of i is its
 // top kbits bits once filled >= kbits or j = i + L, and moving to i + 1 drops character i's codeword and
 // appends the next ones: about one table lookup each way per key instead of one per key character.
+// The keys leave through a per-wave LDS buffer, 16 lanes' runs (128 consecutive keys) at a time, so each
+// store instruction writes 512 contiguous bytes (a store of every lane's r-th key would touch 64 lines).
 constexpr uint32_t CKH_RUN = 8;
 constexpr uint32_t CKH_CHARS = RS_HIST_THREADS * CKH_RUN;
+constexpr uint32_t CKH_PASSES = 5;  // CK_BITS / 8
 __global__ __launch_bounds__(RS_HIST_THREADS) void k_coded_hist_seg(CodedKeys src, uint64_t N, uint32_t lo_bit,
                                                                     uint32_t np, uint32_t seg0_keys, uint32_t wps,
                                                                     uint32_t* __restrict__ part,
                                                                     uint64_t* __restrict__ h2) {
-    __shared__ uint32_t s_h[8 * RS_SEG_BINS];
+    __shared__ uint32_t s_h[CKH_PASSES * RS_SEG_BINS];
     __shared__ uint64_t s_w[(CKH_CHARS + CK_BITS) * 8 / 64 + 3];
     __shared__ uint32_t s_tab[256];
+    __shared__ uint64_t s_k[RS_HIST_THREADS / 64][16 * CKH_RUN];
+    const uint32_t lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
     const uint32_t bins = np * RS_SEG_BINS;
     for (uint32_t i = threadIdx.x; i < bins; i += RS_HIST_THREADS) s_h[i] = 0;
     for (uint32_t i = threadIdx.x; i < 256; i += RS_HIST_THREADS) s_tab[i] = src.ctab[i];
@@ -233,24 +238,46 @@ __global__ __launch_bounds__(RS_HIST_THREADS) void k_coded_hist_seg(CodedKeys sr
         const uint32_t cnt = p0 < end ? (uint32_t)(end - p0 < CKH_RUN ? end - p0 : CKH_RUN) : 0u;
         uint64_t acc = 0;
         uint32_t filled = 0, j = 0;  // acc: the codewords of characters [p0 + r, p0 + j)
-        for (uint32_t r = 0; r < cnt; r++) {
-            while (filled < kbits && j < r + L) {
-                const uint32_t en = entry(j);
-                acc |= (uint64_t)(en & 0xFFFFFFu) << (CK_BITS - filled);
-                filled += en >> 24;
-                j++;
+        uint64_t ks[CKH_RUN];
+#pragma unroll
+        for (uint32_t r = 0; r < CKH_RUN; r++) {
+            ks[r] = 0;
+            if (r < cnt) {
+                while (filled < kbits && j < r + L) {
+                    const uint32_t en = entry(j);
+                    acc |= (uint64_t)(en & 0xFFFFFFu) << (CK_BITS - filled);
+                    filled += en >> 24;
+                    j++;
+                }
+                const uint64_t k = acc & km;
+                ks[r] = k;
+                uint32_t sg = sg0;
+                for (uint32_t p = 0; p < np; p++) {
+                    const uint32_t dg = (uint32_t)(k >> (lo_bit + 8 * p)) & 0xFFu;
+                    atomicAdd(&s_h[p * RS_SEG_BINS + sg * RS_RADIX + dg], 1u);
+                    sg = dg >> RS_SEG_SHIFT;
+                }
+                const uint32_t l0 = entry(r) >> 24;  // character p0 + r leaves the window (j > r)
+                acc <<= l0;
+                filled -= l0;
             }
-            const uint64_t k = acc & km;
-            src.mat[p0 + r] = k;
-            uint32_t sg = sg0;
-            for (uint32_t p = 0; p < np; p++) {
-                const uint32_t dg = (uint32_t)(k >> (lo_bit + 8 * p)) & 0xFFu;
-                atomicAdd(&s_h[p * RS_SEG_BINS + sg * RS_RADIX + dg], 1u);
-                sg = dg >> RS_SEG_SHIFT;
+        }
+        // the wave's 512 positions [wb, wb + 512) out through s_k: lanes 16q .. 16q + 15 hold [wb + 128q,
+        // wb + 128q + 128); the LDS accesses of one wave run in order
+        const uint64_t wb = base + (uint64_t)wv * 64 * CKH_RUN;
+#pragma unroll
+        for (uint32_t q = 0; q < 4; q++) {
+            if ((lane >> 4) == q) {
+#pragma unroll
+                for (uint32_t r = 0; r < CKH_RUN; r++) s_k[wv][(lane & 15) * CKH_RUN + r] = ks[r];
             }
-            const uint32_t l0 = entry(r) >> 24;  // character p0 + r leaves the window (j > r)
-            acc <<= l0;
-            filled -= l0;
+            __builtin_amdgcn_wave_barrier();
+#pragma unroll
+            for (uint32_t t = 0; t < 2; t++) {
+                const uint64_t pos = wb + 128 * q + 64 * t + lane;
+                if (pos < end) src.mat[pos] = s_k[wv][64 * t + lane];
+            }
+            __builtin_amdgcn_wave_barrier();
         }
     }
     __syncthreads();
@@ -1281,6 +1308,8 @@ struct Primitives {
         if (window_aligned(first, N, lo_bit, np)) {
             hist_windows(first, N, lo_bit, np, (uint32_t)seg0_keys, (uint32_t)wps);
         } else {
+            if constexpr (std::is_same<Src, CodedKeys>::value)
+                if (np > CKH_PASSES) throw Error(-4, "prefix-code keys have at most 40 bits");
             if constexpr (std::is_same<Src, CodedKeys>::value)
                 hipLaunchKernelGGL(k_coded_hist_seg, dim3((unsigned)(RS_SEGS * wps)), dim3(RS_HIST_THREADS), 0, stream,
                                    first, N, lo_bit, np, (uint32_t)seg0_keys, (uint32_t)wps, seg_part, seg_h2);
