@@ -196,7 +196,8 @@ __global__ __launch_bounds__(RS_HIST_THREADS) void k_radix_hist_seg(Src src, uin
 constexpr uint32_t CKH_RUN = 8;
 constexpr uint32_t CKH_CHARS = RS_HIST_THREADS * CKH_RUN;
 constexpr uint32_t CKH_PASSES = 5;  // CK_BITS / 8
-__global__ __launch_bounds__(RS_HIST_THREADS) void k_coded_hist_seg(CodedKeys src, uint64_t N, uint32_t lo_bit,
+// (8 waves per SIMD: two workgroups per CU, which the LDS allows, need at most 96 scalar registers)
+__global__ __launch_bounds__(RS_HIST_THREADS, 8) void k_coded_hist_seg(CodedKeys src, uint64_t N, uint32_t lo_bit,
                                                                     uint32_t np, uint32_t seg0_keys, uint32_t wps,
                                                                     uint32_t* __restrict__ part,
                                                                     uint64_t* __restrict__ h2) {
@@ -213,7 +214,9 @@ __global__ __launch_bounds__(RS_HIST_THREADS) void k_coded_hist_seg(CodedKeys sr
     const uint32_t sg0 = blockIdx.x / wps, c = blockIdx.x % wps;
     const uint64_t a = (uint64_t)sg0 * seg0_keys < N ? (uint64_t)sg0 * seg0_keys : N;
     const uint64_t e = (uint64_t)(sg0 + 1) * seg0_keys < N ? (uint64_t)(sg0 + 1) * seg0_keys : N;
-    const uint64_t len = e - a, chunk = (len + wps - 1) / wps;
+    // shares of whole 8-character runs: segment starts are multiples of the radix tile, so every chunk
+    // starts at a multiple of 8
+    const uint64_t len = e - a, chunk = ((len + wps - 1) / wps + 7) & ~7ull;
     const uint64_t lo = a + (c * chunk < len ? c * chunk : len);
     const uint64_t hi = a + ((c + 1) * chunk < len ? (c + 1) * chunk : len);
     const uint32_t bits = src.bits, cm = (1u << bits) - 1, kbits = src.kbits, L = src.L;
@@ -239,27 +242,71 @@ __global__ __launch_bounds__(RS_HIST_THREADS) void k_coded_hist_seg(CodedKeys sr
         uint64_t acc = 0;
         uint32_t filled = 0, j = 0;  // acc: the codewords of characters [p0 + r, p0 + j)
         uint64_t ks[CKH_RUN];
+        if (bits == 8 && (base & 7) == 0) {
+            // byte characters: the run's characters are the word s_w[threadIdx.x], and each later word holds
+            // the next eight; their table entries are looked up eight at a time into a queue (q[0]: the
+            // entry of character p0 + j), and the run's own eight also give the lengths that leave
+            uint32_t q[8], dl[8];
+            uint32_t qn = 8, nextw = threadIdx.x + 1;
+            {
+                const uint64_t w0 = s_w[threadIdx.x];
+#pragma unroll
+                for (int i = 0; i < 8; i++) {
+                    q[i] = s_tab[(uint32_t)(w0 >> (56 - 8 * i)) & 0xFFu];
+                    dl[i] = q[i] >> 24;
+                }
+            }
+#pragma unroll
+            for (uint32_t r = 0; r < CKH_RUN; r++) {
+                ks[r] = 0;
+                if (r < cnt) {
+                    while (filled < kbits && j < r + L) {
+                        if (qn == 0) {
+                            const uint64_t w = s_w[nextw++];
+#pragma unroll
+                            for (int i = 0; i < 8; i++) q[i] = s_tab[(uint32_t)(w >> (56 - 8 * i)) & 0xFFu];
+                            qn = 8;
+                        }
+                        const uint32_t en = q[0];
+#pragma unroll
+                        for (int i = 0; i < 7; i++) q[i] = q[i + 1];
+                        qn--;
+                        acc |= (uint64_t)(en & 0xFFFFFFu) << (CK_BITS - filled);
+                        filled += en >> 24;
+                        j++;
+                    }
+                    ks[r] = acc & km;
+                    acc <<= dl[r];
+                    filled -= dl[r];
+                }
+            }
+        } else {
+#pragma unroll
+            for (uint32_t r = 0; r < CKH_RUN; r++) {
+                ks[r] = 0;
+                if (r < cnt) {
+                    while (filled < kbits && j < r + L) {
+                        const uint32_t en = entry(j);
+                        acc |= (uint64_t)(en & 0xFFFFFFu) << (CK_BITS - filled);
+                        filled += en >> 24;
+                        j++;
+                    }
+                    ks[r] = acc & km;
+                    const uint32_t l0 = entry(r) >> 24;  // character p0 + r leaves the window (j > r)
+                    acc <<= l0;
+                    filled -= l0;
+                }
+            }
+        }
 #pragma unroll
         for (uint32_t r = 0; r < CKH_RUN; r++) {
-            ks[r] = 0;
             if (r < cnt) {
-                while (filled < kbits && j < r + L) {
-                    const uint32_t en = entry(j);
-                    acc |= (uint64_t)(en & 0xFFFFFFu) << (CK_BITS - filled);
-                    filled += en >> 24;
-                    j++;
-                }
-                const uint64_t k = acc & km;
-                ks[r] = k;
                 uint32_t sg = sg0;
                 for (uint32_t p = 0; p < np; p++) {
-                    const uint32_t dg = (uint32_t)(k >> (lo_bit + 8 * p)) & 0xFFu;
+                    const uint32_t dg = (uint32_t)(ks[r] >> (lo_bit + 8 * p)) & 0xFFu;
                     atomicAdd(&s_h[p * RS_SEG_BINS + sg * RS_RADIX + dg], 1u);
                     sg = dg >> RS_SEG_SHIFT;
                 }
-                const uint32_t l0 = entry(r) >> 24;  // character p0 + r leaves the window (j > r)
-                acc <<= l0;
-                filled -= l0;
             }
         }
         // the wave's 512 positions [wb, wb + 512) out through s_k: lanes 16q .. 16q + 15 hold [wb + 128q,
