@@ -203,6 +203,7 @@ __global__ __launch_bounds__(RS_HIST_THREADS, 8) void k_coded_hist_seg(CodedKeys
                                                                     uint64_t* __restrict__ h2) {
     __shared__ uint32_t s_h[CKH_PASSES * RS_SEG_BINS];
     __shared__ uint64_t s_w[(CKH_CHARS + CK_BITS) * 8 / 64 + 3];
+    static_assert((CKH_CHARS + CK_BITS) * 8 / 64 + 3 <= 2 * RS_HIST_THREADS, "two staged words per thread");
     __shared__ uint32_t s_tab[256];
     __shared__ uint64_t s_k[RS_HIST_THREADS / 64][16 * CKH_RUN];
     const uint32_t lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
@@ -221,14 +222,38 @@ __global__ __launch_bounds__(RS_HIST_THREADS, 8) void k_coded_hist_seg(CodedKeys
     const uint64_t hi = a + ((c + 1) * chunk < len ? (c + 1) * chunk : len);
     const uint32_t bits = src.bits, cm = (1u << bits) - 1, kbits = src.kbits, L = src.L;
     const uint64_t km = ~0ull << (64 - kbits);
+    // a chunk's stream words: those of characters [base, end - 1 + halo], and the one after (a window's
+    // second word), at most 2 * RS_HIST_THREADS of them; each thread loads words threadIdx.x and
+    // threadIdx.x + RS_HIST_THREADS of the next chunk into registers while the current one is processed
+    auto chunk_words = [&](uint64_t b0, uint64_t& wf, uint32_t& nw) {
+        const uint64_t e0 = b0 + CKH_CHARS < hi ? b0 + CKH_CHARS : hi;
+        wf = (b0 * bits) >> 6;
+        nw = (uint32_t)((((e0 - 1 + src.halo()) * bits) >> 6) + 2 - wf);
+    };
+    uint64_t pw0 = 0, pw1 = 0;
+    if (lo < hi) {
+        uint64_t wf;
+        uint32_t nw;
+        chunk_words(lo, wf, nw);
+        if (threadIdx.x < nw) pw0 = src.s[wf + threadIdx.x];
+        if (threadIdx.x + RS_HIST_THREADS < nw) pw1 = src.s[wf + threadIdx.x + RS_HIST_THREADS];
+    }
     for (uint64_t base = lo; base < hi; base += CKH_CHARS) {
         const uint64_t end = base + CKH_CHARS < hi ? base + CKH_CHARS : hi;
-        // the words of characters [base, end - 1 + halo], and the one after (a window's second word)
-        const uint64_t wfirst = (base * bits) >> 6;
-        const uint32_t nw = (uint32_t)((((end - 1 + src.halo()) * bits) >> 6) + 2 - wfirst);
+        uint64_t wfirst;
+        uint32_t nw;
+        chunk_words(base, wfirst, nw);
         __syncthreads();  // the previous chunk's words are read (and the table and counts are set)
-        for (uint32_t q = threadIdx.x; q < nw; q += RS_HIST_THREADS) s_w[q] = src.s[wfirst + q];
+        if (threadIdx.x < nw) s_w[threadIdx.x] = pw0;
+        if (threadIdx.x + RS_HIST_THREADS < nw) s_w[threadIdx.x + RS_HIST_THREADS] = pw1;
         __syncthreads();
+        if (end < hi) {
+            uint64_t wf;
+            uint32_t nn;
+            chunk_words(end, wf, nn);
+            if (threadIdx.x < nn) pw0 = src.s[wf + threadIdx.x];
+            if (threadIdx.x + RS_HIST_THREADS < nn) pw1 = src.s[wf + threadIdx.x + RS_HIST_THREADS];
+        }
         const uint64_t p0 = base + (uint64_t)threadIdx.x * CKH_RUN;
         const uint64_t o0 = p0 * bits - (wfirst << 6);  // bit offset of character p0 in s_w
         // code table entry of character p0 + t
