@@ -275,3 +275,88 @@ def test_autotune_searches_x_then_sub_batch_size(monkeypatch):
     n = len(eng.calls)
     assert tuner.x_for("acgt", 0, off, 0) == 21 and len(eng.calls) == n  # cached
     assert autotune.XTuner(eng, candidates=(21,), reps=1).size_for("acgt") == 0
+
+
+_CODE_DRIVER = r"""
+#include <cstdio>
+#include "alphabetic_code.hpp"
+int main() {
+    int m;
+    unsigned max_len;
+    while (std::scanf("%d %u", &m, &max_len) == 2) {
+        std::vector<double> w(m);
+        for (auto& x : w) std::scanf("%lf", &x);
+        std::vector<uint32_t> len, code;
+        const uint32_t longest = dcx::alphabetic_code(w, max_len, len, code);
+        std::printf("%u", longest);
+        for (int i = 0; i < m; i++) std::printf(" %u %u", len[i], code[i]);
+        std::printf("\n");
+    }
+    return 0;
+}
+"""
+
+
+def _optimal_alphabetic_cost(w):
+    """Least weighted depth of a binary tree with leaves w in order, by the plain O(m^3) recurrence."""
+    m = len(w)
+    pre = np.concatenate([[0.0], np.cumsum(w)])
+    cost = [[0.0] * m for _ in range(m)]
+    for d in range(1, m):
+        for i in range(m - d):
+            j = i + d
+            cost[i][j] = min(cost[i][k] + cost[k + 1][j] for k in range(i, j)) + pre[j + 1] - pre[i]
+    return cost[0][m - 1]
+
+
+def test_alphabetic_code_is_ordered_prefix_free_and_optimal(tmp_path):
+    """The round-0 key code (csrc/alphabetic_code.hpp, compiled here with g++): codewords increase with the
+    symbol and none is a prefix of another (so coded strings compare as the strings), the tree is full
+    (Kraft sum 1), the weighted length equals the optimum of the unrestricted O(m^3) recurrence (the
+    Knuth-Yao split bounds lose nothing), and a length limit that the optimum exceeds yields no code."""
+    import shutil
+    import subprocess
+    gxx = shutil.which("g++")
+    if gxx is None:
+        pytest.skip("g++ not available")
+    src = tmp_path / "drv.cpp"
+    src.write_text(_CODE_DRIVER)
+    exe = tmp_path / "drv"
+    subprocess.run([gxx, "-O2", "-std=c++17", "-I", os.path.join(ROOT, "distributed_suffix_sorting_evaluation_amd", "csrc"),
+                    str(src), "-o", str(exe)], check=True)
+    rng = np.random.default_rng(9)
+    cases = [[5.0], [1.0, 1.0], [1.0, 100.0, 1.0]]
+    for t in range(60):
+        m = int(rng.integers(2, 40))
+        kind = t % 3
+        if kind == 0:
+            w = rng.random(m) + 1e-3
+        elif kind == 1:
+            w = np.arange(1, m + 1, dtype=np.float64) ** -1.1
+            rng.shuffle(w)
+        else:
+            w = rng.choice([1.0, 3.0, 1e3], size=m)
+        cases.append([float(x) for x in w])
+    r = np.arange(1, 257, dtype=np.float64) ** -1.1
+    cases.append([float(x) for x in r])  # the byte-alphabet workload's distribution
+    inp = "".join(f"{len(w)} 24 " + " ".join(repr(x) for x in w) + "\n" for w in cases)
+    inp += "3 1 1000000 1 1\n"  # the optimum puts two symbols at depth 2
+    out = subprocess.run([str(exe)], input=inp, capture_output=True, text=True, check=True).stdout.split("\n")
+    for w, line in zip(cases, out):
+        f = [int(x) for x in line.split()]
+        longest, lens, codes = f[0], f[1::2], f[2::2]
+        m = len(w)
+        if m == 1:
+            assert longest == 1 and lens == [1]
+            continue
+        assert longest == max(lens) and 1 <= min(lens) and longest <= 24
+        words = [format(c, f"0{l}b") for c, l in zip(codes, lens)]
+        assert all(c < (1 << l) for c, l in zip(codes, lens))
+        assert words == sorted(words) and len(set(words)) == m
+        for a in range(m - 1):
+            assert not words[a + 1].startswith(words[a])  # sorted: a prefix would sit right before
+        assert abs(sum(2.0 ** -l for l in lens) - 1.0) < 1e-12
+        if m <= 40:
+            got = sum(x * l for x, l in zip(w, lens))
+            assert abs(got - _optimal_alphabetic_cost(w)) <= 1e-9 * got, (w, got)
+    assert out[len(cases)].split()[0] == "0"
