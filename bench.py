@@ -393,10 +393,11 @@ def phase_table(read_dir, write_dir, trace_dir):
 
 def radix_traffic(read_dir, write_dir, trace_dir):
     """HBM bytes per launch of the 64-bit radix pass (k_onesweep with 64-bit keys: the key sources
-    ArrayKeys<unsigned long> and StreamKeys) from the same three rocprofv3 runs as --phase-table, with the
-    launch count and the mean launch time of the trace.  Prints a JSON line."""
+    ArrayKeys<unsigned long>, StreamKeys and CodedKeys) from the same three rocprofv3 runs as --phase-table,
+    with the launch count and the mean launch time of the trace.  Prints a JSON line."""
     def radix(name):
-        return "k_onesweep<" in name and ("ArrayKeys<unsigned long>" in name or "StreamKeys<" in name)
+        return "k_onesweep<" in name and ("ArrayKeys<unsigned long>" in name or "StreamKeys<" in name
+                                          or "CodedKeys" in name)
     rd = {d: v for d, v in _dispatch_counters(read_dir).items() if radix(v[0])}
     wr = {d: v for d, v in _dispatch_counters(write_dir).items() if radix(v[0])}
     trace = [r for r in _rocprof_rows(trace_dir, "kernel_trace.csv") if radix(r["Kernel_Name"])]
