@@ -479,7 +479,27 @@ class Engine {
         uint32_t kbits = 0;
         const uint32_t pbits = bit_width(N - 1);
         const uint32_t cpack = std::min<uint32_t>(c0, (64 - pbits) / bits);
-        if (N >= (1u << 16) && cpack * info >= bit_width(N) + PACKED_SLACK) {
+        // code bits for log2(N) + ROUND0_SLACK bits of information (about entropy / avg_len of it per code
+        // bit), in whole digits; with the position beside them when they leave it room
+        const bool code_ok = code.ctab && bits <= 8 && code.entropy > 0 && N >= (1u << 16) && L > 1;
+        const uint32_t ckb =
+            code_ok ? 8 * (uint32_t)std::ceil((bit_width(N) + ROUND0_SLACK) * code.avg_len / code.entropy / 8) : 0;
+        if (code_ok && ckb <= CODE_KEY_BITS_MAX && ckb <= 64 - pbits) {
+            // round 0 keyed by the code with the position in the bits below (keys-only passes of 16 bytes);
+            // the histogram kernel writes the keys into ka, the first pass reads them from there
+            coded = true;
+            kbits = ckb;
+            DBuf<uint64_t> ka(cache, N), kb(cache, N);
+            uint64_t* k = ka.get();
+            uint64_t* k2 = kb.get();
+            uint32_t* nv = nullptr;
+            uint32_t* nv2 = nullptr;
+            prim.sort_pairs_from(CodedKeys{PT, bits, code.ctab, kbits, L, k, 1u}, k, k2, nv, nv2, N, 64 - kbits, 64,
+                                 false);
+            H = DBuf<uint8_t>(cache, (size_t)N + 1);
+            order = DBuf<uint32_t>(cache, N);
+            k_heads_packed<<<grid(N / 16 + 1ull), 256, 0, stream>>>(k, N, kbits, (1ull << pbits) - 1, H.get(), order.get());
+        } else if (N >= (1u << 16) && cpack * info >= bit_width(N) + PACKED_SLACK) {
             // round 0 sorts one word per position holding the first characters above the position itself
             // (16 instead of 24 bytes moved per pass) when the characters left beside the position still
             // carry PACKED_SLACK bits of information more than log2(N); the positions they leave tied go
@@ -506,7 +526,7 @@ class Engine {
                 const uint32_t digits = std::min<uint32_t>(8, (std::max<uint32_t>(1, want) * bits + 7) / 8);
                 c0 = std::max<uint32_t>(1, std::min<uint32_t>(c0, 8 * digits / bits));
                 kbits = c0 * bits;
-                if (code.ctab && dense && bits <= 8 && code.entropy > 0 && L > c0) {
+                if (code_ok && L > c0) {
                     // the same information from code bits (about entropy / avg_len of it per bit), at
                     // most CODE_KEY_BITS_MAX of them: taken when that saves a radix pass
                     const double need = (bit_width(N) + ROUND0_SLACK) * code.avg_len / code.entropy;
@@ -537,9 +557,9 @@ class Engine {
             uint32_t np;
             if (coded) {
                 // the histogram kernel writes the code keys into k, the first pass reads them from there
-                np = prim.sort_pairs_from(CodedKeys{PT, bits, code.ctab, kbits, L, k}, k, k2, v, v2, N, 64 - kbits,
-                                          64, true, low.get(), ia.get(), ib.get(), H.get(), top_mask(kbits),
-                                          &heads_done);
+                np = prim.sort_pairs_from(CodedKeys{PT, bits, code.ctab, kbits, L, k, 0u}, k, k2, v, v2, N,
+                                          64 - kbits, 64, true, low.get(), ia.get(), ib.get(), H.get(),
+                                          top_mask(kbits), &heads_done);
             } else {
                 np = prim.sort_pairs_from(StreamKeys<false>{PT, bits, top_mask(kbits)}, k, k2, v, v2, N, 64 - kbits,
                                           64, true, low.get(), ia.get(), ib.get(), H.get(), top_mask(kbits),
@@ -988,33 +1008,38 @@ class Engine {
         return e;
     }
 
-    // The round-0 key code of a level-0 text (alphabetic_code) from the sampled counts code_counts of its
-    // characters 1..maxchar, which the dense stream codes as 0..maxchar - 1.  Characters the sample missed
-    // weigh as one in 2^18 sampled characters, which keeps the codewords short; a code whose longest
-    // codeword would exceed CODE_MAX_LEN bits is not used.  The device table goes to d_ctab.
+    // The round-0 key code of a level-0 stream (alphabetic_code).  wj[j]: the sampled count of character j
+    // (0: the padding, which counts nothing); the stream codes character j as j - dense (the padding stays
+    // 0), so with dense words characters 0 and 1 share code 0.  Codes that the sample missed weigh as one in
+    // 2^18 sampled characters, which keeps the codewords short; a code whose longest codeword would exceed
+    // CODE_MAX_LEN bits is not used.  The device table (256 entries) goes to d_ctab; its host copy h_ctab
+    // must stay until the copy has run (the calls are ordered on the stream, and a level builds one code).
     uint32_t h_ctab[256];
-    KeyCode key_code(uint32_t maxchar, DBuf<uint32_t>& d_ctab) {
+    KeyCode key_code(const std::vector<double>& wj, bool dense, DBuf<uint32_t>& d_ctab) {
         KeyCode kc;
-        if (!have_counts || maxchar < 2 || maxchar > 256) return kc;
+        const uint32_t maxchar = (uint32_t)wj.size() - 1;
+        const uint32_t m = maxchar + 1 - (dense ? 1 : 0);  // stream codes 0 .. m - 1
+        if (maxchar < 1 || m < 2 || m > 256) return kc;
+        std::vector<double> h(m, 0.0);
+        for (uint32_t j = 0; j <= maxchar; j++) h[j == 0 ? 0 : j - (dense ? 1 : 0)] += wj[j];
         double total = 0;
-        for (uint32_t c = 1; c <= maxchar; c++) total += (double)code_counts[c];
+        for (double x : h) total += x;
         if (total <= 0) return kc;
         const double floor_w = std::max(1.0, total / (double)(1u << 18));
-        std::vector<double> w(maxchar);
+        std::vector<double> w(m);
         double e = 0;
-        for (uint32_t c = 0; c < maxchar; c++) {
-            const double h = (double)code_counts[c + 1];
-            w[c] = std::max(h, floor_w);
-            if (h > 0) e -= h / total * std::log2(h / total);
+        for (uint32_t d = 0; d < m; d++) {
+            w[d] = std::max(h[d], floor_w);
+            if (h[d] > 0) e -= h[d] / total * std::log2(h[d] / total);
         }
         std::vector<uint32_t> len, cw;
         const uint32_t longest = alphabetic_code(w, CODE_MAX_LEN, len, cw);
         if (longest == 0) return kc;
         double avg = 0;
-        for (uint32_t c = 0; c < maxchar; c++) avg += (double)code_counts[c + 1] * len[c];
+        for (uint32_t d = 0; d < m; d++) avg += h[d] * len[d];
         avg /= total;
         for (uint32_t c = 0; c < 256; c++) {
-            const uint32_t d = c < maxchar ? c : maxchar - 1;  // codes past the alphabet do not occur
+            const uint32_t d = c < m ? c : m - 1;  // codes past the alphabet do not occur
             h_ctab[c] = (len[d] << 24) | (cw[d] << (24 - len[d]));
         }
         d_ctab = DBuf<uint32_t>(cache, 256);
@@ -1024,6 +1049,13 @@ class Engine {
         kc.avg_len = avg;
         kc.entropy = e;
         return kc;
+    }
+    // the same for a single text: characters 1..maxchar are the alphabet codes lut[b] (code_counts)
+    KeyCode key_code_text(uint32_t maxchar, bool dense, DBuf<uint32_t>& d_ctab) {
+        if (!have_counts || maxchar > 256) return KeyCode{};
+        std::vector<double> wj(maxchar + 1, 0.0);
+        for (uint32_t c = 1; c <= maxchar; c++) wj[c] = (double)code_counts[c];
+        return key_code(wj, dense, d_ctab);
     }
 
     // the level's characters as a packed code stream (k_pack_text) of `chars` characters
@@ -1050,10 +1082,11 @@ class Engine {
     template <class CharT>
     // The SA goes to sa_out, or with sa_take != nullptr into *sa_take (the level's own buffer, no copy).
     // A caller that has the level's text only as pieces (a batch) passes `stream_fn`, which packs the
-    // level's character stream of the given width, dense flag and length, and `entropy` (of the codes).
+    // level's character stream of the given width, dense flag and length, `entropy` (of the codes) and the
+    // round-0 key code of its characters (`batch_code`, or none).
     void level(const CharT* text, const uint16_t* lut, uint64_t n, uint32_t maxchar, int depth, uint32_t* sa_out,
                hipEvent_t* ev, DBuf<uint32_t>* sa_take = nullptr, const StreamFn* stream_fn = nullptr,
-               double entropy = -1.0) {
+               double entropy = -1.0, const KeyCode* batch_code = nullptr) {
         const uint32_t X = dcs->X, D = dcs->D;
         const uint32_t bits = bit_width(maxchar);
         // dense merge words (c -> c - 1) when they save a bit per character
@@ -1129,7 +1162,9 @@ class Engine {
                 make_stream(text, lut, n, mbits, dense ? 1 : 0, n + Ls + 2, PT);
             const double info = info_bits(text, lut, n, mbits, entropy);
             DBuf<uint32_t> d_ctab;
-            const KeyCode kc = depth == 0 && !stream_fn && dense && mbits <= 8 ? key_code(maxchar, d_ctab) : KeyCode{};
+            const KeyCode kc = depth == 0 && mbits <= 8
+                                   ? (batch_code ? *batch_code : !stream_fn ? key_code_text(maxchar, dense, d_ctab) : KeyCode{})
+                                   : KeyCode{};
             refine_sort(PT.get(), n, Ls, mbits, dense, info, order, H, tied, kc);
             if (dense && tied.A > 0) boundary_fixup(PT.get(), mbits, Ls, n, dc.bnd, order, H, tied);
         }
@@ -1274,6 +1309,21 @@ class Engine {
         // entropy of the codes, once for the batch (it sizes round 0 of the sub-batches of 2^24 characters
         // and more; shorter ones key at full width)
         const double entropy = total >= (1ull << 24) ? code_entropy(d_texts, dlut.get(), total) : 8.0;
+        // the round-0 key code of the joined characters: text bytes as lut[b] + 1 (the sampled counts),
+        // the separator once per text (sampled at the same rate), the padding
+        DBuf<uint32_t> d_ctab;
+        KeyCode bcode;
+        if (total >= (1ull << 24) && have_counts && maxchar <= 255) {
+            std::vector<double> wj(maxchar + 1, 0.0);
+            double sampled = 0;
+            for (uint32_t c = 1; c <= sig; c++) sampled += (double)code_counts[c];
+            for (uint32_t c = 1; c <= sig; c++) wj[c + 1] = (double)code_counts[c];
+            wj[1] = sampled * (double)count / (double)total;
+            const uint32_t dbits = bit_width(maxchar - 1);
+            bcode = key_code(wj, dbits >= 1 && dbits < bit_width(maxchar), d_ctab);
+            // the second stream reads the table too: it must have arrived before that stream's first kernel
+            DCX_HIP(hipStreamSynchronize(stream));
+        }
         // sub-batches: greedy, at least one text, at most `limit` joined characters each
         std::vector<uint32_t> cuts{0};
         while (cuts.back() < count) {
@@ -1287,7 +1337,7 @@ class Engine {
         }
         const size_t nsub = cuts.size() - 1;
         const BatchInput in{d_texts, offsets, base, maxchar, dlut.get(), ident.get(), entropy, o->sa_on_device != 0,
-                            sa_out};
+                            sa_out, bcode.ctab ? &bcode : nullptr};
         // Two sub-batches in flight when there are several: a second engine on its own stream (created on
         // first use and kept with the handle) takes every other one from a shared counter, in a host thread
         // of its own, so one sub-batch's kernels run while the other waits for a readback.  Both write
@@ -1364,6 +1414,7 @@ class Engine {
         double entropy;
         bool sa_on_device;
         uint32_t* sa_out;
+        const KeyCode* code;      // round-0 key code of the joined characters, shared by the sub-batches
     };
 
     // One sub-batch, texts [k0, k1): joined with separators, one DCX pipeline, partition by text into
@@ -1405,7 +1456,7 @@ class Engine {
                 k_batch_pack<<<(unsigned)std::min<uint64_t>(chunks, 16384), 256, 0, stream>>>(
                     d_texts, d_in.get(), d_co.get(), cnt, N, in.lut, bits, dense, nwords, PT.get());
             };
-            level<uint8_t>(nullptr, in.ident, N, maxchar, 0, nullptr, nullptr, &sa, &fn, in.entropy);
+            level<uint8_t>(nullptr, in.ident, N, maxchar, 0, nullptr, nullptr, &sa, &fn, in.entropy, in.code);
         } else if (maxchar <= 255) {
             DBuf<uint32_t> t_in = owner_table_in();
             DBuf<uint8_t> jt(cache, (size_t)N + 16);

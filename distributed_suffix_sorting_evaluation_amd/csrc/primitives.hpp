@@ -105,8 +105,9 @@ struct StreamKeys {
 // (top-aligned) of the codewords of characters i, i + 1, ..., i + L - 1 of s (`bits` <= 8 bits per
 // character), zeros after the L-th; ctab[c] = (length << 24) | the codeword of character code c,
 // left-aligned in the low 24 bits.  Every codeword has at least one bit, so a key reads fewer than
-// halo() characters from i on.  The sort's histogram kernel (k_coded_hist_seg) computes the keys and
-// writes them to mat (N words), which the first pass then reads.
+// halo() characters from i on.  With packpos the position i itself fills the bits below the code bits
+// (keys-only sorts; i < 2^(64 - kbits)).  The sort's histogram kernel (k_coded_hist_seg) computes the
+// keys and writes them to mat (N words), which the first pass then reads.
 constexpr uint32_t CK_BITS = 40;  // longest key: codewords are placed at bit CK_BITS - (bits filled) and up
 struct CodedKeys {
     const uint64_t* s;
@@ -114,6 +115,7 @@ struct CodedKeys {
     const uint32_t* ctab;
     uint32_t kbits, L;
     uint64_t* mat;
+    uint32_t packpos;
     __device__ __forceinline__ uint64_t operator()(uint64_t i) const { return mat[i]; }
     uint64_t read_bytes(uint64_t n) const { return 8 * n; }
     __host__ __device__ uint32_t halo() const { return L < CK_BITS ? L : CK_BITS; }
@@ -322,6 +324,10 @@ __global__ __launch_bounds__(RS_HIST_THREADS, 8) void k_coded_hist_seg(CodedKeys
                     filled -= l0;
                 }
             }
+        }
+        if (src.packpos) {
+#pragma unroll
+            for (uint32_t r = 0; r < CKH_RUN; r++) ks[r] |= p0 + r;
         }
 #pragma unroll
         for (uint32_t r = 0; r < CKH_RUN; r++) {

@@ -273,10 +273,12 @@ def test_genome_64mib_digest(engine):
 
 
 def _coded_round0_texts():
-    """Texts of 2^24 + a few characters over power-of-two alphabets (dense character words) whose first sort
-    round is keyed by the alphabetic code of their characters: Zipf bytes with a periodic run, bytes that are
-    95% one value (codewords of one bit: 40 code bits reach past the key length), and a skewed 4-letter text
-    (the code resolves 30 characters in 4 radix passes instead of 8)."""
+    """Texts of 2^24 + a few characters whose first sort round is keyed by the alphabetic code of their
+    characters: Zipf bytes with a periodic run (32 code bits above the position, keys-only passes), bytes that
+    are 95% one value (codewords of one bit: 40 code bits reach past the key length) and a skewed 4-letter
+    text (the code resolves 30 characters in 5 radix passes instead of 8), both keyed by 40 code bits beside
+    split positions, and a skewed 5-letter text (sentinel-coded characters: the padding is a code of its
+    own)."""
     rng = np.random.default_rng(77)
     n = (1 << 24) + 4097
     r = np.arange(1, 257, dtype=np.float64) ** -1.1
@@ -286,10 +288,11 @@ def _coded_round0_texts():
     skew = np.where(rng.random(n) < 0.95, 7, rng.integers(0, 256, size=n)).astype(np.uint8)
     skew[:256] = np.arange(256, dtype=np.uint8)  # all 256 byte values occur
     dna = rng.choice(4, size=n, p=[0.82, 0.1, 0.05, 0.03]).astype(np.uint8) + ord("A")
-    return [("zipf", z), ("one_value_95", skew), ("dna_skewed", dna)]
+    five = np.frombuffer(b"ACGTN", dtype=np.uint8)[rng.choice(5, size=n, p=[0.4, 0.25, 0.2, 0.1, 0.05])]
+    return [("zipf", z), ("one_value_95", skew), ("dna_skewed", dna), ("five_letters", five)]
 
 
-@pytest.mark.parametrize("case", [0, 1, 2])
+@pytest.mark.parametrize("case", [0, 1, 2, 3])
 def test_coded_round0_against_oracle(engine, case):
     """Round 0 keyed by an alphabetic code (texts of at least 2^24 characters with dense character words):
     the SA and every level's statistics (names, reduced, recursion) equal the oracle's, so the groups after
