@@ -585,6 +585,7 @@ class Engine {
         uint32_t C = coded ? 0 : c0;
         uint32_t kmin = coded ? std::min<uint32_t>(L, kbits / code.max_len) : 0;
         const RoundChars rk{PT, bits, 0, 0, code.ctab, kbits, L};
+        bool parked = false;  // some large groups left the rounds equal through L (refine_round deep)
         ActiveSet cur;
         active_set(H.get(), nullptr, N, cur);
         DBuf<uint32_t> G;
@@ -623,12 +624,17 @@ class Engine {
             // compare the characters alone
             c = std::min<uint32_t>(cpw, L - kmin - C);
             {
+                // (one bit below the characters is kept for the class bit of deep partitions)
                 const uint32_t nlarge = cur.nlarge;  // counted with the active set
                 const uint32_t lbits = nlarge > 1 ? bit_width(nlarge - 1) : 0;
-                if (lbits + c * bits > 64 && 64 - lbits >= bits) c = (64 - lbits) / bits;
+                if (lbits + c * bits + 1 > 64 && 64 - lbits - 1 >= bits) c = (64 - lbits - 1) / bits;
             }
             DBuf<uint8_t> Ha(cache, (size_t)A + 1);
-            refine_round(RoundChars{PT, bits, C, c * bits, coded ? code.ctab : nullptr, kbits, L}, cur, order, H, Ha);
+            // while some group needs another round after this one, the partitions of large groups look ahead to
+            // character L and park the members equal to their first one through it
+            const bool deep = kmin + C + c < L;
+            parked |= refine_round(RoundChars{PT, bits, C, c * bits, coded ? code.ctab : nullptr, kbits, L}, cur, order,
+                                   H, Ha, deep);
             C += c;
             ActiveSet next;
             active_set(Ha.get(), cur.act.get(), A, next, true);
@@ -640,6 +646,12 @@ class Engine {
                                                            nullptr, km.get());
                 kmin = std::max(kmin, (uint32_t)prim.read_device(km.get(), 4)[0]);
             }
+        }
+        if (parked) {
+            // the parked groups are tied through L as well: the tied set is every group of H with two or
+            // more members
+            cur = ActiveSet();
+            active_set(H.get(), nullptr, N, cur);
         }
         fin = std::move(cur);
     }
@@ -677,8 +689,10 @@ class Engine {
     // each, middle ones (up to RF_WAVE members) by one wave each; the members of large groups go through a
     // radix sort keyed by (rank among the large groups, characters), or two stable sorts when those do not
     // fit 64 bits.
-    void refine_round(const RoundChars& rc, const ActiveSet& cur, DBuf<uint32_t>& order, DBuf<uint8_t>& H,
-                      DBuf<uint8_t>& Ha) {
+    // deep: the large groups' three-way partitions compare their members with the first one up to character
+    // L, and the members equal to it there leave the rounds (returns true when it may have parked some).
+    bool refine_round(const RoundChars& rc, const ActiveSet& cur, DBuf<uint32_t>& order, DBuf<uint8_t>& H,
+                      DBuf<uint8_t>& Ha, bool deep = false) {
         const uint64_t A = cur.A;
         const uint32_t G = cur.G;
         DBuf<uint8_t> large(cache, G);
@@ -693,7 +707,7 @@ class Engine {
         // members of the large groups and their count, known from the active set's group counts
         const uint64_t B = cur.B8;
         const uint32_t nlarge = cur.nlarge;
-        if (B == 0) return;
+        if (B == 0) return false;
         DBuf<uint32_t> slots(cache, B), lrank(cache, B), lfirst(cache, (size_t)nlarge + 1);
         prim.scan(LargeMembersOp{large.get(), cur.gid.get(), slots.get(), lrank.get(), lfirst.get()}, A, 1);
         prim.fill_u32(lfirst.get() + nlarge, (uint32_t)B, 1);  // B = 2^32 reads as 0: spans are differences
@@ -707,6 +721,8 @@ class Engine {
         uint32_t* y2 = rvb.get();
         if (lbits + cbits <= 64) {
             const bool part = B >= LARGE_PARTITION_MIN && B >= (uint64_t)LARGE_PARTITION_AVG * nlarge;
+            const bool dp = deep && part && lbits + cbits + 1 <= 64;  // a class bit below the characters
+            const uint32_t lo = 64 - lbits - cbits - (dp ? 1 : 0);
             DBuf<uint8_t> cls(cache, part ? B : 0);
             DBuf<uint32_t> lk;  // after a coded round 0: k of each large group, decoded once
             if (rc.ctab) {
@@ -715,14 +731,15 @@ class Engine {
                                                             order.get(), lk.get(), nullptr);
             }
             k_keys_large<<<grid(B), 256, 0, stream>>>(rc, B, slots.get(), cur.act.get(), lrank.get(), order.get(),
-                                                      lbits, x, y, lfirst.get(), part ? cls.get() : nullptr, lk.get());
-            if (part && partition_large(cur, slots.get(), lrank.get(), lfirst.get(), cls.get(), B, 64 - lbits - cbits,
-                                        x, y, x2, y2, order, H, Ha))
-                return;
-            prim.sort_pairs(x, x2, y, y2, B, 64 - lbits - cbits, 64);
+                                                      lbits, x, y, lfirst.get(), part ? cls.get() : nullptr, lk.get(),
+                                                      dp ? 1u : 0u);
+            if (part && partition_large(cur, slots.get(), lrank.get(), lfirst.get(), cls.get(), B, lo, x, y, x2, y2,
+                                        order, H, Ha, dp))
+                return dp;
+            prim.sort_pairs(x, x2, y, y2, B, lo, 64);
             k_writeback_large<<<grid(B), 256, 0, stream>>>(x, y, slots.get(), B, cur.act.get(), order.get(), H.get(),
                                                            Ha.get());
-            return;
+            return false;
         }
         // characters first, then a stable sort by the group rank (the members are identified by index)
         DBuf<uint32_t> P(cache, B);
@@ -733,6 +750,7 @@ class Engine {
         prim.sort_pairs(x, x2, y, y2, B, 0, lbits);
         k_writeback_large2<<<grid(B), 256, 0, stream>>>(x, y, P.get(), rc, slots.get(), B, cur.act.get(), order.get(),
                                                         H.get(), Ha.get());
+        return false;
     }
 
     // Large groups by a stable three-way partition around each group's first key (PivotScanOp), when at
@@ -741,14 +759,15 @@ class Engine {
     // x and y untouched) when the caller should sort all members instead.
     bool partition_large(const ActiveSet& cur, const uint32_t* slots, const uint32_t* lrank, const uint32_t* lfirst,
                          const uint8_t* cls, uint64_t B, uint32_t lo_bit, const uint64_t* x, const uint32_t* y,
-                         uint64_t* x2, uint32_t* y2, DBuf<uint32_t>& order, DBuf<uint8_t>& H, DBuf<uint8_t>& Ha) {
+                         uint64_t* x2, uint32_t* y2, DBuf<uint32_t>& order, DBuf<uint8_t>& H, DBuf<uint8_t>& Ha,
+                         bool park = false) {
         DBuf<uint32_t> minpref(cache, (size_t)B + 1), c0pref(cache, (size_t)B + 1);
         prim.scan(PivotScanOp{cls, x, minpref.get(), c0pref.get(), x2, y2}, B, 1);
         const auto tot = prim.totals(1);
         const uint64_t Bm = tot.lo;
         if (Bm * 2 > B) return false;
         if (Bm == 0) {
-            k_uniform_heads<<<grid(B), 256, 0, stream>>>(B, slots, lrank, lfirst, Ha.get());
+            k_uniform_heads<<<grid(B), 256, 0, stream>>>(B, slots, lrank, lfirst, Ha.get(), park ? 1u : 0u);
             return true;
         }
         prim.fill_u32(minpref.get() + B, (uint32_t)Bm, 1);
@@ -761,7 +780,7 @@ class Engine {
         uint32_t* mia = mi2.get();
         prim.sort_pairs(mk, mka, mi, mia, Bm, lo_bit, 64);
         k_place_equal<<<grid(B), 256, 0, stream>>>(B, y, slots, lrank, lfirst, minpref.get(), c0pref.get(),
-                                                   cur.act.get(), order.get(), H.get(), Ha.get());
+                                                   cur.act.get(), order.get(), H.get(), Ha.get(), park ? 1u : 0u);
         k_place_minority<<<grid(Bm), 256, 0, stream>>>(Bm, mk, mi, y, slots, lrank, lfirst, minpref.get(), c0pref.get(),
                                                        cur.act.get(), order.get(), H.get(), Ha.get());
         return true;

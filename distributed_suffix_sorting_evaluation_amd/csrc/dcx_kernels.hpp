@@ -334,6 +334,19 @@ struct RoundChars {
         const StreamKeys<false> w{s, bits, top_mask(nb)};
         return w(start);
     }
+    // -1, 0 or 1: suffix p against suffix q (members of one group, so the same k) on the characters after
+    // this round's, up to character L
+    __device__ __forceinline__ int cmp_after(uint64_t p, uint64_t q, uint32_t k) const {
+        const uint32_t cpw = 64 / bits;
+        const uint32_t c0 = (ctab ? (k + C < L ? k + C : L) : C) + cbits / bits;
+        for (uint32_t t = c0; t < L; t += cpw) {
+            const uint32_t nc = L - t < cpw ? L - t : cpw;
+            const StreamKeys<false> w{s, bits, top_mask(nc * bits)};
+            const uint64_t a = w(p + t), b = w(q + t);
+            if (a != b) return a < b ? -1 : 1;
+        }
+        return 0;
+    }
 };
 
 // k of every group's first member after a coded round 0 (code_chars), lk[g] for g < G; with kmin, their
@@ -1143,19 +1156,31 @@ __global__ __launch_bounds__(256) void k_keys_large(RoundChars chars, uint64_t B
                                                     const uint32_t* __restrict__ order, uint32_t lbits,
                                                     uint64_t* __restrict__ keys, uint32_t* __restrict__ vals,
                                                     const uint32_t* __restrict__ lfirst, uint8_t* __restrict__ cls,
-                                                    const uint32_t* __restrict__ lk) {
+                                                    const uint32_t* __restrict__ lk, uint32_t deep) {
     for (uint64_t b = blockIdx.x * 256ull + threadIdx.x; b < B; b += gridDim.x * 256ull) {
         const uint32_t p = order[act_idx[slots[b]]];
         const uint32_t k = lk ? lk[lrank[b]] : 0u;  // the group's k after a coded round 0
         const uint64_t ch = chars.with_k((uint64_t)p, k);
-        keys[b] = lbits ? (((uint64_t)lrank[b] << (64 - lbits)) | (ch >> lbits)) : ch;
+        uint64_t key = lbits ? (((uint64_t)lrank[b] << (64 - lbits)) | (ch >> lbits)) : ch;
         vals[b] = p;
         if (cls) {
             const uint32_t fb = lfirst[lrank[b]];
-            const uint64_t cf = chars.with_k((uint64_t)order[act_idx[slots[fb]]], k);
+            const uint32_t pf = order[act_idx[slots[fb]]];
+            const uint64_t cf = chars.with_k((uint64_t)pf, k);
             const uint64_t x = ch >> lbits, pv = cf >> lbits;  // the same group rank above both
-            cls[b] = x < pv ? 0 : (x == pv ? 1 : 2);
+            uint8_t c = x < pv ? 0 : (x == pv ? 1 : 2);
+            if (deep) {
+                // against the first member up to character L: 1 only for the members equal to it there; the
+                // class bit below the characters orders the others of this round's key by that comparison
+                if (c == 1) {
+                    const int r = chars.cmp_after((uint64_t)p, (uint64_t)pf, k);
+                    c = r < 0 ? 0 : (r == 0 ? 1 : 2);
+                }
+                if (c == 2) key |= 1ull << (63 - lbits - chars.cbits);
+            }
+            cls[b] = c;
         }
+        keys[b] = key;
     }
 }
 
@@ -1236,7 +1261,8 @@ __global__ __launch_bounds__(256) void k_place_equal(uint64_t B, const uint32_t*
                                                      const uint32_t* __restrict__ minpref,
                                                      const uint32_t* __restrict__ c0pref,
                                                      const uint32_t* __restrict__ act_idx, uint32_t* __restrict__ order,
-                                                     uint8_t* __restrict__ H, uint8_t* __restrict__ Ha) {
+                                                     uint8_t* __restrict__ H, uint8_t* __restrict__ Ha,
+                                                     uint32_t park) {
     for (uint64_t b = blockIdx.x * 256ull + threadIdx.x; b < B; b += gridDim.x * 256ull) {
         if (minpref[b + 1] != minpref[b]) continue;  // minority
         const LargeGroupSpan sp = large_span(b, lrank, lfirst);
@@ -1244,6 +1270,9 @@ __global__ __launch_bounds__(256) void k_place_equal(uint64_t B, const uint32_t*
         // rank among the equal members (list indices below 2^32; differences of the prefix arrays modulo 2^32)
         const uint32_t r1 = ((uint32_t)b - sp.fb) - (minpref[b] - minpref[sp.fb]);
         large_put(sp.fb + n0 + r1, vals[b], r1 == 0, slots, act_idx, order, H, Ha);
+        // park: members equal through character L leave the rounds (singletons of the next active list),
+        // their group stays one group in H
+        if (park) Ha[slots[sp.fb + n0 + r1]] = 1;
     }
 }
 
@@ -1271,11 +1300,13 @@ __global__ __launch_bounds__(256) void k_place_minority(uint64_t Bm, const uint6
 }
 
 // groups whose members all share the first member's key: only the compact heads change
+// (park: the members are equal through character L and all leave the rounds)
 __global__ __launch_bounds__(256) void k_uniform_heads(uint64_t B, const uint32_t* __restrict__ slots,
                                                        const uint32_t* __restrict__ lrank,
-                                                       const uint32_t* __restrict__ lfirst, uint8_t* __restrict__ Ha) {
+                                                       const uint32_t* __restrict__ lfirst, uint8_t* __restrict__ Ha,
+                                                       uint32_t park) {
     for (uint64_t b = blockIdx.x * 256ull + threadIdx.x; b < B; b += gridDim.x * 256ull)
-        Ha[slots[b]] = (uint32_t)b == lfirst[lrank[b]] ? 1 : 0;
+        Ha[slots[b]] = park || (uint32_t)b == lfirst[lrank[b]] ? 1 : 0;
 }
 
 // The same when rank and characters do not fit one word: an LSD sort by the characters (values: member
