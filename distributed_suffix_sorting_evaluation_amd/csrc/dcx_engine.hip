@@ -586,6 +586,8 @@ class Engine {
         uint32_t kmin = coded ? std::min<uint32_t>(L, kbits / code.max_len) : 0;
         const RoundChars rk{PT, bits, 0, 0, code.ctab, kbits, L};
         bool parked = false;  // some large groups left the rounds equal through L (refine_round deep)
+        DBuf<uint32_t> act0;  // the first round's list (its slots hold every group tied later), A0 entries
+        uint64_t A0 = 0;
         ActiveSet cur;
         active_set(H.get(), nullptr, N, cur);
         DBuf<uint32_t> G;
@@ -638,6 +640,10 @@ class Engine {
             C += c;
             ActiveSet next;
             active_set(Ha.get(), cur.act.get(), A, next, true);
+            if (!act0.get()) {
+                act0 = std::move(cur.act);
+                A0 = A;
+            }
             cur = std::move(next);
             if (coded && cur.A > 0 && cur.G <= GROUP_KMIN_MAX) {
                 DBuf<uint32_t> km(cache, 1);
@@ -649,9 +655,9 @@ class Engine {
         }
         if (parked) {
             // the parked groups are tied through L as well: the tied set is every group of H with two or
-            // more members
+            // more members, all of them inside the first round's list
             cur = ActiveSet();
-            active_set(H.get(), nullptr, N, cur);
+            active_set(H.get(), act0.get(), A0, cur);
         }
         fin = std::move(cur);
     }
