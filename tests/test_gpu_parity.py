@@ -307,6 +307,22 @@ def test_coded_round0_against_oracle(engine, case):
         assert same_stats(st["levels"], ost), (name, X, st["levels"], ost)
 
 
+def test_coded_round0_every_x_against_libsais(engine):
+    """The code-keyed round 0 under every difference cover: the byte-alphabet text (32 code bits above the
+    position) and the sentinel-coded 5-letter text of 2^24 + 4,097 characters, their SAs against libsais for
+    all eleven X (L from 3 to 136 characters: the code's character count capped at L, look-ahead partitions
+    of the periodic run over up to 19 rounds)."""
+    if not oracle.sais_available():
+        pytest.skip("libsais not built")
+    texts = _coded_round0_texts()
+    for name, t in (texts[0], texts[3]):
+        ref = oracle.libsais_sa(t, threads=8)
+        for X in ALL_X:
+            say(f"coded round 0 {name} X={X} vs libsais")
+            sa, _ = gpu_sa(engine, t, X=X)
+            assert np.array_equal(sa, ref), (name, X)
+
+
 def test_zipf_16mib_digest_dc21_dc39_dc133(engine):
     """The byte-alphabet workload (Zipf(1.1) over 256 values plus a period-17 block), sigma = 256 (9-bit
     packing), at 16 MiB: every X gives the SA whose libsais digest is committed."""
