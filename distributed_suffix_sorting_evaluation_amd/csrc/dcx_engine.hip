@@ -469,8 +469,10 @@ class Engine {
     // ahead (exact: the last step takes only the L - C characters still missing).  Doubling needs the
     // sentinel coding (dense = false): the empty suffix past the end must be the smallest key.
     // `info`: estimated bits of information per character (at most `bits`), which sizes round 0.
-    // `code`: an order-preserving prefix code of the characters (dense words of at most 8 bits), which
-    // keys round 0 of a large sort when its key then needs fewer radix passes than the raw characters.
+    // `code`: an order-preserving prefix code of the character words (at most 8 bits, the padding's code
+    // included), which keys round 0 when its bits for log2(N) + ROUND0_SLACK bits of information fit beside
+    // the position (keys-only passes), or, in large sorts, when they take fewer radix passes than the raw
+    // characters (40 bits at most, beside the split position).
     void refine_sort(const uint64_t* PT, uint64_t N, uint32_t L, uint32_t bits, bool dense, double info,
                      DBuf<uint32_t>& order, DBuf<uint8_t>& H, ActiveSet& fin, const KeyCode& code = KeyCode{}) {
         const uint32_t cpw = 64 / bits;
