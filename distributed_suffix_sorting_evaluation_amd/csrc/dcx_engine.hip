@@ -1041,6 +1041,10 @@ class Engine {
     // 2^18 sampled characters, which keeps the codewords short; a code whose longest codeword would exceed
     // CODE_MAX_LEN bits is not used.  The device table (256 entries) goes to d_ctab; its host copy h_ctab
     // must stay until the copy has run (the calls are ordered on the stream, and a level builds one code).
+    // the 64-bit device output of the build in progress, written by level 0 (see level), and whether it was
+    uint64_t* out64 = nullptr;
+    bool out64_written = false;
+
     uint32_t h_ctab[256];
     KeyCode key_code(const std::vector<double>& wj, bool dense, DBuf<uint32_t>& d_ctab) {
         KeyCode kc;
@@ -1207,7 +1211,11 @@ class Engine {
                 prim.fill_u32(s_order.get(), m - 1, 1);
                 prim.fill_u32(s_gid.get(), 0, 1);
             }
-            prim.scan(SampleScanOp{order.get(), H.get(), dc, dummy ? 1u : 0u, s_order.get(), s_gid.get()}, n, 0);
+            // level 0 with a 64-bit device output: the scan's down pass writes it from every order entry
+            // (the tied slots change later and are written again after the merge)
+            uint64_t* wide = depth == 0 && out64 && Primitives::scan_two_pass(n) ? out64 : nullptr;
+            prim.scan(SampleScanOp{order.get(), H.get(), dc, dummy ? 1u : 0u, s_order.get(), s_gid.get(), wide}, n, 0);
+            if (wide) out64_written = true;
             DBuf<uint8_t> Hs(cache, (size_t)m + 1);
             k_heads_u32<<<grid(m / 16 + 1ull), 256, 0, stream>>>(s_gid.get(), m, Hs.get());
             s_gid.reset();
@@ -1234,6 +1242,8 @@ class Engine {
         // suffixes with different L_s-character keys are already in order; groups of equal keys are ordered
         // by the DC comparator on the sample ranks
         tie_break(dc, order, H, tied, ranks.get());
+        if (depth == 0 && out64_written && tied.A > 0)
+            k_widen_slots<<<grid(tied.A), 256, 0, stream>>>(order.get(), tied.act.get(), tied.A, out64);
         if (depth == 0) mark(5);
         if (sa_take)
             *sa_take = std::move(order);
@@ -1586,10 +1596,18 @@ class Engine {
         // phase boundaries default to "empty" (a base-case level never reaches them)
         DCX_HIP(hipEventRecord(ev[2], stream));
         DCX_HIP(hipEventRecord(ev[3], stream));
+        // a 64-bit device output is written by level 0 itself when it takes the sample scan's two passes
+        out64 = o->sa_on_device && width == 8 && (reinterpret_cast<uintptr_t>(sa_out) & 7) == 0
+                    ? static_cast<uint64_t*>(sa_out)
+                    : nullptr;
+        out64_written = false;
         level<uint8_t>(d_text, dlut.get(), n, sigma, 0, nullptr, ev, &sa);
+        out64 = nullptr;
         DCX_HIP(hipEventRecord(ev[4], stream));  // SA resident in HBM
         // output in the requested width
-        if (o->sa_on_device) {
+        if (o->sa_on_device && out64_written) {
+            // written during the level
+        } else if (o->sa_on_device) {
             widen(sa.get(), n, width, static_cast<uint8_t*>(sa_out));
         } else if (width == 4) {
             download(sa_out, sa.get(), n * 4);

@@ -606,12 +606,25 @@ struct NamesOp {
 struct SampleScanOp {
     static constexpr bool kBlocked = true;
     static constexpr bool kCompact = true;  // writes only samples (low half 1)
+    static constexpr bool kAll = true;      // sees every order entry (the 64-bit output below)
     const uint32_t* order;
     const uint8_t* H;
     DcDev dc;
     uint32_t shift;
     uint32_t* s_order;
     uint32_t* s_gid;
+    uint64_t* wide;  // optional: every order entry as a 64-bit SA entry (the level-0 output, 8-byte aligned)
+    __device__ __forceinline__ void all(uint64_t base, const uint32_t* pl, uint64_t N) const {
+        if (!wide) return;
+        if (base + SC_ITEMS <= N && (reinterpret_cast<uintptr_t>(wide) & 15) == 0) {
+#pragma unroll
+            for (int k = 0; k < SC_ITEMS; k += 2)
+                reinterpret_cast<uint4*>(wide + base)[k / 2] = make_uint4(pl[k], 0u, pl[k + 1], 0u);
+        } else {
+            for (int k = 0; k < SC_ITEMS; k++)
+                if (base + k < N) wide[base + k] = pl[k];
+        }
+    }
     __device__ __forceinline__ uint64_t value(uint64_t j) const {
         const uint32_t p = order[j];
         const uint32_t q = fast_div(p, dc.X, dc.inv_X);
@@ -1673,6 +1686,16 @@ __global__ void k_phase_mark() {}
 // ------------------------------------------------------------------------------------------------
 // output width conversion: uint32 SA -> 4 / 5 (uint40, uint_types.hpp:262) / 8 bytes little endian
 // ------------------------------------------------------------------------------------------------
+// out[j] = sa[j] as 64 bits for the slots j = slots[i], i < A (the tied slots, whose entries change after
+// the sample scan wrote the others)
+__global__ __launch_bounds__(256) void k_widen_slots(const uint32_t* __restrict__ sa, const uint32_t* __restrict__ slots,
+                                                     uint64_t A, uint64_t* __restrict__ out) {
+    for (uint64_t i = blockIdx.x * 256ull + threadIdx.x; i < A; i += (uint64_t)gridDim.x * 256) {
+        const uint32_t j = slots[i];
+        out[j] = sa[j];
+    }
+}
+
 __global__ __launch_bounds__(256) void k_widen(const uint32_t* __restrict__ sa, uint64_t n, uint32_t width,
                                                uint8_t* __restrict__ out) {
     for (uint64_t j = blockIdx.x * 256ull + threadIdx.x; j < n; j += (uint64_t)gridDim.x * 256) {

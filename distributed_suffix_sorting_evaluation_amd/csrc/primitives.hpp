@@ -976,6 +976,17 @@ struct ScanCompact<Op, decltype((void)Op::kCompact)> {
     static constexpr bool value = Op::kCompact;
 };
 
+// Ops with kAll see every element's payload in the down pass of a two-pass scan: all(base, pl, N) with a
+// thread's SC_ITEMS consecutive elements from base (blocked ops with payloads only).
+template <class Op, class = void>
+struct ScanAll {
+    static constexpr bool value = false;
+};
+template <class Op>
+struct ScanAll<Op, decltype((void)Op::kAll)> {
+    static constexpr bool value = Op::kAll;
+};
+
 template <bool PL = false, class Op>
 __device__ __forceinline__ void scan_load(const Op& op, uint64_t N, uint64_t tbase, uint32_t tid,
                                           uint64_t (&v)[SC_ITEMS], uint32_t* pl = nullptr) {
@@ -1097,6 +1108,7 @@ __global__ __launch_bounds__(SC_THREADS) void k_scan_down(Op op, uint64_t N, uin
         uint64_t v[SC_ITEMS];
         uint32_t pl[PL ? SC_ITEMS : 1];
         scan_load<PL>(op, N, tbase, tid, v, pl);
+        if constexpr (PL && ScanAll<Op>::value) op.all(tbase + (uint64_t)tid * SC_ITEMS, pl, N);
         if constexpr (!Op::kBlocked) {
             // striped values -> blocked through LDS
 #pragma unroll
@@ -1667,6 +1679,8 @@ struct Primitives {
         return (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>(tiles, want));
     }
 
+    // whether scan() over N elements takes the two passes (reduce, down) rather than the single pass
+    static bool scan_two_pass(uint64_t N) { return ceil_div(N, (uint64_t)SC_TILE) > SC_SINGLE_PASS_TILES; }
     template <class Op>
     void scan(const Op& op, uint64_t N, int slot) {
         if (N == 0) {

@@ -166,6 +166,28 @@ def test_sa_widths(engine):
     assert s5.tobytes() == oracle.uint40_bytes(ref)
 
 
+def test_device_output_64bit_from_the_level(engine):
+    """A 64-bit SA in device memory is written by level 0 itself (the sample scan's second pass writes every
+    entry, the tied groups are written again after the DC-comparator merge) when the text takes the two-pass
+    scan (more than 2^20 characters): a text with copies and a periodic run (large tied groups the merge
+    reorders) against libsais, into a 16-byte aligned output (vector stores) and into one 8 bytes off."""
+    import torch
+    t = copies_text(3 << 20, 41, sigma=4, copy_frac=0.5, max_len=4000)
+    t[1_000_000:1_200_000] = np.tile(t[:13], 200_000 // 13 + 1)[:200_000]
+    ref = oracle.libsais_sa(t, threads=8)
+    n = t.size
+    dt = torch.from_numpy(t).cuda()
+    for X in (7, 39):
+        for off in (0, 8):
+            say(f"64-bit device output X={X} offset={off}")
+            buf = torch.zeros(8 * n + 16, dtype=torch.uint8, device="cuda")
+            engine.build_sa_device(dt.data_ptr(), n, buf.data_ptr() + off, 8, X=X)
+            got = buf[off:off + 8 * n].cpu().numpy().view("<u8")
+            assert np.array_equal(got, ref), (X, off)
+    del dt
+    torch.cuda.empty_cache()
+
+
 def test_width5_device_output_tails_and_alignment(engine):
     """40-bit output written on the device: lengths that are not a multiple of 16 and an output pointer
     that is not 16-byte aligned take the per-entry path, the rest the 16-entry vector path."""
