@@ -1041,10 +1041,6 @@ class Engine {
     // 2^18 sampled characters, which keeps the codewords short; a code whose longest codeword would exceed
     // CODE_MAX_LEN bits is not used.  The device table (256 entries) goes to d_ctab; its host copy h_ctab
     // must stay until the copy has run (the calls are ordered on the stream, and a level builds one code).
-    // the 64-bit device output of the build in progress, written by level 0 (see level), and whether it was
-    uint64_t* out64 = nullptr;
-    bool out64_written = false;
-
     uint32_t h_ctab[256];
     KeyCode key_code(const std::vector<double>& wj, bool dense, DBuf<uint32_t>& d_ctab) {
         KeyCode kc;
@@ -1110,6 +1106,16 @@ class Engine {
 
     // characters of the level are 1..maxchar (0 is the padding)
     using StreamFn = std::function<void(uint32_t bits, uint32_t dense, uint64_t chars, DBuf<uint64_t>& PT)>;
+    // A 64-bit SA in device memory for the build in progress (null outside build_sa): level 0 writes it
+    // from the sample scan's second pass when it takes two passes, and sets out64_written
+    uint64_t* out64 = nullptr;
+    bool out64_written = false;
+    struct Out64Scope {  // out64 for one build, cleared however the build ends
+        uint64_t*& p;
+        Out64Scope(uint64_t*& q, uint64_t* v) : p(q) { p = v; }
+        ~Out64Scope() { p = nullptr; }
+    };
+
     template <class CharT>
     // The SA goes to sa_out, or with sa_take != nullptr into *sa_take (the level's own buffer, no copy).
     // A caller that has the level's text only as pieces (a batch) passes `stream_fn`, which packs the
@@ -1184,6 +1190,7 @@ class Engine {
         DBuf<uint32_t> order;
         DBuf<uint8_t> H;
         ActiveSet tied;
+        uint64_t* wide64 = nullptr;  // level 0: the 64-bit device output this level writes
         if (depth == 0) mark(1);
         {
             DBuf<uint64_t> PT;
@@ -1213,9 +1220,9 @@ class Engine {
             }
             // level 0 with a 64-bit device output: the scan's down pass writes it from every order entry
             // (the tied slots change later and are written again after the merge)
-            uint64_t* wide = depth == 0 && out64 && Primitives::scan_two_pass(n) ? out64 : nullptr;
-            prim.scan(SampleScanOp{order.get(), H.get(), dc, dummy ? 1u : 0u, s_order.get(), s_gid.get(), wide}, n, 0);
-            if (wide) out64_written = true;
+            wide64 = depth == 0 && out64 && Primitives::scan_two_pass(n) ? out64 : nullptr;
+            prim.scan(SampleScanOp{order.get(), H.get(), dc, dummy ? 1u : 0u, s_order.get(), s_gid.get(), wide64}, n,
+                      0);
             DBuf<uint8_t> Hs(cache, (size_t)m + 1);
             k_heads_u32<<<grid(m / 16 + 1ull), 256, 0, stream>>>(s_gid.get(), m, Hs.get());
             s_gid.reset();
@@ -1242,8 +1249,10 @@ class Engine {
         // suffixes with different L_s-character keys are already in order; groups of equal keys are ordered
         // by the DC comparator on the sample ranks
         tie_break(dc, order, H, tied, ranks.get());
-        if (depth == 0 && out64_written && tied.A > 0)
-            k_widen_slots<<<grid(tied.A), 256, 0, stream>>>(order.get(), tied.act.get(), tied.A, out64);
+        if (wide64) {
+            if (tied.A > 0) k_widen_slots<<<grid(tied.A), 256, 0, stream>>>(order.get(), tied.act.get(), tied.A, wide64);
+            out64_written = true;
+        }
         if (depth == 0) mark(5);
         if (sa_take)
             *sa_take = std::move(order);
@@ -1597,12 +1606,13 @@ class Engine {
         DCX_HIP(hipEventRecord(ev[2], stream));
         DCX_HIP(hipEventRecord(ev[3], stream));
         // a 64-bit device output is written by level 0 itself when it takes the sample scan's two passes
-        out64 = o->sa_on_device && width == 8 && (reinterpret_cast<uintptr_t>(sa_out) & 7) == 0
-                    ? static_cast<uint64_t*>(sa_out)
-                    : nullptr;
         out64_written = false;
-        level<uint8_t>(d_text, dlut.get(), n, sigma, 0, nullptr, ev, &sa);
-        out64 = nullptr;
+        {
+            Out64Scope scope(out64, o->sa_on_device && width == 8 && (reinterpret_cast<uintptr_t>(sa_out) & 7) == 0
+                                        ? static_cast<uint64_t*>(sa_out)
+                                        : nullptr);
+            level<uint8_t>(d_text, dlut.get(), n, sigma, 0, nullptr, ev, &sa);
+        }
         DCX_HIP(hipEventRecord(ev[4], stream));  // SA resident in HBM
         // output in the requested width
         if (o->sa_on_device && out64_written) {

@@ -170,7 +170,8 @@ def test_device_output_64bit_from_the_level(engine):
     """A 64-bit SA in device memory is written by level 0 itself (the sample scan's second pass writes every
     entry, the tied groups are written again after the DC-comparator merge) when the text takes the two-pass
     scan (more than 2^20 characters): a text with copies and a periodic run (large tied groups the merge
-    reorders) against libsais, into a 16-byte aligned output (vector stores) and into one 8 bytes off."""
+    reorders) against libsais, into a 16-byte aligned output (vector stores) and into one 8 bytes off.  A
+    batched build and a 32-bit build on the same engine afterwards must not touch that output again."""
     import torch
     t = copies_text(3 << 20, 41, sigma=4, copy_frac=0.5, max_len=4000)
     t[1_000_000:1_200_000] = np.tile(t[:13], 200_000 // 13 + 1)[:200_000]
@@ -184,7 +185,18 @@ def test_device_output_64bit_from_the_level(engine):
             engine.build_sa_device(dt.data_ptr(), n, buf.data_ptr() + off, 8, X=X)
             got = buf[off:off + 8 * n].cpu().numpy().view("<u8")
             assert np.array_equal(got, ref), (X, off)
-    del dt
+    # the same engine next: a batch (level 0 of every sub-batch) and a 32-bit device build
+    say("batch and 32-bit build after the 64-bit device output")
+    off = np.array([0, 1 << 20, 2 << 20, n], dtype=np.uint64)
+    ds = torch.zeros(n, dtype=torch.int32, device="cuda")
+    engine.build_sa_batch_device(dt.data_ptr(), off, ds.data_ptr(), X=39)
+    got = ds.cpu().numpy().view(np.uint32)
+    for k in range(3):
+        a, b = int(off[k]), int(off[k + 1])
+        assert np.array_equal(got[a:b].astype(np.uint64), oracle.libsais_sa(t[a:b], threads=8)), k
+    engine.build_sa_device(dt.data_ptr(), n, ds.data_ptr(), 4, X=39)
+    assert np.array_equal(ds.cpu().numpy().view(np.uint32).astype(np.uint64), ref)
+    del dt, ds
     torch.cuda.empty_cache()
 
 
