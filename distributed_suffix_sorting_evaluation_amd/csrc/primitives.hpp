@@ -275,35 +275,36 @@ __global__ __launch_bounds__(RS_HIST_THREADS, 8) void k_coded_hist_seg(CodedKeys
         uint64_t ks[CKH_RUN];
         if (bits == 8 && (base & 7) == 0) {
             // byte characters: the run's characters are the word s_w[threadIdx.x], and each later word holds
-            // the next eight; their table entries are looked up eight at a time into a queue (q[0]: the
-            // entry of character p0 + j), and the run's own eight also give the lengths that leave
-            uint32_t q[8], dl[8];
-            uint32_t qn = 8, nextw = threadIdx.x + 1;
-            {
-                const uint64_t w0 = s_w[threadIdx.x];
+            // the next eight.  The table entry of the next character is read one step ahead (its LDS
+            // latency overlaps the current append), the next word one word ahead; the run's own eight
+            // entries give the lengths that leave.  Word indices stay below the staged array (clamped).
+            constexpr uint32_t SW_LAST = (CKH_CHARS + CK_BITS) * 8 / 64 + 2;
+            uint64_t w = s_w[threadIdx.x];
+            uint32_t dl[8];
 #pragma unroll
-                for (int i = 0; i < 8; i++) {
-                    q[i] = s_tab[(uint32_t)(w0 >> (56 - 8 * i)) & 0xFFu];
-                    dl[i] = q[i] >> 24;
-                }
-            }
+            for (int i = 0; i < 8; i++) dl[i] = s_tab[(uint32_t)(w >> (56 - 8 * i)) & 0xFFu] >> 24;
+            uint32_t wn = threadIdx.x + 2;
+            uint64_t w2 = s_w[threadIdx.x + 1];
+            uint32_t en = s_tab[(uint32_t)(w >> 56)];  // entry of character p0 + j
+            w <<= 8;
+            uint32_t left = 7;  // characters still in w
 #pragma unroll
             for (uint32_t r = 0; r < CKH_RUN; r++) {
                 ks[r] = 0;
                 if (r < cnt) {
                     while (filled < kbits && j < r + L) {
-                        if (qn == 0) {
-                            const uint64_t w = s_w[nextw++];
-#pragma unroll
-                            for (int i = 0; i < 8; i++) q[i] = s_tab[(uint32_t)(w >> (56 - 8 * i)) & 0xFFu];
-                            qn = 8;
+                        const uint32_t cur = en;
+                        if (left == 0) {
+                            w = w2;
+                            w2 = s_w[wn < SW_LAST ? wn : SW_LAST];
+                            wn++;
+                            left = 8;
                         }
-                        const uint32_t en = q[0];
-#pragma unroll
-                        for (int i = 0; i < 7; i++) q[i] = q[i + 1];
-                        qn--;
-                        acc |= (uint64_t)(en & 0xFFFFFFu) << (CK_BITS - filled);
-                        filled += en >> 24;
+                        en = s_tab[(uint32_t)(w >> 56)];
+                        w <<= 8;
+                        left--;
+                        acc |= (uint64_t)(cur & 0xFFFFFFu) << (CK_BITS - filled);
+                        filled += cur >> 24;
                         j++;
                     }
                     ks[r] = acc & km;
