@@ -196,10 +196,12 @@ __global__ __launch_bounds__(RS_HIST_THREADS) void k_radix_hist_seg(Src src, uin
 // and the code table in LDS.  A thread takes CKH_RUN consecutive positions and slides one bit window along
 // them: acc holds the codewords of characters [i, j) from its top bit (filled bits), the key of i is its
 // top kbits bits once filled >= kbits or j = i + L, and moving to i + 1 drops character i's codeword and
-// appends the next ones: about one table lookup each way per key instead of one per key character.
+// appends the next ones: about one table lookup each way per key instead of one per key character.  Byte
+// characters with L >= CK_BITS go right to left instead: one window for the position after the run, then
+// each key is its character's codeword followed by the next key.
 // The keys leave through a per-wave LDS buffer, 16 lanes' runs (128 consecutive keys) at a time, so each
 // store instruction writes 512 contiguous bytes (a store of every lane's r-th key would touch 64 lines).
-constexpr uint32_t CKH_RUN = 8;
+constexpr uint32_t CKH_RUN = 8;  // a stream word of byte characters
 constexpr uint32_t CKH_CHARS = RS_HIST_THREADS * CKH_RUN;
 constexpr uint32_t CKH_PASSES = 5;  // CK_BITS / 8
 // (8 waves per SIMD: two workgroups per CU, which the LDS allows, need at most 96 scalar registers)
@@ -273,7 +275,46 @@ __global__ __launch_bounds__(RS_HIST_THREADS, 8) void k_coded_hist_seg(CodedKeys
         uint64_t acc = 0;
         uint32_t filled = 0, j = 0;  // acc: the codewords of characters [p0 + r, p0 + j)
         uint64_t ks[CKH_RUN];
-        if (bits == 8 && (base & 7) == 0) {
+        if (bits == 8 && (base & 7) == 0 && L >= CK_BITS) {
+            // byte characters, and no key is cut short by L (a window of L >= CK_BITS characters holds at
+            // least kbits bits): key(i) is the codeword of character i followed by the top bits of
+            // key(i + 1).  One fill gives the key of p0 + 8 (the next run's first position); the run's
+            // keys follow from it right to left, a codeword each (no loop whose length varies by lane).
+            // Characters at or past 40 of a position do not reach its key, so the fill may read past the
+            // stream's end.  Word indices stay below the staged array (clamped).
+            constexpr uint32_t SW_LAST = (CKH_CHARS + CK_BITS) * 8 / 64 + 2;
+            const uint64_t w = s_w[threadIdx.x];
+            uint32_t e8[8];
+#pragma unroll
+            for (int i = 0; i < 8; i++) e8[i] = s_tab[(uint32_t)(w >> (56 - 8 * i)) & 0xFFu];
+            uint64_t k = 0;
+            if (cnt) {
+                uint64_t w1 = s_w[threadIdx.x + 1], w2 = s_w[threadIdx.x + 2];
+                uint32_t wn = threadIdx.x + 3;
+                uint32_t en = s_tab[(uint32_t)(w1 >> 56)];  // the next character's entry, one step ahead
+                w1 <<= 8;
+                uint32_t left = 7, f = 0;
+                while (f < kbits) {
+                    const uint32_t cur = en;
+                    if (left == 0) {
+                        w1 = w2;
+                        w2 = s_w[wn < SW_LAST ? wn : SW_LAST];
+                        wn++;
+                        left = 8;
+                    }
+                    en = s_tab[(uint32_t)(w1 >> 56)];
+                    w1 <<= 8;
+                    left--;
+                    k |= (uint64_t)(cur & 0xFFFFFFu) << (CK_BITS - f);
+                    f += cur >> 24;
+                }
+            }
+#pragma unroll
+            for (int r = CKH_RUN - 1; r >= 0; r--) {
+                k = ((uint64_t)(e8[r] & 0xFFFFFFu) << CK_BITS) | (k >> (e8[r] >> 24));
+                ks[r] = (uint32_t)r < cnt ? k & km : 0ull;
+            }
+        } else if (bits == 8 && (base & 7) == 0) {
             // byte characters: the run's characters are the word s_w[threadIdx.x], and each later word holds
             // the next eight.  The table entry of the next character is read one step ahead (its LDS
             // latency overlaps the current append), the next word one word ahead; the run's own eight
