@@ -196,9 +196,9 @@ __global__ __launch_bounds__(RS_HIST_THREADS) void k_radix_hist_seg(Src src, uin
 // and the code table in LDS.  A thread takes CKH_RUN consecutive positions and slides one bit window along
 // them: acc holds the codewords of characters [i, j) from its top bit (filled bits), the key of i is its
 // top kbits bits once filled >= kbits or j = i + L, and moving to i + 1 drops character i's codeword and
-// appends the next ones: about one table lookup each way per key instead of one per key character.  Byte
-// characters with L >= CK_BITS go right to left instead: one window for the position after the run, then
-// each key is its character's codeword followed by the next key.
+// appends the next ones: about one table lookup each way per key instead of one per key character.  With
+// L >= CK_BITS the keys go right to left instead: one window for the position after the run, then each key
+// is its character's codeword followed by the next key.
 // The keys leave through a per-wave LDS buffer, 16 lanes' runs (128 consecutive keys) at a time, so each
 // store instruction writes 512 contiguous bytes (a store of every lane's r-th key would touch 64 lines).
 constexpr uint32_t CKH_RUN = 8;  // a stream word of byte characters
@@ -352,6 +352,26 @@ __global__ __launch_bounds__(RS_HIST_THREADS, 8) void k_coded_hist_seg(CodedKeys
                     acc <<= dl[r];
                     filled -= dl[r];
                 }
+            }
+        } else if (L >= CK_BITS) {
+            // the same right to left for other character widths (characters past 40 of the last position
+            // read stale words of the staged array, which do not reach a key)
+            uint32_t e8[CKH_RUN];
+#pragma unroll
+            for (uint32_t r = 0; r < CKH_RUN; r++) e8[r] = entry(r);
+            uint64_t k = 0;
+            if (cnt) {
+                uint32_t f = 0, t = CKH_RUN;
+                while (f < kbits) {
+                    const uint32_t en = entry(t++);
+                    k |= (uint64_t)(en & 0xFFFFFFu) << (CK_BITS - f);
+                    f += en >> 24;
+                }
+            }
+#pragma unroll
+            for (int r = CKH_RUN - 1; r >= 0; r--) {
+                k = ((uint64_t)(e8[r] & 0xFFFFFFu) << CK_BITS) | (k >> (e8[r] >> 24));
+                ks[r] = (uint32_t)r < cnt ? k & km : 0ull;
             }
         } else {
 #pragma unroll
