@@ -92,7 +92,7 @@ constexpr uint32_t LARGE_PARTITION_AVG = 64;
 // length of the sampled characters, entropy: their order-0 entropy.
 struct KeyCode {
     const uint32_t* ctab = nullptr;
-    uint32_t max_len = 0;
+    uint32_t max_len = 0, min_len = 0;  // longest and shortest codeword
     double avg_len = 0, entropy = 0;
 };
 
@@ -496,7 +496,7 @@ class Engine {
             uint64_t* k2 = kb.get();
             uint32_t* nv = nullptr;
             uint32_t* nv2 = nullptr;
-            prim.sort_pairs_from(CodedKeys{PT, bits, code.ctab, kbits, L, k, 1u}, k, k2, nv, nv2, N, 64 - kbits, 64,
+            prim.sort_pairs_from(CodedKeys{PT, bits, code.ctab, kbits, L, k, 1u, L * code.min_len >= kbits ? 1u : 0u}, k, k2, nv, nv2, N, 64 - kbits, 64,
                                  false);
             H = DBuf<uint8_t>(cache, (size_t)N + 1);
             order = DBuf<uint32_t>(cache, N);
@@ -559,7 +559,7 @@ class Engine {
             uint32_t np;
             if (coded) {
                 // the histogram kernel writes the code keys into k, the first pass reads them from there
-                np = prim.sort_pairs_from(CodedKeys{PT, bits, code.ctab, kbits, L, k, 0u}, k, k2, v, v2, N,
+                np = prim.sort_pairs_from(CodedKeys{PT, bits, code.ctab, kbits, L, k, 0u, L * code.min_len >= kbits ? 1u : 0u}, k, k2, v, v2, N,
                                           64 - kbits, 64, true, low.get(), ia.get(), ib.get(), H.get(),
                                           top_mask(kbits), &heads_done);
             } else {
@@ -1073,6 +1073,7 @@ class Engine {
         DCX_HIP(hipMemcpyAsync(d_ctab.get(), h_ctab, sizeof(h_ctab), hipMemcpyHostToDevice, stream));
         kc.ctab = d_ctab.get();
         kc.max_len = longest;
+        kc.min_len = *std::min_element(len.begin(), len.end());
         kc.avg_len = avg;
         kc.entropy = e;
         return kc;

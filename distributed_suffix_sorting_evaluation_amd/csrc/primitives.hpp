@@ -120,6 +120,7 @@ struct CodedKeys {
     uint32_t kbits, L;
     uint64_t* mat;
     uint32_t packpos;
+    uint32_t rtl;  // every window of L characters holds kbits bits (L x the shortest codeword >= kbits)
     __device__ __forceinline__ uint64_t operator()(uint64_t i) const { return mat[i]; }
     uint64_t read_bytes(uint64_t n) const { return 8 * n; }
     __host__ __device__ uint32_t halo() const { return L < CK_BITS ? L : CK_BITS; }
@@ -196,9 +197,9 @@ __global__ __launch_bounds__(RS_HIST_THREADS) void k_radix_hist_seg(Src src, uin
 // and the code table in LDS.  A thread takes CKH_RUN consecutive positions and slides one bit window along
 // them: acc holds the codewords of characters [i, j) from its top bit (filled bits), the key of i is its
 // top kbits bits once filled >= kbits or j = i + L, and moving to i + 1 drops character i's codeword and
-// appends the next ones: about one table lookup each way per key instead of one per key character.  With
-// L >= CK_BITS the keys go right to left instead: one window for the position after the run, then each key
-// is its character's codeword followed by the next key.
+// appends the next ones: about one table lookup each way per key instead of one per key character.  When
+// every window of L characters holds kbits bits (rtl) the keys go right to left instead: one window for the
+// position after the run, then each key is its character's codeword followed by the next key.
 // The keys leave through a per-wave LDS buffer, 16 lanes' runs (128 consecutive keys) at a time, so each
 // store instruction writes 512 contiguous bytes (a store of every lane's r-th key would touch 64 lines).
 constexpr uint32_t CKH_RUN = 8;  // a stream word of byte characters
@@ -275,13 +276,13 @@ __global__ __launch_bounds__(RS_HIST_THREADS, 8) void k_coded_hist_seg(CodedKeys
         uint64_t acc = 0;
         uint32_t filled = 0, j = 0;  // acc: the codewords of characters [p0 + r, p0 + j)
         uint64_t ks[CKH_RUN];
-        if (bits == 8 && (base & 7) == 0 && L >= CK_BITS) {
-            // byte characters, and no key is cut short by L (a window of L >= CK_BITS characters holds at
-            // least kbits bits): key(i) is the codeword of character i followed by the top bits of
-            // key(i + 1).  One fill gives the key of p0 + 8 (the next run's first position); the run's
-            // keys follow from it right to left, a codeword each (no loop whose length varies by lane).
-            // Characters at or past 40 of a position do not reach its key, so the fill may read past the
-            // stream's end.  Word indices stay below the staged array (clamped).
+        if (bits == 8 && (base & 7) == 0 && src.rtl) {
+            // byte characters, and no key is cut short by L (every window of L characters holds kbits
+            // bits): key(i) is the codeword of character i followed by the top bits of key(i + 1).  One
+            // fill gives the key of p0 + 8 (the next run's first position); the run's keys follow from it
+            // right to left, a codeword each (no loop whose length varies by lane).  Characters at or past
+            // L of a position do not reach its key, so the fill may read past the stream's end and past
+            // the staged characters.  Word indices stay below the staged array (clamped).
             constexpr uint32_t SW_LAST = (CKH_CHARS + CK_BITS) * 8 / 64 + 2;
             const uint64_t w = s_w[threadIdx.x];
             uint32_t e8[8];
@@ -353,8 +354,8 @@ __global__ __launch_bounds__(RS_HIST_THREADS, 8) void k_coded_hist_seg(CodedKeys
                     filled -= dl[r];
                 }
             }
-        } else if (L >= CK_BITS) {
-            // the same right to left for other character widths (characters past 40 of the last position
+        } else if (src.rtl) {
+            // the same right to left for other character widths (characters past L of the last position
             // read stale words of the staged array, which do not reach a key)
             uint32_t e8[CKH_RUN];
 #pragma unroll
