@@ -614,15 +614,18 @@ struct SampleScanOp {
     uint32_t* s_order;
     uint32_t* s_gid;
     uint64_t* wide;  // optional: every order entry as a 64-bit SA entry (the level-0 output, 8-byte aligned)
-    __device__ __forceinline__ void all(uint64_t base, const uint32_t* pl, uint64_t N) const {
-        if (!wide) return;
-        if (base + SC_ITEMS <= N && (reinterpret_cast<uintptr_t>(wide) & 15) == 0) {
+    __device__ __forceinline__ bool wants_all() const { return wide != nullptr; }
+    // the tile's entries [tbase, tbase + SC_TILE) from p (LDS): two per lane and store, consecutive lanes on
+    // consecutive pairs (1 KB per wave and store where the output is 16-byte aligned)
+    __device__ __forceinline__ void all(uint64_t tbase, const uint32_t* p, uint64_t N) const {
+        if (tbase + SC_TILE <= N && (reinterpret_cast<uintptr_t>(wide) & 15) == 0) {
 #pragma unroll
-            for (int k = 0; k < SC_ITEMS; k += 2)
-                reinterpret_cast<uint4*>(wide + base)[k / 2] = make_uint4(pl[k], 0u, pl[k + 1], 0u);
+            for (uint32_t q = 0; q < SC_TILE / 2; q += SC_THREADS) {
+                const uint2 pr = reinterpret_cast<const uint2*>(p)[q + threadIdx.x];
+                reinterpret_cast<uint4*>(wide + tbase)[q + threadIdx.x] = make_uint4(pr.x, 0u, pr.y, 0u);
+            }
         } else {
-            for (int k = 0; k < SC_ITEMS; k++)
-                if (base + k < N) wide[base + k] = pl[k];
+            for (uint32_t e = threadIdx.x; e < SC_TILE && tbase + e < N; e += SC_THREADS) wide[tbase + e] = p[e];
         }
     }
     __device__ __forceinline__ uint64_t value(uint64_t j) const {

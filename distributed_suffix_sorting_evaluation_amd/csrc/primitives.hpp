@@ -1043,8 +1043,9 @@ struct ScanCompact<Op, decltype((void)Op::kCompact)> {
     static constexpr bool value = Op::kCompact;
 };
 
-// Ops with kAll see every element's payload in the down pass of a two-pass scan: all(base, pl, N) with a
-// thread's SC_ITEMS consecutive elements from base (blocked ops with payloads only).
+// Ops with kAll see every element's payload in the down pass of a two-pass scan (blocked compacting ops
+// with payloads only): when wants_all(), all(tbase, p, N) with the tile's SC_TILE payloads from tbase in
+// LDS, called by every thread of the workgroup.
 template <class Op, class = void>
 struct ScanAll {
     static constexpr bool value = false;
@@ -1155,6 +1156,7 @@ __global__ __launch_bounds__(SC_THREADS) void k_scan_down(Op op, uint64_t N, uin
                                                           const uint64_t* __restrict__ prefix) {
     constexpr bool PL = ScanPayload<Op>::value && Op::kBlocked;
     constexpr bool COMPACT = ScanCompact<Op>::value && Op::kBlocked;
+    static_assert(!ScanAll<Op>::value || (PL && COMPACT), "every payload is staged through the compaction buffer");
     constexpr uint32_t PAD = SC_TILE + SC_TILE / 16;  // slot of the tile's end prefix
     __shared__ uint64_t s_tmp[SC_THREADS / 64];
     __shared__ uint64_t s_e[COMPACT ? 1 : PAD + 1];  // padded to spread blocked accesses over banks
@@ -1165,7 +1167,7 @@ __global__ __launch_bounds__(SC_THREADS) void k_scan_down(Op op, uint64_t N, uin
     static_assert(SC_TILE <= (1 << 15), "compaction entries hold tile offsets in 15 bits");
     __shared__ uint16_t s_ci[COMPACT ? SC_TILE : 1];
     __shared__ uint16_t s_chi[COMPACT ? SC_TILE : 1];
-    __shared__ uint32_t s_cp[COMPACT && PL ? SC_TILE : 1];  // COMPACT: payloads
+    __shared__ __align__(16) uint32_t s_cp[COMPACT && PL ? SC_TILE : 1];  // COMPACT: payloads (16-byte stores)
     const uint32_t tid = threadIdx.x;
     const uint64_t t0 = (uint64_t)blockIdx.x * tpb;
     uint64_t carry = prefix[blockIdx.x];
@@ -1175,7 +1177,6 @@ __global__ __launch_bounds__(SC_THREADS) void k_scan_down(Op op, uint64_t N, uin
         uint64_t v[SC_ITEMS];
         uint32_t pl[PL ? SC_ITEMS : 1];
         scan_load<PL>(op, N, tbase, tid, v, pl);
-        if constexpr (PL && ScanAll<Op>::value) op.all(tbase + (uint64_t)tid * SC_ITEMS, pl, N);
         if constexpr (!Op::kBlocked) {
             // striped values -> blocked through LDS
 #pragma unroll
@@ -1217,6 +1218,19 @@ __global__ __launch_bounds__(SC_THREADS) void k_scan_down(Op op, uint64_t N, uin
                     op.write(tbase + (ci & 0x7FFFu), excl, vv, s_cp[e]);
                 else
                     op.write(tbase + (ci & 0x7FFFu), excl, vv);
+            }
+            if constexpr (PL && ScanAll<Op>::value) {
+                if (op.wants_all()) {
+                    // every payload of the tile through s_cp, then written striped (the tile's entries in
+                    // order, consecutive lanes on consecutive entries)
+                    __syncthreads();  // the compacted payloads are written out
+#pragma unroll
+                    for (int k = 0; k < SC_ITEMS; k += 4)
+                        reinterpret_cast<uint4*>(s_cp)[(tid * SC_ITEMS + k) / 4] =
+                            make_uint4(pl[k], pl[k + 1], pl[k + 2], pl[k + 3]);
+                    __syncthreads();
+                    op.all(tbase, s_cp, N);
+                }
             }
         } else {
 #pragma unroll
