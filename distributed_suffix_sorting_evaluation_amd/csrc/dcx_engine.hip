@@ -917,7 +917,7 @@ class Engine {
         if (n > 80 && (double)nu / (double)m <= thr) {
             const uint32_t nc = 2 * nu + 1;
             DBuf<uint32_t> cand(cache, nc);
-            prim.scan(NonUniqueOp{Hs.get(), s_order.get(), dc, m, cand.get()}, m, 0);
+            prim.scan_after_total(NonUniqueOp{Hs.get(), s_order.get(), dc, m, cand.get()}, m, 0);
             prim.fill_u32(cand.get() + 2 * nu, 0, 1);  // element 0 is always kept
             DBuf<uint32_t> k1(cache, nc), v1(cache, nc), k2(cache, nc), v2(cache, nc);
             uint32_t *ks = nullptr, *vs = nullptr;

@@ -1806,6 +1806,22 @@ struct Primitives {
                            scalars + 2 * slot);
     }
 
+    // scan() right after total() over the same N and the same values (ops of one type that differ only in
+    // where they write), with no scan in between: the block prefixes total() left in scan_prefix are
+    // reused, so only the down pass runs (short inputs take the single pass again)
+    template <class Op>
+    void scan_after_total(const Op& op, uint64_t N, int slot) {
+        const uint64_t tiles = ceil_div(N, SC_TILE);
+        if (N == 0 || tiles <= SC_SINGLE_PASS_TILES) {
+            scan(op, N, slot);
+            return;
+        }
+        const uint32_t tpb = (uint32_t)ceil_div(tiles, (uint64_t)scan_blocks<Op>(tiles));
+        const uint32_t nb = (uint32_t)ceil_div(tiles, (uint64_t)tpb);
+        hipLaunchKernelGGL(k_scan_down<Op>, dim3(nb), dim3(SC_THREADS), 0, stream, op, N, tpb,
+                           (const uint64_t*)scan_prefix);
+    }
+
     // `bytes` (at most 8 * MAIL_WORDS) of device memory at `src` (8-byte aligned, readable up to the next
     // multiple of 8), ordered after the work on the stream: published by k_publish into the mailbox and
     // awaited by spinning on its sequence number.  The stream is queried now and then, so that a failed
